@@ -1,0 +1,1168 @@
+/*
+ * npme_oracle.c -- TEST INFRASTRUCTURE ONLY (see npme_oracle.h).
+ *
+ * FP64 CPU restatement of the NPME CalcV path.  Written from the
+ * reference's algorithm description; every function cites the reference
+ * file:line it follows.  OpenMP parallelism is over independent targets or
+ * grid lines only, so every result is deterministic.
+ */
+#include "npme_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "rsem_table.inc"
+
+typedef double complex cplx;
+
+static const double PI = 3.14159265358979323846;
+
+/* ======================================================================
+ * B-splines.  Cardinal B-spline M_n(w) on [0, n], zero outside
+ * (src/Bspline.cpp:16714 dispatch; piecewise polynomials e.g. n=6 at
+ * :127-175, outside-support test at :15421).  Restated with the
+ * Cox-de Boor recursion evaluated for all integer shifts of the
+ * fractional part.  Pinned by: exact integer-point values, partition of
+ * unity, symmetry, derivative identity (tests/test_oracle_core.py).
+ * ==================================================================== */
+static void bspline_shifts(int n, double t, double *out)
+/* out[j] = M_n(t + j), j = 0..n-1, for t in [0, 1] */
+{
+  double m[64];
+  if (n == 1) { out[0] = 1.0; return; }
+  m[0] = t;          /* M_2(t)     */
+  m[1] = 1.0 - t;    /* M_2(t + 1) */
+  for (int k = 3; k <= n; k++) {
+    /* M_k(t+j) = ((t+j) M_{k-1}(t+j) + (k-t-j) M_{k-1}(t+j-1)) / (k-1) */
+    double nm[64];
+    for (int j = 0; j < k; j++) {
+      double left  = (j < k - 1) ? m[j] : 0.0;      /* M_{k-1}(t+j)   */
+      double right = (j >= 1) ? m[j - 1] : 0.0;     /* M_{k-1}(t+j-1) */
+      nm[j] = ((t + j) * left + (k - t - j) * right) / (k - 1);
+    }
+    for (int j = 0; j < k; j++) m[j] = nm[j];
+  }
+  for (int j = 0; j < n; j++) out[j] = m[j];
+}
+
+double npo_bspline(int n, double w)
+{
+  if (w < 0.0 || w > (double) n) return 0.0;
+  if (n == 1) return (w < 1.0) ? 1.0 : 0.0;
+  long j0 = (long) floor(w);
+  if (j0 >= n) return 0.0;                /* w == n */
+  double sh[64];
+  bspline_shifts(n, w - (double) j0, sh);
+  return sh[j0];
+}
+
+double npo_bspline_dw(int n, double w, double *dBdw)
+/* dM_n/dw = M_{n-1}(w) - M_{n-1}(w-1)  (src/Bspline.cpp:16766) */
+{
+  *dBdw = npo_bspline(n - 1, w) - npo_bspline(n - 1, w - 1.0);
+  return npo_bspline(n, w);
+}
+
+/* ======================================================================
+ * Derivative-match split.
+ * radial derivatives f_n = (1/r d/dr)^n f at r = Rdir:
+ *   r^alpha   src/FunctionDerivMatch.cpp:78-91
+ *   Helmholtz src/FunctionDerivMatch.cpp:42-55 (Leibniz recurrence)
+ * even series P(r) = sum_p a_p r^{2p} with (1/r d/dr)^n P (Rdir) = f_n,
+ * an upper-triangular system C_{n,p} = 2^n p!/(p-n)! Rdir^{2(p-n)}
+ * (src/FunctionDerivMatch.cpp:226-266), b_i = 2(i+1) a_{i+1}.
+ * Pinned by: preprint Table 1 known answer, derivative-match identity,
+ * the scaling law a_n(R) = a_n(1) R^{alpha-2n}.
+ * ==================================================================== */
+static double fact(int n) { double f = 1; for (int i = 2; i <= n; i++) f *= i; return f; }
+
+static double dm_C(int n, int p, double R)
+{
+  if (n > p) return 0.0;
+  return pow(R * R, p - n) * pow(2.0, n) * fact(p) / fact(p - n);
+}
+
+static void dm_solve(int Nder, double R, const cplx *f, cplx *a, cplx *b)
+{
+  for (int n = Nder; n >= 0; n--) {
+    cplx s = f[n];
+    for (int p = n + 1; p <= Nder; p++) s -= dm_C(n, p, R) * a[p];
+    a[n] = s / dm_C(n, n, R);
+  }
+  for (int i = 0; i < Nder; i++) b[i] = 2.0 * (i + 1) * a[i + 1];
+  b[Nder] = 0;
+}
+
+int npo_kernel_init(npo_kernel *k, int kind, double alpha, double k0r,
+                    double k0i, int Nder, double Rdir)
+{
+  memset(k, 0, sizeof(*k));
+  if (Nder < 2 || Nder > 30) return -1;
+  k->kind = kind;
+  k->alpha = (kind == NPO_LAPLACE) ? -1.0 : alpha;
+  k->k0r = k0r;
+  k->k0i = k0i;
+  k->Nder = Nder;
+  k->Rdir = Rdir;
+  cplx f[32], a[32], b[32];
+  const double r2 = Rdir * Rdir;
+  if (kind == NPO_HELMHOLTZ) {
+    const cplx k0 = k0r + I * k0i;
+    const cplx C = cexp(I * k0 * Rdir);
+    f[0] = C / Rdir;
+    if (Nder >= 1) f[1] = -C / (Rdir * r2) + I * k0 * C / r2;
+    for (int n = 2; n <= Nder; n++)
+      f[n] = -(2.0 * n - 1.0) / r2 * f[n - 1] - k0 * k0 / r2 * f[n - 2];
+  } else {
+    f[0] = pow(Rdir, k->alpha);
+    for (int n = 1; n <= Nder; n++)
+      f[n] = f[n - 1] * (k->alpha - 2 * n + 2) / r2;
+  }
+  dm_solve(Nder, Rdir, f, a, b);
+  for (int i = 0; i <= Nder; i++) {
+    k->a[2 * i] = creal(a[i]); k->a[2 * i + 1] = cimag(a[i]);
+    k->b[2 * i] = creal(b[i]); k->b[2 * i + 1] = cimag(b[i]);
+  }
+  return 0;
+}
+
+static cplx horner(const double *c, int deg, double r2)
+{
+  cplx s = c[2 * deg] + I * c[2 * deg + 1];
+  for (int i = deg - 1; i >= 0; i--) s = s * r2 + (c[2 * i] + I * c[2 * i + 1]);
+  return s;
+}
+
+static void full_kernel(const npo_kernel *k, double r2, cplx *f0, cplx *f1)
+/* Laplace src/KernelFunctionLaplace.cpp (NPME_Kfunc_Laplace::Calc),
+ * r^alpha src/KernelFunctionRalpha.cpp:57-82,
+ * Helmholtz src/KernelFunctionHelmholtz.cpp:162-200 */
+{
+  const double r = sqrt(fabs(r2));
+  if (k->kind == NPO_LAPLACE) {
+    *f0 = 1.0 / r;
+    *f1 = -1.0 / (r * r2);
+  } else if (k->kind == NPO_RALPHA) {
+    double v = pow(r2, k->alpha / 2);
+    *f0 = v;
+    *f1 = k->alpha * v / r2;
+  } else {
+    const cplx k0 = k->k0r + I * k->k0i;
+    cplx v = cexp(I * k0 * r) / r;
+    *f0 = v;
+    *f1 = (I * k0 * v - v / r) / r;
+  }
+}
+
+void npo_kernel_eval(const npo_kernel *k, int part, double r2, double *f0o,
+                     double *f1o)
+/* LR/SR cutoff tests follow each kernel class:
+ *  Laplace   LR/SR: r > Rdir is outside  (KernelFunctionLaplace.cpp:394-437,
+ *            :762-803)
+ *  r^alpha   LR/SR: r2 > Rdir^2 is outside (KernelFunctionRalpha.cpp:343-385,
+ *            :718-758)
+ *  Helmholtz LR: r > Rdir outside; SR: r < Rdir inside
+ *            (KernelFunctionHelmholtz.cpp:506-560, :962-1029) */
+{
+  cplx f0 = 0, f1 = 0;
+  const double R = k->Rdir;
+  const double r = sqrt(fabs(r2));
+  int inside;
+  if (k->kind == NPO_RALPHA) inside = !(r2 > R * R);
+  else if (k->kind == NPO_HELMHOLTZ && part == NPO_SR) inside = (r < R);
+  else inside = !(r > R);
+
+  if (part == NPO_FULL) {
+    full_kernel(k, r2, &f0, &f1);
+  } else if (part == NPO_LR) {
+    if (inside) {
+      f0 = horner(k->a, k->Nder, r2);
+      f1 = horner(k->b, k->Nder - 1, r2);
+    } else {
+      full_kernel(k, r2, &f0, &f1);
+    }
+  } else {
+    if (inside) {
+      cplx e0, e1;
+      full_kernel(k, r2, &e0, &e1);
+      f0 = e0 - horner(k->a, k->Nder, r2);
+      f1 = e1 - horner(k->b, k->Nder - 1, r2);
+    }
+  }
+  if (k->kind == NPO_HELMHOLTZ) {
+    f0o[0] = creal(f0); f0o[1] = cimag(f0);
+    f1o[0] = creal(f1); f1o[1] = cimag(f1);
+  } else {
+    f0o[0] = creal(f0);
+    f1o[0] = creal(f1);
+  }
+}
+
+/* ======================================================================
+ * FFT size picker: smallest multiple of 4 >= n whose prime factors are in
+ * {2,3,5,7}; block size m with N % 2m == 0, m <= N/4 closest to the ideal
+ * (first minimum wins).  src/SupportFunctions.cpp:384-455.
+ * Pinned by brute-force enumeration in the tests.
+ * ==================================================================== */
+static int smooth_2357(long n)
+{
+  const long p[4] = {2, 3, 5, 7};
+  for (int i = 0; i < 4; i++) while (n % p[i] == 0) n /= p[i];
+  return n == 1;
+}
+
+long npo_optimal_grid_size(long n)
+{
+  for (long N = 4; N <= 2 * n + 4; N += 4)
+    if (N >= n && smooth_2357(N)) return N;
+  return -1;
+}
+
+void npo_fft_size_block(long Nideal, long nideal, long *N, long *nblk)
+{
+  long NN = npo_optimal_grid_size(Nideal);
+  long minDiff = NN, best = 1;
+  for (long m = 2; m < NN; m++) {
+    if (NN % (2 * m) == 0 && m <= NN / 4) {
+      long d = labs(m - nideal);
+      if (minDiff > d) { minDiff = d; best = m; }
+    }
+  }
+  *N = NN;
+  *nblk = best;
+}
+
+/* ======================================================================
+ * Geometry.  Box extent and centre (src/RecSumSupportFunctions.cpp:149-200);
+ * PME-corrected half-box X = max(X0, X0/a1 - del/2 a2/a1),
+ * a1 = 1 - 2(n+1)/N, a2 = 1 - 4(n+1)/N, R = R0 + (X0 - X)/2
+ * (src/RecSumSupportFunctions.cpp:137-147, :276-296).
+ * ==================================================================== */
+void npo_box_dim_center(size_t n, const double *coord, double X0[3],
+                        double R0[3])
+{
+  double mx[3] = {-1e12, -1e12, -1e12}, mn[3] = {1e12, 1e12, 1e12};
+  for (size_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      double x = coord[3 * i + d];
+      if (mx[d] < x) mx[d] = x;
+      if (mn[d] > x) mn[d] = x;
+    }
+  for (int d = 0; d < 3; d++) {
+    X0[d] = mx[d] - mn[d];
+    R0[d] = 0.5 * (mx[d] + mn[d]);
+  }
+}
+
+static double min_x_compact(double X0, double del, long Bn, long N)
+{
+  const double a1 = 1.0 - 2.0 * (Bn + 1) / N;
+  const double a2 = 1.0 - 4.0 * (Bn + 1) / N;
+  double X = X0 / a1 - del / 2.0 * a2 / a1;
+  if (X0 > X) X = X0;
+  return X;
+}
+
+void npo_pme_corr_box(const double X0[3], const double R0[3], const long N[3],
+                      const double del[3], long Bn, double X[3], double R[3])
+{
+  for (int d = 0; d < 3; d++) {
+    X[d] = min_x_compact(X0[d], del[d], Bn, N[d]);
+    R[d] = R0[d] + 0.5 * (X0[d] - X[d]);
+  }
+}
+
+/* ======================================================================
+ * RSEM grid chooser.
+ * eps_V = Q_d Rdir^alpha (Rdir/L_d)^1.5 exp(P(u_d)), u_d = N_d Rdir / L_d,
+ * P a quartic (static) or a bi-quartic in (u, w = |k0| Rdir) (Helmholtz);
+ * 10 fixed-point iterations starting from N = 100 and the uncorrected box,
+ * then the size/block picker.  src/EstimateParm.cpp:62-102 (u search),
+ * :110-181 (min-grid Nder choice), :209-417 (iteration), :421-558
+ * (predicted error), :2571-2888 (Helmholtz).
+ * Pinned by: grid choices replayed independently for the metric and
+ * config inputs (N = 120^3, 512^3, 432^3, 280^3).
+ * ==================================================================== */
+static int find_u(double *u, double y0, const double a[5], double uMin,
+                  double uMax)
+{
+  const int NumTrial = 1000;
+  const double del = (uMax - uMin) / (NumTrial - 1);
+  double minDiff = 1.0e6, yPrev = 1.0e6;
+  *u = uMin;
+  for (int n = 0; n < NumTrial; n++) {
+    double ut = uMin + n * del;
+    double y = 0, up = 1;
+    for (int p = 0; p <= 4; p++) { y += a[p] * up; up *= ut; }
+    double diff = fabs(y - y0);
+    if (minDiff > diff) { minDiff = diff; *u = ut; }
+    if (minDiff < 0.1) break;
+    if (y > yPrev) break;
+    yPrev = y;
+  }
+  return 0;
+}
+
+static double qsum_real(size_t n, const double *q)
+{
+  double s = 0; for (size_t i = 0; i < n; i++) s += q[i] * q[i]; return sqrt(s);
+}
+
+static double qsum_cplx(size_t n, const double *qc)
+{
+  double s = 0;
+  for (size_t i = 0; i < n; i++) s += qc[2 * i] * qc[2 * i] + qc[2 * i + 1] * qc[2 * i + 1];
+  return sqrt(s);
+}
+
+static void rsem_iterate(size_t n, const double *coord, double Q0, long Bn,
+                         double alphaExp, double tol, double Rdir,
+                         const double a[5], double uMin, double uMax,
+                         long N[3], long nblk[3])
+/* alphaExp: the Rdir^(-alpha) factor exponent (Helmholtz uses -1) */
+{
+  const double del = Rdir;
+  double X0[3], R0[3];
+  npo_box_dim_center(n, coord, X0, R0);
+  N[0] = N[1] = N[2] = 100;
+  for (int it = 0; it < 10; it++) {
+    double X[3], R[3], dl[3] = {del, del, del};
+    if (it == 0) { X[0] = X0[0]; X[1] = X0[1]; X[2] = X0[2]; }
+    else npo_pme_corr_box(X0, R0, N, dl, Bn, X, R);
+    double L[3];
+    for (int d = 0; d < 3; d++) L[d] = 2 * (X[d] + del);
+    const double V = L[0] * L[1] * L[2];
+    const double C = tol * pow(Rdir, -alphaExp);
+    for (int d = 0; d < 3; d++) {
+      double Qd = Q0 * sqrt(L[d] * L[d] * L[d] / V);
+      double t = C / Qd * pow(L[d] / Rdir, 1.5);
+      double u;
+      find_u(&u, log(t), a, uMin, uMax);
+      N[d] = (long) (u * L[d] / Rdir) + 1;
+      if (N[d] < 4 * Bn + 1) N[d] = 4 * Bn + 1;
+    }
+  }
+  for (int d = 0; d < 3; d++) npo_fft_size_block(N[d], 20, &N[d], &nblk[d]);
+}
+
+int npo_rsem_static(size_t n, const double *q, const double *coord, long Bn,
+                    double alpha, double tol, double Rdir, int nDerivIn,
+                    long N[3], long nblk[3], int *NderOut)
+{
+  const double Q0 = qsum_real(n, q);
+  long bestSize = -1;
+  int found = 0;
+  for (int r = 0; r < NPME_RSEM_N_STATIC; r++) {
+    const double *row = npme_rsem_static[r];
+    if ((long) row[0] != Bn || fabs(alpha - row[2]) >= 1e-6) continue;
+    if (nDerivIn >= 0 && (int) row[1] != nDerivIn) continue;
+    long Nt[3], nt[3];
+    rsem_iterate(n, coord, Q0, Bn, alpha, tol, Rdir, row + 5, row[3], row[4], Nt, nt);
+    long sz = Nt[0] * Nt[1] * Nt[2];
+    if (!found || bestSize > sz) {
+      bestSize = sz;
+      for (int d = 0; d < 3; d++) { N[d] = Nt[d]; nblk[d] = nt[d]; }
+      *NderOut = (int) row[1];
+      found = 1;
+    }
+    if (nDerivIn >= 0) break;
+  }
+  return found ? 0 : -1;
+}
+
+int npo_rsem_static_pred_error(size_t n, const double *q, const double *coord,
+                               long Bn, double alpha, int Nder, double Rdir,
+                               const long N[3], double *eps, int *reliable)
+{
+  const double *row = NULL;
+  for (int r = 0; r < NPME_RSEM_N_STATIC; r++) {
+    const double *t = npme_rsem_static[r];
+    if ((long) t[0] == Bn && (int) t[1] == Nder && fabs(alpha - t[2]) < 1e-6) { row = t; break; }
+  }
+  if (!row) return -1;
+  const double del = Rdir, Q0 = qsum_real(n, q);
+  double X0[3], R0[3], X[3], R[3], dl[3] = {del, del, del}, L[3];
+  npo_box_dim_center(n, coord, X0, R0);
+  npo_pme_corr_box(X0, R0, N, dl, Bn, X, R);
+  for (int d = 0; d < 3; d++) L[d] = 2 * (X[d] + del);
+  const double V = L[0] * L[1] * L[2];
+  *reliable = 1;
+  double e = 0;
+  for (int d = 0; d < 3; d++) {
+    double u = N[d] * Rdir / L[d];
+    if (u < row[3] || u > row[4]) *reliable = 0;
+    double P = 0, up = 1;
+    for (int p = 0; p <= 4; p++) { P += row[5 + p] * up; up *= u; }
+    double Qd = Q0 * sqrt(L[d] * L[d] * L[d] / V);
+    double ed = Qd * pow(Rdir, alpha) * pow(Rdir / L[d], 1.5) * exp(P);
+    if (ed > e) e = ed;
+  }
+  *eps = e;
+  return 0;
+}
+
+int npo_rsem_helmholtz(size_t n, const double *qc, const double *coord,
+                       long Bn, double k0r, double k0i, double tol,
+                       double Rdir, int nDerivIn, long N[3], long nblk[3],
+                       int *NderOut)
+{
+  const double Q0 = qsum_cplx(n, qc);
+  const double w = hypot(k0r, k0i) * Rdir;
+  long bestSize = -1;
+  int found = 0;
+  for (int r = 0; r < NPME_RSEM_N_HELM; r++) {
+    const double *row = npme_rsem_helm[r];
+    if ((long) row[0] != Bn) continue;
+    if (nDerivIn >= 0 && (int) row[1] != nDerivIn) continue;
+    if (w < row[4] || w > row[5]) return -2;
+    double b[5], wq[5];
+    wq[0] = 1; for (int q = 1; q < 5; q++) wq[q] = wq[q - 1] * w;
+    for (int p = 0; p < 5; p++) {
+      b[p] = 0;
+      for (int q = 0; q < 5; q++) b[p] += row[6 + p * 5 + q] * wq[q];
+    }
+    long Nt[3], nt[3];
+    rsem_iterate(n, coord, Q0, Bn, -1.0, tol, Rdir, b, row[2], row[3], Nt, nt);
+    long sz = Nt[0] * Nt[1] * Nt[2];
+    if (!found || bestSize > sz) {
+      bestSize = sz;
+      for (int d = 0; d < 3; d++) { N[d] = Nt[d]; nblk[d] = nt[d]; }
+      *NderOut = (int) row[1];
+      found = 1;
+    }
+    if (nDerivIn >= 0) break;
+  }
+  return found ? 0 : -1;
+}
+
+/* ======================================================================
+ * Embedded cell partition.  cellB = Rdir/nNeigh, cellA = nCellClust1D
+ * cellB; cell index (size_t)(x / cellSize) in FP64 after translating by the
+ * box minimum, with the 1e-14 upper-edge correction.
+ * src/PartitionBox.cpp:146-194 (box), :311-351 (index),
+ * src/PartitionEmbeddedBox.cpp:300-455 (embedded indices).
+ * Pinned by: CheckCoordInCell-style containment invariant in the tests.
+ * ==================================================================== */
+static int coord2cell(double x, double cs, unsigned long ncell, unsigned long *m)
+{
+  const double tol = 1.0e-14;
+  if (x < -tol) return -1;
+  unsigned long v = (unsigned long) (x / cs);
+  if (v > ncell) return -1;
+  if (v == ncell) {
+    if (fabs(x - v * cs) < tol) v--;
+    else return -1;
+  }
+  *m = v;
+  return 0;
+}
+
+int npo_cell_index(size_t n, const double *coord, double Rdir, long nNeigh,
+                   long nCellClust1D, long nCellA[3], unsigned long *totA,
+                   unsigned long *totB)
+{
+  if (n < 1) return -1;
+  const double csB = Rdir / nNeigh;
+  const unsigned long nP = (unsigned long) nCellClust1D;
+  const double csA = nP * csB;
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = coord[d];
+  for (size_t i = 1; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      double x = coord[3 * i + d];
+      if (mn[d] > x) mn[d] = x;
+      if (mx[d] < x) mx[d] = x;
+    }
+  unsigned long nA[3];
+  for (int d = 0; d < 3; d++) {
+    double L = mx[d] - mn[d];
+    nA[d] = (unsigned long) (L / csA) + 1;
+    nCellA[d] = (long) nA[d];
+  }
+  const unsigned long Nc = nP * nP * nP;
+  for (size_t i = 0; i < n; i++) {
+    unsigned long a[3], b[3];
+    for (int d = 0; d < 3; d++) {
+      double x1 = coord[3 * i + d] + (-mn[d]);
+      if (coord2cell(x1, csA, nA[d], &a[d])) return -1;
+      double x2 = x1 - a[d] * csA;
+      if (coord2cell(x2, csB, nP, &b[d])) return -1;
+    }
+    unsigned long tA = (a[0] * nA[1] + a[1]) * nA[2] + a[2];
+    unsigned long loc = (b[0] * nP + b[1]) * nP + b[2];
+    totA[i] = tA;
+    totB[i] = tA * Nc + loc;
+  }
+  return 0;
+}
+
+/* ======================================================================
+ * FFT: in-place complex DFT, X[k] = sum_n x[n] exp(sign 2 pi i n k / N),
+ * no normalisation.  This replaces the MKL DFTI calls
+ * NPME_1D_FFT_NoNorm / NPME_3D_FFT_NoNorm (src/ExtLibrary.cpp:930-1033,
+ * forward = sign -1).  Recursive mixed-radix Cooley-Tukey with an FP64
+ * root table.  Pinned by numpy.fft golden vectors and a brute-force DFT.
+ * ==================================================================== */
+static long small_factor(long N)
+{
+  for (long p = 2; p * p <= N; p++) if (N % p == 0) return p;
+  return N;
+}
+
+static void fft_rec(long N, const cplx *in, long s, cplx *out,
+                    const cplx *tw, long N0)
+/* out[k] = sum_n in[n s] w_N^{n k}, w_N = tw[N0/N] */
+{
+  if (N == 1) { out[0] = in[0]; return; }
+  const long p = small_factor(N), m = N / p, step = N0 / N;
+  for (long r = 0; r < p; r++) fft_rec(m, in + r * s, s * p, out + r * m, tw, N0);
+  cplx t[64], u[64];
+  cplx *tp = (p <= 64) ? t : (cplx *) malloc(sizeof(cplx) * p);
+  cplx *up = (p <= 64) ? u : (cplx *) malloc(sizeof(cplx) * p);
+  for (long k = 0; k < m; k++) {
+    for (long r = 0; r < p; r++) tp[r] = out[r * m + k] * tw[((r * k) % N) * step];
+    for (long q = 0; q < p; q++) {
+      cplx acc = 0;
+      for (long r = 0; r < p; r++) acc += tp[r] * tw[((r * q * m) % N) * step];
+      up[q] = acc;
+    }
+    for (long q = 0; q < p; q++) out[k + q * m] = up[q];
+  }
+  if (p > 64) { free(tp); free(up); }
+}
+
+static cplx *make_roots(long N, int sign)
+{
+  cplx *tw = (cplx *) malloc(sizeof(cplx) * N);
+  for (long j = 0; j < N; j++) {
+    double ang = 2.0 * PI * (double) j / (double) N;
+    tw[j] = cos(ang) + I * (sign * sin(ang));
+  }
+  return tw;
+}
+
+static void fft1d_tw(long N, cplx *z, const cplx *tw, cplx *tmp)
+{
+  memcpy(tmp, z, sizeof(cplx) * N);
+  fft_rec(N, tmp, 1, z, tw, N);
+}
+
+void npo_fft1d(long N, double *zd, int sign)
+{
+  cplx *z = (cplx *) zd;
+  cplx *tw = make_roots(N, sign);
+  cplx *tmp = (cplx *) malloc(sizeof(cplx) * N);
+  fft1d_tw(N, z, tw, tmp);
+  free(tmp);
+  free(tw);
+}
+
+static void fft_axis(long n1, long L, long n3, cplx *z, int sign)
+/* transform the middle axis of z[n1][L][n3] */
+{
+  cplx *tw = make_roots(L, sign);
+  long c;
+#pragma omp parallel for schedule(static)
+  for (c = 0; c < n1 * n3; c++) {
+    long i1 = c / n3, i3 = c % n3;
+    cplx *buf = (cplx *) malloc(sizeof(cplx) * L * 2);
+    cplx *col = buf, *tmp = buf + L;
+    for (long i = 0; i < L; i++) col[i] = z[(i1 * L + i) * n3 + i3];
+    fft1d_tw(L, col, tw, tmp);
+    for (long i = 0; i < L; i++) z[(i1 * L + i) * n3 + i3] = col[i];
+    free(buf);
+  }
+  free(tw);
+}
+
+void npo_fft3d(long N1, long N2, long N3, double *zd, int sign)
+{
+  cplx *z = (cplx *) zd;
+  fft_axis(N1 * N2, N3, 1, z, sign);
+  fft_axis(N1, N2, N3, z, sign);
+  fft_axis(1, N1, N2 * N3, z, sign);
+}
+
+/* ======================================================================
+ * Compact Fourier grid of the smooth kernel.
+ * Real-space values on the octant x_n = n L/N, n = 0..N/2, as the sum over
+ * the 8 overlapped images weighted by smooth top-hats
+ * (src/RecSumGrid.cpp:45-162 CalcGenFunc, :388-484 CompactReal_Func,
+ * :1079-1096 xArray, :1124-1231 Hramp/Tsmooth/OverlapSmoothFunc), then a
+ * forward 1-D DFT along each axis of the even extension, folded back
+ * (src/RecSumGrid.cpp:483-582).
+ * Pinned by: real-space values against f_LR inside the flat region, the
+ * even-extension DFT against numpy, and the whole path against the exact
+ * sum.
+ * ==================================================================== */
+static double hramp(double x, double a)
+{
+  const double erf_max = 6.0;
+  if (x < -1.0) return 0.0;
+  if (x > 1.0) return 1.0;
+  const double z = erf_max / a;
+  const double xMax = z / sqrt(1.0 + z * z);
+  if (x >= xMax) return 1.0;
+  if (x <= -xMax) return 0.0;
+  return 0.5 * (1.0 + erf(a * x / sqrt(1.0 - x * x)));
+}
+
+static int tsmooth(double x, double X, double Xmax, double a, double *T)
+{
+  const double eps = 1.0e-12 * Xmax;
+  if (x < -Xmax - eps) return -1;
+  if (x < -X) {
+    double del = (Xmax - X) * 0.5;
+    *T = hramp((x + X + del) / del, a);
+  } else if (x <= X) {
+    *T = 1.0;
+  } else if (x < Xmax + eps) {
+    double del = (Xmax - X) * 0.5;
+    *T = hramp((-x + X + del) / del, a);
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+static int overlap_smooth(long M, const double *x1, double X, double Xmax,
+                          double a, double *T1, double *T2, double *x2)
+{
+  const double eps = 1.0e-12 * Xmax;
+  const double Xt = 2 * Xmax - X;
+  for (long i = 0; i < M; i++) {
+    if (x1[i] < -Xmax - eps || x1[i] > Xmax + eps) return -1;
+    if (x1[i] > X) {
+      if (tsmooth(x1[i], X, Xt, a, &T1[i])) return -1;
+      x2[i] = x1[i] - 2 * Xmax;
+      if (tsmooth(x2[i], X, Xt, a, &T2[i])) return -1;
+    } else if (x1[i] > -X) {
+      T1[i] = 1.0; x2[i] = 1.0; T2[i] = 0.0;
+    } else {
+      if (tsmooth(x1[i], X, Xt, a, &T1[i])) return -1;
+      x2[i] = x1[i] + 2 * Xmax;
+      if (tsmooth(x2[i], X, Xt, a, &T2[i])) return -1;
+    }
+  }
+  return 0;
+}
+
+static void even_dft_axis(long n1, long L2, long n3, long Nfull, cplx *f)
+/* each line of length L2 = Nfull/2+1 along the middle axis: even-extend,
+ * forward DFT, fold back (src/RecSumGrid.cpp:483-512 Compact2Full /
+ * Full2Compact) */
+{
+  cplx *tw = make_roots(Nfull, -1);
+  long c;
+#pragma omp parallel for schedule(static)
+  for (c = 0; c < n1 * n3; c++) {
+    long i1 = c / n3, i3 = c % n3;
+    cplx *buf = (cplx *) malloc(sizeof(cplx) * Nfull * 2);
+    cplx *xF = buf, *tmp = buf + Nfull;
+    const long h = Nfull / 2;
+    for (long i = 0; i < h; i++) xF[i] = f[(i1 * L2 + i) * n3 + i3];
+    for (long i = h; i < Nfull; i++) xF[i] = f[(i1 * L2 + (Nfull - i)) * n3 + i3];
+    fft1d_tw(Nfull, xF, tw, tmp);
+    f[(i1 * L2 + 0) * n3 + i3] = xF[0];
+    f[(i1 * L2 + h) * n3 + i3] = xF[h];
+    for (long i = 1; i < h; i++) f[(i1 * L2 + i) * n3 + i3] = 0.5 * (xF[i] + xF[Nfull - i]);
+    free(buf);
+  }
+  free(tw);
+}
+
+int npo_fgrid_compact(const npo_kernel *k, const long N[3], const double a[3],
+                      const double del[3], const double X[3], double *Fd)
+{
+  cplx *F = (cplx *) Fd;
+  long M[3];
+  double *x1[3], *x2[3], *T1[3], *T2[3];
+  int err = 0;
+  for (int d = 0; d < 3; d++) {
+    M[d] = N[d] / 2 + 1;
+    x1[d] = (double *) malloc(sizeof(double) * M[d]);
+    x2[d] = (double *) malloc(sizeof(double) * M[d]);
+    T1[d] = (double *) malloc(sizeof(double) * M[d]);
+    T2[d] = (double *) malloc(sizeof(double) * M[d]);
+    const double Xmax = X[d] + del[d];
+    const double L = 2 * Xmax, dx = L / N[d];
+    for (long i = 0; i < M[d]; i++) x1[d][i] = i * dx;
+    if (overlap_smooth(M[d], x1[d], X[d], Xmax, a[d], T1[d], T2[d], x2[d])) err = -1;
+  }
+  if (!err) {
+    long c;
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < M[0] * M[1]; c++) {
+      long i1 = c / M[1], i2 = c % M[1];
+      for (long i3 = 0; i3 < M[2]; i3++) {
+        cplx s = 0;
+        for (int ax = 0; ax < 2; ax++)
+          for (int ay = 0; ay < 2; ay++)
+            for (int az = 0; az < 2; az++) {
+              double tx = ax ? T2[0][i1] : T1[0][i1];
+              double ty = ay ? T2[1][i2] : T1[1][i2];
+              double tz = az ? T2[2][i3] : T1[2][i3];
+              double x = ax ? x2[0][i1] : x1[0][i1];
+              double y = ay ? x2[1][i2] : x1[1][i2];
+              double z = az ? x2[2][i3] : x1[2][i3];
+              double f0[2], f1[2];
+              npo_kernel_eval(k, NPO_LR, x * x + y * y + z * z, f0, f1);
+              cplx fv = (k->kind == NPO_HELMHOLTZ) ? (f0[0] + I * f0[1]) : f0[0];
+              s += tx * ty * tz * fv;
+            }
+        F[(i1 * M[1] + i2) * M[2] + i3] = s;
+      }
+    }
+    even_dft_axis(M[0] * M[1], M[2], 1, N[2], F);
+    even_dft_axis(M[0], M[1], M[2], N[1], F);
+    even_dft_axis(1, M[0], M[1] * M[2], N[0], F);
+  }
+  for (int d = 0; d < 3; d++) { free(x1[d]); free(x2[d]); free(T1[d]); free(T2[d]); }
+  return err;
+}
+
+/* ======================================================================
+ * |lambda(k)|^2 = 1 / |sum_{p=0}^{n} M_n(p) exp(-2 pi i k p / N)|^2 with k
+ * the signed frequency (src/RecSumV.cpp:565-581).
+ * Pinned by: direct evaluation of the definition in numpy.
+ * ==================================================================== */
+void npo_lambda2(long N, long Bn, double *lam2)
+{
+  for (long idx = 0; idx < N; idx++) {
+    long kk = (idx >= N / 2) ? idx - N : idx;
+    double z = (double) kk / (double) N;
+    cplx den = 0;
+    for (int p = 0; p <= (int) Bn; p++) den += npo_bspline((int) Bn, p) * cexp(-2.0 * PI * I * z * p);
+    cplx l = 1.0 / den;
+    lam2[idx] = creal(l * conj(l));
+  }
+}
+
+/* ======================================================================
+ * Spreading / interpolation index helpers
+ * (src/RecSumSupportFunctions.h:123-171):
+ *   mStart = (wX < 0) ? (long)(-wX) + 1 : (long)(-wX),
+ *   m in [mStart, mStart+n-1] must lie in [-N/4, N/4-1],
+ *   compact array index p = m < 0 ? m + N/2 : m.
+ * ==================================================================== */
+static long qstart(double w) { return (w < 0) ? (long) (-w) + 1 : (long) (-w); }
+static int qcheck(long ms, long N, long Bn) { return (ms >= -N / 4) && (ms + Bn - 1 <= N / 4 - 1); }
+static long dft2arr(long m, long M) { return (m < 0) ? m + M : m; }
+
+int npo_spread_basic(size_t n, const double *q, int isComplex,
+                     const double *coord, const long N[3], const double L[3],
+                     long Bn, const double R[3], double *Qd)
+/* src/RecSumQ.cpp:93-168 */
+{
+  cplx *Q = (cplx *) Qd;
+  const long M1 = N[0] / 2, M2 = N[1] / 2, M3 = N[2] / 2;
+  memset(Q, 0, sizeof(cplx) * M1 * M2 * M3);
+  for (size_t i = 0; i < n; i++) {
+    double w[3];
+    long ms[3];
+    double B[3][64];
+    for (int d = 0; d < 3; d++) {
+      w[d] = N[d] / L[d] * (coord[3 * i + d] - R[d]);
+      ms[d] = qstart(w[d]);
+      if (!qcheck(ms[d], N[d], Bn)) return -1;
+      for (long j = 0; j < Bn; j++) B[d][j] = npo_bspline((int) Bn, w[d] + ms[d] + j);
+    }
+    cplx qi = isComplex ? conj(q[2 * i] + I * q[2 * i + 1]) : q[i];
+    for (long j1 = 0; j1 < Bn; j1++) {
+      long p1 = dft2arr(ms[0] + j1, M1);
+      for (long j2 = 0; j2 < Bn; j2++) {
+        long p2 = dft2arr(ms[1] + j2, M2);
+        cplx c = qi * B[0][j1] * B[1][j2];
+        for (long j3 = 0; j3 < Bn; j3++) {
+          long p3 = dft2arr(ms[2] + j3, M3);
+          Q[(p1 * M2 + p2) * M3 + p3] += c * B[2][j3];
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+void npo_theta_8parity(const long N[3], long Bn, const double *Fd,
+                       const double *Qd, double *Td)
+/* src/RecSumV.cpp:766-840 (CalcTheta), src/RecSumQ.cpp:456-522 (C twiddle,
+ * CalcQfft), src/RecSumV.cpp:639-707 (compact-F k-space multiply),
+ * :709-761 (twiddle accumulate) */
+{
+  const cplx *F = (const cplx *) Fd;
+  const cplx *Q = (const cplx *) Qd;
+  cplx *T = (cplx *) Td;
+  const long M1 = N[0] / 2, M2 = N[1] / 2, M3 = N[2] / 2, Mt = M1 * M2 * M3;
+  const long F2 = N[1] / 2 + 1, F3 = N[2] / 2 + 1;
+  double *lam[3];
+  cplx *C[3];
+  const long Md[3] = {M1, M2, M3};
+  for (int d = 0; d < 3; d++) {
+    lam[d] = (double *) malloc(sizeof(double) * N[d]);
+    npo_lambda2(N[d], Bn, lam[d]);
+    C[d] = (cplx *) malloc(sizeof(cplx) * Md[d]);
+    for (long j = 0; j < Md[d]; j++) {
+      long m = (j >= Md[d] / 2) ? j - Md[d] : j;
+      C[d][j] = cexp(-2.0 * PI * I / N[d] * m);
+    }
+  }
+  cplx *X = (cplx *) malloc(sizeof(cplx) * Mt);
+  memset(T, 0, sizeof(cplx) * Mt);
+  const double norm = 1.0 / ((double) N[0] * N[1] * N[2]);
+  for (int a1 = 0; a1 <= 1; a1++)
+    for (int a2 = 0; a2 <= 1; a2++)
+      for (int a3 = 0; a3 <= 1; a3++) {
+        long c;
+#pragma omp parallel for schedule(static)
+        for (c = 0; c < M1 * M2; c++) {
+          long n1 = c / M2, n2 = c % M2;
+          cplx cc = 1.0;
+          if (a1) cc = C[0][n1];
+          if (a2) cc *= C[1][n2];
+          for (long n3 = 0; n3 < M3; n3++) {
+            long idx = c * M3 + n3;
+            X[idx] = a3 ? Q[idx] * cc * C[2][n3] : Q[idx] * cc;
+          }
+        }
+        npo_fft3d(M1, M2, M3, (double *) X, -1);
+#pragma omp parallel for schedule(static)
+        for (c = 0; c < M1 * M2; c++) {
+          long n1 = c / M2, n2 = c % M2;
+          long m1 = (n1 >= M1 / 2) ? n1 - M1 : n1;
+          long m2 = (n2 >= M2 / 2) ? n2 - M2 : n2;
+          long t1 = 2 * m1 + a1, t2 = 2 * m2 + a2;
+          long k1 = dft2arr(t1, N[0]), k2 = dft2arr(t2, N[1]);
+          double cl = lam[0][k1] * lam[1][k2] * norm;
+          for (long n3 = 0; n3 < M3; n3++) {
+            long m3 = (n3 >= M3 / 2) ? n3 - M3 : n3;
+            long t3 = 2 * m3 + a3, k3 = dft2arr(t3, N[2]);
+            long fi = (labs(t1) * F2 + labs(t2)) * F3 + labs(t3);
+            long idx = c * M3 + n3;
+            X[idx] = cl * conj(X[idx]) * F[fi] * lam[2][k3];
+          }
+        }
+        npo_fft3d(M1, M2, M3, (double *) X, -1);
+#pragma omp parallel for schedule(static)
+        for (c = 0; c < M1 * M2; c++) {
+          long n1 = c / M2, n2 = c % M2;
+          cplx cc = 1.0;
+          if (a1) cc = C[0][n1];
+          if (a2) cc *= C[1][n2];
+          for (long n3 = 0; n3 < M3; n3++) {
+            long idx = c * M3 + n3;
+            T[idx] += a3 ? X[idx] * cc * C[2][n3] : X[idx] * cc;
+          }
+        }
+      }
+  free(X);
+  for (int d = 0; d < 3; d++) { free(lam[d]); free(C[d]); }
+}
+
+int npo_interp_basic(size_t n, const double *coord, const double *thetad,
+                     const long N[3], const double L[3], long Bn,
+                     const double R[3], double *V1d)
+/* src/RecSumV.cpp:44-149 */
+{
+  const cplx *th = (const cplx *) thetad;
+  cplx *V1 = (cplx *) V1d;
+  const long M1 = N[0] / 2, M2 = N[1] / 2, M3 = N[2] / 2;
+  const double C1 = N[0] / L[0], C2 = N[1] / L[1], C3 = N[2] / L[2];
+  int err = 0;
+  long i;
+#pragma omp parallel for schedule(static)
+  for (i = 0; i < (long) n; i++) {
+    double w[3], B[3][64], dB[3][64];
+    long ms[3];
+    int bad = 0;
+    for (int d = 0; d < 3; d++) {
+      w[d] = N[d] / L[d] * (coord[3 * i + d] - R[d]);
+      ms[d] = qstart(w[d]);
+      if (!qcheck(ms[d], N[d], Bn)) bad = 1;
+      for (long j = 0; j < Bn; j++) B[d][j] = npo_bspline_dw((int) Bn, w[d] + ms[d] + j, &dB[d][j]);
+    }
+    if (bad) { err = -1; continue; }
+    cplx v[4] = {0, 0, 0, 0};
+    for (long j1 = 0; j1 < Bn; j1++) {
+      long p1 = dft2arr(ms[0] + j1, M1);
+      for (long j2 = 0; j2 < Bn; j2++) {
+        long p2 = dft2arr(ms[1] + j2, M2);
+        double A1 = B[0][j1] * B[1][j2], A2 = dB[0][j1] * B[1][j2], A3 = B[0][j1] * dB[1][j2];
+        cplx tb = 0, tdb = 0;
+        for (long j3 = 0; j3 < Bn; j3++) {
+          long p3 = dft2arr(ms[2] + j3, M3);
+          cplx t = th[(p1 * M2 + p2) * M3 + p3];
+          tb += t * B[2][j3];
+          tdb += t * dB[2][j3];
+        }
+        v[0] += A1 * tb; v[1] += A2 * tb; v[2] += A3 * tb; v[3] += A1 * tdb;
+      }
+    }
+    V1[4 * i] = v[0];
+    V1[4 * i + 1] = v[1] * C1;
+    V1[4 * i + 2] = v[2] * C2;
+    V1[4 * i + 3] = v[3] * C3;
+  }
+  return err;
+}
+
+/* ======================================================================
+ * Direct (short-range) and exact sums.
+ * V_i += f(r_ij) q_j, grad_i V += (r_i - r_j) f1(r_ij) q_j for every
+ * ordered pair i != j (src/PotentialLaplace.cpp:488-612 scalar pair/self:
+ * V1 += f0 q2, V1_x += x f1 q2 with x = x1 - x2).  The neighbour search
+ * here is a plain uniform grid of cell size >= Rdir; it only has to find
+ * every pair inside the cutoff, since the SR kernel is zero outside.
+ * Pinned by: brute-force O(N^2) evaluation of the same SR kernel on small
+ * inputs, and the whole path against the exact sum.
+ * ==================================================================== */
+typedef struct {
+  long nc[3];
+  double mn[3], cs;
+  long *start, *list;
+} ugrid;
+
+static void ugrid_build(ugrid *g, size_t n, const double *coord, double cs)
+{
+  for (int d = 0; d < 3; d++) {
+    g->mn[d] = 1e300;
+    double mx = -1e300;
+    for (size_t i = 0; i < n; i++) {
+      double x = coord[3 * i + d];
+      if (x < g->mn[d]) g->mn[d] = x;
+      if (x > mx) mx = x;
+    }
+    g->nc[d] = (long) ((mx - g->mn[d]) / cs) + 1;
+  }
+  g->cs = cs;
+  long nt = g->nc[0] * g->nc[1] * g->nc[2];
+  if (nt < 1) nt = 1;
+  g->start = (long *) calloc(nt + 1, sizeof(long));
+  g->list = (long *) malloc(sizeof(long) * (n ? n : 1));
+  long *cell = (long *) malloc(sizeof(long) * (n ? n : 1));
+  for (size_t i = 0; i < n; i++) {
+    long c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = (long) ((coord[3 * i + d] - g->mn[d]) / cs);
+      if (c[d] >= g->nc[d]) c[d] = g->nc[d] - 1;
+    }
+    cell[i] = (c[0] * g->nc[1] + c[1]) * g->nc[2] + c[2];
+    g->start[cell[i] + 1]++;
+  }
+  for (long c = 0; c < nt; c++) g->start[c + 1] += g->start[c];
+  long *fill = (long *) calloc(nt, sizeof(long));
+  for (size_t i = 0; i < n; i++) g->list[g->start[cell[i]] + fill[cell[i]]++] = (long) i;
+  free(fill);
+  free(cell);
+}
+
+static void ugrid_free(ugrid *g) { free(g->start); free(g->list); }
+
+void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
+                   const double *q, double *V)
+{
+  const int cx = (k->kind == NPO_HELMHOLTZ);
+  const double R = k->Rdir;
+  ugrid g;
+  ugrid_build(&g, n, coord, R * 1.0000001);
+  long i;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i = 0; i < (long) n; i++) {
+    const double *ri = coord + 3 * i;
+    long c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = (long) ((ri[d] - g.mn[d]) / g.cs);
+      if (c[d] >= g.nc[d]) c[d] = g.nc[d] - 1;
+    }
+    cplx v[4] = {0, 0, 0, 0};
+    for (long d1 = -1; d1 <= 1; d1++)
+      for (long d2 = -1; d2 <= 1; d2++)
+        for (long d3 = -1; d3 <= 1; d3++) {
+          long a = c[0] + d1, b = c[1] + d2, e = c[2] + d3;
+          if (a < 0 || b < 0 || e < 0 || a >= g.nc[0] || b >= g.nc[1] || e >= g.nc[2]) continue;
+          long cc = (a * g.nc[1] + b) * g.nc[2] + e;
+          for (long t = g.start[cc]; t < g.start[cc + 1]; t++) {
+            long j = g.list[t];
+            if (j == i) continue;
+            double x = ri[0] - coord[3 * j], y = ri[1] - coord[3 * j + 1], z = ri[2] - coord[3 * j + 2];
+            double r2 = x * x + y * y + z * z;
+            if (r2 > R * R * 1.0001) continue;
+            double f0[2], f1[2];
+            npo_kernel_eval(k, NPO_SR, r2, f0, f1);
+            cplx F0 = cx ? f0[0] + I * f0[1] : f0[0];
+            cplx F1 = cx ? f1[0] + I * f1[1] : f1[0];
+            cplx qj = cx ? q[2 * j] + I * q[2 * j + 1] : q[j];
+            v[0] += F0 * qj;
+            v[1] += x * F1 * qj;
+            v[2] += y * F1 * qj;
+            v[3] += z * F1 * qj;
+          }
+        }
+    for (int p = 0; p < 4; p++) {
+      if (cx) { V[8 * i + 2 * p] = creal(v[p]); V[8 * i + 2 * p + 1] = cimag(v[p]); }
+      else V[4 * i + p] = creal(v[p]);
+    }
+  }
+  ugrid_free(&g);
+}
+
+void npo_exact(const npo_kernel *k, size_t n, const double *coord,
+               const double *q, size_t nt, const long *idx, double *V)
+{
+  const int cx = (k->kind == NPO_HELMHOLTZ);
+  long t;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (t = 0; t < (long) nt; t++) {
+    const long i = idx ? idx[t] : t;
+    const double *ri = coord + 3 * i;
+    cplx v[4] = {0, 0, 0, 0};
+    for (size_t j = 0; j < n; j++) {
+      if ((long) j == i) continue;
+      double x = ri[0] - coord[3 * j], y = ri[1] - coord[3 * j + 1], z = ri[2] - coord[3 * j + 2];
+      double f0[2], f1[2];
+      npo_kernel_eval(k, NPO_FULL, x * x + y * y + z * z, f0, f1);
+      cplx F0 = cx ? f0[0] + I * f0[1] : f0[0];
+      cplx F1 = cx ? f1[0] + I * f1[1] : f1[0];
+      cplx qj = cx ? q[2 * j] + I * q[2 * j + 1] : q[j];
+      v[0] += F0 * qj; v[1] += x * F1 * qj; v[2] += y * F1 * qj; v[3] += z * F1 * qj;
+    }
+    for (int p = 0; p < 4; p++) {
+      if (cx) { V[8 * t + 2 * p] = creal(v[p]); V[8 * t + 2 * p + 1] = cimag(v[p]); }
+      else V[4 * t + p] = creal(v[p]);
+    }
+  }
+}
+
+double npo_count_pairs(size_t n, const double *coord, double Rdir)
+{
+  ugrid g;
+  ugrid_build(&g, n, coord, Rdir * 1.0000001);
+  double total = 0;
+  long i;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : total)
+  for (i = 0; i < (long) n; i++) {
+    const double *ri = coord + 3 * i;
+    long c[3];
+    for (int d = 0; d < 3; d++) {
+      c[d] = (long) ((ri[d] - g.mn[d]) / g.cs);
+      if (c[d] >= g.nc[d]) c[d] = g.nc[d] - 1;
+    }
+    long cnt = 0;
+    for (long d1 = -1; d1 <= 1; d1++)
+      for (long d2 = -1; d2 <= 1; d2++)
+        for (long d3 = -1; d3 <= 1; d3++) {
+          long a = c[0] + d1, b = c[1] + d2, e = c[2] + d3;
+          if (a < 0 || b < 0 || e < 0 || a >= g.nc[0] || b >= g.nc[1] || e >= g.nc[2]) continue;
+          long cc = (a * g.nc[1] + b) * g.nc[2] + e;
+          for (long t = g.start[cc]; t < g.start[cc + 1]; t++) {
+            long j = g.list[t];
+            if (j == i) continue;
+            double x = ri[0] - coord[3 * j], y = ri[1] - coord[3 * j + 1], z = ri[2] - coord[3 * j + 2];
+            if (x * x + y * y + z * z < Rdir * Rdir) cnt++;
+          }
+        }
+    total += (double) cnt;
+  }
+  ugrid_free(&g);
+  return total;
+}
+
+/* ======================================================================
+ * Whole path: V = SR direct sum - f_LR(0) q_i + reciprocal sum
+ * (src/Interface.cpp:2017-2100 Laplace DM, :1719-1800 GenFunc,
+ * :2853-2938 Helmholtz; rec sum src/RecSumInterface.cpp:340-722).
+ * The reference permutes charges into cell order first; that changes only
+ * the order of summation, so this restatement works in input order.
+ * ==================================================================== */
+int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
+              const double *q, const double *coord, double *V)
+{
+  const int cx = (k->kind == NPO_HELMHOLTZ);
+  const int w = cx ? 8 : 4;
+  npo_direct_sr(k, n, coord, q, V);
+  double fs0[2], fs1[2];
+  npo_kernel_eval(k, NPO_LR, 0.0, fs0, fs1);
+  for (size_t i = 0; i < n; i++) {
+    if (cx) {
+      cplx fs = fs0[0] + I * fs0[1], qi = q[2 * i] + I * q[2 * i + 1];
+      cplx v = V[w * i] + I * V[w * i + 1] - fs * qi;
+      V[w * i] = creal(v); V[w * i + 1] = cimag(v);
+    } else {
+      V[w * i] -= fs0[0] * q[i];
+    }
+  }
+  double X0[3], R0[3], X[3], R[3], L[3];
+  npo_box_dim_center(n, coord, X0, R0);
+  npo_pme_corr_box(X0, R0, p->N, p->del, p->Bn, X, R);
+  for (int d = 0; d < 3; d++) L[d] = 2 * (X[d] + p->del[d]);
+  const long M = (p->N[0] / 2) * (p->N[1] / 2) * (p->N[2] / 2);
+  const long MF = (p->N[0] / 2 + 1) * (p->N[1] / 2 + 1) * (p->N[2] / 2 + 1);
+  double *F = (double *) malloc(sizeof(double) * 2 * MF);
+  double *Q = (double *) malloc(sizeof(double) * 2 * M);
+  double *T = (double *) malloc(sizeof(double) * 2 * M);
+  double *V1 = (double *) malloc(sizeof(double) * 8 * (n ? n : 1));
+  int err = npo_fgrid_compact(k, p->N, p->a, p->del, X, F);
+  if (!err) err = npo_spread_basic(n, q, cx, coord, p->N, L, p->Bn, R, Q);
+  if (!err) {
+    npo_theta_8parity(p->N, p->Bn, F, Q, T);
+    err = npo_interp_basic(n, coord, T, p->N, L, p->Bn, R, V1);
+  }
+  if (!err) {
+    for (size_t i = 0; i < n; i++)
+      for (int c = 0; c < 4; c++) {
+        if (cx) {
+          V[8 * i + 2 * c] += V1[8 * i + 2 * c];
+          V[8 * i + 2 * c + 1] += V1[8 * i + 2 * c + 1];
+        } else {
+          V[4 * i + c] += V1[8 * i + 2 * c];
+        }
+      }
+  }
+  free(F); free(Q); free(T); free(V1);
+  return err;
+}
+
+/* ======================================================================
+ * compareV metric (src/SupportFunctions.cpp:88-305, app/compareV.cpp).
+ * ==================================================================== */
+void npo_compare_v(size_t n, int isComplex, const double *Vref,
+                   const double *V, double out[5])
+{
+  double eV = 0, edV = 0, magV = 0, magdV = 0, mx = 0;
+  for (size_t i = 0; i < n; i++) {
+    double di0 = 0;
+    for (int p = 0; p < 4; p++) {
+      double d, m;
+      if (isComplex) {
+        d = hypot(V[8 * i + 2 * p] - Vref[8 * i + 2 * p], V[8 * i + 2 * p + 1] - Vref[8 * i + 2 * p + 1]);
+        m = hypot(Vref[8 * i + 2 * p], Vref[8 * i + 2 * p + 1]);
+      } else {
+        d = fabs(V[4 * i + p] - Vref[4 * i + p]);
+        m = fabs(Vref[4 * i + p]);
+      }
+      if (p == 0) { eV += d * d; magV += m; di0 = d; }
+      else edV += d * d;
+      if (p <= 2) magdV += m;   /* reference quirk: components 0..2 */
+    }
+    if (di0 > mx) mx = di0;
+  }
+  out[0] = sqrt(eV / n);
+  out[1] = sqrt(edV / (3.0 * n));
+  out[2] = magV / n;
+  out[3] = magdV / (3.0 * n);
+  out[4] = (out[2] > 0) ? mx / out[2] : 0;
+}
+
+void npo_splitmix_uniform(unsigned long long seed, size_t n, double *out)
+{
+  unsigned long long s = seed;
+  for (size_t i = 0; i < n; i++) {
+    s += 0x9E3779B97F4A7C15ULL;
+    unsigned long long z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    out[i] = (double) (z >> 11) * (1.0 / 9007199254740992.0);
+  }
+}

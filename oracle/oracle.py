@@ -1,0 +1,335 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/libnpme_oracle.so (built by oracle/Makefile).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg use this module, as
+the checker or as the timed CPU baseline; the product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnpme_oracle.so")
+
+LAPLACE, RALPHA, HELMHOLTZ = 0, 1, 2
+FULL, LR, SR = 0, 1, 2
+
+
+class Kernel(C.Structure):
+    _fields_ = [("kind", C.c_int), ("alpha", C.c_double), ("k0r", C.c_double),
+                ("k0i", C.c_double), ("Nder", C.c_int), ("Rdir", C.c_double),
+                ("a", C.c_double * 62), ("b", C.c_double * 62)]
+
+    def coeffs(self):
+        n = self.Nder + 1
+        a = np.array(self.a[:2 * n]).reshape(n, 2)
+        b = np.array(self.b[:2 * n]).reshape(n, 2)
+        if self.kind == HELMHOLTZ:
+            return a[:, 0] + 1j * a[:, 1], b[:, 0] + 1j * b[:, 1]
+        return a[:, 0].copy(), b[:, 0].copy()
+
+
+class RecSumParm(C.Structure):
+    _fields_ = [("N", C.c_long * 3), ("nblk", C.c_long * 3), ("Bn", C.c_long),
+                ("del_", C.c_double * 3), ("a", C.c_double * 3)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        d, p, l, i = C.c_double, C.c_void_p, C.c_long, C.c_int
+        sz = C.c_size_t
+        L.npo_bspline.restype = d
+        L.npo_bspline.argtypes = [i, d]
+        L.npo_bspline_dw.restype = d
+        L.npo_bspline_dw.argtypes = [i, d, C.POINTER(d)]
+        L.npo_kernel_init.argtypes = [C.POINTER(Kernel), i, d, d, d, i, d]
+        L.npo_kernel_eval.argtypes = [C.POINTER(Kernel), i, d, p, p]
+        L.npo_optimal_grid_size.restype = l
+        L.npo_optimal_grid_size.argtypes = [l]
+        L.npo_fft_size_block.argtypes = [l, l, C.POINTER(l), C.POINTER(l)]
+        L.npo_box_dim_center.argtypes = [sz, p, p, p]
+        L.npo_pme_corr_box.argtypes = [p, p, p, p, l, p, p]
+        L.npo_rsem_static.argtypes = [sz, p, p, l, d, d, d, i, p, p, C.POINTER(i)]
+        L.npo_rsem_static_pred_error.argtypes = [sz, p, p, l, d, i, d, p, C.POINTER(d), C.POINTER(i)]
+        L.npo_rsem_helmholtz.argtypes = [sz, p, p, l, d, d, d, d, i, p, p, C.POINTER(i)]
+        L.npo_cell_index.argtypes = [sz, p, d, l, l, p, p, p]
+        L.npo_fft1d.argtypes = [l, p, i]
+        L.npo_fft3d.argtypes = [l, l, l, p, i]
+        L.npo_fgrid_compact.argtypes = [C.POINTER(Kernel), p, p, p, p, p]
+        L.npo_lambda2.argtypes = [l, l, p]
+        L.npo_spread_basic.argtypes = [sz, p, i, p, p, p, l, p, p]
+        L.npo_theta_8parity.argtypes = [p, l, p, p, p]
+        L.npo_interp_basic.argtypes = [sz, p, p, p, p, l, p, p]
+        L.npo_direct_sr.argtypes = [C.POINTER(Kernel), sz, p, p, p]
+        L.npo_exact.argtypes = [C.POINTER(Kernel), sz, p, p, sz, p, p]
+        L.npo_count_pairs.restype = d
+        L.npo_count_pairs.argtypes = [sz, p, d]
+        L.npo_calcv.argtypes = [C.POINTER(Kernel), C.POINTER(RecSumParm), sz, p, p, p]
+        L.npo_compare_v.argtypes = [sz, i, p, p, p]
+        L.npo_splitmix_uniform.argtypes = [C.c_ulonglong, sz, p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _c128_as_f64(q):
+    q = np.ascontiguousarray(q, dtype=np.complex128)
+    return q.view(np.float64)
+
+
+# ---------------------------------------------------------------- basics
+def bspline(n, w):
+    return lib().npo_bspline(int(n), float(w))
+
+
+def bspline_dw(n, w):
+    d = C.c_double()
+    v = lib().npo_bspline_dw(int(n), float(w), C.byref(d))
+    return v, d.value
+
+
+def kernel(kind, Nder, Rdir, alpha=-1.0, k0=0.0):
+    k = Kernel()
+    k0 = complex(k0)
+    rc = lib().npo_kernel_init(C.byref(k), int(kind), float(alpha), k0.real, k0.imag,
+                               int(Nder), float(Rdir))
+    if rc != 0:
+        raise ValueError("bad kernel parameters")
+    return k
+
+
+def kernel_eval(k, part, r2):
+    f0 = np.zeros(2)
+    f1 = np.zeros(2)
+    lib().npo_kernel_eval(C.byref(k), int(part), float(r2), _ptr(f0), _ptr(f1))
+    if k.kind == HELMHOLTZ:
+        return complex(f0[0], f0[1]), complex(f1[0], f1[1])
+    return f0[0], f1[0]
+
+
+def optimal_grid_size(n):
+    return lib().npo_optimal_grid_size(int(n))
+
+
+def fft_size_block(Nideal, nideal=20):
+    N, n = C.c_long(), C.c_long()
+    lib().npo_fft_size_block(int(Nideal), int(nideal), C.byref(N), C.byref(n))
+    return N.value, n.value
+
+
+def box_dim_center(coord):
+    coord = _f64(coord)
+    X0, R0 = np.zeros(3), np.zeros(3)
+    lib().npo_box_dim_center(len(coord), _ptr(coord), _ptr(X0), _ptr(R0))
+    return X0, R0
+
+
+def pme_corr_box(X0, R0, N, dl, Bn):
+    X0, R0, dl = _f64(X0), _f64(R0), _f64(dl)
+    Nn = np.ascontiguousarray(N, dtype=np.int64)
+    X, R = np.zeros(3), np.zeros(3)
+    lib().npo_pme_corr_box(_ptr(X0), _ptr(R0), _ptr(Nn), _ptr(dl), int(Bn), _ptr(X), _ptr(R))
+    return X, R
+
+
+def rsem_static(q, coord, Bn, alpha, tol, Rdir, nDeriv=-1):
+    q, coord = _f64(q), _f64(coord)
+    N = np.zeros(3, np.int64)
+    nb = np.zeros(3, np.int64)
+    nd = C.c_int()
+    rc = lib().npo_rsem_static(len(q), _ptr(q), _ptr(coord), int(Bn), float(alpha), float(tol),
+                               float(Rdir), int(nDeriv), _ptr(N), _ptr(nb), C.byref(nd))
+    if rc != 0:
+        raise ValueError("no RSEM row for Bn=%d alpha=%g" % (Bn, alpha))
+    return tuple(int(x) for x in N), tuple(int(x) for x in nb), nd.value
+
+
+def rsem_static_pred_error(q, coord, Bn, alpha, Nder, Rdir, N):
+    q, coord = _f64(q), _f64(coord)
+    Nn = np.ascontiguousarray(N, dtype=np.int64)
+    e, rel = C.c_double(), C.c_int()
+    rc = lib().npo_rsem_static_pred_error(len(q), _ptr(q), _ptr(coord), int(Bn), float(alpha),
+                                          int(Nder), float(Rdir), _ptr(Nn), C.byref(e), C.byref(rel))
+    if rc != 0:
+        raise ValueError("no RSEM row")
+    return e.value, bool(rel.value)
+
+
+def rsem_helmholtz(qc, coord, Bn, k0, tol, Rdir, nDeriv=-1):
+    qf, coord = _c128_as_f64(qc), _f64(coord)
+    N = np.zeros(3, np.int64)
+    nb = np.zeros(3, np.int64)
+    nd = C.c_int()
+    k0 = complex(k0)
+    rc = lib().npo_rsem_helmholtz(len(coord), _ptr(qf), _ptr(coord), int(Bn), k0.real, k0.imag,
+                                  float(tol), float(Rdir), int(nDeriv), _ptr(N), _ptr(nb),
+                                  C.byref(nd))
+    if rc != 0:
+        raise ValueError("Helmholtz RSEM failed (%d)" % rc)
+    return tuple(int(x) for x in N), tuple(int(x) for x in nb), nd.value
+
+
+def cell_index(coord, Rdir, nNeigh=2, nCellClust1D=3):
+    coord = _f64(coord)
+    n = len(coord)
+    nA = np.zeros(3, np.int64)
+    tA = np.zeros(n, np.uint64)
+    tB = np.zeros(n, np.uint64)
+    rc = lib().npo_cell_index(n, _ptr(coord), float(Rdir), int(nNeigh), int(nCellClust1D),
+                              _ptr(nA), _ptr(tA), _ptr(tB))
+    if rc != 0:
+        raise ValueError("point outside box")
+    return nA, tA, tB
+
+
+def fft1d(z, sign=-1):
+    z = np.ascontiguousarray(z, dtype=np.complex128).copy()
+    lib().npo_fft1d(len(z), _ptr(z), int(sign))
+    return z
+
+
+def fft3d(z, sign=-1):
+    z = np.ascontiguousarray(z, dtype=np.complex128).copy()
+    n1, n2, n3 = z.shape
+    lib().npo_fft3d(n1, n2, n3, _ptr(z), int(sign))
+    return z
+
+
+def fgrid_compact(k, N, a, dl, X):
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    F = np.zeros((N[0] // 2 + 1, N[1] // 2 + 1, N[2] // 2 + 1), np.complex128)
+    rc = lib().npo_fgrid_compact(C.byref(k), _ptr(N), _ptr(_f64(a)), _ptr(_f64(dl)),
+                                 _ptr(_f64(X)), _ptr(F))
+    if rc != 0:
+        raise ValueError("F grid failed")
+    return F
+
+
+def lambda2(N, Bn):
+    out = np.zeros(N)
+    lib().npo_lambda2(int(N), int(Bn), _ptr(out))
+    return out
+
+
+def spread_basic(q, coord, N, L, Bn, R):
+    cx = np.iscomplexobj(q)
+    qq = _c128_as_f64(q) if cx else _f64(q)
+    coord = _f64(coord)
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    Q = np.zeros((N[0] // 2, N[1] // 2, N[2] // 2), np.complex128)
+    rc = lib().npo_spread_basic(len(coord), _ptr(qq), int(cx), _ptr(coord), _ptr(N),
+                                _ptr(_f64(L)), int(Bn), _ptr(_f64(R)), _ptr(Q))
+    if rc != 0:
+        raise ValueError("spline window outside the compact grid")
+    return Q
+
+
+def theta_8parity(N, Bn, F, Q):
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    Q = np.ascontiguousarray(Q, dtype=np.complex128)
+    T = np.zeros_like(Q)
+    lib().npo_theta_8parity(_ptr(N), int(Bn), _ptr(F), _ptr(Q), _ptr(T))
+    return T
+
+
+def interp_basic(coord, theta, N, L, Bn, R):
+    coord = _f64(coord)
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    th = np.ascontiguousarray(theta, dtype=np.complex128)
+    V1 = np.zeros((len(coord), 4), np.complex128)
+    rc = lib().npo_interp_basic(len(coord), _ptr(coord), _ptr(th), _ptr(N), _ptr(_f64(L)),
+                                int(Bn), _ptr(_f64(R)), _ptr(V1))
+    if rc != 0:
+        raise ValueError("spline window outside the compact grid")
+    return V1
+
+
+def _vshape(k, n):
+    return (n, 4) if k.kind != HELMHOLTZ else (n, 4)
+
+
+def direct_sr(k, coord, q):
+    coord = _f64(coord)
+    cx = k.kind == HELMHOLTZ
+    qq = _c128_as_f64(q) if cx else _f64(q)
+    V = np.zeros((len(coord), 4), np.complex128 if cx else np.float64)
+    lib().npo_direct_sr(C.byref(k), len(coord), _ptr(coord), _ptr(qq), _ptr(V))
+    return V
+
+
+def exact(k, coord, q, idx=None):
+    coord = _f64(coord)
+    cx = k.kind == HELMHOLTZ
+    qq = _c128_as_f64(q) if cx else _f64(q)
+    if idx is None:
+        nt, ip = len(coord), None
+    else:
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        nt, ip = len(idx), _ptr(idx)
+    V = np.zeros((nt, 4), np.complex128 if cx else np.float64)
+    lib().npo_exact(C.byref(k), len(coord), _ptr(coord), _ptr(qq), nt, ip, _ptr(V))
+    return V
+
+
+def count_pairs(coord, Rdir):
+    coord = _f64(coord)
+    return lib().npo_count_pairs(len(coord), _ptr(coord), float(Rdir))
+
+
+def calcv(k, N, Bn, coord, q, nblk=(20, 20, 20), dl=None, a=(4.0, 4.0, 4.0)):
+    p = RecSumParm()
+    for d in range(3):
+        p.N[d] = int(N[d])
+        p.nblk[d] = int(nblk[d])
+        p.del_[d] = float(k.Rdir if dl is None else dl[d])
+        p.a[d] = float(a[d])
+    p.Bn = int(Bn)
+    coord = _f64(coord)
+    cx = k.kind == HELMHOLTZ
+    qq = _c128_as_f64(q) if cx else _f64(q)
+    V = np.zeros((len(coord), 4), np.complex128 if cx else np.float64)
+    rc = lib().npo_calcv(C.byref(k), C.byref(p), len(coord), _ptr(qq), _ptr(coord), _ptr(V))
+    if rc != 0:
+        raise ValueError("oracle CalcV failed")
+    return V
+
+
+def compare_v(Vref, V):
+    cx = np.iscomplexobj(Vref) or np.iscomplexobj(V)
+    if cx:
+        a = _c128_as_f64(Vref)
+        b = _c128_as_f64(V)
+    else:
+        a, b = _f64(Vref), _f64(V)
+    out = np.zeros(5)
+    lib().npo_compare_v(len(Vref), int(cx), _ptr(a), _ptr(b), _ptr(out))
+    return {"errV": out[0], "errdV": out[1], "magV": out[2], "magdV": out[3],
+            "relV": out[0] / out[2] if out[2] else 0.0,
+            "reldV": out[1] / out[3] if out[3] else 0.0, "maxrelV": out[4]}
+
+
+def splitmix_uniform(seed, n):
+    out = np.zeros(n)
+    lib().npo_splitmix_uniform(int(seed), int(n), _ptr(out))
+    return out
