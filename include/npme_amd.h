@@ -1,0 +1,125 @@
+/*
+ * npme_amd.h -- C ABI of the MI355X NPME potential evaluation.
+ *
+ * Drop-in boundary for the NPME library calls of ElkingD/npme
+ * (src/Interface.h).  Plain pointers and sizes only.  Complex data is
+ * interleaved (re, im) doubles, the memory layout of C99 `_Complex double`
+ * used by the reference.  V arrays are [nCharge][4] = (V, dV/dx, dV/dy,
+ * dV/dz) per charge, complex ones [nCharge][4][2].
+ *
+ * Every entry point returns an int status (NPME_OK = 0) and never exits the
+ * process; npme_last_error() returns the message of the last failure on the
+ * calling thread.  One handle owns one device-resident state (grids, cell
+ * lists, twiddles); calls on one handle are serialised by the library,
+ * distinct handles may be used concurrently.
+ */
+#ifndef NPME_AMD_H
+#define NPME_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define NPME_OK              0
+#define NPME_ERR_KEYWORD     1   /* missing / contradictory keyword        */
+#define NPME_ERR_GRID        2   /* FFT grid or B-spline constraint        */
+#define NPME_ERR_NOMEM       3   /* device memory / admission refused      */
+#define NPME_ERR_DEVICE      4   /* HIP runtime error                      */
+#define NPME_ERR_ARG         5   /* bad argument (size mismatch, NULL ...) */
+#define NPME_ERR_UNSUPPORTED 6   /* combination not provided               */
+
+typedef struct npme_keywords npme_keywords;
+typedef struct npme_handle   npme_handle;
+
+/* ---- keyword set: NPME_KeywordInput (src/ReadPrint.h:28-102) ---------- */
+npme_keywords *npme_keywords_create(void);
+void npme_keywords_destroy(npme_keywords *kw);
+/* AddKeyword (src/ReadPrint.h:64-98); replaces an existing value */
+int  npme_keywords_set(npme_keywords *kw, const char *key, const char *value);
+/* GetValue: copies the value, returns 1 if the key exists, 0 if not */
+int  npme_keywords_get(const npme_keywords *kw, const char *key, char *buf,
+                       size_t buflen);
+/* NPME_KeywordInput(file): "@Keyword" section, "key value [comment]" lines
+ * (src/ReadPrint.cpp:38-66) */
+int  npme_keywords_read_file(npme_keywords *kw, const char *path);
+/* number of keywords and the i-th key (for printing / copying) */
+int  npme_keywords_count(const npme_keywords *kw);
+int  npme_keywords_key(const npme_keywords *kw, int i, char *buf, size_t buflen);
+
+/* ---- NPME_AddMissingKeywords (src/Interface.h:37-51,
+ *      src/Interface.cpp:53-517): defaults, validation and the RSEM grid
+ *      choice.  isChargeReal selects chargeReal[n] or chargeComplex[n][2];
+ *      the other pointer may be NULL. ------------------------------------ */
+int npme_add_missing_keywords(npme_keywords *kw, size_t nCharge,
+                              int isChargeReal, const double *coord,
+                              const double *chargeReal,
+                              const double *chargeComplex);
+
+/* ---- set-up: NPME_Interface_SetUpRealKernel / SetUpComplexKernel
+ *      (src/Interface.h:800-829) selecting NPME_InterfaceReal_Laplace_DM,
+ *      NPME_InterfaceReal_GenFunc (r^alpha) or
+ *      NPME_InterfaceComplex_Helmholtz_DM from the keywords (funcType,
+ *      EwaldSplit), then ::SetUp (src/Interface.cpp:1606-1671, :1847-1956,
+ *      :2661-2791).  The keyword set must already hold every value
+ *      (npme_add_missing_keywords).  Extension keyword: precision = mixed
+ *      (default; FP32 spreading/FFT/direct sum with compensated
+ *      accumulation) or fp64. ------------------------------------------ */
+int npme_create(npme_handle **h, const npme_keywords *kw, size_t nCharge,
+                const double *coord);
+/* ::ResetCoord (src/Interface.h:186-187; src/Interface.cpp:1958-1987) */
+int npme_reset_coord(npme_handle *h, size_t nCharge, const double *coord);
+void npme_destroy(npme_handle *h);
+
+/* ---- ::CalcV (src/Interface.h:373-375 real, :715-717 complex).
+ * Host arrays.  coord must be the geometry given to create/reset_coord
+ * (the device copy is used); charge and coord are not modified. -------- */
+int npme_calcv_real(npme_handle *h, double *V, size_t nCharge,
+                    const double *charge, const double *coord);
+int npme_calcv_complex(npme_handle *h, double *V, size_t nCharge,
+                       const double *charge, const double *coord);
+
+/* ---- ::CalcV_exact (src/Interface.h:380-383, :722-725): exact O(N^2)
+ * FP64 sum of the full kernel on the GPU. ------------------------------- */
+int npme_calcv_exact_real(npme_handle *h, double *V, size_t nCharge,
+                          const double *charge, const double *coord);
+int npme_calcv_exact_complex(npme_handle *h, double *V, size_t nCharge,
+                             const double *charge, const double *coord);
+/* exact sum for the targets idx[0..nTarget) only; V is [nTarget][4]
+ * (complex: [nTarget][4][2]).  No reference counterpart: the subset form of
+ * CalcV_exact used to check large systems. */
+int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget,
+                            const long *idx, const double *charge);
+
+/* ---- device-resident CalcV: dCharge and dV are device pointers
+ * (charges in input order, FP64; V [n][4] FP64 or [n][4][2]).  No host
+ * traffic; returns when the result is complete.  No reference
+ * counterpart: the same CalcV with inputs already in HBM. -------------- */
+int npme_calcv_device(npme_handle *h, double *dV, const double *dCharge);
+
+/* ---- parameters: Get_N1 ... Get_n3, Get_BnOrder (src/Interface.h:89-115),
+ * plus Nder and the device grid footprint in GB. ----------------------- */
+int npme_get_params(const npme_handle *h, long N[3], long nblk[3], long *Bn,
+                    int *Nder, double *Rdir, double *deviceGB);
+/* ::Print (src/Interface.h:385): human-readable summary into buf */
+int npme_print(const npme_handle *h, char *buf, size_t buflen);
+
+/* ---- number of ordered pairs i != j closer than Rdir, the work of the
+ * direct sum (the reference logs it as nInteract,
+ * src/Interface.cpp:1329-1334). ---------------------------------------- */
+int npme_count_pairs(npme_handle *h, double *pairs);
+
+/* ---- per-stage device times of the last CalcV when profiling is on:
+ * ms[0] gather, [1] direct sum, [2] spread, [3..7] FFT passes A..E,
+ * [8] interpolate+combine; returns the number of stages written. ------- */
+int npme_set_profiling(npme_handle *h, int on);
+int npme_get_stage_times(const npme_handle *h, double *ms, int n);
+
+const char *npme_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

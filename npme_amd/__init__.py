@@ -1,0 +1,295 @@
+"""MI355X NPME potential evaluation -- Python mirror of the reference's
+library interface (src/Interface.h), over the C ABI of libnpme_amd.so.
+
+The names follow the reference: NPME_KeywordInput, NPME_AddMissingKeywords,
+NPME_InterfaceReal_Laplace_DM, NPME_InterfaceReal_GenFunc (r^alpha),
+NPME_InterfaceComplex_Helmholtz_DM, NPME_Interface_SetUpRealKernel /
+ComplexKernel, with SetUp / ResetCoord / CalcV / CalcV_exact / Print and the
+Get_N1 ... getters.  Methods return bool like the reference; the reason of a
+failure is in npme_last_error() (the reference prints it on std::cout).
+
+Charges and potentials are numpy arrays: coord[n][3], charge[n] (real) or
+complex128[n], V[n][4] float64 or complex128.  The library needs the HIP
+build (python -m npme_amd.build); importing without it raises, there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnpme_amd.so")
+
+NPME_OK = 0
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libnpme_amd.so is not built: run `python -m npme_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        p, s, i, d = C.c_void_p, C.c_size_t, C.c_int, C.c_double
+        L.npme_last_error.restype = C.c_char_p
+        L.npme_keywords_create.restype = p
+        L.npme_keywords_destroy.argtypes = [p]
+        L.npme_keywords_set.argtypes = [p, C.c_char_p, C.c_char_p]
+        L.npme_keywords_get.argtypes = [p, C.c_char_p, C.c_char_p, s]
+        L.npme_keywords_read_file.argtypes = [p, C.c_char_p]
+        L.npme_keywords_count.argtypes = [p]
+        L.npme_keywords_key.argtypes = [p, i, C.c_char_p, s]
+        L.npme_add_missing_keywords.argtypes = [p, s, i, p, p, p]
+        L.npme_create.argtypes = [C.POINTER(p), p, s, p]
+        L.npme_reset_coord.argtypes = [p, s, p]
+        L.npme_destroy.argtypes = [p]
+        L.npme_calcv_real.argtypes = [p, p, s, p, p]
+        L.npme_calcv_complex.argtypes = [p, p, s, p, p]
+        L.npme_calcv_exact_real.argtypes = [p, p, s, p, p]
+        L.npme_calcv_exact_complex.argtypes = [p, p, s, p, p]
+        L.npme_calcv_exact_subset.argtypes = [p, p, s, p, p]
+        L.npme_calcv_device.argtypes = [p, p, p]
+        L.npme_get_params.argtypes = [p, p, p, p, p, p, p]
+        L.npme_print.argtypes = [p, C.c_char_p, s]
+        L.npme_set_profiling.argtypes = [p, i]
+        L.npme_count_pairs.argtypes = [p, p]
+        L.npme_get_stage_times.argtypes = [p, p, i]
+        _lib = L
+    return _lib
+
+
+def npme_last_error():
+    return lib().npme_last_error().decode()
+
+
+class NPMEError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__("%s failed (status %d): %s" % (where, code, npme_last_error()))
+        self.code = code
+
+
+def _check(rc, where):
+    if rc != NPME_OK:
+        raise NPMEError(rc, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _coord(coord):
+    c = np.ascontiguousarray(coord, dtype=np.float64)
+    if c.ndim != 2 or c.shape[1] != 3:
+        c = c.reshape(-1, 3)
+    return c
+
+
+# ----------------------------------------------------------------- keywords
+class NPME_KeywordInput:
+    """Keyword map (src/ReadPrint.h:28-102)."""
+
+    def __init__(self, keywordFile=None, **kw):
+        self._h = C.c_void_p(lib().npme_keywords_create())
+        if keywordFile is not None:
+            _check(lib().npme_keywords_read_file(self._h, keywordFile.encode()), "read keywords")
+        for k, v in kw.items():
+            self.AddKeyword(k, v)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().npme_keywords_destroy(self._h)
+            self._h = None
+
+    def AddKeyword(self, key, value):
+        if isinstance(value, float):
+            value = repr(value)
+        _check(lib().npme_keywords_set(self._h, key.encode(), str(value).encode()), "AddKeyword")
+
+    def GetValue(self, key, default=None):
+        buf = C.create_string_buffer(512)
+        if lib().npme_keywords_get(self._h, key.encode(), buf, 512):
+            return buf.value.decode()
+        return default
+
+    def keys(self):
+        buf = C.create_string_buffer(512)
+        out = []
+        for i in range(lib().npme_keywords_count(self._h)):
+            lib().npme_keywords_key(self._h, i, buf, 512)
+            out.append(buf.value.decode())
+        return out
+
+    def as_dict(self):
+        return {k: self.GetValue(k) for k in self.keys()}
+
+    def PrintKeywords(self):
+        return "\n".join("  %-14s %s" % (k, v) for k, v in self.as_dict().items())
+
+
+def NPME_AddMissingKeywords(keyword, nCharge, coord, chargeReal=None, chargeComplex=None):
+    """Defaults, validation and the RSEM grid choice (src/Interface.cpp:53-517).
+    Returns True/False like the reference."""
+    coord = _coord(coord)
+    isReal = chargeComplex is None
+    qr = None if chargeReal is None else np.ascontiguousarray(chargeReal, dtype=np.float64)
+    qc = None if chargeComplex is None else np.ascontiguousarray(chargeComplex, dtype=np.complex128)
+    rc = lib().npme_add_missing_keywords(keyword._h, int(nCharge), int(isReal), _ptr(coord),
+                                         None if qr is None else _ptr(qr),
+                                         None if qc is None else _ptr(qc))
+    return rc == NPME_OK
+
+
+# ----------------------------------------------------------------- interfaces
+class _InterfaceBase:
+    _complex = False
+    _funcType = None
+
+    def __init__(self):
+        self._h = None
+        self.nCharge = 0
+
+    def __del__(self):
+        self._free()
+
+    def _free(self):
+        if getattr(self, "_h", None):
+            lib().npme_destroy(self._h)
+            self._h = None
+
+    def SetUp(self, keyword, nCharge, coord, printLog=False):
+        ft = keyword.GetValue("funcType")
+        if self._funcType is not None and ft not in self._funcType:
+            return False
+        self._free()
+        coord = _coord(coord)
+        h = C.c_void_p()
+        rc = lib().npme_create(C.byref(h), keyword._h, int(nCharge), _ptr(coord))
+        if rc != NPME_OK:
+            return False
+        self._h = h
+        self.nCharge = int(nCharge)
+        return True
+
+    def ResetCoord(self, nCharge, coord, printLog=False):
+        coord = _coord(coord)
+        return lib().npme_reset_coord(self._h, int(nCharge), _ptr(coord)) == NPME_OK
+
+    def _params(self):
+        N = np.zeros(3, np.int64)
+        nb = np.zeros(3, np.int64)
+        Bn = C.c_long()
+        nd = C.c_int()
+        R = C.c_double()
+        gb = C.c_double()
+        _check(lib().npme_get_params(self._h, _ptr(N), _ptr(nb), C.byref(Bn), C.byref(nd),
+                                     C.byref(R), C.byref(gb)), "get_params")
+        return N, nb, Bn.value, nd.value, R.value, gb.value
+
+    def Get_N1(self): return int(self._params()[0][0])
+    def Get_N2(self): return int(self._params()[0][1])
+    def Get_N3(self): return int(self._params()[0][2])
+    def Get_n1(self): return int(self._params()[1][0])
+    def Get_n2(self): return int(self._params()[1][1])
+    def Get_n3(self): return int(self._params()[1][2])
+    def Get_BnOrder(self): return int(self._params()[2])
+    def Get_Nder(self): return int(self._params()[3])
+    def GetRdir(self): return float(self._params()[4])
+    def GetNumCharge(self): return self.nCharge
+    def DeviceGB(self): return float(self._params()[5])
+
+    def Print(self):
+        buf = C.create_string_buffer(4096)
+        lib().npme_print(self._h, buf, 4096)
+        return buf.value.decode()
+
+    def _qarr(self, charge):
+        if self._complex:
+            return np.ascontiguousarray(charge, dtype=np.complex128)
+        return np.ascontiguousarray(charge, dtype=np.float64)
+
+    def _varr(self, V, n):
+        dt = np.complex128 if self._complex else np.float64
+        if V is None:
+            return np.zeros((n, 4), dt)
+        if V.dtype != dt or not V.flags.c_contiguous or V.size != 4 * n:
+            raise ValueError("V must be a contiguous %s array of shape (n, 4)" % np.dtype(dt).name)
+        return V
+
+    def CalcV(self, V, nCharge, charge, coord, printLog=False):
+        """Fills V[n][4]; returns True/False (src/Interface.h:373-375)."""
+        q = self._qarr(charge)
+        V = self._varr(V, nCharge)
+        f = lib().npme_calcv_complex if self._complex else lib().npme_calcv_real
+        return f(self._h, _ptr(V), int(nCharge), _ptr(q), None) == NPME_OK
+
+    def CalcV_exact(self, V, nCharge, charge, coord, printLog=False):
+        q = self._qarr(charge)
+        V = self._varr(V, nCharge)
+        f = lib().npme_calcv_exact_complex if self._complex else lib().npme_calcv_exact_real
+        return f(self._h, _ptr(V), int(nCharge), _ptr(q), None) == NPME_OK
+
+    # conveniences (no reference counterpart)
+    def calcv(self, charge):
+        V = self._varr(None, self.nCharge)
+        if not self.CalcV(V, self.nCharge, charge, None):
+            raise NPMEError(-1, "CalcV")
+        return V
+
+    def calcv_exact_subset(self, charge, idx):
+        q = self._qarr(charge)
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        V = self._varr(None, len(idx))
+        _check(lib().npme_calcv_exact_subset(self._h, _ptr(V), len(idx), _ptr(idx), _ptr(q)),
+               "exact_subset")
+        return V
+
+    def calcv_device(self, dV_ptr, dq_ptr):
+        """Device pointers (e.g. torch tensor data_ptr()): charges in input
+        order, V [n][4] FP64 (complex interleaved)."""
+        _check(lib().npme_calcv_device(self._h, C.c_void_p(dV_ptr), C.c_void_p(dq_ptr)), "calcv_device")
+
+    def count_pairs(self):
+        """Ordered pairs i != j with r < Rdir (the direct-sum work)."""
+        out = C.c_double()
+        _check(lib().npme_count_pairs(self._h, C.byref(out)), "count_pairs")
+        return int(out.value)
+
+    def set_profiling(self, on=True):
+        lib().npme_set_profiling(self._h, int(on))
+
+    def stage_times(self):
+        ms = np.zeros(16)
+        k = lib().npme_get_stage_times(self._h, _ptr(ms), 16)
+        return ms[:k]
+
+
+class NPME_InterfaceReal_Laplace_DM(_InterfaceBase):
+    _funcType = ("Laplace",)
+
+
+class NPME_InterfaceReal_GenFunc(_InterfaceBase):
+    """Generic real kernel; built-in r^alpha (funcType Ralpha) and Laplace."""
+    _funcType = ("Ralpha", "Laplace")
+
+
+class NPME_InterfaceComplex_Helmholtz_DM(_InterfaceBase):
+    _complex = True
+    _funcType = ("Helmholtz",)
+
+
+def NPME_Interface_SetUpRealKernel(keyword, nCharge, coord, useGenericKernel=False,
+                                   printLog=False):
+    """Allocates and sets up the interface class for a real kernel
+    (src/Interface.h:800-815).  Returns the object or None."""
+    ft = keyword.GetValue("funcType")
+    cls = NPME_InterfaceReal_Laplace_DM if (ft == "Laplace" and not useGenericKernel) \
+        else NPME_InterfaceReal_GenFunc
+    obj = cls()
+    return obj if obj.SetUp(keyword, nCharge, coord, printLog) else None
+
+
+def NPME_Interface_SetUpComplexKernel(keyword, nCharge, coord, useGenericKernel=False,
+                                      printLog=False):
+    obj = NPME_InterfaceComplex_Helmholtz_DM()
+    return obj if obj.SetUp(keyword, nCharge, coord, printLog) else None

@@ -1,0 +1,221 @@
+// Internal declarations shared by the host set-up code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/npme_amd.h"
+
+namespace npme {
+
+// ------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+struct Status {
+  int code;
+  std::string msg;
+};
+
+#define NPME_HIP_CHECK(expr)                                                  \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      throw npme::DeviceError(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct Error {
+  int code;
+  std::string msg;
+};
+struct DeviceError {
+  std::string msg;
+  explicit DeviceError(std::string m) : msg(std::move(m)) {}
+};
+
+// ------------------------------------------------------------- kernels
+enum KernelKind { KIND_LAPLACE = 0, KIND_RALPHA = 1, KIND_HELMHOLTZ = 2 };
+
+// Derivative-match split of a radial kernel (host, FP64).
+struct DMKernel {
+  int kind = KIND_LAPLACE;
+  double alpha = -1.0;
+  double k0r = 0, k0i = 0;
+  int Nder = 0;
+  double Rdir = 0;
+  double a[2 * 31] = {0};  // (re, im) pairs
+  double b[2 * 31] = {0};
+};
+bool dm_kernel_init(DMKernel &k, int kind, double alpha, double k0r, double k0i, int Nder,
+                    double Rdir, std::string &err);
+// f_LR at r2 (FP64): f0[2] (re, im)
+void dm_eval_lr(const DMKernel &k, double r2, double f0[2], double f1[2]);
+
+// ------------------------------------------------------------- host math
+double bspline(int n, double w);
+void bspline_shifts(int n, double t, double *out);  // out[j] = M_n(t + j)
+long optimal_grid_size(long n);
+void fft_size_block(long Nideal, long nideal, long &N, long &nblk);
+void box_dim_center(size_t n, const double *coord, double X0[3], double R0[3]);
+void pme_corr_box(const double X0[3], const double R0[3], const long N[3], const double del[3],
+                  long Bn, double X[3], double R[3]);
+bool rsem_static(size_t n, const double *q, const double *coord, long Bn, double alpha, double tol,
+                 double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder);
+bool rsem_static_pred_error(size_t n, const double *q, const double *coord, long Bn, double alpha,
+                            int Nder, double Rdir, const long N[3], double &eps, bool &reliable);
+bool rsem_helmholtz(size_t n, const double *qc, const double *coord, long Bn, double k0r,
+                    double k0i, double tol, double Rdir, int nDerivIn, long N[3], long nblk[3],
+                    int &Nder, std::string &err);
+void lambda2(long N, long Bn, double *lam2);
+
+// ------------------------------------------------------------- keywords
+struct Keywords {
+  std::vector<std::pair<std::string, std::string>> kv;  // insertion order kept
+  bool get(const std::string &k, std::string &v) const;
+  bool get(const std::string &k, double &v) const;
+  bool get(const std::string &k, long &v) const;
+  bool get(const std::string &k, int &v) const;
+  void set(const std::string &k, const std::string &v);
+  void set(const std::string &k, double v);
+  void set(const std::string &k, long v);
+};
+
+// ------------------------------------------------------------- FFT plan
+struct FFTPlan1D {
+  int N = 0;
+  int nst = 0;
+  int radix[24] = {0};
+};
+FFTPlan1D make_plan(int N);
+
+// ------------------------------------------------------------- device state
+struct Handle;
+
+// one CalcV pipeline instance (device buffers for one geometry)
+struct Handle {
+  std::mutex mtx;
+  // configuration
+  Keywords kw;
+  DMKernel ker;
+  bool complexK = false;
+  bool fp64 = false;
+  long Bn = 6;
+  long N[3] = {0, 0, 0}, nblk[3] = {0, 0, 0}, M[3] = {0, 0, 0}, K3 = 0;
+  double del[3] = {0, 0, 0}, ta[3] = {4, 4, 4};
+  long nNeigh = 2, nCellClust1D = 3;
+  size_t n = 0;
+  double fself[2] = {0, 0};
+  // geometry
+  double X0[3], R0[3], X[3], R[3], L[3];
+  // cell grid (global cellB grid)
+  int nb[3] = {0, 0, 0};
+  double csB = 0;
+  double boxMin[3] = {0, 0, 0};
+  // host copies
+  std::vector<int> perm;  // sorted position -> input index
+  // spreading tiles
+  int T[3] = {8, 8, 8};
+  int nt[3] = {0, 0, 0};
+  // device buffers
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int *d_perm = nullptr;         // [n]
+  void *d_src = nullptr;         // [n] float4/double4 (local xyz, q) ; complex: q re only here
+  void *d_qi = nullptr;          // [n] imag part (float/double) for complex kernels
+  int4 *d_cell = nullptr;        // [n] cell coords (x,y,z), pad
+  int *d_cellStart = nullptr;    // [ncell + 1]
+  int4 *d_ms = nullptr;          // [n] stencil start u0 per dim (u = m + N/4)
+  void *d_frac = nullptr;        // [n] float4/double4 fractional spline arguments
+  int *d_tileStart = nullptr;    // [ntiles + 1]
+  int *d_tileList = nullptr;     // [nlist]
+  void *d_grid = nullptr;        // Q / theta: M reals (real) or M complex
+  void *d_buf1 = nullptr;        // M1 x M2 x K3 complex (real) / M1 x M2 x N3 (complex)
+  void *d_buf2 = nullptr;        // M1 x N2 x K3 complex (real) / M1 x N2 x N3 (complex)
+  void *d_G = nullptr;           // k-space multiplier (N1/2+1)(N2/2+1)(K3 or N3/2+1)
+  void *d_tw[3] = {nullptr, nullptr, nullptr};  // twiddle tables per axis length
+  double *d_Vdir = nullptr;      // [n][4] (x2 for complex) direct sum, sorted order
+  double *d_V = nullptr;         // [n][4] (x2) output staging, input order
+  double *d_qin = nullptr;       // [n] (x2) input charges staging
+  double *d_coordS = nullptr;    // [n][3] sorted FP64 coordinates (exact sums)
+  FFTPlan1D plan[3];
+  size_t deviceBytes = 0;
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;
+  double stage_ms[16] = {0};
+  int nstage = 0;
+};
+
+void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord);
+void handle_free_device(Handle &h);
+void handle_calcv(Handle &h, const double *d_q, double *d_Vout);
+void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout);
+
+
+// ------------------------------------------------------------- launches
+// FFT passes (fft.hip)
+struct PassGeom {
+  int N;           // FFT length
+  int Lin, Lout;   // valid input / output positions
+  long nCols;      // number of columns
+  long nInner;     // columns per outer group
+  long inOuter, inInner, inPos;     // input strides (elements)
+  long outOuter, outInner, outPos;  // output strides
+};
+void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const void *tw,
+                      const void *in, void *out, bool backward, hipStream_t s);
+void fft_pass_r2c_rows(bool fp64, int N, int Lin, long nRows, long inRowStride,
+                       long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
+                       void *out, hipStream_t s);
+void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
+                       long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
+                       void *out, hipStream_t s);
+// middle pass: forward FFT along the axis, multiply by G, backward FFT
+void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
+                       int N2, int G2, long K3, const FFTPlan1D &plan, const void *tw,
+                       const void *G, void *data, hipStream_t s);
+// even-extension DFT of the compact kernel grid along one axis (set-up, FP64)
+void fft_pass_even(int N, long nCols, long nInner, long outer, long inner, long pos,
+                   const FFTPlan1D &plan, const void *tw, void *data, hipStream_t s);
+
+// particle kernels (particles.hip)
+struct DirectParams {
+  int kind;
+  int Nder;
+  double Rdir;
+  double alpha;
+  double k0r, k0i;
+  double a[2 * 31], b[2 * 31];
+  int nNeigh;
+  double cs;
+  int nb[3];
+};
+double handle_count_pairs(Handle &h);
+DirectParams direct_params(const Handle &h);
+void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
+                   void *src, void *qi, hipStream_t s);
+void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *qi,
+                   const int4 *cell, const int *cellStart, double *Vdir, hipStream_t s);
+void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
+                        const int *perm, const double *coord, unsigned long long *total,
+                        hipStream_t s);
+void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
+                   const int nt[3], const int *tileStart, const int *tileList, const void *src,
+                   const void *qi, const int4 *ms, const void *frac, void *grid, hipStream_t s);
+void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3],
+                   size_t n, const int4 *ms, const void *frac, const void *grid,
+                   const double *Vdir, const int *perm, const double *q_in, const double fself[2],
+                   double *Vout, hipStream_t s);
+void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
+                  const double *coordS, const int *perm, const double *q_in, size_t nt,
+                  const long *idx, double *Vout, hipStream_t s);
+void launch_fgrid_real_space(const DMKernel &k, const long Mf[3], const double *d_tab,
+                             void *F, hipStream_t s);
+void launch_make_G(bool fp64, bool complexK, const long Mf[3], long K3, const double *F,
+                   const double *lam, void *G, hipStream_t s);
+
+}  // namespace npme

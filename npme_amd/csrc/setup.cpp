@@ -1,0 +1,541 @@
+// Per-geometry set-up and the CalcV driver.
+//
+// Set-up (host FP64, integer indices bit-identical to the reference):
+//   * keywords -> kernel, DM fit, FFT grid, B-spline order
+//   * embedded cell partition (cell = Rdir/nNeigh inside clusters of
+//     nCellClust1D^3 cells) and the stable sort of the charges by cell,
+//     src/PartitionEmbeddedBox.cpp:266-455, src/PartitionBox.cpp:311-351
+//   * PME-corrected box and spline windows mStart per charge,
+//     src/RecSumSupportFunctions.cpp:276-296, .h:123-149
+//   * per-tile charge lists for the atomic-free spreading
+//   * kernel grid G = F(|k|) lam^2 / (N1 N2 N3): real-space compact grid of
+//     f_LR in FP64 on the device, then an even-extension DFT per axis
+//     (src/RecSumGrid.cpp:45-651), lam^2 from src/RecSumV.cpp:565-581
+// Device layout: the compact grid is stored in m-order, u = m + N/4
+// (a fixed cyclic shift of the reference's DFT order p = m mod N/2), so a
+// spline window is always a contiguous box and the zero padding of the
+// full grid is the tail [N/2, N) of every axis.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <sstream>
+
+#include "npme_internal.h"
+
+namespace npme {
+
+static void kw_error(const std::string &m) { throw Error{NPME_ERR_KEYWORD, m}; }
+
+template <typename T> static T *dalloc(Handle &h, size_t count)
+{
+  void *p = nullptr;
+  const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    std::ostringstream os;
+    os << "device allocation of " << bytes << " bytes failed: " << hipGetErrorString(e);
+    throw Error{NPME_ERR_NOMEM, os.str()};
+  }
+  h.deviceBytes += bytes;
+  return (T *)p;
+}
+
+void handle_free_device(Handle &h)
+{
+  void *ptrs[] = {h.d_perm, h.d_src, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
+                  h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
+                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  h.d_perm = nullptr;
+  h.d_src = h.d_qi = nullptr;
+  h.d_cell = nullptr;
+  h.d_cellStart = nullptr;
+  h.d_ms = nullptr;
+  h.d_frac = nullptr;
+  h.d_tileStart = h.d_tileList = nullptr;
+  h.d_grid = h.d_buf1 = h.d_buf2 = h.d_G = nullptr;
+  h.d_tw[0] = h.d_tw[1] = h.d_tw[2] = nullptr;
+  h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
+  h.deviceBytes = 0;
+}
+
+// ---------------------------------------------------------------- cells
+// cell index (size_t)(x / cs) with the 1e-14 upper-edge correction
+// (src/PartitionBox.cpp:311-351)
+static unsigned long coord2cell(double x, double cs, unsigned long ncell)
+{
+  const double tol = 1.0e-14;
+  if (x < -tol) throw Error{NPME_ERR_ARG, "coordinate below the cell box"};
+  unsigned long v = (unsigned long)(x / cs);
+  if (v > ncell) throw Error{NPME_ERR_ARG, "coordinate above the cell box"};
+  if (v == ncell) {
+    if (std::fabs(x - v * cs) < tol) v--;
+    else throw Error{NPME_ERR_ARG, "coordinate above the cell box"};
+  }
+  return v;
+}
+
+static std::vector<double> twiddles(int N)
+{
+  const double PI = 3.14159265358979323846;
+  std::vector<double> t(2 * (size_t)N);
+  for (int k = 0; k < N; k++) {
+    const double a = 2.0 * PI * (double)k / (double)N;
+    t[2 * k] = std::cos(a);
+    t[2 * k + 1] = -std::sin(a);
+  }
+  return t;
+}
+
+static void *upload_twiddles(Handle &h, int N, bool fp64)
+{
+  std::vector<double> t = twiddles(N);
+  if (fp64) {
+    double *d = dalloc<double>(h, t.size());
+    NPME_HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    return d;
+  }
+  std::vector<float> f(t.begin(), t.end());
+  float *d = dalloc<float>(h, f.size());
+  NPME_HIP_CHECK(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+  return d;
+}
+
+// top-hat smoothing of the Fourier extension (src/RecSumGrid.cpp:1124-1231)
+static double hramp(double x, double a)
+{
+  const double erf_max = 6.0;
+  if (x < -1.0) return 0.0;
+  if (x > 1.0) return 1.0;
+  const double z = erf_max / a;
+  const double xMax = z / std::sqrt(1.0 + z * z);
+  if (x >= xMax) return 1.0;
+  if (x <= -xMax) return 0.0;
+  return 0.5 * (1.0 + std::erf(a * x / std::sqrt(1.0 - x * x)));
+}
+
+static double tsmooth(double x, double X, double Xmax, double a)
+{
+  const double eps = 1.0e-12 * Xmax;
+  if (x < -Xmax - eps || x > Xmax + eps)
+    throw Error{NPME_ERR_GRID, "top-hat argument outside the Fourier extension"};
+  const double del = (Xmax - X) * 0.5;
+  if (x < -X) return hramp((x + X + del) / del, a);
+  if (x <= X) return 1.0;
+  return hramp((-x + X + del) / del, a);
+}
+
+// ---------------------------------------------------------------- set-up
+void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
+{
+  h.kw = kw;
+  h.n = n;
+  if (n < 2) throw Error{NPME_ERR_ARG, "nCharge must be at least 2"};
+  std::string funcType, split, prec;
+  if (!kw.get("funcType", funcType)) kw_error("missing keyword 'funcType'");
+  if (!kw.get("EwaldSplit", split)) kw_error("missing keyword 'EwaldSplit'");
+  if (split != "DerivMatch")
+    throw Error{NPME_ERR_UNSUPPORTED, "EwaldSplit = " + split + " is not provided (DerivMatch only)"};
+  double Rdir;
+  int Nder;
+  if (!kw.get("Rdir", Rdir)) kw_error("missing keyword 'Rdir'");
+  if (!kw.get("nDeriv", Nder)) kw_error("missing keyword 'nDeriv'");
+  if (!kw.get("BnOrder", h.Bn)) kw_error("missing keyword 'BnOrder'");
+  const char *nk[3] = {"N1", "N2", "N3"}, *bk[3] = {"n1", "n2", "n3"};
+  const char *dk[3] = {"del1", "del2", "del3"}, *ak[3] = {"a1", "a2", "a3"};
+  for (int d = 0; d < 3; d++) {
+    if (!kw.get(nk[d], h.N[d])) kw_error(std::string("missing keyword '") + nk[d] + "'");
+    if (!kw.get(bk[d], h.nblk[d])) kw_error(std::string("missing keyword '") + bk[d] + "'");
+    if (!kw.get(dk[d], h.del[d])) kw_error(std::string("missing keyword '") + dk[d] + "'");
+    if (!kw.get(ak[d], h.ta[d])) kw_error(std::string("missing keyword '") + ak[d] + "'");
+  }
+  if (!kw.get("nNeigh", h.nNeigh)) kw_error("missing keyword 'nNeigh'");
+  if (!kw.get("nCellClust1D", h.nCellClust1D)) kw_error("missing keyword 'nCellClust1D'");
+  long useCompactF = 1;
+  if (kw.get("useCompactF", useCompactF) && useCompactF != 1)
+    throw Error{NPME_ERR_UNSUPPORTED, "useCompactF = 0 is not provided"};
+  h.fp64 = kw.get("precision", prec) && prec == "fp64";
+  if (kw.get("precision", prec) && prec != "fp64" && prec != "mixed")
+    kw_error("precision must be 'mixed' or 'fp64'");
+
+  int kind;
+  double alpha = -1.0, k0r = 0, k0i = 0;
+  if (funcType == "Laplace") kind = KIND_LAPLACE;
+  else if (funcType == "Ralpha") {
+    kind = KIND_RALPHA;
+    if (!kw.get("alpha", alpha)) kw_error("funcType = Ralpha needs 'alpha'");
+  } else if (funcType == "Helmholtz") {
+    kind = KIND_HELMHOLTZ;
+    if (!kw.get("k0_r", k0r) || !kw.get("k0_i", k0i)) kw_error("funcType = Helmholtz needs k0_r, k0_i");
+    if (k0i < -1.0e-12) kw_error("k0_i is negative");
+  } else {
+    kw_error("funcType = " + funcType + " is undefined");
+    return;
+  }
+  h.complexK = (kind == KIND_HELMHOLTZ);
+  std::string err;
+  if (!dm_kernel_init(h.ker, kind, alpha, k0r, k0i, Nder, Rdir, err)) kw_error(err);
+  {
+    double f1[2];
+    dm_eval_lr(h.ker, 0.0, h.fself, f1);
+  }
+
+  // NPME_CheckFFTParm (src/RecSumSupportFunctions.cpp:37-135)
+  for (int d = 0; d < 3; d++) {
+    const long N = h.N[d], b = h.nblk[d];
+    std::ostringstream os;
+    if (N % 4 != 0) os << "N" << d + 1 << " = " << N << " is not a multiple of 4";
+    else if (h.Bn >= N / 4) os << "BnOrder = " << h.Bn << " >= N" << d + 1 << "/4";
+    else if (b > N / 4) os << "block size n" << d + 1 << " = " << b << " > N/4";
+    else if (b < 1 || N % (2 * b) != 0) os << "N" << d + 1 << " must be a multiple of 2 n" << d + 1;
+    if (!os.str().empty()) throw Error{NPME_ERR_GRID, os.str()};
+    if (make_plan((int)N).nst < 0) throw Error{NPME_ERR_GRID, "FFT size has a prime factor > 7"};
+    h.M[d] = N / 2;
+  }
+  if (h.Bn < 2 || h.Bn > 16) throw Error{NPME_ERR_UNSUPPORTED, "BnOrder must be in [2, 16]"};
+  if (h.N[0] + 1 > (h.fp64 ? 16 : 32) * 256 || h.N[1] + 1 > (h.fp64 ? 16 : 32) * 256 ||
+      h.N[2] + 1 > (h.fp64 ? 16 : 32) * 256)
+    throw Error{NPME_ERR_UNSUPPORTED, "FFT length too large for one LDS column"};
+  h.K3 = h.complexK ? h.N[2] : h.N[2] / 2 + 1;
+
+  // ---- geometry of the reciprocal sum
+  box_dim_center(n, coord, h.X0, h.R0);
+  pme_corr_box(h.X0, h.R0, h.N, h.del, h.Bn, h.X, h.R);
+  for (int d = 0; d < 3; d++) h.L[d] = 2 * (h.X[d] + h.del[d]);
+
+  // ---- embedded cell partition and stable sort by embedded cell index
+  const double csB = Rdir / h.nNeigh;
+  const unsigned long nP = (unsigned long)h.nCellClust1D;
+  const double csA = nP * csB;
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = coord[d];
+  for (size_t i = 1; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      mn[d] = std::min(mn[d], coord[3 * i + d]);
+      mx[d] = std::max(mx[d], coord[3 * i + d]);
+    }
+  unsigned long nA[3];
+  for (int d = 0; d < 3; d++) {
+    nA[d] = (unsigned long)((mx[d] - mn[d]) / csA) + 1;
+    h.nb[d] = (int)(nA[d] * nP);
+    h.boxMin[d] = mn[d];
+  }
+  h.csB = csB;
+  const unsigned long Nc = nP * nP * nP;
+  std::vector<unsigned long> key(n);
+  std::vector<int> gcell(3 * n);
+  for (size_t i = 0; i < n; i++) {
+    unsigned long a[3], b[3];
+    for (int d = 0; d < 3; d++) {
+      const double x1 = coord[3 * i + d] + (-mn[d]);
+      a[d] = coord2cell(x1, csA, nA[d]);
+      const double x2 = x1 - a[d] * csA;
+      b[d] = coord2cell(x2, csB, nP);
+      gcell[3 * i + d] = (int)(a[d] * nP + b[d]);
+    }
+    const unsigned long tA = (a[0] * nA[1] + a[1]) * nA[2] + a[2];
+    key[i] = tA * Nc + (b[0] * nP + b[1]) * nP + b[2];
+  }
+  h.perm.resize(n);
+  std::iota(h.perm.begin(), h.perm.end(), 0);
+  std::stable_sort(h.perm.begin(), h.perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+
+  const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
+  std::vector<int> cellSE(2 * (size_t)ncell, 0);
+  std::vector<int4> cellOf(n);
+  std::vector<double> loc(4 * n);
+  for (size_t s = 0; s < n; s++) {
+    const int i = h.perm[s];
+    const int g0 = gcell[3 * i], g1 = gcell[3 * i + 1], g2 = gcell[3 * i + 2];
+    const long c = ((long)g0 * h.nb[1] + g1) * h.nb[2] + g2;
+    if (cellSE[2 * c + 1] == 0) cellSE[2 * c] = (int)s;
+    cellSE[2 * c + 1]++;
+    cellOf[s] = make_int4(g0, g1, g2, 0);
+    // coordinates relative to the low corner of the charge's cell
+    loc[4 * s + 0] = (coord[3 * i + 0] - mn[0]) - g0 * csB;
+    loc[4 * s + 1] = (coord[3 * i + 1] - mn[1]) - g1 * csB;
+    loc[4 * s + 2] = (coord[3 * i + 2] - mn[2]) - g2 * csB;
+    loc[4 * s + 3] = 0;
+  }
+
+  // ---- spline windows (u = m + N/4) and per-tile lists
+  std::vector<int4> ms(n);
+  std::vector<double> frac(4 * n);
+  for (size_t s = 0; s < n; s++) {
+    const int i = h.perm[s];
+    int u0[3];
+    for (int d = 0; d < 3; d++) {
+      const double w = h.N[d] / h.L[d] * (coord[3 * i + d] - h.R[d]);
+      const long m0 = (w < 0) ? (long)(-w) + 1 : (long)(-w);
+      if (m0 < -h.N[d] / 4 || m0 + h.Bn - 1 > h.N[d] / 4 - 1)
+        throw Error{NPME_ERR_GRID, "spline window outside the compact grid"};
+      u0[d] = (int)(m0 + h.N[d] / 4);
+      frac[4 * s + d] = w + (double)m0;
+    }
+    frac[4 * s + 3] = 0;
+    ms[s] = make_int4(u0[0], u0[1], u0[2], 0);
+  }
+  for (int d = 0; d < 3; d++) h.nt[d] = (int)((h.M[d] + 7) / 8);
+  const long ntiles = (long)h.nt[0] * h.nt[1] * h.nt[2];
+  std::vector<int> tcount(ntiles + 1, 0);
+  auto tile_range = [&](int u0, int &lo, int &hi) {
+    lo = u0 / 8;
+    hi = (u0 + (int)h.Bn - 1) / 8;
+  };
+  for (size_t s = 0; s < n; s++) {
+    int lo[3], hi[3];
+    tile_range(ms[s].x, lo[0], hi[0]);
+    tile_range(ms[s].y, lo[1], hi[1]);
+    tile_range(ms[s].z, lo[2], hi[2]);
+    for (int a = lo[0]; a <= hi[0]; a++)
+      for (int b = lo[1]; b <= hi[1]; b++)
+        for (int c = lo[2]; c <= hi[2]; c++) tcount[((long)a * h.nt[1] + b) * h.nt[2] + c + 1]++;
+  }
+  for (long t = 0; t < ntiles; t++) tcount[t + 1] += tcount[t];
+  std::vector<int> tlist(tcount[ntiles]);
+  {
+    std::vector<int> fill(tcount.begin(), tcount.end() - 1);
+    for (size_t s = 0; s < n; s++) {
+      int lo[3], hi[3];
+      tile_range(ms[s].x, lo[0], hi[0]);
+      tile_range(ms[s].y, lo[1], hi[1]);
+      tile_range(ms[s].z, lo[2], hi[2]);
+      for (int a = lo[0]; a <= hi[0]; a++)
+        for (int b = lo[1]; b <= hi[1]; b++)
+          for (int c = lo[2]; c <= hi[2]; c++) tlist[fill[((long)a * h.nt[1] + b) * h.nt[2] + c]++] = (int)s;
+    }
+  }
+
+  // ---- device state
+  NPME_HIP_CHECK(hipGetDevice(&h.device));
+  if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  handle_free_device(h);
+  const bool f64 = h.fp64;
+  const size_t rs = f64 ? 8 : 4;
+  h.d_perm = dalloc<int>(h, n);
+  NPME_HIP_CHECK(hipMemcpy(h.d_perm, h.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+  h.d_src = dalloc<char>(h, 4 * rs * n);
+  h.d_frac = dalloc<char>(h, 4 * rs * n);
+  if (f64) {
+    NPME_HIP_CHECK(hipMemcpy(h.d_src, loc.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+    NPME_HIP_CHECK(hipMemcpy(h.d_frac, frac.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> lf(loc.begin(), loc.end()), ff(frac.begin(), frac.end());
+    NPME_HIP_CHECK(hipMemcpy(h.d_src, lf.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+    NPME_HIP_CHECK(hipMemcpy(h.d_frac, ff.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+  }
+  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
+  h.d_cell = dalloc<int4>(h, n);
+  NPME_HIP_CHECK(hipMemcpy(h.d_cell, cellOf.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+  h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
+  NPME_HIP_CHECK(hipMemcpy(h.d_cellStart, cellSE.data(), 2 * (size_t)ncell * sizeof(int), hipMemcpyHostToDevice));
+  h.d_ms = dalloc<int4>(h, n);
+  NPME_HIP_CHECK(hipMemcpy(h.d_ms, ms.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+  h.d_tileStart = dalloc<int>(h, ntiles + 1);
+  NPME_HIP_CHECK(hipMemcpy(h.d_tileStart, tcount.data(), (ntiles + 1) * sizeof(int), hipMemcpyHostToDevice));
+  h.d_tileList = dalloc<int>(h, tlist.size());
+  NPME_HIP_CHECK(hipMemcpy(h.d_tileList, tlist.data(), tlist.size() * sizeof(int), hipMemcpyHostToDevice));
+  const int vw = h.complexK ? 8 : 4;
+  h.d_Vdir = dalloc<double>(h, vw * n);
+  h.d_V = dalloc<double>(h, vw * n);
+  h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
+  h.d_coordS = dalloc<double>(h, 3 * n);
+  NPME_HIP_CHECK(hipMemcpy(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+
+  const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2];
+  const size_t cs = 2 * rs;  // complex element size
+  h.d_grid = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)(M1 * M2 * M3));
+  h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.K3));
+  h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.K3));
+  for (int d = 0; d < 3; d++) {
+    h.plan[d] = make_plan((int)h.N[d]);
+    h.d_tw[d] = upload_twiddles(h, (int)h.N[d], f64);
+  }
+
+  // ---- kernel grid G (FP64 build, stored in the working precision)
+  const long Mf[3] = {h.N[0] / 2 + 1, h.N[1] / 2 + 1, h.N[2] / 2 + 1};
+  const long totF = Mf[0] * Mf[1] * Mf[2];
+  std::vector<double> tab;
+  for (int d = 0; d < 3; d++) {
+    const double Xmax = h.X[d] + h.del[d], Xt = 2 * Xmax - h.X[d];
+    const double dx = 2 * Xmax / h.N[d];
+    std::vector<double> x1(Mf[d]), x2(Mf[d]), T1(Mf[d]), T2(Mf[d]);
+    for (long i = 0; i < Mf[d]; i++) {
+      x1[i] = i * dx;
+      const double eps = 1.0e-12 * Xmax;
+      if (x1[i] > Xmax + eps) throw Error{NPME_ERR_GRID, "kernel grid beyond the extension"};
+      if (x1[i] > h.X[d]) {
+        T1[i] = tsmooth(x1[i], h.X[d], Xt, h.ta[d]);
+        x2[i] = x1[i] - 2 * Xmax;
+        T2[i] = tsmooth(x2[i], h.X[d], Xt, h.ta[d]);
+      } else {
+        T1[i] = 1.0;
+        x2[i] = 1.0;
+        T2[i] = 0.0;
+      }
+    }
+    tab.insert(tab.end(), x1.begin(), x1.end());
+    tab.insert(tab.end(), x2.begin(), x2.end());
+    tab.insert(tab.end(), T1.begin(), T1.end());
+    tab.insert(tab.end(), T2.begin(), T2.end());
+  }
+  double *d_tab = nullptr, *d_F = nullptr, *d_lam = nullptr, *d_tw64[3] = {nullptr, nullptr, nullptr};
+  try {
+    NPME_HIP_CHECK(hipMalloc(&d_tab, tab.size() * sizeof(double)));
+    NPME_HIP_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    NPME_HIP_CHECK(hipMalloc(&d_F, 2 * totF * sizeof(double)));
+    launch_fgrid_real_space(h.ker, Mf, d_tab, d_F, h.stream);
+    for (int d = 0; d < 3; d++) {
+      std::vector<double> t = twiddles((int)h.N[d]);
+      NPME_HIP_CHECK(hipMalloc(&d_tw64[d], t.size() * sizeof(double)));
+      NPME_HIP_CHECK(hipMemcpy(d_tw64[d], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // even-extension DFT along axis 3, 2, 1 of F[Mf0][Mf1][Mf2]
+    fft_pass_even((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], d_tw64[2], d_F, h.stream);
+    fft_pass_even((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], d_tw64[1], d_F, h.stream);
+    fft_pass_even((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], d_tw64[0], d_F, h.stream);
+    std::vector<double> lam;
+    for (int d = 0; d < 3; d++) {
+      std::vector<double> l(h.N[d]);
+      lambda2(h.N[d], h.Bn, l.data());
+      lam.insert(lam.end(), l.begin(), l.begin() + Mf[d]);
+    }
+    NPME_HIP_CHECK(hipMalloc(&d_lam, lam.size() * sizeof(double)));
+    NPME_HIP_CHECK(hipMemcpy(d_lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+    h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)totF);
+    launch_make_G(f64, h.complexK, Mf, h.K3, d_F, d_lam, h.d_G, h.stream);
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  } catch (...) {
+    (void)hipFree(d_tab);
+    (void)hipFree(d_F);
+    (void)hipFree(d_lam);
+    for (int d = 0; d < 3; d++) (void)hipFree(d_tw64[d]);
+    throw;
+  }
+  (void)hipFree(d_tab);
+  (void)hipFree(d_F);
+  (void)hipFree(d_lam);
+  for (int d = 0; d < 3; d++) (void)hipFree(d_tw64[d]);
+}
+
+// ---------------------------------------------------------------- CalcV
+static void mark(Handle &h, int i)
+{
+  if (!h.profiling) return;
+  while ((int)h.ev.size() <= i) {
+    hipEvent_t e;
+    NPME_HIP_CHECK(hipEventCreate(&e));
+    h.ev.push_back(e);
+  }
+  NPME_HIP_CHECK(hipEventRecord(h.ev[i], h.stream));
+}
+
+DirectParams direct_params(const Handle &h)
+{
+  DirectParams dp;
+  dp.kind = h.ker.kind;
+  dp.Nder = h.ker.Nder;
+  dp.Rdir = h.ker.Rdir;
+  dp.alpha = h.ker.alpha;
+  dp.k0r = h.ker.k0r;
+  dp.k0i = h.ker.k0i;
+  std::memcpy(dp.a, h.ker.a, sizeof(dp.a));
+  std::memcpy(dp.b, h.ker.b, sizeof(dp.b));
+  dp.nNeigh = (int)h.nNeigh;
+  dp.cs = h.csB;
+  for (int d = 0; d < 3; d++) dp.nb[d] = h.nb[d];
+  return dp;
+}
+
+double handle_count_pairs(Handle &h)
+{
+  unsigned long long *d_tot = nullptr, tot = 0;
+  NPME_HIP_CHECK(hipMalloc(&d_tot, sizeof(unsigned long long)));
+  try {
+    NPME_HIP_CHECK(hipMemsetAsync(d_tot, 0, sizeof(unsigned long long), h.stream));
+    launch_count_pairs(direct_params(h), h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, d_tot,
+                       h.stream);
+    NPME_HIP_CHECK(hipMemcpyAsync(&tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  } catch (...) {
+    (void)hipFree(d_tot);
+    throw;
+  }
+  (void)hipFree(d_tot);
+  return (double)tot;
+}
+
+void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
+{
+  const bool f64 = h.fp64, cx = h.complexK;
+  hipStream_t s = h.stream;
+  const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2], N1 = h.N[0], N2 = h.N[1], N3 = h.N[2];
+  const long K3 = h.K3;
+  int st = 0;
+  mark(h, st++);
+  launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
+  mark(h, st++);
+  const DirectParams dp = direct_params(h);
+  launch_direct(f64, dp, h.n, h.d_src, h.d_qi, h.d_cell, h.d_cellStart, h.d_Vdir, s);
+  mark(h, st++);
+  launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_src, h.d_qi,
+                h.d_ms, h.d_frac, h.d_grid, s);
+  mark(h, st++);
+  if (!cx) {
+    fft_pass_r2c_rows(f64, (int)N3, (int)M3, M1 * M2, M3, K3, h.plan[2], h.d_tw[2], h.d_grid,
+                      h.d_buf1, s);
+    mark(h, st++);
+    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * K3, K3, M2 * K3, 1, K3, N2 * K3, 1, K3};
+    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s);
+    mark(h, st++);
+    fft_pass_convolve(f64, false, (int)N1, (int)M1, N2 * K3, N2 * K3, (int)N2, (int)(N2 / 2 + 1),
+                      K3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s);
+    mark(h, st++);
+    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * K3, K3, N2 * K3, 1, K3, M2 * K3, 1, K3};
+    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s);
+    mark(h, st++);
+    fft_pass_c2r_rows(f64, (int)N3, (int)M3, M1 * M2, K3, M3, h.plan[2], h.d_tw[2], h.d_buf1,
+                      h.d_grid, s);
+    mark(h, st++);
+  } else {
+    PassGeom a{(int)N3, (int)M3, (int)N3, M1 * M2, M1 * M2, 0, M3, 1, 0, N3, 1};
+    fft_pass_complex(f64, a, h.plan[2], h.d_tw[2], h.d_grid, h.d_buf1, false, s);
+    mark(h, st++);
+    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * N3, N3, M2 * N3, 1, N3, N2 * N3, 1, N3};
+    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s);
+    mark(h, st++);
+    fft_pass_convolve(f64, true, (int)N1, (int)M1, N2 * N3, N2 * N3, (int)N2, (int)(N2 / 2 + 1),
+                      N3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s);
+    mark(h, st++);
+    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * N3, N3, N2 * N3, 1, N3, M2 * N3, 1, N3};
+    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s);
+    mark(h, st++);
+    PassGeom e{(int)N3, (int)N3, (int)M3, M1 * M2, M1 * M2, 0, N3, 1, 0, M3, 1};
+    fft_pass_complex(f64, e, h.plan[2], h.d_tw[2], h.d_buf1, h.d_grid, true, s);
+    mark(h, st++);
+  }
+  const double C[3] = {N1 / h.L[0], N2 / h.L[1], N3 / h.L[2]};
+  launch_interp(f64, cx, h.Bn, h.M, C, h.n, h.d_ms, h.d_frac, h.d_grid, h.d_Vdir, h.d_perm, d_q,
+                h.fself, d_Vout, s);
+  mark(h, st++);
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  if (h.profiling) {
+    h.nstage = st - 1;
+    for (int i = 0; i + 1 < st; i++) {
+      float ms = 0;
+      NPME_HIP_CHECK(hipEventElapsedTime(&ms, h.ev[i], h.ev[i + 1]));
+      h.stage_ms[i] = ms;
+    }
+  }
+}
+
+void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout)
+{
+  launch_exact(h.ker.kind, h.ker.alpha, h.ker.k0r, h.ker.k0i, h.n, h.d_coordS, h.d_perm, d_q, nt,
+               d_idx, d_Vout, h.stream);
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
+}  // namespace npme

@@ -9,6 +9,7 @@
 #include "npme_oracle.h"
 
 #include <complex.h>
+#include <omp.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1075,9 +1076,17 @@ double npo_count_pairs(size_t n, const double *coord, double Rdir)
  * The reference permutes charges into cell order first; that changes only
  * the order of summation, so this restatement works in input order.
  * ==================================================================== */
+static double g_times[3];   /* direct, kernel-grid set-up, reciprocal */
+
+void npo_last_calcv_times(double t[3])
+{
+  t[0] = g_times[0]; t[1] = g_times[1]; t[2] = g_times[2];
+}
+
 int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
               const double *q, const double *coord, double *V)
 {
+  double t0 = omp_get_wtime();
   const int cx = (k->kind == NPO_HELMHOLTZ);
   const int w = cx ? 8 : 4;
   npo_direct_sr(k, n, coord, q, V);
@@ -1102,7 +1111,9 @@ int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
   double *Q = (double *) malloc(sizeof(double) * 2 * M);
   double *T = (double *) malloc(sizeof(double) * 2 * M);
   double *V1 = (double *) malloc(sizeof(double) * 8 * (n ? n : 1));
+  double t1 = omp_get_wtime();
   int err = npo_fgrid_compact(k, p->N, p->a, p->del, X, F);
+  double t2 = omp_get_wtime();
   if (!err) err = npo_spread_basic(n, q, cx, coord, p->N, L, p->Bn, R, Q);
   if (!err) {
     npo_theta_8parity(p->N, p->Bn, F, Q, T);
@@ -1120,6 +1131,10 @@ int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
       }
   }
   free(F); free(Q); free(T); free(V1);
+  double t3 = omp_get_wtime();
+  g_times[0] = t1 - t0;
+  g_times[1] = t2 - t1;
+  g_times[2] = t3 - t2;
   return err;
 }
 

@@ -138,6 +138,10 @@ typedef struct {
 int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
               const double *q, const double *coord, double *V);
 
+/* wall times of the last npo_calcv: [0] direct sum + self term,
+ * [1] kernel-grid set-up (F), [2] spreading + convolution + interpolation */
+void npo_last_calcv_times(double t[3]);
+
 /* compareV metric (src/SupportFunctions.cpp:88-305):
  * out[0] = errorV, out[1] = errordVdr, out[2] = mean|V_ref|,
  * out[3] = mean over p=0..2 of |V_ref[4i+p]| (reference quirk),

@@ -79,6 +79,7 @@ def lib():
         L.npo_count_pairs.argtypes = [sz, p, d]
         L.npo_calcv.argtypes = [C.POINTER(Kernel), C.POINTER(RecSumParm), sz, p, p, p]
         L.npo_compare_v.argtypes = [sz, i, p, p, p]
+        L.npo_last_calcv_times.argtypes = [p]
         L.npo_splitmix_uniform.argtypes = [C.c_ulonglong, sz, p]
         _lib = L
     return _lib
@@ -313,6 +314,12 @@ def calcv(k, N, Bn, coord, q, nblk=(20, 20, 20), dl=None, a=(4.0, 4.0, 4.0)):
     if rc != 0:
         raise ValueError("oracle CalcV failed")
     return V
+
+
+def last_calcv_times():
+    t = np.zeros(3)
+    lib().npo_last_calcv_times(_ptr(t))
+    return {"direct": t[0], "setup": t[1], "recip": t[2]}
 
 
 def compare_v(Vref, V):

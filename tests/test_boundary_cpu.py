@@ -1,0 +1,107 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry
+point declared in include/npme_amd.h, and its host-only logic (keyword
+defaults, validation and the RSEM grid choice of NPME_AddMissingKeywords)
+agrees with the oracle.  No device calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, uniform_box
+
+HEADER = os.path.join(ROOT, "include", "npme_amd.h")
+
+
+def _declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(npme_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import npme_amd
+    L = ctypes.CDLL(npme_amd.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _kw(**extra):
+    import npme_amd as N
+    k = N.NPME_KeywordInput()
+    base = dict(calcType="pme", funcType="Laplace", Rdir=8.0, BnOrder=6, tol=1e-6)
+    base.update(extra)
+    for a, b in base.items():
+        k.AddKeyword(a, b)
+    return k
+
+
+def test_add_missing_keywords_defaults_and_rsem(oracle):
+    import npme_amd as N
+    coord, q = uniform_box(10000, 40.0)
+    k = _kw()
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+    d = k.as_dict()
+    # defaults of src/Interface.cpp:53-517 and src/Constant.h:48-63
+    assert d["EwaldSplit"] == "DerivMatch" and d["useCompactF"] == "1"
+    assert float(d["FFTmemGB"]) == 10.0 and d["nNeigh"] == "2" and d["nCellClust1D"] == "3"
+    assert all(float(d[x]) == 4.0 for x in ("a1", "a2", "a3"))
+    assert all(float(d[x]) == 8.0 for x in ("del1", "del2", "del3"))
+    grid = tuple(int(d[x]) for x in ("N1", "N2", "N3"))
+    blk = tuple(int(d[x]) for x in ("n1", "n2", "n3"))
+    assert (grid, blk, int(d["nDeriv"])) == oracle.rsem_static(q, coord, 6, -1.0, 1e-6, 8.0)
+    # default BnOrder is 8
+    k8 = N.NPME_KeywordInput()
+    for a, b in dict(calcType="pme", funcType="Laplace", Rdir=8.0).items():
+        k8.AddKeyword(a, b)
+    assert N.NPME_AddMissingKeywords(k8, len(q), coord, chargeReal=q)
+    assert k8.GetValue("BnOrder") == "8"
+
+
+def test_add_missing_keywords_explicit_grid_nderiv_default():
+    import npme_amd as N
+    coord, q = uniform_box(500, 10.0)
+    for bn, nd in ((4, 3), (6, 6), (8, 8)):
+        k = _kw(BnOrder=bn, N1=64, N2=64, N3=64, n1=16, n2=16, n3=16)
+        assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+        assert k.GetValue("nDeriv") == str(nd)
+
+
+@pytest.mark.parametrize("bad", [dict(calcType="fast"), dict(funcType="Yukawa"),
+                                 dict(EwaldSplit="Other"), dict(funcType="Ralpha"),
+                                 dict(funcType="Helmholtz", k0_r=1.0, k0_i=0.0)])
+def test_add_missing_keywords_rejects(bad):
+    import npme_amd as N
+    coord, q = uniform_box(500, 10.0)
+    k = _kw(**bad)
+    assert not N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+    assert N.npme_last_error()
+
+
+def test_add_missing_keywords_helmholtz(oracle):
+    import npme_amd as N
+    coord, _ = uniform_box(4000, 30.0)
+    rng = np.random.default_rng(3)
+    qc = rng.standard_normal(4000) + 1j * rng.standard_normal(4000)
+    k = _kw(funcType="Helmholtz", k0_r=0.5, k0_i=0.0, Rdir=10.0, BnOrder=8)
+    assert N.NPME_AddMissingKeywords(k, len(qc), coord, chargeComplex=qc)
+    d = k.as_dict()
+    got = (tuple(int(d[x]) for x in ("N1", "N2", "N3")), tuple(int(d[x]) for x in ("n1", "n2", "n3")),
+           int(d["nDeriv"]))
+    assert got == oracle.rsem_helmholtz(qc, coord, 8, 0.5, 1e-6, 10.0)
+    # negative imaginary wave number is refused (src/Interface.cpp:160-165)
+    k2 = _kw(funcType="Helmholtz", k0_r=0.5, k0_i=-0.1, Rdir=10.0, BnOrder=8)
+    assert not N.NPME_AddMissingKeywords(k2, len(qc), coord, chargeComplex=qc)
+
+
+def test_keyword_file_roundtrip(tmp_path):
+    import npme_amd as N
+    p = tmp_path / "kw.txt"
+    p.write_text("@Keyword\n  BnOrder  6  (B-spline order)\n  calcType pme_exact\n"
+                 "  funcType Laplace\n  Rdir 4.0   (direct space cutoff)\n")
+    k = N.NPME_KeywordInput(str(p))
+    assert k.GetValue("BnOrder") == "6" and k.GetValue("Rdir") == "4.0"
+    assert k.GetValue("calcType") == "pme_exact" and k.GetValue("missing") is None
