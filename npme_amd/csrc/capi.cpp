@@ -392,6 +392,10 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  fself " << H.fself[0] << " " << H.fself[1] << "\n";
   os << "  cells " << H.nb[0] << "x" << H.nb[1] << "x" << H.nb[2] << " of " << H.csB
      << "  device GB " << H.deviceBytes / 1e9 << "\n";
+  if (H.useNlist)
+    os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";
+  else
+    os << "  direct sum: cell rows, " << H.nSegments << " segments\n";
   std::snprintf(buf, buflen, "%s", os.str().c_str());
   return NPME_OK;
 }

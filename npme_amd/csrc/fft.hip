@@ -11,10 +11,16 @@
 //   E  C2R rows along axis 3, M3 real outputs
 // Backward transforms use conj(DFT(conj(x))), so only forward butterflies
 // exist.  The twiddle table holds exp(-2 pi i t/N) computed in FP64 on the
-// host.  One workgroup transforms B columns held in LDS (column-major,
-// stride N+1 elements); every thread keeps all butterflies of a stage in
-// registers between the read and the write barrier, so one LDS buffer
-// suffices.
+// host.
+//
+// A workgroup of 256 threads transforms B columns (B a power of two) held in
+// LDS; every thread owns one column and a strided subset of its butterflies,
+// so all per-column index arithmetic is done once, with shifts.  Strided
+// passes keep LDS as [n][c] (B consecutive columns side by side: global rows
+// of B complex values, 128 B for B = 16 in FP32, map to contiguous LDS);
+// row passes keep [c][n] with one element of padding per column.  Between
+// the read and the write barrier of a stage every thread holds its
+// butterflies in registers, so one LDS buffer suffices.
 #include "npme_internal.h"
 
 namespace npme {
@@ -37,8 +43,7 @@ template <typename T> __device__ __forceinline__ T cmul(T a, T b)
   return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 template <typename T> __device__ __forceinline__ T cconj(T a) { return mk<T>(a.x, -a.y); }
-// multiply by -i
-template <typename T> __device__ __forceinline__ T mul_mi(T a) { return mk<T>(a.y, -a.x); }
+template <typename T> __device__ __forceinline__ T mul_mi(T a) { return mk<T>(a.y, -a.x); }  // * -i
 
 constexpr int NT = 256;  // threads per workgroup
 
@@ -109,9 +114,25 @@ template <typename T> struct DFT<T, 3> {
   }
 };
 
-// odd radix R = 5, 7: pairwise symmetric form
+// odd radix R = 5, 7: pairwise symmetric form with constant cos/sin
+template <int R> struct OddTab;
+template <> struct OddTab<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                  -0.80901699437494742410, 0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                  -0.58778525229247312917, -0.95105651629515357212};
+};
+template <> struct OddTab<7> {
+  static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                  -0.90096886790241912624, -0.90096886790241912624,
+                                  -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                  0.43388373911755812048, -0.43388373911755812048,
+                                  -0.97492791218182360702, -0.78183148246802980871};
+};
+
 template <typename T, int R> struct DFTOdd {
-  __device__ static void run(T *x, const double *cs, const double *sn)
+  __device__ static void run(T *x)
   {
     using S = decltype(T::x);
     constexpr int H = (R - 1) / 2;
@@ -131,7 +152,7 @@ template <typename T, int R> struct DFTOdd {
       T A = x[0], B = mk<T>(0, 0);
 #pragma unroll
       for (int j = 1; j <= H; j++) {
-        const S c = (S)cs[(j * k) % R], s = (S)sn[(j * k) % R];
+        const S c = (S)OddTab<R>::c[(j * k) % R], s = (S)OddTab<R>::s[(j * k) % R];
         A.x += c * Sj[j - 1].x;
         A.y += c * Sj[j - 1].y;
         B.x += s * Dj[j - 1].x;
@@ -145,305 +166,311 @@ template <typename T, int R> struct DFTOdd {
   }
 };
 
-__device__ __constant__ double c5_cos[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
-                                            -0.80901699437494742410, 0.30901699437494742410};
-__device__ __constant__ double c5_sin[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
-                                            -0.58778525229247312917, -0.95105651629515357212};
-__device__ __constant__ double c7_cos[7] = {1.0,
-                                            0.62348980185873353053,
-                                            -0.22252093395631440429,
-                                            -0.90096886790241912624,
-                                            -0.90096886790241912624,
-                                            -0.22252093395631440429,
-                                            0.62348980185873353053};
-__device__ __constant__ double c7_sin[7] = {0.0,
-                                            0.78183148246802980871,
-                                            0.97492791218182360702,
-                                            0.43388373911755812048,
-                                            -0.43388373911755812048,
-                                            -0.97492791218182360702,
-                                            -0.78183148246802980871};
-
 template <typename T, int R> __device__ __forceinline__ void dft_any(T *x)
 {
-  if constexpr (R == 5)
-    DFTOdd<T, 5>::run(x, c5_cos, c5_sin);
-  else if constexpr (R == 7)
-    DFTOdd<T, 7>::run(x, c7_cos, c7_sin);
+  if constexpr (R == 5 || R == 7)
+    DFTOdd<T, R>::run(x);
   else
     DFT<T, R>::run(x);
 }
 
+// ------------------------------------------------------------ column access
+// element n of the thread's column: [c][n] (row layout, stride ld = N + 1)
+// or [n][c] (strided layout, B columns side by side)
+struct ColRef {
+  int base, step;  // lds index = base + n * step
+  __device__ __forceinline__ int at(int n) const { return base + n * step; }
+};
+
 // ------------------------------------------------------------ Stockham stage
-// Columns c in [0, B) of length N at lds[c*ld + n].  Butterfly j of a column
-// reads x[j + r N/R], twiddles by w^(r k N/(Ns R)) with k = j mod Ns, and
-// writes y[(j/Ns) Ns R + k + r Ns].
+// butterfly j of a column reads x[j + r N/R], twiddles by w^(r k N/(Ns R)),
+// k = j mod Ns, and writes y[(j - k) R + k + r Ns].
 template <typename T, int R, int BPT>
-__device__ __forceinline__ void stage(T *lds, int ld, int B, int N, int Ns, const T *tw)
+__device__ __forceinline__ void stage(T *lds, ColRef col, int tj, int tpc, int N, int Ns,
+                                      const T *__restrict__ tw)
 {
   const int nb = N / R;
-  const int total = B * nb;
   const int stride = N / (Ns * R);
   T v[BPT][R];
-  int cj[BPT];
 #pragma unroll
   for (int u = 0; u < BPT; u++) {
-    const int b = threadIdx.x + u * NT;
-    if (b < total) {
-      const int c = b / nb, j = b - c * nb;
-      cj[u] = c * ld + j;
-      const T *col = lds + c * ld;
+    const int j = tj + u * tpc;
+    if (j < nb) {
 #pragma unroll
-      for (int r = 0; r < R; r++) v[u][r] = col[j + r * nb];
+      for (int r = 0; r < R; r++) v[u][r] = lds[col.at(j + r * nb)];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int u = 0; u < BPT; u++) {
-    const int b = threadIdx.x + u * NT;
-    if (b < total) {
-      const int c = b / nb, j = b - c * nb;
-      const int k = j % Ns;
+    const int j = tj + u * tpc;
+    if (j < nb) {
+      const int k = (Ns == 1) ? 0 : j % Ns;
       if (k != 0) {
 #pragma unroll
         for (int r = 1; r < R; r++) v[u][r] = cmul(v[u][r], tw[r * k * stride]);
       }
       dft_any<T, R>(v[u]);
-      T *col = lds + c * ld;
-      const int idx = (j / Ns) * Ns * R + k;
+      const int idx = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; r++) col[idx + r * Ns] = v[u][r];
+      for (int r = 0; r < R; r++) lds[col.at(idx + r * Ns)] = v[u][r];
     }
   }
-  (void)cj;
   __syncthreads();
 }
 
+// all stages of the plan on every column of the workgroup; EMAX bounds the
+// elements one thread holds (N / tpc <= EMAX)
 template <typename T, int EMAX>
-__device__ void fft_columns(T *lds, int ld, int B, const PlanDev &p, const T *tw)
+__device__ void fft_columns(T *lds, ColRef col, int tj, int tpc, const PlanDev &p,
+                            const T *__restrict__ tw)
 {
   int Ns = 1;
   for (int s = 0; s < p.nst; s++) {
     const int R = p.radix[s];
     switch (R) {
-      case 8: stage<T, 8, (EMAX + 7) / 8>(lds, ld, B, p.N, Ns, tw); break;
-      case 4: stage<T, 4, (EMAX + 3) / 4>(lds, ld, B, p.N, Ns, tw); break;
-      case 2: stage<T, 2, (EMAX + 1) / 2>(lds, ld, B, p.N, Ns, tw); break;
-      case 3: stage<T, 3, (EMAX + 2) / 3>(lds, ld, B, p.N, Ns, tw); break;
-      case 5: stage<T, 5, (EMAX + 4) / 5>(lds, ld, B, p.N, Ns, tw); break;
-      case 7: stage<T, 7, (EMAX + 6) / 7>(lds, ld, B, p.N, Ns, tw); break;
+      case 8: stage<T, 8, (EMAX + 7) / 8>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 4: stage<T, 4, (EMAX + 3) / 4>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 2: stage<T, 2, (EMAX + 1) / 2>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 3: stage<T, 3, (EMAX + 2) / 3>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 5: stage<T, 5, (EMAX + 4) / 5>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 7: stage<T, 7, (EMAX + 6) / 7>(lds, col, tj, tpc, p.N, Ns, tw); break;
       default: break;
     }
     Ns *= R;
   }
 }
 
-template <typename Real> constexpr int emax() { return sizeof(Real) == 4 ? 32 : 16; }
+// at most 4096 complex values (32 KB FP32, 64 KB FP64) per workgroup, so
+// several workgroups share a CU and overlap their load, LDS and store phases
+template <typename Real> constexpr int emax() { return 16; }
+
+// workgroups are dealt round-robin to the 8 XCDs; remap so each XCD walks a
+// contiguous range of columns (adjacent column groups share L2 lines)
+__device__ __forceinline__ int xcd_wg()
+{
+  const int wg = blockIdx.x, nwg = gridDim.x;
+  const int x = wg & 7, local = wg >> 3, q = nwg >> 3, r = nwg & 7;
+  return x * q + min(x, r) + local;
+}
+
+// thread -> (column, index within column) for the two layouts
+struct Lane {
+  int c, tj, tpc;
+  ColRef col;
+};
+__device__ __forceinline__ Lane lane_strided(int lgB, int N)
+{
+  Lane L;
+  const int B = 1 << lgB;
+  L.c = threadIdx.x & (B - 1);
+  L.tj = threadIdx.x >> lgB;
+  L.tpc = NT >> lgB;
+  L.col.base = L.c;
+  L.col.step = B;
+  (void)N;
+  return L;
+}
+__device__ __forceinline__ Lane lane_rows(int lgB, int N)
+{
+  Lane L;
+  L.tpc = NT >> lgB;
+  L.c = threadIdx.x >> (8 - lgB);  // NT = 2^8
+  L.tj = threadIdx.x & (L.tpc - 1);
+  L.col.base = L.c * (N + 1);
+  L.col.step = 1;
+  return L;
+}
 
 // ------------------------------------------------------------ pass kernels
-// generic complex pass along strided or contiguous columns
-template <typename Real>
-__global__ __launch_bounds__(NT) void k_fft_complex(PassGeom g, PlanDev p, const typename Cx<Real>::T *tw,
-                                                    const typename Cx<Real>::T *in,
-                                                    typename Cx<Real>::T *out, int B, int backward)
+// generic complex pass.  Column col = o * nInner + i;
+// element n at o * outer + i * inner + n * pos (all element counts < 2^31).
+struct CGeom {
+  int N, Lin, Lout;
+  int nCols, nInner;
+  int inOuter, inInner, inPos, outOuter, outInner, outPos;
+};
+
+template <typename Real, bool ROWS>
+__global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const typename Cx<Real>::T *tw,
+                                                    const typename Cx<Real>::T *__restrict__ in,
+                                                    typename Cx<Real>::T *__restrict__ out, int lgB,
+                                                    int backward)
 {
   using T = typename Cx<Real>::T;
   extern __shared__ char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
-  const int N = g.N, ld = N + 1;
-  const long col0 = (long)blockIdx.x * B;
-  const bool rowMode = (g.inPos == 1);
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    int c, n;
-    if (rowMode) { c = e / N; n = e - c * N; }
-    else { n = e / B; c = e - n * B; }
-    const long col = col0 + c;
+  const int N = g.N;
+  const Lane L = ROWS ? lane_rows(lgB, N) : lane_strided(lgB, N);
+  const int colg = (xcd_wg() << lgB) + L.c;
+  const bool valid = colg < g.nCols;
+  const int o = valid ? colg / g.nInner : 0, i = valid ? colg - o * g.nInner : 0;
+  const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
+  for (int n = L.tj; n < N; n += L.tpc) {
     T v = mk<T>(0, 0);
-    if (n < g.Lin && col < g.nCols) {
-      const long o = col / g.nInner, i = col - o * g.nInner;
-      v = in[o * g.inOuter + i * g.inInner + (long)n * g.inPos];
+    if (valid && n < g.Lin) {
+      v = src[(long)n * g.inPos];
       if (backward) v = cconj(v);
     }
-    lds[c * ld + n] = v;
+    lds[L.col.at(n)] = v;
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, ld, B, p, tw);
-  const bool rowOut = (g.outPos == 1);
-  for (int e = threadIdx.x; e < B * g.Lout; e += NT) {
-    int c, n;
-    if (rowOut) { c = e / g.Lout; n = e - c * g.Lout; }
-    else { n = e / B; c = e - n * B; }
-    const long col = col0 + c;
-    if (col < g.nCols) {
-      const long o = col / g.nInner, i = col - o * g.nInner;
-      T v = lds[c * ld + n];
-      if (backward) v = cconj(v);
-      out[o * g.outOuter + i * g.outInner + (long)n * g.outPos] = v;
-    }
+  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!valid) return;
+  T *dst = out + ((long)o * g.outOuter + (long)i * g.outInner);
+  for (int n = L.tj; n < g.Lout; n += L.tpc) {
+    T v = lds[L.col.at(n)];
+    if (backward) v = cconj(v);
+    dst[(long)n * g.outPos] = v;
   }
 }
 
 // pass A: two real rows per complex column, R2C by the two-for-one split
 template <typename Real>
-__global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, long nPairs, long inRow, long outRow,
+__global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, int nPairs, int inRow, int outRow,
                                                 PlanDev p, const typename Cx<Real>::T *tw,
-                                                const Real *in, typename Cx<Real>::T *out, int B)
+                                                const Real *__restrict__ in,
+                                                typename Cx<Real>::T *__restrict__ out, int lgB)
 {
   using T = typename Cx<Real>::T;
   extern __shared__ char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
-  const int ld = N + 1;
-  const long pr0 = (long)blockIdx.x * B;
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    const int c = e / N, n = e - c * N;
-    const long pr = pr0 + c;
+  const Lane L = lane_rows(lgB, N);
+  const int pr = (xcd_wg() << lgB) + L.c;
+  const bool valid = pr < nPairs;
+  const Real *ra = in + (long)(2 * pr) * inRow, *rb = ra + inRow;
+  for (int n = L.tj; n < N; n += L.tpc) {
     T v = mk<T>(0, 0);
-    if (n < Lin && pr < nPairs) v = mk<T>(in[(2 * pr) * inRow + n], in[(2 * pr + 1) * inRow + n]);
-    lds[c * ld + n] = v;
+    if (valid && n < Lin) v = mk<T>(ra[n], rb[n]);
+    lds[L.col.at(n)] = v;
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, ld, B, p, tw);
+  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!valid) return;
   const int K = N / 2 + 1;
   const Real h = (Real)0.5;
-  for (int e = threadIdx.x; e < B * K; e += NT) {
-    const int c = e / K, k = e - c * K;
-    const long pr = pr0 + c;
-    if (pr < nPairs) {
-      const T z = lds[c * ld + k];
-      const T w = cconj(lds[c * ld + (k == 0 ? 0 : N - k)]);
-      const T s = cadd(z, w), d = csub(z, w);
-      out[(2 * pr) * outRow + k] = mk<T>(h * s.x, h * s.y);
-      out[(2 * pr + 1) * outRow + k] = mk<T>(h * d.y, -h * d.x);  // -i d / 2
-    }
+  T *oa = out + (long)(2 * pr) * outRow, *ob = oa + outRow;
+  for (int k = L.tj; k < K; k += L.tpc) {
+    const T z = lds[L.col.at(k)];
+    const T w = cconj(lds[L.col.at(k == 0 ? 0 : N - k)]);
+    const T s = cadd(z, w), d = csub(z, w);
+    oa[k] = mk<T>(h * s.x, h * s.y);
+    ob[k] = mk<T>(h * d.y, -h * d.x);  // -i d / 2
   }
 }
 
 // pass E: Hermitian half rows -> two real rows (backward transform)
 template <typename Real>
-__global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, long nPairs, long inRow, long outRow,
+__global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int inRow, int outRow,
                                                 PlanDev p, const typename Cx<Real>::T *tw,
-                                                const typename Cx<Real>::T *in, Real *out, int B)
+                                                const typename Cx<Real>::T *__restrict__ in,
+                                                Real *__restrict__ out, int lgB)
 {
   using T = typename Cx<Real>::T;
   extern __shared__ char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
-  const int ld = N + 1;
-  const long pr0 = (long)blockIdx.x * B;
+  const Lane L = lane_rows(lgB, N);
+  const int pr = (xcd_wg() << lgB) + L.c;
+  const bool valid = pr < nPairs;
   const int h = N / 2;
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    const int c = e / N, n = e - c * N;
-    const long pr = pr0 + c;
+  const T *ia = in + (long)(2 * pr) * inRow, *ib = ia + inRow;
+  for (int n = L.tj; n < N; n += L.tpc) {
     T v = mk<T>(0, 0);
-    if (pr < nPairs) {
+    if (valid) {
       const int m = (n <= h) ? n : N - n;
-      const T a = in[(2 * pr) * inRow + m], b = in[(2 * pr + 1) * inRow + m];
+      const T a = ia[m], b = ib[m];
       // Z = Xa + i Xb (n <= N/2), conj(Xa) + i conj(Xb) above; store conj(Z)
       if (n <= h) v = mk<T>(a.x - b.y, -(a.y + b.x));
-      else v = mk<T>(a.x + b.y, -(-a.y + b.x));
+      else v = mk<T>(a.x + b.y, a.y - b.x);
     }
-    lds[c * ld + n] = v;
+    lds[L.col.at(n)] = v;
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, ld, B, p, tw);
-  for (int e = threadIdx.x; e < B * Lout; e += NT) {
-    const int c = e / Lout, n = e - c * Lout;
-    const long pr = pr0 + c;
-    if (pr < nPairs) {
-      const T z = lds[c * ld + n];  // conj of the backward transform
-      out[(2 * pr) * outRow + n] = z.x;
-      out[(2 * pr + 1) * outRow + n] = -z.y;
-    }
+  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!valid) return;
+  Real *oa = out + (long)(2 * pr) * outRow, *ob = oa + outRow;
+  for (int n = L.tj; n < Lout; n += L.tpc) {
+    const T z = lds[L.col.at(n)];  // conj of the backward transform
+    oa[n] = z.x;
+    ob[n] = -z.y;
   }
 }
 
 // pass C: forward along axis 1, X <- X G, backward, keep the first Lin
 // outputs.  Columns c = k2 * C3 + k3; G index (|k1|, |k2|, k3 or |k3|).
 template <typename Real, bool CG>
-__global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, long nCols, long pos, int N2, int G2,
-                                                 long C3, int N3fold, PlanDev p,
+__global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int pos, int N2, int G2,
+                                                 int C3, int N3fold, PlanDev p,
                                                  const typename Cx<Real>::T *tw, const void *Gv,
-                                                 typename Cx<Real>::T *data, int B)
+                                                 typename Cx<Real>::T *__restrict__ data, int lgB)
 {
   using T = typename Cx<Real>::T;
   extern __shared__ char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
-  const int ld = N + 1;
-  const long col0 = (long)blockIdx.x * B;
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    const int n = e / B, c = e - n * B;
-    const long col = col0 + c;
+  const Lane L = lane_strided(lgB, N);
+  const int colg = (xcd_wg() << lgB) + L.c;
+  const bool valid = colg < nCols;
+  T *col = data + (valid ? colg : 0);
+  for (int n = L.tj; n < N; n += L.tpc) {
     T v = mk<T>(0, 0);
-    if (n < Lin && col < nCols) v = data[col + (long)n * pos];
-    lds[c * ld + n] = v;
+    if (valid && n < Lin) v = col[(long)n * pos];
+    lds[L.col.at(n)] = v;
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, ld, B, p, tw);
+  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
   // multiply by G and conjugate (for the backward transform)
-  const long G3 = N3fold > 0 ? (N3fold / 2 + 1) : C3;
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    const int n = e / B, c = e - n * B;
-    const long col = col0 + c;
-    if (col < nCols) {
-      const long k2 = col / C3, k3 = col - k2 * C3;
-      const long a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
-      const long a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
-      const long a1 = (n <= N / 2) ? n : N - n;
-      const long gi = (a1 * G2 + a2) * G3 + a3;
-      T x = lds[c * ld + n];
-      if constexpr (CG) {
-        const T gg = reinterpret_cast<const T *>(Gv)[gi];
-        x = cconj(cmul(x, gg));
-      } else {
-        const Real gg = reinterpret_cast<const Real *>(Gv)[gi];
-        x = mk<T>(gg * x.x, -gg * x.y);
-      }
-      lds[c * ld + n] = x;
+  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : C3;
+  const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
+  const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
+  const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
+  const int gbase = a2 * G3 + a3, gstep = G2 * G3;
+  for (int n = L.tj; n < N; n += L.tpc) {
+    const int a1 = (n <= N / 2) ? n : N - n;
+    T x = lds[L.col.at(n)];
+    if constexpr (CG) {
+      const T gg = reinterpret_cast<const T *>(Gv)[(long)a1 * gstep + gbase];
+      x = cconj(cmul(x, gg));
+    } else {
+      const Real gg = reinterpret_cast<const Real *>(Gv)[(long)a1 * gstep + gbase];
+      x = mk<T>(gg * x.x, -gg * x.y);
     }
+    lds[L.col.at(n)] = x;
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, ld, B, p, tw);
-  for (int e = threadIdx.x; e < B * Lin; e += NT) {
-    const int n = e / B, c = e - n * B;
-    const long col = col0 + c;
-    if (col < nCols) data[col + (long)n * pos] = cconj(lds[c * ld + n]);
-  }
+  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!valid) return;
+  for (int n = L.tj; n < Lin; n += L.tpc) col[(long)n * pos] = cconj(lds[L.col.at(n)]);
 }
 
 // set-up: even extension of a compact line (length N/2+1) to length N,
 // forward DFT, folded back to the compact line, in place (FP64)
-__global__ __launch_bounds__(NT) void k_fft_even(int N, long nCols, long nInner, long outer, long inner,
-                                                 long pos, PlanDev p, const double2 *tw,
-                                                 double2 *data, int B)
+__global__ __launch_bounds__(NT) void k_fft_even(int N, int nCols, int nInner, int outer, int inner,
+                                                 int pos, PlanDev p, const double2 *tw,
+                                                 double2 *__restrict__ data, int lgB)
 {
   extern __shared__ char smem_raw[];
   double2 *lds = reinterpret_cast<double2 *>(smem_raw);
-  const int ld = N + 1, h = N / 2;
-  const long col0 = (long)blockIdx.x * B;
-  for (int e = threadIdx.x; e < B * N; e += NT) {
-    const int n = e / B, c = e - n * B;
-    const long col = col0 + c;
+  const int h = N / 2;
+  const Lane L = lane_strided(lgB, N);
+  const int colg = (xcd_wg() << lgB) + L.c;
+  const bool valid = colg < nCols;
+  const int o = valid ? colg / nInner : 0, i = valid ? colg - o * nInner : 0;
+  double2 *line = data + ((long)o * outer + (long)i * inner);
+  for (int n = L.tj; n < N; n += L.tpc) {
     double2 v = mk<double2>(0, 0);
-    if (col < nCols) {
-      const long o = col / nInner, i = col - o * nInner;
-      const int m = (n <= h) ? n : N - n;
-      v = data[o * outer + i * inner + (long)m * pos];
-    }
-    lds[c * ld + n] = v;
+    if (valid) v = line[(long)((n <= h) ? n : N - n) * pos];
+    lds[L.col.at(n)] = v;
   }
   __syncthreads();
-  fft_columns<double2, 16>(lds, ld, B, p, tw);
-  for (int e = threadIdx.x; e < B * (h + 1); e += NT) {
-    const int n = e / B, c = e - n * B;
-    const long col = col0 + c;
-    if (col < nCols) {
-      const long o = col / nInner, i = col - o * nInner;
-      double2 v = lds[c * ld + n];
-      if (n != 0 && n != h) {
-        const double2 w = lds[c * ld + (N - n)];
-        v = mk<double2>(0.5 * (v.x + w.x), 0.5 * (v.y + w.y));
-      }
-      data[o * outer + i * inner + (long)n * pos] = v;
+  fft_columns<double2, 16>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!valid) return;
+  for (int n = L.tj; n <= h; n += L.tpc) {
+    double2 v = lds[L.col.at(n)];
+    if (n != 0 && n != h) {
+      const double2 w = lds[L.col.at(N - n)];
+      v = mk<double2>(0.5 * (v.x + w.x), 0.5 * (v.y + w.y));
     }
+    line[(long)n * pos] = v;
   }
 }
 
@@ -457,29 +484,60 @@ static PlanDev to_dev(const FFTPlan1D &p)
   return d;
 }
 
-// columns per workgroup: power of two, B (N+1) <= EMAX * NT
-static int batch_for(int N, bool fp64)
+// columns per workgroup: a power of two with B * N <= EMAX * NT (strided
+// layout) or B * (N + 1) <= EMAX * NT (row layout), at most 64
+static int lg_batch(int N, bool fp64, bool rows)
 {
-  const int cap = (fp64 ? 16 : 32) * NT;
-  int B = 1;
-  while (B * 2 * (N + 1) <= cap && B < 64) B *= 2;
-  return B;
+  const int cap = 16 * NT;
+  (void)fp64;
+  const int per = rows ? N + 1 : N;
+  int lg = 0;
+  while (lg < 6 && (2 << lg) * per <= cap) lg++;
+  return lg;
 }
 
-static size_t lds_bytes(int N, int B, bool fp64) { return (size_t)B * (N + 1) * (fp64 ? 16 : 8); }
+static size_t lds_bytes(int N, int lgB, bool fp64, bool rows)
+{
+  return (size_t)(1 << lgB) * (rows ? N + 1 : N) * (fp64 ? 16 : 8);
+}
+
+static void check_int(long v, const char *what)
+{
+  if (v < 0 || v > 0x7fffffffL) throw Error{NPME_ERR_UNSUPPORTED, std::string(what) + " exceeds 2^31 elements"};
+}
 
 void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const void *tw,
                       const void *in, void *out, bool backward, hipStream_t s)
 {
-  const int B = batch_for(g.N, fp64);
-  const dim3 grid((unsigned)((g.nCols + B - 1) / B));
-  const size_t sh = lds_bytes(g.N, B, fp64);
-  if (fp64)
-    hipLaunchKernelGGL(k_fft_complex<double>, grid, dim3(NT), sh, s, g, to_dev(plan),
-                       (const double2 *)tw, (const double2 *)in, (double2 *)out, B, (int)backward);
-  else
-    hipLaunchKernelGGL(k_fft_complex<float>, grid, dim3(NT), sh, s, g, to_dev(plan),
-                       (const float2 *)tw, (const float2 *)in, (float2 *)out, B, (int)backward);
+  const bool rows = (g.inPos == 1 && g.outPos == 1);
+  const int lgB = lg_batch(g.N, fp64, rows);
+  CGeom c;
+  c.N = g.N;
+  c.Lin = g.Lin;
+  c.Lout = g.Lout;
+  c.nCols = (int)g.nCols;
+  c.nInner = (int)g.nInner;
+  c.inOuter = (int)g.inOuter;
+  c.inInner = (int)g.inInner;
+  c.inPos = (int)g.inPos;
+  c.outOuter = (int)g.outOuter;
+  c.outInner = (int)g.outInner;
+  c.outPos = (int)g.outPos;
+  check_int(g.nCols * std::max<long>(1, g.N), "FFT pass");
+  const dim3 grid((unsigned)((g.nCols + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(g.N, lgB, fp64, rows);
+#define NPME_FFTC(R, ROWS)                                                                     \
+  hipLaunchKernelGGL((k_fft_complex<R, ROWS>), grid, dim3(NT), sh, s, c, to_dev(plan),         \
+                     (const typename Cx<R>::T *)tw, (const typename Cx<R>::T *)in,            \
+                     (typename Cx<R>::T *)out, lgB, (int)backward)
+  if (fp64) {
+    if (rows) NPME_FFTC(double, true);
+    else NPME_FFTC(double, false);
+  } else {
+    if (rows) NPME_FFTC(float, true);
+    else NPME_FFTC(float, false);
+  }
+#undef NPME_FFTC
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -487,18 +545,19 @@ void fft_pass_r2c_rows(bool fp64, int N, int Lin, long nRows, long inRowStride,
                        long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
                        void *out, hipStream_t s)
 {
-  const long nPairs = nRows / 2;
-  const int B = batch_for(N, fp64);
-  const dim3 grid((unsigned)((nPairs + B - 1) / B));
-  const size_t sh = lds_bytes(N, B, fp64);
+  check_int(nRows * std::max(inRowStride, outRowStride), "FFT pass");
+  const int nPairs = (int)(nRows / 2);
+  const int lgB = lg_batch(N, fp64, true);
+  const dim3 grid((unsigned)((nPairs + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(N, lgB, fp64, true);
   if (fp64)
-    hipLaunchKernelGGL(k_fft_r2c<double>, grid, dim3(NT), sh, s, N, Lin, nPairs, inRowStride,
-                       outRowStride, to_dev(plan), (const double2 *)tw, (const double *)in,
-                       (double2 *)out, B);
+    hipLaunchKernelGGL(k_fft_r2c<double>, grid, dim3(NT), sh, s, N, Lin, nPairs, (int)inRowStride,
+                       (int)outRowStride, to_dev(plan), (const double2 *)tw, (const double *)in,
+                       (double2 *)out, lgB);
   else
-    hipLaunchKernelGGL(k_fft_r2c<float>, grid, dim3(NT), sh, s, N, Lin, nPairs, inRowStride,
-                       outRowStride, to_dev(plan), (const float2 *)tw, (const float *)in,
-                       (float2 *)out, B);
+    hipLaunchKernelGGL(k_fft_r2c<float>, grid, dim3(NT), sh, s, N, Lin, nPairs, (int)inRowStride,
+                       (int)outRowStride, to_dev(plan), (const float2 *)tw, (const float *)in,
+                       (float2 *)out, lgB);
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -506,18 +565,19 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
                        long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
                        void *out, hipStream_t s)
 {
-  const long nPairs = nRows / 2;
-  const int B = batch_for(N, fp64);
-  const dim3 grid((unsigned)((nPairs + B - 1) / B));
-  const size_t sh = lds_bytes(N, B, fp64);
+  check_int(nRows * std::max(inRowStride, outRowStride), "FFT pass");
+  const int nPairs = (int)(nRows / 2);
+  const int lgB = lg_batch(N, fp64, true);
+  const dim3 grid((unsigned)((nPairs + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(N, lgB, fp64, true);
   if (fp64)
-    hipLaunchKernelGGL(k_fft_c2r<double>, grid, dim3(NT), sh, s, N, Lout, nPairs, inRowStride,
-                       outRowStride, to_dev(plan), (const double2 *)tw, (const double2 *)in,
-                       (double *)out, B);
+    hipLaunchKernelGGL(k_fft_c2r<double>, grid, dim3(NT), sh, s, N, Lout, nPairs, (int)inRowStride,
+                       (int)outRowStride, to_dev(plan), (const double2 *)tw, (const double2 *)in,
+                       (double *)out, lgB);
   else
-    hipLaunchKernelGGL(k_fft_c2r<float>, grid, dim3(NT), sh, s, N, Lout, nPairs, inRowStride,
-                       outRowStride, to_dev(plan), (const float2 *)tw, (const float2 *)in,
-                       (float *)out, B);
+    hipLaunchKernelGGL(k_fft_c2r<float>, grid, dim3(NT), sh, s, N, Lout, nPairs, (int)inRowStride,
+                       (int)outRowStride, to_dev(plan), (const float2 *)tw, (const float2 *)in,
+                       (float *)out, lgB);
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -527,40 +587,35 @@ void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, lon
 {
   // C3: number of axis-3 entries per axis-2 row in the data; for complex
   // kernels the axis-3 index is folded (|k3|) against N3 = C3.
-  const int B = batch_for(N, fp64);
-  const dim3 grid((unsigned)((nCols + B - 1) / B));
-  const size_t sh = lds_bytes(N, B, fp64);
+  check_int(nCols * (long)Lin, "FFT pass");
+  const int lgB = lg_batch(N, fp64, false);
+  const dim3 grid((unsigned)((nCols + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(N, lgB, fp64, false);
   const int N3fold = complexG ? (int)C3 : 0;
+#define NPME_CONV(R, CGV)                                                                      \
+  hipLaunchKernelGGL((k_fft_conv<R, CGV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,          \
+                     (int)pos_stride, N2, G2, (int)C3, N3fold, to_dev(plan),                    \
+                     (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data, lgB)
   if (fp64) {
-    if (complexG)
-      hipLaunchKernelGGL((k_fft_conv<double, true>), grid, dim3(NT), sh, s, N, Lin, nCols,
-                         pos_stride, N2, G2, C3, N3fold, to_dev(plan), (const double2 *)tw, G,
-                         (double2 *)data, B);
-    else
-      hipLaunchKernelGGL((k_fft_conv<double, false>), grid, dim3(NT), sh, s, N, Lin, nCols,
-                         pos_stride, N2, G2, C3, N3fold, to_dev(plan), (const double2 *)tw, G,
-                         (double2 *)data, B);
+    if (complexG) NPME_CONV(double, true);
+    else NPME_CONV(double, false);
   } else {
-    if (complexG)
-      hipLaunchKernelGGL((k_fft_conv<float, true>), grid, dim3(NT), sh, s, N, Lin, nCols,
-                         pos_stride, N2, G2, C3, N3fold, to_dev(plan), (const float2 *)tw, G,
-                         (float2 *)data, B);
-    else
-      hipLaunchKernelGGL((k_fft_conv<float, false>), grid, dim3(NT), sh, s, N, Lin, nCols,
-                         pos_stride, N2, G2, C3, N3fold, to_dev(plan), (const float2 *)tw, G,
-                         (float2 *)data, B);
+    if (complexG) NPME_CONV(float, true);
+    else NPME_CONV(float, false);
   }
+#undef NPME_CONV
   NPME_HIP_CHECK(hipGetLastError());
 }
 
 void fft_pass_even(int N, long nCols, long nInner, long outer, long inner, long pos,
                    const FFTPlan1D &plan, const void *tw, void *data, hipStream_t s)
 {
-  const int B = batch_for(N, true);
-  const dim3 grid((unsigned)((nCols + B - 1) / B));
-  const size_t sh = lds_bytes(N, B, true);
-  hipLaunchKernelGGL(k_fft_even, grid, dim3(NT), sh, s, N, nCols, nInner, outer, inner, pos,
-                     to_dev(plan), (const double2 *)tw, (double2 *)data, B);
+  check_int(nCols * (long)(N / 2 + 1), "FFT pass");
+  const int lgB = lg_batch(N, true, false);
+  const dim3 grid((unsigned)((nCols + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(N, lgB, true, false);
+  hipLaunchKernelGGL(k_fft_even, grid, dim3(NT), sh, s, N, (int)nCols, (int)nInner, (int)outer,
+                     (int)inner, (int)pos, to_dev(plan), (const double2 *)tw, (double2 *)data, lgB);
   NPME_HIP_CHECK(hipGetLastError());
 }
 

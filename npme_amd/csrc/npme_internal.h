@@ -113,7 +113,7 @@ struct Handle {
   double X0[3], R0[3], X[3], R[3], L[3];
   // cell grid (global cellB grid)
   int nb[3] = {0, 0, 0};
-  double csB = 0;
+  double csB = 0, csA = 0;
   double boxMin[3] = {0, 0, 0};
   // host copies
   std::vector<int> perm;  // sorted position -> input index
@@ -125,6 +125,16 @@ struct Handle {
   int device = 0;
   int *d_perm = nullptr;         // [n]
   void *d_src = nullptr;         // [n] float4/double4 (local xyz, q) ; complex: q re only here
+  void *d_srcLo = nullptr;       // [n] float4 low parts of the FP32 positions (mixed)
+  void *d_blockBox = nullptr;    // int4 source segments of the 8-target blocks
+  int *d_segStart = nullptr;     // [ceil(n/8) + 1] first segment of each block
+  size_t nSegments = 0;
+  // sliced-ELLPACK neighbour list (used when it fits the memory budget)
+  bool useNlist = false;
+  unsigned *d_nlist = nullptr;
+  long *d_sliceStart = nullptr;
+  int *d_sliceLen = nullptr;
+  size_t nlistEntries = 0;
   void *d_qi = nullptr;          // [n] imag part (float/double) for complex kernels
   int4 *d_cell = nullptr;        // [n] cell coords (x,y,z), pad
   int *d_cellStart = nullptr;    // [ncell + 1]
@@ -198,8 +208,16 @@ double handle_count_pairs(Handle &h);
 DirectParams direct_params(const Handle &h);
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s);
-void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *qi,
-                   const int4 *cell, const int *cellStart, double *Vdir, hipStream_t s);
+void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
+                   const void *srcLo, const void *qi, const int4 *cell, const int *segStart,
+                   const void *segs, int nP, double csA, double *Vdir, hipStream_t s);
+void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
+                  const int *perm, const double *coord, int nP, int *count, const long *sliceStart,
+                  unsigned *list, hipStream_t s);
+void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src,
+                      const void *srcLo, const void *qi, const long *sliceStart,
+                      const int *sliceLen, const unsigned *list, double csA, double *Vdir,
+                      hipStream_t s);
 void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                         const int *perm, const double *coord, unsigned long long *total,
                         hipStream_t s);

@@ -22,9 +22,42 @@ template <> struct V4t<double> { using T = double4; };
 struct DirectDev {
   int kind, Nder, nNeigh;
   double Rdir, alpha, k0r, k0i, cs;
-  double a[62], b[62];
+  double a[62], b[62];    // DM coefficients (re, im), zero beyond Nder
+  float af[62], bf[62];   // the same in FP32
   int nb0, nb1, nb2;
 };
+
+// coefficient array of the working precision
+template <typename Real> __device__ __forceinline__ const Real *coef_a(const DirectDev &p);
+template <> __device__ __forceinline__ const float *coef_a<float>(const DirectDev &p) { return p.af; }
+template <> __device__ __forceinline__ const double *coef_a<double>(const DirectDev &p) { return p.a; }
+template <typename Real> __device__ __forceinline__ const Real *coef_b(const DirectDev &p);
+template <> __device__ __forceinline__ const float *coef_b<float>(const DirectDev &p) { return p.bf; }
+template <> __device__ __forceinline__ const double *coef_b<double>(const DirectDev &p) { return p.b; }
+
+// Horner in r2 of degree D with compile-time indices (the coefficients are
+// zero above Nder), so the kernel-argument loads are hoisted out of the
+// pair loop
+template <typename Real, int D>
+__device__ __forceinline__ Real horner_fix(const Real *c, Real r2)
+{
+  Real s = c[2 * D];
+#pragma unroll
+  for (int i = D - 1; i >= 0; i--) s = s * r2 + c[2 * i];
+  return s;
+}
+template <typename Real, int D>
+__device__ __forceinline__ void horner_fix_cx(const Real *c, Real r2, Real &re, Real &im)
+{
+  Real sr = c[2 * D], si = c[2 * D + 1];
+#pragma unroll
+  for (int i = D - 1; i >= 0; i--) {
+    sr = sr * r2 + c[2 * i];
+    si = si * r2 + c[2 * i + 1];
+  }
+  re = sr;
+  im = si;
+}
 
 // ------------------------------------------------------------------ gather
 template <typename Real, bool CX>
@@ -91,120 +124,172 @@ __device__ __forceinline__ void horner_cx(const double *c, int deg, Real r2, Rea
   im = si;
 }
 
-// one target per lane; cells of the reference's cell grid (size Rdir/nNeigh)
-// within +-nNeigh are visited unless their box is farther than Rdir from the
-// target.  Coordinates are local to each cell, so the FP32 differences stay
-// accurate for any box size.
-template <typename Real, int KIND>
+// Short-range DM direct sum.  One wavefront takes a block of 8 consecutive
+// cell-sorted targets; lane (i, j) = (lane / 8, lane % 8) pairs target i with
+// every 8th source of the block's source stream.  The stream is a list of
+// contiguous segments of the sorted charges, built at set-up: for every row
+// of cells (x, y fixed) within Rdir of the box spanned by the block's
+// targets, the run of cells along z whose slab is within Rdir, cut where the
+// run crosses a cluster of nCellClust1D^3 cells (inside one cluster such a
+// run is contiguous in the sorted order).  All loops are uniform over the
+// wavefront; only the r < Rdir mask differs per lane.  Positions are
+// relative to the low corner of their cluster and, in the mixed form,
+// carried as FP32 hi + lo parts, so the pair difference keeps FP32 relative
+// accuracy in r for close pairs.  Per segment the lanes accumulate in the
+// working precision; segment partials are added in FP64 and the 8 lanes of
+// a target reduced by shuffles at the end.  Workgroups are remapped so that
+// each XCD works through a contiguous range of blocks (its L2 then holds the
+// shared neighbourhoods).
+__device__ __forceinline__ int xcd_remap(int wg, int nwg)
+{
+  const int x = wg & 7, local = wg >> 3, q = nwg >> 3, r = nwg & 7;
+  return x * q + min(x, r) + local;
+}
+
+template <typename Real, int KIND, bool HILO, int MAXD>
 __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
                                                 const typename V4t<Real>::T *__restrict__ src,
+                                                const float4 *__restrict__ srcLo,
                                                 const Real *__restrict__ qi,
                                                 const int4 *__restrict__ cell,
-                                                const int2 *__restrict__ cellSE,
-                                                double *__restrict__ Vdir)
+                                                const int *__restrict__ segStart,
+                                                const int4 *__restrict__ seg, int nBlocks,
+                                                int nP, double csA, double *__restrict__ Vdir)
 {
   using V4 = typename V4t<Real>::T;
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
   constexpr bool CX = (KIND == KIND_HELMHOLTZ);
-  const V4 me = src[s];
-  const int4 c = cell[s];
-  const Real cs = (Real)p.cs;
+  const int lane = threadIdx.x & 63;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int blk = wg * 4 + (threadIdx.x >> 6);
+  if (blk >= nBlocks) return;  // uniform per wavefront
+  const int ti = lane >> 3, tj = lane & 7;
+  const size_t t = (size_t)blk * 8 + ti;
+  const bool tvalid = t < n;
+  const size_t tt = tvalid ? t : (size_t)blk * 8;
+  const V4 me = src[tt];
+  float4 meL = make_float4(0, 0, 0, 0);
+  if (HILO) meL = srcLo[tt];
+  const int4 ct = cell[tt];
+  const int atx = ct.x / nP, aty = ct.y / nP, atz = ct.z / nP;
   const Real R2 = (Real)(p.Rdir * p.Rdir);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  const int nN = p.nNeigh;
-  for (int dx = -nN; dx <= nN; dx++) {
-    const int gx = c.x + dx;
-    if (gx < 0 || gx >= p.nb0) continue;
-    const Real ox = (Real)(dx * p.cs);
-    const Real ex = fmax(fmax(ox - me.x, me.x - (ox + cs)), (Real)0);
-    for (int dy = -nN; dy <= nN; dy++) {
-      const int gy = c.y + dy;
-      if (gy < 0 || gy >= p.nb1) continue;
-      const Real oy = (Real)(dy * p.cs);
-      const Real ey = fmax(fmax(oy - me.y, me.y - (oy + cs)), (Real)0);
-      if (ex * ex + ey * ey > R2) continue;
-      for (int dz = -nN; dz <= nN; dz++) {
-        const int gz = c.z + dz;
-        if (gz < 0 || gz >= p.nb2) continue;
-        const Real oz = (Real)(dz * p.cs);
-        const Real ez = fmax(fmax(oz - me.z, me.z - (oz + cs)), (Real)0);
-        if (ex * ex + ey * ey + ez * ez > R2) continue;
-        const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
-        Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int j = se.x; j < se.x + se.y; j++) {
-          if ((size_t)j == s) continue;
-          const V4 o = src[j];
-          const Real rx = me.x - o.x - ox;
-          const Real ry = me.y - o.y - oy;
-          const Real rz = me.z - o.z - oz;
+  const int sb = segStart[blk], se = segStart[blk + 1];
+  for (int g0 = sb; g0 < se; g0 += 64) {
+    // one descriptor per lane, then broadcast
+    const int4 mine = (g0 + lane < se) ? seg[g0 + lane] : make_int4(0, 0, 0, 0);
+    const int cnt = min(64, se - g0);
+    for (int k = 0; k < cnt; k++) {
+      const int segB = __shfl(mine.x, k, 64), segE = __shfl(mine.y, k, 64);
+      const int ax = __shfl(mine.z, k, 64), ayz = __shfl(mine.w, k, 64);
+      const Real offx = (Real)((ax - atx) * csA);
+      const Real offy = (Real)(((ayz >> 16) - aty) * csA);
+      const Real offz = (Real)(((ayz & 0xffff) - atz) * csA);
+      Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int js = segB + tj;
+      V4 o;
+      float4 oL;
+      if (js < segE) {
+        o = src[js];
+        if (HILO) oL = srcLo[js];
+      }
+      for (int s0 = segB; s0 < segE; s0 += 8) {
+        const int jn = js + 8;
+        V4 on;
+        float4 oLn;
+        if (jn < segE) {  // prefetch the next source of this lane
+          on = src[jn];
+          if (HILO) oLn = srcLo[jn];
+        }
+        if (js < segE && tvalid && (size_t)js != t) {
+          Real rx = me.x - (o.x + offx);
+          Real ry = me.y - (o.y + offy);
+          Real rz = me.z - (o.z + offz);
+          if (HILO) {
+            rx += (Real)(meL.x - oL.x);
+            ry += (Real)(meL.y - oL.y);
+            rz += (Real)(meL.z - oL.z);
+          }
           const Real r2 = rx * rx + ry * ry + rz * rz;
-          if (!(r2 < R2)) continue;
-          if constexpr (KIND == KIND_LAPLACE) {
-            const Real ri = rsqrt_r<Real>(r2);
-            const Real f0 = ri - horner_re<Real>(p.a, p.Nder, r2);
-            const Real f1 = -ri * ri * ri - horner_re<Real>(p.b, p.Nder - 1, r2);
-            const Real q = o.w, fq = f1 * q;
-            part[0] += f0 * q;
-            part[1] += rx * fq;
-            part[2] += ry * fq;
-            part[3] += rz * fq;
-          } else if constexpr (KIND == KIND_RALPHA) {
-            const Real fe = pow(r2, (Real)(0.5 * p.alpha));
-            const Real f0 = fe - horner_re<Real>(p.a, p.Nder, r2);
-            const Real f1 = (Real)p.alpha * fe / r2 - horner_re<Real>(p.b, p.Nder - 1, r2);
-            const Real q = o.w, fq = f1 * q;
-            part[0] += f0 * q;
-            part[1] += rx * fq;
-            part[2] += ry * fq;
-            part[3] += rz * fq;
-          } else {
-            // exp(i k0 r)/r with the phase k0r r reduced modulo 2 pi in FP64
-            const Real r = sqrt(r2);
-            const Real ri = (Real)1 / r;
-            const double ph = p.k0r * (double)r;
-            const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
-            Real sn, cn;
-            sincos_r((Real)red, &sn, &cn);
-            const Real damp = exp((Real)(-p.k0i) * r) * ri;
-            const Real fr = damp * cn, fi = damp * sn;
-            // f1 = (i k0 f - f/r)/r
-            const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
-            const Real ikr = -k0i - ri, iki = k0r;  // i k0 - 1/r
-            const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
-            Real pr, pi, br, bi;
-            horner_cx<Real>(p.a, p.Nder, r2, pr, pi);
-            horner_cx<Real>(p.b, p.Nder - 1, r2, br, bi);
-            const Real s0r = fr - pr, s0i = fi - pi, s1r = f1r - br, s1i = f1i - bi;
-            const Real qr = o.w, qim = qi[j];
-            const Real vr = s0r * qr - s0i * qim, vi = s0r * qim + s0i * qr;
-            const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
-            part[0] += vr;
-            part[1] += vi;
-            part[2] += rx * gr;
-            part[3] += rx * gi;
-            part[4] += ry * gr;
-            part[5] += ry * gi;
-            part[6] += rz * gr;
-            part[7] += rz * gi;
+          if (r2 < R2) {
+            if constexpr (KIND == KIND_LAPLACE) {
+              const Real ri = rsqrt_r<Real>(r2);
+              const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+              const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+              const Real q = o.w, fq = f1 * q;
+              part[0] += f0 * q;
+              part[1] += rx * fq;
+              part[2] += ry * fq;
+              part[3] += rz * fq;
+            } else if constexpr (KIND == KIND_RALPHA) {
+              const Real fe = pow(r2, (Real)(0.5 * p.alpha));
+              const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+              const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+              const Real q = o.w, fq = f1 * q;
+              part[0] += f0 * q;
+              part[1] += rx * fq;
+              part[2] += ry * fq;
+              part[3] += rz * fq;
+            } else {
+              // exp(i k0 r)/r with the phase k0r r reduced modulo 2 pi in FP64
+              const Real r = sqrt(r2);
+              const Real ri = (Real)1 / r;
+              const double ph = p.k0r * (double)r;
+              const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
+              Real sn, cn;
+              sincos_r((Real)red, &sn, &cn);
+              const Real damp = exp((Real)(-p.k0i) * r) * ri;
+              const Real fr = damp * cn, fi = damp * sn;
+              const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
+              const Real ikr = -k0i - ri, iki = k0r;  // i k0 - 1/r
+              const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
+              Real pr, pim, br, bi;
+              horner_fix_cx<Real, MAXD>(coef_a<Real>(p), r2, pr, pim);
+              horner_fix_cx<Real, MAXD>(coef_b<Real>(p), r2, br, bi);
+              const Real s0r = fr - pr, s0i = fi - pim, s1r = f1r - br, s1i = f1i - bi;
+              const Real qr = o.w, qim = qi[js];
+              part[0] += s0r * qr - s0i * qim;
+              part[1] += s0r * qim + s0i * qr;
+              const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
+              part[2] += rx * gr;
+              part[3] += rx * gi;
+              part[4] += ry * gr;
+              part[5] += ry * gi;
+              part[6] += rz * gr;
+              part[7] += rz * gi;
+            }
           }
         }
-#pragma unroll
-        for (int k = 0; k < (CX ? 8 : 4); k++) acc[k] += (double)part[k];
+        js = jn;
+        o = on;
+        if (HILO) oL = oLn;
       }
+#pragma unroll
+      for (int c = 0; c < (CX ? 8 : 4); c++) acc[c] += (double)part[c];
     }
   }
-  if (CX) {
+  // reduce over the 8 source lanes of each target
 #pragma unroll
-    for (int k = 0; k < 8; k++) Vdir[8 * s + k] = acc[k];
-  } else {
+  for (int c = 0; c < (CX ? 8 : 4); c++) {
+    double v = acc[c];
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    acc[c] = v;
+  }
+  if (tvalid && tj == 0) {
+    if (CX) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) Vdir[4 * s + k] = acc[k];
+      for (int c = 0; c < 8; c++) Vdir[8 * t + c] = acc[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) Vdir[4 * t + c] = acc[c];
+    }
   }
 }
 
-void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *qi,
-                   const int4 *cell, const int *cellStart, double *Vdir, hipStream_t s)
+void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
+                   const void *srcLo, const void *qi, const int4 *cell, const int *segStart,
+                   const void *segs, int nP, double csA, double *Vdir, hipStream_t s)
 {
   DirectDev d;
   d.kind = p.kind;
@@ -216,27 +301,282 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
   d.k0i = p.k0i;
   d.cs = p.cs;
   for (int i = 0; i < 62; i++) {
-    d.a[i] = p.a[i];
-    d.b[i] = p.b[i];
+    const bool used = i < 2 * (p.Nder + 1);
+    d.a[i] = used ? p.a[i] : 0.0;
+    d.b[i] = used ? p.b[i] : 0.0;
+    d.af[i] = (float)d.a[i];
+    d.bf[i] = (float)d.b[i];
+  }
+  d.nb0 = p.nb[0];
+  d.nb1 = p.nb[1];
+  d.nb2 = p.nb[2];
+  const int nBlocks = (int)((n + 7) / 8);
+  const dim3 grid((unsigned)((nBlocks + 3) / 4));  // 4 wavefronts per workgroup
+  const int4 *sg = (const int4 *)segs;
+#define NPME_DIRECT_D(R, K, HL, D)                                                              \
+  hipLaunchKernelGGL((k_direct<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                      \
+                     (const typename V4t<R>::T *)src, (const float4 *)srcLo, (const R *)qi,    \
+                     cell, segStart, sg, nBlocks, nP, csA, Vdir)
+#define NPME_DIRECT(R, K, HL)                                                                   \
+  do {                                                                                          \
+    if (p.Nder <= 8) NPME_DIRECT_D(R, K, HL, 8);                                                \
+    else if (p.Nder <= 16) NPME_DIRECT_D(R, K, HL, 16);                                         \
+    else NPME_DIRECT_D(R, K, HL, 30);                                                           \
+  } while (0)
+  if (fp64) {
+    if (p.kind == KIND_LAPLACE) NPME_DIRECT(double, KIND_LAPLACE, false);
+    else if (p.kind == KIND_RALPHA) NPME_DIRECT(double, KIND_RALPHA, false);
+    else NPME_DIRECT(double, KIND_HELMHOLTZ, false);
+  } else {
+    if (p.kind == KIND_LAPLACE) NPME_DIRECT(float, KIND_LAPLACE, true);
+    else if (p.kind == KIND_RALPHA) NPME_DIRECT(float, KIND_RALPHA, true);
+    else NPME_DIRECT(float, KIND_HELMHOLTZ, true);
+  }
+#undef NPME_DIRECT_D
+#undef NPME_DIRECT
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ neighbour list
+// The geometry is fixed between CalcV calls, so the pairs inside the cutoff
+// are found once at set-up and stored as a sliced-ELLPACK list: the targets
+// are cut into slices of 64 consecutive sorted charges (one wavefront), each
+// slice is padded to its longest list, and entry k of lane l sits at
+// sliceStart + k * 64 + l, so step k of a wavefront reads 256 contiguous
+// bytes.  An entry packs the source index (low 27 bits) with the offset of
+// the source's cluster from the target's cluster, (dx+1) * 9 + (dy+1) * 3 +
+// (dz+1) in the high 5 bits; padding entries are 0xffffffff.  The pairs are
+// found in FP64 with a cutoff inflated by 1e-9 relative; the direct-sum
+// kernel applies r < Rdir in its working precision.
+constexpr unsigned NL_PAD = 0xffffffffu;
+constexpr int NL_SHIFT = 27;
+
+// walk the cells of the stencil of sorted target s in FP64 input
+// coordinates; COUNT mode counts, FILL mode writes entries
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4 *__restrict__ cell,
+                                               const int2 *__restrict__ cellSE,
+                                               const int *__restrict__ perm,
+                                               const double *__restrict__ coord, int nP,
+                                               int *__restrict__ count,
+                                               const long *__restrict__ sliceStart,
+                                               unsigned *__restrict__ list)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int4 c = cell[s];
+  const long i = perm[s];
+  const double xi = coord[3 * i], yi = coord[3 * i + 1], zi = coord[3 * i + 2];
+  const double R = p.Rdir * (1.0 + 1.0e-9);
+  const double R2 = R * R;
+  const int nN = p.nNeigh;
+  const int ax = c.x / nP, ay = c.y / nP, az = c.z / nP;
+  unsigned *out = nullptr;
+  if (FILL) out = list + sliceStart[s >> 6] + (s & 63);
+  int k = 0;
+  for (int dx = -nN; dx <= nN; dx++)
+    for (int dy = -nN; dy <= nN; dy++)
+      for (int dz = -nN; dz <= nN; dz++) {
+        const int gx = c.x + dx, gy = c.y + dy, gz = c.z + dz;
+        if (gx < 0 || gy < 0 || gz < 0 || gx >= p.nb0 || gy >= p.nb1 || gz >= p.nb2) continue;
+        const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+        const unsigned code = (unsigned)((gx / nP - ax + 1) * 9 + (gy / nP - ay + 1) * 3 + (gz / nP - az + 1));
+        for (int j = se.x; j < se.x + se.y; j++) {
+          if ((size_t)j == s) continue;
+          const long o = perm[j];
+          const double rx = xi - coord[3 * o], ry = yi - coord[3 * o + 1], rz = zi - coord[3 * o + 2];
+          if (rx * rx + ry * ry + rz * rz < R2) {
+            if (FILL) out[(long)k * 64] = (code << NL_SHIFT) | (unsigned)j;
+            k++;
+          }
+        }
+      }
+  if (!FILL) count[s] = k;
+}
+
+void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
+                  const int *perm, const double *coord, int nP, int *count, const long *sliceStart,
+                  unsigned *list, hipStream_t s)
+{
+  DirectDev d;
+  d.Rdir = p.Rdir;
+  d.nNeigh = p.nNeigh;
+  d.nb0 = p.nb[0];
+  d.nb1 = p.nb[1];
+  d.nb2 = p.nb[2];
+  const dim3 g((unsigned)((n + 255) / 256));
+  if (fill)
+    hipLaunchKernelGGL(k_nlist<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, perm,
+                       coord, nP, count, sliceStart, list);
+  else
+    hipLaunchKernelGGL(k_nlist<false>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, perm,
+                       coord, nP, count, sliceStart, list);
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+// direct sum over the neighbour list: one target per lane, the 64 targets of
+// a slice walk their lists in lockstep (lists of one slice have similar
+// lengths, so little of the wavefront idles)
+template <typename Real, int KIND, bool HILO, int MAXD>
+__global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
+                                                   const typename V4t<Real>::T *__restrict__ src,
+                                                   const float4 *__restrict__ srcLo,
+                                                   const Real *__restrict__ qi,
+                                                   const long *__restrict__ sliceStart,
+                                                   const int *__restrict__ sliceLen,
+                                                   const unsigned *__restrict__ list, double csA,
+                                                   double *__restrict__ Vdir)
+{
+  using V4 = typename V4t<Real>::T;
+  constexpr bool CX = (KIND == KIND_HELMHOLTZ);
+  const size_t s = (size_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const size_t slice = s >> 6;
+  if (slice * 64 >= n) return;  // uniform per wavefront
+  const bool tvalid = s < n;
+  const size_t tt = tvalid ? s : slice * 64;
+  const V4 me = src[tt];
+  float4 meL = make_float4(0, 0, 0, 0);
+  if (HILO) meL = srcLo[tt];
+  const Real R2 = (Real)(p.Rdir * p.Rdir);
+  const Real cA = (Real)csA;
+  const int len = sliceLen[slice];
+  const unsigned *e = list + sliceStart[slice] + (s & 63);
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned ent = (len > 0) ? e[0] : NL_PAD;
+  for (int k = 0; k < len; k++) {
+    const unsigned nxt = (k + 1 < len) ? e[(long)(k + 1) * 64] : NL_PAD;
+    if (ent != NL_PAD && tvalid) {
+      const int j = (int)(ent & ((1u << NL_SHIFT) - 1));
+      const int code = (int)(ent >> NL_SHIFT);
+      const Real ox = (Real)(code / 9 - 1) * cA, oy = (Real)((code / 3) % 3 - 1) * cA,
+                 oz = (Real)(code % 3 - 1) * cA;
+      const V4 o = src[j];
+      Real rx = me.x - (o.x + ox);
+      Real ry = me.y - (o.y + oy);
+      Real rz = me.z - (o.z + oz);
+      if (HILO) {
+        const float4 oL = srcLo[j];
+        rx += (Real)(meL.x - oL.x);
+        ry += (Real)(meL.y - oL.y);
+        rz += (Real)(meL.z - oL.z);
+      }
+      const Real r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 < R2) {
+        if constexpr (KIND == KIND_LAPLACE) {
+          const Real ri = rsqrt_r<Real>(r2);
+          const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+          const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+          const Real q = o.w, fq = f1 * q;
+          part[0] += f0 * q;
+          part[1] += rx * fq;
+          part[2] += ry * fq;
+          part[3] += rz * fq;
+        } else if constexpr (KIND == KIND_RALPHA) {
+          const Real fe = pow(r2, (Real)(0.5 * p.alpha));
+          const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+          const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+          const Real q = o.w, fq = f1 * q;
+          part[0] += f0 * q;
+          part[1] += rx * fq;
+          part[2] += ry * fq;
+          part[3] += rz * fq;
+        } else {
+          const Real r = sqrt(r2);
+          const Real ri = (Real)1 / r;
+          const double ph = p.k0r * (double)r;
+          const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
+          Real sn, cn;
+          sincos_r((Real)red, &sn, &cn);
+          const Real damp = exp((Real)(-p.k0i) * r) * ri;
+          const Real fr = damp * cn, fi = damp * sn;
+          const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
+          const Real ikr = -k0i - ri, iki = k0r;
+          const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
+          Real pr, pim, br, bi;
+          horner_fix_cx<Real, MAXD>(coef_a<Real>(p), r2, pr, pim);
+          horner_fix_cx<Real, MAXD>(coef_b<Real>(p), r2, br, bi);
+          const Real s0r = fr - pr, s0i = fi - pim, s1r = f1r - br, s1i = f1i - bi;
+          const Real qr = o.w, qim = qi[j];
+          part[0] += s0r * qr - s0i * qim;
+          part[1] += s0r * qim + s0i * qr;
+          const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
+          part[2] += rx * gr;
+          part[3] += rx * gi;
+          part[4] += ry * gr;
+          part[5] += ry * gi;
+          part[6] += rz * gr;
+          part[7] += rz * gi;
+        }
+      }
+    }
+    ent = nxt;
+    if ((k & 15) == 15) {  // FP64 carry every 16 pairs
+#pragma unroll
+      for (int c = 0; c < (CX ? 8 : 4); c++) {
+        acc[c] += (double)part[c];
+        part[c] = 0;
+      }
+    }
+  }
+  if (!tvalid) return;
+#pragma unroll
+  for (int c = 0; c < (CX ? 8 : 4); c++) acc[c] += (double)part[c];
+  if (CX) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) Vdir[8 * s + c] = acc[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 4; c++) Vdir[4 * s + c] = acc[c];
+  }
+}
+
+void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src,
+                      const void *srcLo, const void *qi, const long *sliceStart,
+                      const int *sliceLen, const unsigned *list, double csA, double *Vdir,
+                      hipStream_t s)
+{
+  DirectDev d;
+  d.kind = p.kind;
+  d.Nder = p.Nder;
+  d.nNeigh = p.nNeigh;
+  d.Rdir = p.Rdir;
+  d.alpha = p.alpha;
+  d.k0r = p.k0r;
+  d.k0i = p.k0i;
+  d.cs = p.cs;
+  for (int i = 0; i < 62; i++) {
+    const bool used = i < 2 * (p.Nder + 1);
+    d.a[i] = used ? p.a[i] : 0.0;
+    d.b[i] = used ? p.b[i] : 0.0;
+    d.af[i] = (float)d.a[i];
+    d.bf[i] = (float)d.b[i];
   }
   d.nb0 = p.nb[0];
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
   const dim3 grid((unsigned)((n + 255) / 256));
-  const int2 *se = (const int2 *)cellStart;
-#define NPME_DIRECT(R, K)                                                                  \
-  hipLaunchKernelGGL((k_direct<R, K>), grid, dim3(256), 0, s, d, n,                        \
-                     (const typename V4t<R>::T *)src, (const R *)qi, cell, se, Vdir)
+#define NPME_DNL_D(R, K, HL, D)                                                                 \
+  hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \
+                     (const typename V4t<R>::T *)src, (const float4 *)srcLo, (const R *)qi,    \
+                     sliceStart, sliceLen, list, csA, Vdir)
+#define NPME_DNL(R, K, HL)                                                                      \
+  do {                                                                                          \
+    if (p.Nder <= 8) NPME_DNL_D(R, K, HL, 8);                                                   \
+    else if (p.Nder <= 16) NPME_DNL_D(R, K, HL, 16);                                            \
+    else NPME_DNL_D(R, K, HL, 30);                                                              \
+  } while (0)
   if (fp64) {
-    if (p.kind == KIND_LAPLACE) NPME_DIRECT(double, KIND_LAPLACE);
-    else if (p.kind == KIND_RALPHA) NPME_DIRECT(double, KIND_RALPHA);
-    else NPME_DIRECT(double, KIND_HELMHOLTZ);
+    if (p.kind == KIND_LAPLACE) NPME_DNL(double, KIND_LAPLACE, false);
+    else if (p.kind == KIND_RALPHA) NPME_DNL(double, KIND_RALPHA, false);
+    else NPME_DNL(double, KIND_HELMHOLTZ, false);
   } else {
-    if (p.kind == KIND_LAPLACE) NPME_DIRECT(float, KIND_LAPLACE);
-    else if (p.kind == KIND_RALPHA) NPME_DIRECT(float, KIND_RALPHA);
-    else NPME_DIRECT(float, KIND_HELMHOLTZ);
+    if (p.kind == KIND_LAPLACE) NPME_DNL(float, KIND_LAPLACE, true);
+    else if (p.kind == KIND_RALPHA) NPME_DNL(float, KIND_RALPHA, true);
+    else NPME_DNL(float, KIND_HELMHOLTZ, true);
   }
-#undef NPME_DIRECT
+#undef NPME_DNL
+#undef NPME_DNL_D
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -333,7 +673,7 @@ __global__ __launch_bounds__(64) void k_spread(int Bn, long M1, long M2, long M3
 {
   __shared__ Real w[64][3 * TS];
   __shared__ Real qs[64][2];
-  const int t = blockIdx.x;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int t3 = t % nt3, t2 = (t / nt3) % nt2, t1 = t / (nt3 * nt2);
   const int u1 = t1 * TS, u2 = t2 * TS, u3 = t3 * TS;
   const int li = threadIdx.x / TS, lj = threadIdx.x % TS;
@@ -430,7 +770,7 @@ __global__ __launch_bounds__(256) void k_interp(int Bn, long M1, long M2, long M
                                                 const double *__restrict__ q, double fsr,
                                                 double fsi, double *__restrict__ Vout)
 {
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t s = (size_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int4 m0 = ms[s];
   const typename V4t<Real>::T f = frac[s];

@@ -43,13 +43,20 @@ template <typename T> static T *dalloc(Handle &h, size_t count)
 
 void handle_free_device(Handle &h)
 {
-  void *ptrs[] = {h.d_perm, h.d_src, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
+  void *ptrs[] = {h.d_perm, h.d_src, h.d_srcLo, h.d_blockBox, h.d_segStart, h.d_nlist,
+                  h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
                   h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   h.d_perm = nullptr;
-  h.d_src = h.d_qi = nullptr;
+  h.d_src = h.d_qi = h.d_srcLo = h.d_blockBox = nullptr;
+  h.d_segStart = nullptr;
+  h.d_nlist = nullptr;
+  h.d_sliceStart = nullptr;
+  h.d_sliceLen = nullptr;
+  h.useNlist = false;
+  h.nlistEntries = 0;
   h.d_cell = nullptr;
   h.d_cellStart = nullptr;
   h.d_ms = nullptr;
@@ -127,6 +134,57 @@ static double tsmooth(double x, double X, double Xmax, double a)
   return hramp((-x + X + del) / del, a);
 }
 
+// ---------------------------------------------------------------- neighbour list
+// Built when the keyword neighborList is not 'off', the cluster offsets fit
+// the entry code (nNeigh <= nCellClust1D) and the list fits a quarter of the
+// free device memory ('on' lifts the memory limit); otherwise the direct
+// sum walks the cell rows of each block (k_direct).
+static void build_neighbor_list(Handle &h, const Keywords &kw)
+{
+  std::string mode = "auto";
+  kw.get("neighborList", mode);
+  if (mode == "off" || h.nNeigh > h.nCellClust1D || h.n >= (1u << 27)) return;
+  const DirectParams dp = direct_params(h);
+  const int nP = (int)h.nCellClust1D;
+  int *d_count = nullptr;
+  NPME_HIP_CHECK(hipMalloc(&d_count, h.n * sizeof(int)));
+  std::vector<int> cnt(h.n);
+  try {
+    launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP, d_count,
+                 nullptr, nullptr, h.stream);
+    NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  } catch (...) {
+    (void)hipFree(d_count);
+    throw;
+  }
+  (void)hipFree(d_count);
+  const size_t nSlices = (h.n + 63) / 64;
+  std::vector<long> sliceStart(nSlices + 1, 0);
+  std::vector<int> sliceLen(nSlices, 0);
+  for (size_t sl = 0; sl < nSlices; sl++) {
+    int m = 0;
+    for (size_t s = 64 * sl; s < std::min(h.n, 64 * sl + 64); s++) m = std::max(m, cnt[s]);
+    sliceLen[sl] = m;
+    sliceStart[sl + 1] = sliceStart[sl] + 64L * m;
+  }
+  const size_t entries = (size_t)sliceStart[nSlices];
+  size_t freeB = 0, totalB = 0;
+  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  if (mode != "on" && entries * sizeof(unsigned) > freeB / 4) return;
+  h.d_nlist = dalloc<unsigned>(h, entries);
+  h.d_sliceStart = dalloc<long>(h, nSlices + 1);
+  h.d_sliceLen = dalloc<int>(h, nSlices);
+  NPME_HIP_CHECK(hipMemcpy(h.d_sliceStart, sliceStart.data(), (nSlices + 1) * sizeof(long), hipMemcpyHostToDevice));
+  NPME_HIP_CHECK(hipMemcpy(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int), hipMemcpyHostToDevice));
+  NPME_HIP_CHECK(hipMemsetAsync(h.d_nlist, 0xff, entries * sizeof(unsigned), h.stream));
+  launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP, nullptr,
+               h.d_sliceStart, h.d_nlist, h.stream);
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  h.nlistEntries = entries;
+  h.useNlist = true;
+}
+
 // ---------------------------------------------------------------- set-up
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
 {
@@ -195,9 +253,10 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.M[d] = N / 2;
   }
   if (h.Bn < 2 || h.Bn > 16) throw Error{NPME_ERR_UNSUPPORTED, "BnOrder must be in [2, 16]"};
-  if (h.N[0] + 1 > (h.fp64 ? 16 : 32) * 256 || h.N[1] + 1 > (h.fp64 ? 16 : 32) * 256 ||
-      h.N[2] + 1 > (h.fp64 ? 16 : 32) * 256)
-    throw Error{NPME_ERR_UNSUPPORTED, "FFT length too large for one LDS column"};
+  // one FFT column must fit a workgroup's LDS in FP64 (kernel-grid set-up)
+  for (int d = 0; d < 3; d++)
+    if (h.N[d] + 1 > 16 * 256)
+      throw Error{NPME_ERR_UNSUPPORTED, "FFT length above 4095 is not provided"};
   h.K3 = h.complexK ? h.N[2] : h.N[2] / 2 + 1;
 
   // ---- geometry of the reciprocal sum
@@ -243,22 +302,85 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   std::stable_sort(h.perm.begin(), h.perm.end(), [&](int x, int y) { return key[x] < key[y]; });
 
   const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
+  // (start, count) of every cell of the dense cell grid in the sorted order;
+  // an empty cell starts where its key would be, so a run of consecutive
+  // cells of one cluster row is the range [start(first), start(last)+count)
+  std::vector<unsigned long> skey(n);
+  for (size_t s = 0; s < n; s++) skey[s] = key[h.perm[s]];
   std::vector<int> cellSE(2 * (size_t)ncell, 0);
+  for (int g0 = 0; g0 < h.nb[0]; g0++)
+    for (int g1 = 0; g1 < h.nb[1]; g1++)
+      for (int g2 = 0; g2 < h.nb[2]; g2++) {
+        const unsigned long a0 = g0 / nP, a1 = g1 / nP, a2 = g2 / nP;
+        const unsigned long kc = ((a0 * nA[1] + a1) * nA[2] + a2) * Nc +
+                                 ((g0 % nP) * nP + (g1 % nP)) * nP + (g2 % nP);
+        auto lo = std::lower_bound(skey.begin(), skey.end(), kc);
+        auto hi = std::upper_bound(lo, skey.end(), kc);
+        const long c = ((long)g0 * h.nb[1] + g1) * h.nb[2] + g2;
+        cellSE[2 * c] = (int)(lo - skey.begin());
+        cellSE[2 * c + 1] = (int)(hi - lo);
+      }
   std::vector<int4> cellOf(n);
   std::vector<double> loc(4 * n);
   for (size_t s = 0; s < n; s++) {
     const int i = h.perm[s];
-    const int g0 = gcell[3 * i], g1 = gcell[3 * i + 1], g2 = gcell[3 * i + 2];
-    const long c = ((long)g0 * h.nb[1] + g1) * h.nb[2] + g2;
-    if (cellSE[2 * c + 1] == 0) cellSE[2 * c] = (int)s;
-    cellSE[2 * c + 1]++;
-    cellOf[s] = make_int4(g0, g1, g2, 0);
-    // coordinates relative to the low corner of the charge's cell
-    loc[4 * s + 0] = (coord[3 * i + 0] - mn[0]) - g0 * csB;
-    loc[4 * s + 1] = (coord[3 * i + 1] - mn[1]) - g1 * csB;
-    loc[4 * s + 2] = (coord[3 * i + 2] - mn[2]) - g2 * csB;
+    int g[3];
+    for (int d = 0; d < 3; d++) {
+      g[d] = gcell[3 * i + d];
+      // position relative to the low corner of the charge's cluster, as the
+      // reference forms it (x2 = x1 - n cellSizeA)
+      loc[4 * s + d] = (coord[3 * i + d] + (-mn[d])) - (double)(g[d] / (int)nP) * csA;
+    }
     loc[4 * s + 3] = 0;
+    cellOf[s] = make_int4(g[0], g[1], g[2], 0);
   }
+  // source segments of every block of 8 consecutive sorted targets: rows of
+  // cells within Rdir (slightly inflated, the kernel applies the exact
+  // cutoff) of the box spanned by the block's positions, split per cluster
+  const long nBlocks = ((long)n + 7) / 8;
+  std::vector<int> segStart(nBlocks + 1, 0);
+  std::vector<int4> segs;
+  segs.reserve((size_t)nBlocks * 48);
+  {
+    const double Rm = Rdir * (1.0 + 1.0e-9) + 1.0e-12;
+    for (long b = 0; b < nBlocks; b++) {
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (size_t s = 8 * b; s < std::min(n, (size_t)(8 * b + 8)); s++) {
+        const int i = h.perm[s];
+        for (int d = 0; d < 3; d++) {
+          const double x = coord[3 * i + d] + (-mn[d]);
+          lo[d] = std::min(lo[d], x);
+          hi[d] = std::max(hi[d], x);
+        }
+      }
+      const int gx0 = std::max(0, (int)std::floor((lo[0] - Rm) / csB));
+      const int gx1 = std::min(h.nb[0] - 1, (int)std::floor((hi[0] + Rm) / csB));
+      const int gy0 = std::max(0, (int)std::floor((lo[1] - Rm) / csB));
+      const int gy1 = std::min(h.nb[1] - 1, (int)std::floor((hi[1] + Rm) / csB));
+      for (int gx = gx0; gx <= gx1; gx++) {
+        const double gapx = std::max(0.0, std::max(gx * csB - hi[0], lo[0] - (gx + 1) * csB));
+        if (gapx >= Rm) continue;
+        for (int gy = gy0; gy <= gy1; gy++) {
+          const double gapy = std::max(0.0, std::max(gy * csB - hi[1], lo[1] - (gy + 1) * csB));
+          const double rem = Rm * Rm - gapx * gapx - gapy * gapy;
+          if (rem <= 0) continue;
+          const double sr = std::sqrt(rem);
+          const int zlo = std::max(0, (int)std::floor((lo[2] - sr) / csB));
+          const int zhi = std::min(h.nb[2] - 1, (int)std::floor((hi[2] + sr) / csB));
+          const long row = ((long)gx * h.nb[1] + gy) * h.nb[2];
+          for (int az = zlo / (int)nP; az <= zhi / (int)nP; az++) {
+            const int z0 = std::max(zlo, az * (int)nP), z1 = std::min(zhi, az * (int)nP + (int)nP - 1);
+            const int sB = cellSE[2 * (row + z0)];
+            const int sE = cellSE[2 * (row + z1)] + cellSE[2 * (row + z1) + 1];
+            if (sE > sB) segs.push_back(make_int4(sB, sE, gx / (int)nP, ((gy / (int)nP) << 16) | az));
+          }
+        }
+      }
+      if (segs.size() > 0x7fffffffUL) throw Error{NPME_ERR_NOMEM, "too many direct-sum segments"};
+      segStart[b + 1] = (int)segs.size();
+    }
+  }
+  h.csA = csA;
 
   // ---- spline windows (u = m + N/4) and per-tile lists
   std::vector<int4> ms(n);
@@ -322,10 +444,22 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     NPME_HIP_CHECK(hipMemcpy(h.d_src, loc.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
     NPME_HIP_CHECK(hipMemcpy(h.d_frac, frac.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
   } else {
-    std::vector<float> lf(loc.begin(), loc.end()), ff(frac.begin(), frac.end());
+    // FP32 hi part and the FP32 remainder: hi + lo carries ~48 bits
+    std::vector<float> lf(4 * n), lo(4 * n), ff(frac.begin(), frac.end());
+    for (size_t k = 0; k < 4 * n; k++) {
+      lf[k] = (float)loc[k];
+      lo[k] = (float)(loc[k] - (double)lf[k]);
+    }
     NPME_HIP_CHECK(hipMemcpy(h.d_src, lf.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+    h.d_srcLo = dalloc<float>(h, 4 * n);
+    NPME_HIP_CHECK(hipMemcpy(h.d_srcLo, lo.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
     NPME_HIP_CHECK(hipMemcpy(h.d_frac, ff.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
   }
+  h.d_segStart = dalloc<int>(h, segStart.size());
+  NPME_HIP_CHECK(hipMemcpy(h.d_segStart, segStart.data(), segStart.size() * sizeof(int), hipMemcpyHostToDevice));
+  h.d_blockBox = dalloc<int4>(h, segs.size());
+  NPME_HIP_CHECK(hipMemcpy(h.d_blockBox, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice));
+  h.nSegments = segs.size();
   if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
   h.d_cell = dalloc<int4>(h, n);
   NPME_HIP_CHECK(hipMemcpy(h.d_cell, cellOf.data(), n * sizeof(int4), hipMemcpyHostToDevice));
@@ -418,6 +552,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   (void)hipFree(d_F);
   (void)hipFree(d_lam);
   for (int d = 0; d < 3; d++) (void)hipFree(d_tw64[d]);
+  build_neighbor_list(h, kw);
 }
 
 // ---------------------------------------------------------------- CalcV
@@ -478,7 +613,12 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
   launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
   mark(h, st++);
   const DirectParams dp = direct_params(h);
-  launch_direct(f64, dp, h.n, h.d_src, h.d_qi, h.d_cell, h.d_cellStart, h.d_Vdir, s);
+  if (h.useNlist)
+    launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcLo, h.d_qi, h.d_sliceStart, h.d_sliceLen,
+                     h.d_nlist, h.csA, h.d_Vdir, s);
+  else
+    launch_direct(f64, dp, h.n, h.d_src, h.d_srcLo, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
+                  (int)h.nCellClust1D, h.csA, h.d_Vdir, s);
   mark(h, st++);
   launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_src, h.d_qi,
                 h.d_ms, h.d_frac, h.d_grid, s);

@@ -129,3 +129,15 @@ def test_calcv_rejects_wrong_size():
     V = np.zeros((1999, 4))
     assert not h.CalcV(V, 1999, q[:1999], coord[:1999])
     assert "nCharge" in N.npme_last_error()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_direct_sum_paths_agree(oracle, prec):
+    """The neighbour-list and the cell-row direct sums give the same V."""
+    coord, q = uniform_box(6000, 35.0)
+    hl = _real_setup(coord, q, prec, Rdir=7.0, neighborList="on")
+    hc = _real_setup(coord, q, prec, Rdir=7.0, neighborList="off")
+    assert "neighbour list" in hl.Print() and "cell rows" in hc.Print()
+    Vl, Vc = hl.calcv(q), hc.calcv(q)
+    m = oracle.compare_v(Vl, Vc)
+    assert m["relV"] < (1e-13 if prec == "fp64" else 2e-7) and m["reldV"] < (1e-13 if prec == "fp64" else 1e-6), m
