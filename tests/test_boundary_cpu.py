@@ -105,3 +105,26 @@ def test_keyword_file_roundtrip(tmp_path):
     k = N.NPME_KeywordInput(str(p))
     assert k.GetValue("BnOrder") == "6" and k.GetValue("Rdir") == "4.0"
     assert k.GetValue("calcType") == "pme_exact" and k.GetValue("missing") is None
+
+
+def test_cpp_dropin_header_host_calls(oracle, tmp_path):
+    """include/npme_amd.hpp compiles as the reference's NPME_Library
+    interface, links against the library and returns the same RSEM grid as
+    the oracle (host calls only)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "npme_amd")
+    exe = str(tmp_path / "wrapper_host")
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "wrapper_host.cpp"), "-L", libdir, "-lnpme_amd",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    coord, q = uniform_box(10000, 40.0)
+    inp = tmp_path / "charges.txt"
+    np.savetxt(inp, np.column_stack([coord, q]), fmt="%.17g")
+    out = subprocess.run([exe, str(inp)], check=True, capture_output=True, text=True).stdout.split()
+    N, nb, nd = oracle.rsem_static(q, coord, 6, -1.0, 1e-6, 8.0)
+    assert [int(v) for v in out] == list(N) + list(nb) + [nd]
