@@ -42,6 +42,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-accuracy", action="store_true")
+    p.add_argument("--calib-copy", action="store_true",
+                   help="after the steps, copy a 256 MiB device buffer 3 times (a plain copy "
+                        "kernel of known size, to calibrate profiler byte counters)")
     return p.parse_args()
 
 
@@ -117,6 +120,14 @@ def main():
     Nder = h.Get_Nder()
     log("grid", Ngrid, "Nder", Nder, "setup %.3f s" % t_setup)
     ms = time_steps(h, dV, dq, a.steps, a.warmup)
+    if a.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x %d bytes read and written" % (src.numel() * 4))
+        del src, dst
     stages = stage_profile(h, dV, dq)
     P = h.count_pairs()
     V = dV.cpu().numpy()

@@ -333,26 +333,12 @@ int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget, const lon
       if (idx[t] < 0 || (size_t)idx[t] >= H.n) throw Error{NPME_ERR_ARG, "target index out of range"};
     const bool cx = H.complexK;
     const size_t qn = (cx ? 2 : 1) * H.n, vw = cx ? 8 : 4;
-    long *d_idx = nullptr;
-    double *d_out = nullptr;
-    NPME_HIP_CHECK(hipMalloc(&d_idx, std::max<size_t>(nTarget, 1) * sizeof(long)));
-    hipError_t e2 = hipMalloc(&d_out, std::max<size_t>(nTarget, 1) * vw * sizeof(double));
-    if (e2 != hipSuccess) {
-      (void)hipFree(d_idx);
-      throw DeviceError(hipGetErrorString(e2));
-    }
-    try {
-      NPME_HIP_CHECK(hipMemcpyAsync(H.d_qin, charge, qn * sizeof(double), hipMemcpyHostToDevice, H.stream));
-      NPME_HIP_CHECK(hipMemcpyAsync(d_idx, idx, nTarget * sizeof(long), hipMemcpyHostToDevice, H.stream));
-      handle_exact(H, H.d_qin, nTarget, d_idx, d_out);
-      NPME_HIP_CHECK(hipMemcpy(V, d_out, nTarget * vw * sizeof(double), hipMemcpyDeviceToHost));
-    } catch (...) {
-      (void)hipFree(d_idx);
-      (void)hipFree(d_out);
-      throw;
-    }
-    (void)hipFree(d_idx);
-    (void)hipFree(d_out);
+    DevBuf d_idx(std::max<size_t>(nTarget, 1) * sizeof(long));
+    DevBuf d_out(std::max<size_t>(nTarget, 1) * vw * sizeof(double));
+    NPME_HIP_CHECK(hipMemcpyAsync(H.d_qin, charge, qn * sizeof(double), hipMemcpyHostToDevice, H.stream));
+    NPME_HIP_CHECK(hipMemcpyAsync(d_idx.p, idx, nTarget * sizeof(long), hipMemcpyHostToDevice, H.stream));
+    handle_exact(H, H.d_qin, nTarget, (const long *)d_idx.p, (double *)d_out.p);
+    NPME_HIP_CHECK(hipMemcpy(V, d_out.p, nTarget * vw * sizeof(double), hipMemcpyDeviceToHost));
   });
 }
 
@@ -392,6 +378,7 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  fself " << H.fself[0] << " " << H.fself[1] << "\n";
   os << "  cells " << H.nb[0] << "x" << H.nb[1] << "x" << H.nb[2] << " of " << H.csB
      << "  device GB " << H.deviceBytes / 1e9 << "\n";
+  os << "  per-charge set-up on the " << (H.deviceSetup ? "device" : "host") << "\n";
   if (H.useNlist)
     os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";
   else

@@ -1,0 +1,372 @@
+// Set-up of one geometry on the device: embedded cell keys, the stable sort
+// of the charges by cell, the per-cell ranges, cluster-relative positions,
+// spline windows and the per-tile charge lists of spreading and
+// interpolation.  Every index is computed with the same FP64 operations as
+// the host set-up (no contraction into FMAs), and the sorts are stable, so
+// the permutation and every list are bit-identical to the host's; the
+// per-charge cell index follows src/PartitionBox.cpp:311-351 and
+// src/PartitionEmbeddedBox.cpp:266-455, the spline window
+// src/RecSumSupportFunctions.h:123-149.
+#include <cstring>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "npme_internal.h"
+
+namespace npme {
+
+// error bits written by the set-up kernels
+enum { GEO_ERR_CELL = 1, GEO_ERR_WINDOW = 2, GEO_ERR_FIXED = 4 };
+
+__device__ __forceinline__ bool d_coord2cell(double x, double cs, unsigned long ncell,
+                                             unsigned long &v)
+{
+#pragma clang fp contract(off)
+  const double tol = 1.0e-14;
+  if (x < -tol) return false;
+  unsigned long t = (unsigned long)(x / cs);
+  if (t > ncell) return false;
+  if (t == ncell) {
+    if (fabs(x - (double)t * cs) < tol) t--;
+    else return false;
+  }
+  v = t;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_cell_keys(size_t n, const double *__restrict__ coord,
+                                                   GeoParams g, unsigned long *__restrict__ key,
+                                                   int *__restrict__ idx,
+                                                   int4 *__restrict__ gcell, int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  unsigned long a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+  bool ok = true;
+  for (int d = 0; d < 3; d++) {
+    const double x1 = coord[3 * i + d] + (-g.mn[d]);
+    ok = ok && d_coord2cell(x1, g.csA, g.nA[d], a[d]);
+    const double x2 = x1 - (double)a[d] * g.csA;
+    ok = ok && d_coord2cell(x2, g.csB, g.nP, b[d]);
+  }
+  if (!ok) {
+    err[0] = GEO_ERR_CELL;
+    a[0] = a[1] = a[2] = b[0] = b[1] = b[2] = 0;
+  }
+  const unsigned long Nc = g.nP * g.nP * g.nP;
+  const unsigned long tA = (a[0] * g.nA[1] + a[1]) * g.nA[2] + a[2];
+  key[i] = tA * Nc + (b[0] * g.nP + b[1]) * g.nP + b[2];
+  idx[i] = (int)i;
+  gcell[i] = make_int4((int)(a[0] * g.nP + b[0]), (int)(a[1] * g.nP + b[1]),
+                       (int)(a[2] * g.nP + b[2]), 0);
+}
+
+template <typename K>
+__device__ __forceinline__ size_t lower_bound_d(const K *a, size_t n, K v)
+{
+  size_t lo = 0, hi = n;
+  while (lo < hi) {
+    const size_t mid = (lo + hi) >> 1;
+    if (a[mid] < v) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// (start, count) of every cell of the dense cell grid; an empty cell starts
+// where its key would be
+__global__ __launch_bounds__(256) void k_cell_ranges(long ncell, GeoParams g,
+                                                     const unsigned long *__restrict__ skey,
+                                                     size_t n, int *__restrict__ cellSE)
+{
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const long g2 = c % g.nb[2], g1 = (c / g.nb[2]) % g.nb[1], g0 = c / ((long)g.nb[1] * g.nb[2]);
+  const unsigned long nP = g.nP, Nc = nP * nP * nP;
+  const unsigned long a0 = g0 / nP, a1 = g1 / nP, a2 = g2 / nP;
+  const unsigned long kc =
+      ((a0 * g.nA[1] + a1) * g.nA[2] + a2) * Nc + ((g0 % nP) * nP + (g1 % nP)) * nP + (g2 % nP);
+  const size_t lo = lower_bound_d(skey, n, kc);
+  const size_t hi = lower_bound_d(skey, n, kc + 1);
+  cellSE[2 * c] = (int)lo;
+  cellSE[2 * c + 1] = (int)(hi - lo);
+}
+
+// cell of each sorted charge
+__global__ __launch_bounds__(256) void k_gather_cells(size_t n, const int *__restrict__ perm,
+                                                      const int4 *__restrict__ gcell,
+                                                      int4 *__restrict__ cellOf)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) cellOf[s] = gcell[perm[s]];
+}
+
+// per sorted charge: cluster-relative position (FP64 or int32 fixed
+// point), spline window start u0 = m0 + N/4 and fractional argument, and
+// the number of 8^3 spreading tiles its window touches
+template <bool FP64>
+__global__ __launch_bounds__(256) void k_charge_setup(size_t n, const int *__restrict__ perm,
+                                                      const double *__restrict__ coord,
+                                                      const int4 *__restrict__ cellOf, GeoParams g,
+                                                      void *__restrict__ src,
+                                                      float4 *__restrict__ srcRes,
+                                                      int4 *__restrict__ ms,
+                                                      void *__restrict__ frac,
+                                                      int *__restrict__ ntile,
+                                                      int *__restrict__ err)
+{
+#pragma clang fp contract(off)
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = perm[s];
+  const int4 gc = cellOf[s];
+  const int gcv[3] = {gc.x, gc.y, gc.z};
+  double loc[3], fr[3];
+  int u0[3];
+  int cnt = 1;
+  for (int d = 0; d < 3; d++) {
+    const double x = coord[3 * (size_t)i + d];
+    loc[d] = (x + (-g.mn[d])) - (double)(gcv[d] / (int)g.nP) * g.csA;
+    const double w = g.NoverL[d] * (x - g.R[d]);
+    const long m0 = (w < 0) ? (long)(-w) + 1 : (long)(-w);
+    if (m0 < -g.N[d] / 4 || m0 + g.Bn - 1 > g.N[d] / 4 - 1) {
+      err[0] = GEO_ERR_WINDOW;
+      u0[d] = 0;
+      fr[d] = 0;
+    } else {
+      u0[d] = (int)(m0 + g.N[d] / 4);
+      fr[d] = w + (double)m0;
+    }
+    cnt *= (u0[d] + (int)g.Bn - 1) / 8 - u0[d] / 8 + 1;
+  }
+  ms[s] = make_int4(u0[0], u0[1], u0[2], 0);
+  ntile[s] = cnt;
+  if (FP64) {
+    ((double4 *)src)[s] = make_double4(loc[0], loc[1], loc[2], 0.0);
+    ((double4 *)frac)[s] = make_double4(fr[0], fr[1], fr[2], 0.0);
+  } else {
+    int v[3];
+    float res[3];
+    for (int d = 0; d < 3; d++) {
+      const double u = loc[d] * g.fxScale;
+      const long r = lround(u);
+      if (r < -(1L << 20) || r > (1L << g.fxShift) + (1L << 20)) {
+        err[0] = GEO_ERR_FIXED;
+        v[d] = 0;
+      } else {
+        v[d] = (int)r;
+      }
+      res[d] = (float)(u - (double)r);
+    }
+    ((int4 *)src)[s] = make_int4(v[0], v[1], v[2], 0);
+    srcRes[s] = make_float4(res[0], res[1], res[2], 0.0f);
+    ((float4 *)frac)[s] = make_float4((float)fr[0], (float)fr[1], (float)fr[2], 0.0f);
+  }
+}
+
+// (tile, charge) entries of the spreading lists, in charge order
+__global__ __launch_bounds__(256) void k_tile_emit(size_t n, const int4 *__restrict__ ms,
+                                                   const int *__restrict__ off, GeoParams g,
+                                                   unsigned *__restrict__ tkey,
+                                                   int *__restrict__ tval)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int4 u = ms[s];
+  const int lo0 = u.x / 8, hi0 = (u.x + (int)g.Bn - 1) / 8;
+  const int lo1 = u.y / 8, hi1 = (u.y + (int)g.Bn - 1) / 8;
+  const int lo2 = u.z / 8, hi2 = (u.z + (int)g.Bn - 1) / 8;
+  int o = off[s];
+  for (int a = lo0; a <= hi0; a++)
+    for (int b = lo1; b <= hi1; b++)
+      for (int c = lo2; c <= hi2; c++) {
+        tkey[o] = (unsigned)(((long)a * g.nt[1] + b) * g.nt[2] + c);
+        tval[o] = (int)s;
+        o++;
+      }
+}
+
+// interpolation tile of each charge (the tile holding its window start)
+__global__ __launch_bounds__(256) void k_itile_key(size_t n, const int4 *__restrict__ ms,
+                                                   GeoParams g, unsigned *__restrict__ key,
+                                                   int *__restrict__ val)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int4 u = ms[s];
+  key[s] = (unsigned)(((long)(u.x / g.TI) * g.nti[1] + u.y / g.TI) * g.nti[2] + u.z / g.TI);
+  val[s] = (int)s;
+}
+
+// start[t] = number of entries with key < t, t = 0..nt
+__global__ __launch_bounds__(256) void k_starts(long nt, const unsigned *__restrict__ skey,
+                                                size_t m, int *__restrict__ start)
+{
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > nt) return;
+  start[t] = (int)lower_bound_d(skey, m, (unsigned)t);
+}
+
+// neighbour-list slices: the length of a slice is its longest list; the
+// scan input is 64 * length (the last slot stays 0 for the total)
+__global__ __launch_bounds__(256) void k_slice_len(size_t n, const int *__restrict__ count,
+                                                   int *__restrict__ sliceLen,
+                                                   long *__restrict__ sliceWords)
+{
+  const size_t sl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t nSlices = (n + 63) / 64;
+  if (sl > nSlices) return;
+  if (sl == nSlices) {
+    sliceWords[sl] = 0;
+    return;
+  }
+  int m = 0;
+  const size_t e = (64 * sl + 64 < n) ? 64 * sl + 64 : n;
+  for (size_t s = 64 * sl; s < e; s++) m = max(m, count[s]);
+  sliceLen[sl] = m;
+  sliceWords[sl] = 64L * m;
+}
+
+static dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+static int key_bits(unsigned long maxKey)
+{
+  int b = 1;
+  while (b < 64 && (maxKey >> b) != 0) b++;
+  return b;
+}
+
+// stable sort of (key, value) pairs; temporary storage from the pool
+template <typename K>
+static void sort_pairs(const K *kin, K *kout, const int *vin, int *vout, size_t m, int bits,
+                       hipStream_t s)
+{
+  size_t bytes = 0;
+  NPME_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, m, 0, bits, s));
+  DevBuf tmp(bytes);
+  NPME_HIP_CHECK(rocprim::radix_sort_pairs(tmp.p, bytes, kin, kout, vin, vout, m, 0, bits, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));  // tmp is released on return
+}
+
+template <typename T, typename S>
+static void exclusive_scan(const T *in, S *out, size_t m, hipStream_t s)
+{
+  size_t bytes = 0;
+  NPME_HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, (S)0, m, rocprim::plus<S>(), s));
+  DevBuf tmp(bytes);
+  NPME_HIP_CHECK(rocprim::exclusive_scan(tmp.p, bytes, in, out, (S)0, m, rocprim::plus<S>(), s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+static void check_err(const int *d_err, hipStream_t s)
+{
+  int e = 0;
+  NPME_HIP_CHECK(hipMemcpyAsync(&e, d_err, sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  if (e & GEO_ERR_CELL) throw Error{NPME_ERR_ARG, "coordinate outside the cell box"};
+  if (e & GEO_ERR_WINDOW) throw Error{NPME_ERR_GRID, "spline window outside the compact grid"};
+  if (e & GEO_ERR_FIXED) throw Error{NPME_ERR_ARG, "position outside its cluster"};
+}
+
+void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, int *d_perm,
+                           int4 *d_cellOf, int *d_cellSE, hipStream_t s)
+{
+  DevBuf key(n * sizeof(unsigned long)), skey(n * sizeof(unsigned long)), idx(n * sizeof(int)),
+      err(sizeof(int)), gcell(n * sizeof(int4));
+  NPME_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_cell_keys, grid_for(n), dim3(256), 0, s, n, d_coord, g,
+                     (unsigned long *)key.p, (int *)idx.p, (int4 *)gcell.p, (int *)err.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  check_err((const int *)err.p, s);
+  const unsigned long Nc = g.nP * g.nP * g.nP;
+  const unsigned long maxKey = g.nA[0] * g.nA[1] * g.nA[2] * Nc;
+  sort_pairs((const unsigned long *)key.p, (unsigned long *)skey.p, (const int *)idx.p, d_perm,
+             n, key_bits(maxKey), s);
+  const long ncell = (long)g.nb[0] * g.nb[1] * g.nb[2];
+  hipLaunchKernelGGL(k_cell_ranges, grid_for(ncell), dim3(256), 0, s, ncell, g,
+                     (const unsigned long *)skey.p, n, d_cellSE);
+  NPME_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_gather_cells, grid_for(n), dim3(256), 0, s, n, (const int *)d_perm,
+                     (const int4 *)gcell.p, d_cellOf);
+  NPME_HIP_CHECK(hipGetLastError());
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const double *d_coord,
+                             const int *d_perm, const int4 *d_cellOf, void *d_src, void *d_srcRes,
+                             int4 *d_ms, void *d_frac, DevBuf &tileStart, DevBuf &tileList,
+                             DevBuf &itileStart, DevBuf &itileList, size_t &nTileEntries,
+                             hipStream_t s)
+{
+  DevBuf ntile(n * sizeof(int)), off((n + 1) * sizeof(int)), err(sizeof(int));
+  NPME_HIP_CHECK(hipMemsetAsync(err.p, 0, sizeof(int), s));
+  if (fp64)
+    hipLaunchKernelGGL(k_charge_setup<true>, grid_for(n), dim3(256), 0, s, n, d_perm, d_coord,
+                       d_cellOf, g, d_src, (float4 *)d_srcRes, d_ms, d_frac, (int *)ntile.p,
+                       (int *)err.p);
+  else
+    hipLaunchKernelGGL(k_charge_setup<false>, grid_for(n), dim3(256), 0, s, n, d_perm, d_coord,
+                       d_cellOf, g, d_src, (float4 *)d_srcRes, d_ms, d_frac, (int *)ntile.p,
+                       (int *)err.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  check_err((const int *)err.p, s);
+
+  // spreading lists: emit (tile, charge) in charge order, stable sort by tile
+  exclusive_scan((const int *)ntile.p, (int *)off.p, n, s);
+  int last[2] = {0, 0};
+  NPME_HIP_CHECK(hipMemcpyAsync(&last[0], (int *)off.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipMemcpyAsync(&last[1], (int *)ntile.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  const size_t m = (size_t)last[0] + (size_t)last[1];
+  nTileEntries = m;
+  {
+    DevBuf tk(m * sizeof(unsigned)), tv(m * sizeof(int)), sk(m * sizeof(unsigned));
+    tileList.reset(m * sizeof(int));
+    hipLaunchKernelGGL(k_tile_emit, grid_for(n), dim3(256), 0, s, n, d_ms, (const int *)off.p, g,
+                       (unsigned *)tk.p, (int *)tv.p);
+    NPME_HIP_CHECK(hipGetLastError());
+    const long ntiles = (long)g.nt[0] * g.nt[1] * g.nt[2];
+    sort_pairs((const unsigned *)tk.p, (unsigned *)sk.p, (const int *)tv.p, (int *)tileList.p, m,
+               key_bits((unsigned long)ntiles), s);
+    tileStart.reset((ntiles + 1) * sizeof(int));
+    hipLaunchKernelGGL(k_starts, grid_for(ntiles + 1), dim3(256), 0, s, ntiles,
+                       (const unsigned *)sk.p, m, (int *)tileStart.p);
+    NPME_HIP_CHECK(hipGetLastError());
+    NPME_HIP_CHECK(hipStreamSynchronize(s));  // before the scratch returns to the pool
+  }
+  // interpolation lists: one tile per charge
+  {
+    DevBuf ik(n * sizeof(unsigned)), iv(n * sizeof(int)), sk(n * sizeof(unsigned));
+    itileList.reset(n * sizeof(int));
+    hipLaunchKernelGGL(k_itile_key, grid_for(n), dim3(256), 0, s, n, d_ms, g, (unsigned *)ik.p,
+                       (int *)iv.p);
+    NPME_HIP_CHECK(hipGetLastError());
+    const long nit = (long)g.nti[0] * g.nti[1] * g.nti[2];
+    sort_pairs((const unsigned *)ik.p, (unsigned *)sk.p, (const int *)iv.p, (int *)itileList.p, n,
+               key_bits((unsigned long)nit), s);
+    itileStart.reset((nit + 1) * sizeof(int));
+    hipLaunchKernelGGL(k_starts, grid_for(nit + 1), dim3(256), 0, s, nit, (const unsigned *)sk.p,
+                       n, (int *)itileStart.p);
+    NPME_HIP_CHECK(hipGetLastError());
+    NPME_HIP_CHECK(hipStreamSynchronize(s));
+  }
+}
+
+size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
+                           hipStream_t s)
+{
+  const size_t nSlices = (n + 63) / 64;
+  DevBuf words((nSlices + 1) * sizeof(long));
+  hipLaunchKernelGGL(k_slice_len, grid_for(nSlices + 1), dim3(256), 0, s, n, d_count, d_sliceLen,
+                     (long *)words.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  exclusive_scan((const long *)words.p, d_sliceStart, nSlices + 1, s);
+  long total = 0;
+  NPME_HIP_CHECK(hipMemcpyAsync(&total, d_sliceStart + nSlices, sizeof(long), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  return (size_t)total;
+}
+
+}  // namespace npme

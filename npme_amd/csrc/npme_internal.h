@@ -84,6 +84,39 @@ struct Keywords {
   void set(const std::string &k, long v);
 };
 
+// ------------------------------------------------------------- device memory
+// caching allocator (devmem.cpp): blocks return to per-device free lists;
+// a block may be freed only when no queued work still uses it
+void *dev_alloc(size_t bytes);  // nullptr when the device is out of memory
+void dev_free(void *p);
+size_t dev_block_bytes(size_t bytes);
+void dev_trim();
+void dev_stats(size_t *inUse, size_t *cached);
+
+// RAII block from the pool
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() {}
+  explicit DevBuf(size_t b) { reset(b); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { dev_free(p); }
+  void reset(size_t b)
+  {
+    dev_free(p);
+    p = dev_alloc(b);
+    bytes = b;
+    if (!p) throw Error{NPME_ERR_NOMEM, "device allocation of " + std::to_string(b) + " bytes failed"};
+  }
+  void *release()
+  {
+    void *q = p;
+    p = nullptr;
+    return q;
+  }
+};
+
 // ------------------------------------------------------------- FFT plan
 struct FFTPlan1D {
   int N = 0;
@@ -115,8 +148,6 @@ struct Handle {
   int nb[3] = {0, 0, 0};
   double csB = 0, csA = 0;
   double boxMin[3] = {0, 0, 0};
-  // host copies
-  std::vector<int> perm;  // sorted position -> input index
   // spreading tiles
   int T[3] = {8, 8, 8};
   int nt[3] = {0, 0, 0};
@@ -124,8 +155,9 @@ struct Handle {
   hipStream_t stream = nullptr;
   int device = 0;
   int *d_perm = nullptr;         // [n]
-  void *d_src = nullptr;         // [n] float4/double4 (local xyz, q) ; complex: q re only here
-  void *d_srcLo = nullptr;       // [n] float4 low parts of the FP32 positions (mixed)
+  void *d_src = nullptr;         // [n] double4 (local xyz, q) or mixed: int32 fixed-point xyz + FP32 q
+  int fxShift = 29;              // mixed: one cluster edge = 2^fxShift fixed-point units
+  void *d_srcRes = nullptr;      // mixed: [n] float4 sub-unit residuals of the positions
   void *d_blockBox = nullptr;    // int4 source segments of the 8-target blocks
   int *d_segStart = nullptr;     // [ceil(n/8) + 1] first segment of each block
   size_t nSegments = 0;
@@ -142,6 +174,10 @@ struct Handle {
   void *d_frac = nullptr;        // [n] float4/double4 fractional spline arguments
   int *d_tileStart = nullptr;    // [ntiles + 1]
   int *d_tileList = nullptr;     // [nlist]
+  int TI = 16;                   // interpolation tile edge
+  int nti[3] = {0, 0, 0};
+  int *d_itileStart = nullptr;   // [ntiles + 1] charges per interpolation tile
+  int *d_itileList = nullptr;    // [n]
   void *d_grid = nullptr;        // Q / theta: M reals (real) or M complex
   void *d_buf1 = nullptr;        // M1 x M2 x K3 complex (real) / M1 x M2 x N3 (complex)
   void *d_buf2 = nullptr;        // M1 x N2 x K3 complex (real) / M1 x N2 x N3 (complex)
@@ -153,6 +189,7 @@ struct Handle {
   double *d_coordS = nullptr;    // [n][3] sorted FP64 coordinates (exact sums)
   FFTPlan1D plan[3];
   size_t deviceBytes = 0;
+  bool deviceSetup = false;      // per-charge set-up ran on the device
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -161,6 +198,32 @@ struct Handle {
 };
 
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord);
+
+// geometry of the cell partition and the grids, shared by the host and the
+// device set-up (geometry.hip)
+struct GeoParams {
+  double mn[3];            // low corner of the cell box
+  double csA, csB;         // cluster and cell edges
+  unsigned long nA[3];     // clusters per axis
+  unsigned long nP;        // cells per cluster edge
+  int nb[3];               // cells per axis
+  double NoverL[3], R[3];  // spline argument w = N/L (x - R)
+  long N[3];
+  long Bn;
+  double fxScale;          // mixed form: 2^fxShift / csA
+  int fxShift;
+  int nt[3];               // 8^3 spreading tiles per axis
+  int TI, nti[3];          // interpolation tiles
+};
+void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, int *d_perm,
+                           int4 *d_cellOf, int *d_cellSE, hipStream_t s);
+void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const double *d_coord,
+                             const int *d_perm, const int4 *d_cellOf, void *d_src, void *d_srcRes,
+                             int4 *d_ms,
+                             void *d_frac, DevBuf &tileStart, DevBuf &tileList, DevBuf &itileStart,
+                             DevBuf &itileList, size_t &nTileEntries, hipStream_t s);
+size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
+                           hipStream_t s);
 void handle_free_device(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout);
@@ -203,31 +266,32 @@ struct DirectParams {
   int nNeigh;
   double cs;
   int nb[3];
+  int fxShift;    // mixed form: one cluster edge = 2^fxShift fixed-point units
+  double fxInv;   // cluster edge / 2^fxShift
 };
 double handle_count_pairs(Handle &h);
 DirectParams direct_params(const Handle &h);
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s);
-void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
-                   const void *srcLo, const void *qi, const int4 *cell, const int *segStart,
-                   const void *segs, int nP, double csA, double *Vdir, hipStream_t s);
+void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
+                   const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
+                   double *Vdir, hipStream_t s);
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const int *perm, const double *coord, int nP, int *count, const long *sliceStart,
                   unsigned *list, hipStream_t s);
-void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src,
-                      const void *srcLo, const void *qi, const long *sliceStart,
-                      const int *sliceLen, const unsigned *list, double csA, double *Vdir,
-                      hipStream_t s);
+void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
+                      const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
+                      double csA, double *Vdir, hipStream_t s);
 void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                         const int *perm, const double *coord, unsigned long long *total,
                         hipStream_t s);
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
                    const int nt[3], const int *tileStart, const int *tileList, const void *src,
                    const void *qi, const int4 *ms, const void *frac, void *grid, hipStream_t s);
-void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3],
-                   size_t n, const int4 *ms, const void *frac, const void *grid,
-                   const double *Vdir, const int *perm, const double *q_in, const double fself[2],
-                   double *Vout, hipStream_t s);
+void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3], int T,
+                   const int nt[3], const int *tileStart, const int *tileList, const int4 *ms,
+                   const void *frac, const void *grid, const double *Vdir, const int *perm,
+                   const double *q_in, const double fself[2], double *Vout, hipStream_t s);
 void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
                   const double *coordS, const int *perm, const double *q_in, size_t nt,
                   const long *idx, double *Vout, hipStream_t s);

@@ -25,6 +25,9 @@ struct DirectDev {
   double a[62], b[62];    // DM coefficients (re, im), zero beyond Nder
   float af[62], bf[62];   // the same in FP32
   int nb0, nb1, nb2;
+  int fxUnit;      // 2^S: one cluster edge in fixed-point units (mixed form)
+  float fxInv;     // cluster edge / 2^S: length of one fixed-point unit
+  float fxClose2;  // below this r^2 a pair adds the sub-unit residuals
 };
 
 // coefficient array of the working precision
@@ -133,9 +136,14 @@ __device__ __forceinline__ void horner_cx(const double *c, int deg, Real r2, Rea
 // run crosses a cluster of nCellClust1D^3 cells (inside one cluster such a
 // run is contiguous in the sorted order).  All loops are uniform over the
 // wavefront; only the r < Rdir mask differs per lane.  Positions are
-// relative to the low corner of their cluster and, in the mixed form,
-// carried as FP32 hi + lo parts, so the pair difference keeps FP32 relative
-// accuracy in r for close pairs.  Per segment the lanes accumulate in the
+// relative to the low corner of their cluster.  In the mixed form they are
+// int32 fixed point (one cluster edge = 2^S units) in the x, y, z words of
+// the float4 whose w word is the FP32 charge: the pair difference is exact in
+// integers and rounded to FP32 once.  The rounding of the positions to whole
+// units (an absolute error of one unit) only matters for pairs closer than
+// 2^24 units, where it would exceed FP32 relative accuracy; those rare pairs
+// add the sub-unit residuals of both positions (a second array, read only
+// then), and their integer difference converts to FP32 exactly.  Per segment the lanes accumulate in the
 // working precision; segment partials are added in FP64 and the 8 lanes of
 // a target reduced by shuffles at the end.  Workgroups are remapped so that
 // each XCD works through a contiguous range of blocks (its L2 then holds the
@@ -146,10 +154,10 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg)
   return x * q + min(x, r) + local;
 }
 
-template <typename Real, int KIND, bool HILO, int MAXD>
+template <typename Real, int KIND, bool FIXP, int MAXD>
 __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
                                                 const typename V4t<Real>::T *__restrict__ src,
-                                                const float4 *__restrict__ srcLo,
+                                                const float4 *__restrict__ res,
                                                 const Real *__restrict__ qi,
                                                 const int4 *__restrict__ cell,
                                                 const int *__restrict__ segStart,
@@ -167,8 +175,14 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
   const bool tvalid = t < n;
   const size_t tt = tvalid ? t : (size_t)blk * 8;
   const V4 me = src[tt];
-  float4 meL = make_float4(0, 0, 0, 0);
-  if (HILO) meL = srcLo[tt];
+  int mx = 0, my = 0, mz = 0;
+  float4 mres = make_float4(0, 0, 0, 0);
+  if constexpr (FIXP) {
+    mx = __float_as_int(me.x);
+    my = __float_as_int(me.y);
+    mz = __float_as_int(me.z);
+    mres = res[tt];
+  }
   const int4 ct = cell[tt];
   const int atx = ct.x / nP, aty = ct.y / nP, atz = ct.z / nP;
   const Real R2 = (Real)(p.Rdir * p.Rdir);
@@ -184,32 +198,38 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
       const Real offx = (Real)((ax - atx) * csA);
       const Real offy = (Real)(((ayz >> 16) - aty) * csA);
       const Real offz = (Real)(((ayz & 0xffff) - atz) * csA);
+      const int iox = mx - (ax - atx) * p.fxUnit, ioy = my - ((ayz >> 16) - aty) * p.fxUnit,
+                ioz = mz - ((ayz & 0xffff) - atz) * p.fxUnit;
       Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       int js = segB + tj;
       V4 o;
-      float4 oL;
-      if (js < segE) {
-        o = src[js];
-        if (HILO) oL = srcLo[js];
-      }
+      if (js < segE) o = src[js];
       for (int s0 = segB; s0 < segE; s0 += 8) {
         const int jn = js + 8;
         V4 on;
-        float4 oLn;
-        if (jn < segE) {  // prefetch the next source of this lane
-          on = src[jn];
-          if (HILO) oLn = srcLo[jn];
-        }
+        if (jn < segE) on = src[jn];  // prefetch the next source of this lane
         if (js < segE && tvalid && (size_t)js != t) {
-          Real rx = me.x - (o.x + offx);
-          Real ry = me.y - (o.y + offy);
-          Real rz = me.z - (o.z + offz);
-          if (HILO) {
-            rx += (Real)(meL.x - oL.x);
-            ry += (Real)(meL.y - oL.y);
-            rz += (Real)(meL.z - oL.z);
+          Real rx, ry, rz, r2;
+          if constexpr (FIXP) {
+            const int dx = iox - __float_as_int(o.x), dy = ioy - __float_as_int(o.y),
+                      dz = ioz - __float_as_int(o.z);
+            rx = (Real)dx * (Real)p.fxInv;
+            ry = (Real)dy * (Real)p.fxInv;
+            rz = (Real)dz * (Real)p.fxInv;
+            r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 < (Real)p.fxClose2) {
+              const float4 ro = res[js];
+              rx = ((Real)dx + (mres.x - ro.x)) * (Real)p.fxInv;
+              ry = ((Real)dy + (mres.y - ro.y)) * (Real)p.fxInv;
+              rz = ((Real)dz + (mres.z - ro.z)) * (Real)p.fxInv;
+              r2 = rx * rx + ry * ry + rz * rz;
+            }
+          } else {
+            rx = me.x - (o.x + offx);
+            ry = me.y - (o.y + offy);
+            rz = me.z - (o.z + offz);
+            r2 = rx * rx + ry * ry + rz * rz;
           }
-          const Real r2 = rx * rx + ry * ry + rz * rz;
           if (r2 < R2) {
             if constexpr (KIND == KIND_LAPLACE) {
               const Real ri = rsqrt_r<Real>(r2);
@@ -261,7 +281,6 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
         }
         js = jn;
         o = on;
-        if (HILO) oL = oLn;
       }
 #pragma unroll
       for (int c = 0; c < (CX ? 8 : 4); c++) acc[c] += (double)part[c];
@@ -287,9 +306,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
   }
 }
 
-void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
-                   const void *srcLo, const void *qi, const int4 *cell, const int *segStart,
-                   const void *segs, int nP, double csA, double *Vdir, hipStream_t s)
+static DirectDev direct_dev(const DirectParams &p)
 {
   DirectDev d;
   d.kind = p.kind;
@@ -310,13 +327,26 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
   d.nb0 = p.nb[0];
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
+  d.fxUnit = 1 << p.fxShift;
+  d.fxInv = (float)p.fxInv;
+  // pairs closer than 2^24 units: one unit exceeds FP32 relative accuracy
+  const double rc = std::ldexp(p.fxInv, 24);
+  d.fxClose2 = (float)(rc * rc);
+  return d;
+}
+
+void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
+                   const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
+                   double *Vdir, hipStream_t s)
+{
+  const DirectDev d = direct_dev(p);
   const int nBlocks = (int)((n + 7) / 8);
   const dim3 grid((unsigned)((nBlocks + 3) / 4));  // 4 wavefronts per workgroup
   const int4 *sg = (const int4 *)segs;
 #define NPME_DIRECT_D(R, K, HL, D)                                                              \
   hipLaunchKernelGGL((k_direct<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                      \
-                     (const typename V4t<R>::T *)src, (const float4 *)srcLo, (const R *)qi,    \
-                     cell, segStart, sg, nBlocks, nP, csA, Vdir)
+                     (const typename V4t<R>::T *)src, (const float4 *)res, (const R *)qi, cell, \
+                     segStart, sg, nBlocks, nP, csA, Vdir)
 #define NPME_DIRECT(R, K, HL)                                                                   \
   do {                                                                                          \
     if (p.Nder <= 8) NPME_DIRECT_D(R, K, HL, 8);                                                \
@@ -343,13 +373,13 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src,
 // are cut into slices of 64 consecutive sorted charges (one wavefront), each
 // slice is padded to its longest list, and entry k of lane l sits at
 // sliceStart + k * 64 + l, so step k of a wavefront reads 256 contiguous
-// bytes.  An entry packs the source index (low 27 bits) with the offset of
-// the source's cluster from the target's cluster, (dx+1) * 9 + (dy+1) * 3 +
-// (dz+1) in the high 5 bits; padding entries are 0xffffffff.  The pairs are
+// bytes.  An entry packs the source index (low 26 bits) with the offset of
+// the source's cluster from the target's cluster, dx+1, dy+1 and dz+1 in
+// two bits each (bits 26, 28, 30); padding entries are 0xffffffff.  The pairs are
 // found in FP64 with a cutoff inflated by 1e-9 relative; the direct-sum
 // kernel applies r < Rdir in its working precision.
 constexpr unsigned NL_PAD = 0xffffffffu;
-constexpr int NL_SHIFT = 27;
+constexpr int NL_SHIFT = 26;
 
 // walk the cells of the stencil of sorted target s in FP64 input
 // coordinates; COUNT mode counts, FILL mode writes entries
@@ -380,7 +410,8 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
         const int gx = c.x + dx, gy = c.y + dy, gz = c.z + dz;
         if (gx < 0 || gy < 0 || gz < 0 || gx >= p.nb0 || gy >= p.nb1 || gz >= p.nb2) continue;
         const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
-        const unsigned code = (unsigned)((gx / nP - ax + 1) * 9 + (gy / nP - ay + 1) * 3 + (gz / nP - az + 1));
+        const unsigned code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) |
+                                         ((gz / nP - az + 1) << 4));
         for (int j = se.x; j < se.x + se.y; j++) {
           if ((size_t)j == s) continue;
           const long o = perm[j];
@@ -414,13 +445,67 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   NPME_HIP_CHECK(hipGetLastError());
 }
 
+// short-range DM pair term of one source (charge qr + i qim) at offset
+// (rx, ry, rz), r2 = |r|^2 < Rdir^2, added to part: V and grad (real), or
+// (re, im) of V, d/dx, d/dy, d/dz (complex)
+template <typename Real, int KIND, int MAXD>
+__device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Real rz, Real r2,
+                                        Real qr, Real qim, Real *part)
+{
+  if constexpr (KIND == KIND_LAPLACE) {
+    const Real ri = rsqrt_r<Real>(r2);
+    const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+    const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+    const Real fq = f1 * qr;
+    part[0] += f0 * qr;
+    part[1] += rx * fq;
+    part[2] += ry * fq;
+    part[3] += rz * fq;
+  } else if constexpr (KIND == KIND_RALPHA) {
+    const Real fe = pow(r2, (Real)(0.5 * p.alpha));
+    const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
+    const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+    const Real fq = f1 * qr;
+    part[0] += f0 * qr;
+    part[1] += rx * fq;
+    part[2] += ry * fq;
+    part[3] += rz * fq;
+  } else {
+    // exp(i k0 r)/r with the phase k0r r reduced modulo 2 pi in FP64
+    const Real r = sqrt(r2);
+    const Real ri = (Real)1 / r;
+    const double ph = p.k0r * (double)r;
+    const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
+    Real sn, cn;
+    sincos_r((Real)red, &sn, &cn);
+    const Real damp = exp((Real)(-p.k0i) * r) * ri;
+    const Real fr = damp * cn, fi = damp * sn;
+    const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
+    const Real ikr = -k0i - ri, iki = k0r;  // i k0 - 1/r
+    const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
+    Real pr, pim, br, bi;
+    horner_fix_cx<Real, MAXD>(coef_a<Real>(p), r2, pr, pim);
+    horner_fix_cx<Real, MAXD>(coef_b<Real>(p), r2, br, bi);
+    const Real s0r = fr - pr, s0i = fi - pim, s1r = f1r - br, s1i = f1i - bi;
+    part[0] += s0r * qr - s0i * qim;
+    part[1] += s0r * qim + s0i * qr;
+    const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
+    part[2] += rx * gr;
+    part[3] += rx * gi;
+    part[4] += ry * gr;
+    part[5] += ry * gi;
+    part[6] += rz * gr;
+    part[7] += rz * gi;
+  }
+}
+
 // direct sum over the neighbour list: one target per lane, the 64 targets of
 // a slice walk their lists in lockstep (lists of one slice have similar
 // lengths, so little of the wavefront idles)
-template <typename Real, int KIND, bool HILO, int MAXD>
+template <typename Real, int KIND, bool FIXP, int MAXD>
 __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
                                                    const typename V4t<Real>::T *__restrict__ src,
-                                                   const float4 *__restrict__ srcLo,
+                                                   const float4 *__restrict__ res,
                                                    const Real *__restrict__ qi,
                                                    const long *__restrict__ sliceStart,
                                                    const int *__restrict__ sliceLen,
@@ -435,83 +520,69 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
   const bool tvalid = s < n;
   const size_t tt = tvalid ? s : slice * 64;
   const V4 me = src[tt];
-  float4 meL = make_float4(0, 0, 0, 0);
-  if (HILO) meL = srcLo[tt];
+  // fixed-point target position shifted by one cluster edge, so that the
+  // offset code c in {0, 1, 2} enters as (me + unit) - c * unit - o
+  int mx = 0, my = 0, mz = 0;
+  float4 mres = make_float4(0, 0, 0, 0);
+  if constexpr (FIXP) {
+    mx = __float_as_int(me.x) + p.fxUnit;
+    my = __float_as_int(me.y) + p.fxUnit;
+    mz = __float_as_int(me.z) + p.fxUnit;
+    mres = res[tt];
+  }
   const Real R2 = (Real)(p.Rdir * p.Rdir);
   const Real cA = (Real)csA;
   const int len = sliceLen[slice];
   const unsigned *e = list + sliceStart[slice] + (s & 63);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  unsigned ent = (len > 0) ? e[0] : NL_PAD;
-  for (int k = 0; k < len; k++) {
-    const unsigned nxt = (k + 1 < len) ? e[(long)(k + 1) * 64] : NL_PAD;
-    if (ent != NL_PAD && tvalid) {
-      const int j = (int)(ent & ((1u << NL_SHIFT) - 1));
-      const int code = (int)(ent >> NL_SHIFT);
-      const Real ox = (Real)(code / 9 - 1) * cA, oy = (Real)((code / 3) % 3 - 1) * cA,
-                 oz = (Real)(code % 3 - 1) * cA;
-      const V4 o = src[j];
-      Real rx = me.x - (o.x + ox);
-      Real ry = me.y - (o.y + oy);
-      Real rz = me.z - (o.z + oz);
-      if (HILO) {
-        const float4 oL = srcLo[j];
-        rx += (Real)(meL.x - oL.x);
-        ry += (Real)(meL.y - oL.y);
-        rz += (Real)(meL.z - oL.z);
-      }
-      const Real r2 = rx * rx + ry * ry + rz * rz;
-      if (r2 < R2) {
-        if constexpr (KIND == KIND_LAPLACE) {
-          const Real ri = rsqrt_r<Real>(r2);
-          const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-          const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
-          const Real q = o.w, fq = f1 * q;
-          part[0] += f0 * q;
-          part[1] += rx * fq;
-          part[2] += ry * fq;
-          part[3] += rz * fq;
-        } else if constexpr (KIND == KIND_RALPHA) {
-          const Real fe = pow(r2, (Real)(0.5 * p.alpha));
-          const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-          const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
-          const Real q = o.w, fq = f1 * q;
-          part[0] += f0 * q;
-          part[1] += rx * fq;
-          part[2] += ry * fq;
-          part[3] += rz * fq;
-        } else {
-          const Real r = sqrt(r2);
-          const Real ri = (Real)1 / r;
-          const double ph = p.k0r * (double)r;
-          const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
-          Real sn, cn;
-          sincos_r((Real)red, &sn, &cn);
-          const Real damp = exp((Real)(-p.k0i) * r) * ri;
-          const Real fr = damp * cn, fi = damp * sn;
-          const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
-          const Real ikr = -k0i - ri, iki = k0r;
-          const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
-          Real pr, pim, br, bi;
-          horner_fix_cx<Real, MAXD>(coef_a<Real>(p), r2, pr, pim);
-          horner_fix_cx<Real, MAXD>(coef_b<Real>(p), r2, br, bi);
-          const Real s0r = fr - pr, s0i = fi - pim, s1r = f1r - br, s1i = f1i - bi;
-          const Real qr = o.w, qim = qi[j];
-          part[0] += s0r * qr - s0i * qim;
-          part[1] += s0r * qim + s0i * qr;
-          const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
-          part[2] += rx * gr;
-          part[3] += rx * gi;
-          part[4] += ry * gr;
-          part[5] += ry * gi;
-          part[6] += rz * gr;
-          part[7] += rz * gi;
-        }
-      }
+  // U entries per step: their source loads are independent, so each
+  // wavefront keeps U gathers in flight
+  constexpr int U = 4;
+  for (int k0 = 0; k0 < len; k0 += U) {
+    unsigned ent[U];
+    V4 o[U];
+    Real oqi[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) ent[u] = (k0 + u < len) ? e[(long)(k0 + u) * 64] : NL_PAD;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      // padding entries load source 0 (always present) and are skipped below
+      const int j = (ent[u] == NL_PAD) ? 0 : (int)(ent[u] & ((1u << NL_SHIFT) - 1));
+      o[u] = src[j];
+      if constexpr (CX) oqi[u] = qi[j];
     }
-    ent = nxt;
-    if ((k & 15) == 15) {  // FP64 carry every 16 pairs
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (ent[u] == NL_PAD || !tvalid) continue;
+      const unsigned en = ent[u];
+      const int cx = (int)((en >> NL_SHIFT) & 3), cy = (int)((en >> (NL_SHIFT + 2)) & 3),
+                cz = (int)(en >> (NL_SHIFT + 4));
+      Real rx, ry, rz, r2;
+      if constexpr (FIXP) {
+        const int dx = mx - cx * p.fxUnit - __float_as_int(o[u].x),
+                  dy = my - cy * p.fxUnit - __float_as_int(o[u].y),
+                  dz = mz - cz * p.fxUnit - __float_as_int(o[u].z);
+        rx = (Real)dx * (Real)p.fxInv;
+        ry = (Real)dy * (Real)p.fxInv;
+        rz = (Real)dz * (Real)p.fxInv;
+        r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 < (Real)p.fxClose2) {  // rare: add the sub-unit residuals
+          const float4 ro = res[en & ((1u << NL_SHIFT) - 1)];
+          rx = ((Real)dx + (mres.x - ro.x)) * (Real)p.fxInv;
+          ry = ((Real)dy + (mres.y - ro.y)) * (Real)p.fxInv;
+          rz = ((Real)dz + (mres.z - ro.z)) * (Real)p.fxInv;
+          r2 = rx * rx + ry * ry + rz * rz;
+        }
+      } else {
+        rx = me.x - (o[u].x + (Real)(cx - 1) * cA);
+        ry = me.y - (o[u].y + (Real)(cy - 1) * cA);
+        rz = me.z - (o[u].z + (Real)(cz - 1) * cA);
+        r2 = rx * rx + ry * ry + rz * rz;
+      }
+      if (r2 < R2) pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o[u].w, CX ? oqi[u] : (Real)0, part);
+    }
+    if (((k0 + U) & 15) == 0) {  // FP64 carry every 16 entries
 #pragma unroll
       for (int c = 0; c < (CX ? 8 : 4); c++) {
         acc[c] += (double)part[c];
@@ -531,34 +602,15 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
   }
 }
 
-void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src,
-                      const void *srcLo, const void *qi, const long *sliceStart,
-                      const int *sliceLen, const unsigned *list, double csA, double *Vdir,
-                      hipStream_t s)
+void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
+                      const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
+                      double csA, double *Vdir, hipStream_t s)
 {
-  DirectDev d;
-  d.kind = p.kind;
-  d.Nder = p.Nder;
-  d.nNeigh = p.nNeigh;
-  d.Rdir = p.Rdir;
-  d.alpha = p.alpha;
-  d.k0r = p.k0r;
-  d.k0i = p.k0i;
-  d.cs = p.cs;
-  for (int i = 0; i < 62; i++) {
-    const bool used = i < 2 * (p.Nder + 1);
-    d.a[i] = used ? p.a[i] : 0.0;
-    d.b[i] = used ? p.b[i] : 0.0;
-    d.af[i] = (float)d.a[i];
-    d.bf[i] = (float)d.b[i];
-  }
-  d.nb0 = p.nb[0];
-  d.nb1 = p.nb[1];
-  d.nb2 = p.nb[2];
+  const DirectDev d = direct_dev(p);
   const dim3 grid((unsigned)((n + 255) / 256));
 #define NPME_DNL_D(R, K, HL, D)                                                                 \
   hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \
-                     (const typename V4t<R>::T *)src, (const float4 *)srcLo, (const R *)qi,    \
+                     (const typename V4t<R>::T *)src, (const float4 *)res, (const R *)qi,      \
                      sliceStart, sliceLen, list, csA, Vdir)
 #define NPME_DNL(R, K, HL)                                                                      \
   do {                                                                                          \
@@ -653,6 +705,29 @@ template <typename Real> __device__ __forceinline__ void bsp_shifts(int n, Real 
   }
 }
 
+// the same with the order known at compile time (fully unrolled, registers)
+template <typename Real, int N> __device__ __forceinline__ void bsp_shifts_c(Real t, Real *m)
+{
+  if constexpr (N == 1) {
+    m[0] = (Real)1;
+  } else {
+    m[0] = t;
+    m[1] = (Real)1 - t;
+#pragma unroll
+    for (int k = 3; k <= N; k++) {
+      const Real inv = (Real)1 / (Real)(k - 1);
+      Real prev = (Real)0;
+#pragma unroll
+      for (int j = 0; j < k; j++) {
+        const Real left = (j < k - 1) ? m[j] : (Real)0;
+        const Real val = ((t + j) * left + ((Real)k - t - j) * prev) * inv;
+        prev = left;
+        m[j] = val;
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ spread
 // tile = 8 x 8 x 8 grid points, 64 lanes; lane (i, j) owns the 8-point
 // column (u1 + i, u2 + j, u3 .. u3+7).  Charges whose spline window meets the
@@ -661,8 +736,8 @@ template <typename Real> __device__ __forceinline__ void bsp_shifts(int n, Real 
 // q w1[i] w2[j] w3[k] into its column.
 constexpr int TS = 8;
 
-template <typename Real, bool CX>
-__global__ __launch_bounds__(64) void k_spread(int Bn, long M1, long M2, long M3, int nt1, int nt2,
+template <typename Real, bool CX, int BN>
+__global__ __launch_bounds__(64) void k_spread(int BnRun, long M1, long M2, long M3, int nt1, int nt2,
                                                int nt3, const int *__restrict__ tileStart,
                                                const int *__restrict__ tileList,
                                                const typename V4t<Real>::T *__restrict__ src,
@@ -671,8 +746,11 @@ __global__ __launch_bounds__(64) void k_spread(int Bn, long M1, long M2, long M3
                                                const typename V4t<Real>::T *__restrict__ frac,
                                                Real *__restrict__ grid)
 {
-  __shared__ Real w[64][3 * TS];
+  // per staged charge: w1[8], w2[8], w3[8] restricted to the tile, q, q_im
+  __shared__ __align__(16) Real w[64][3 * TS];
   __shared__ Real qs[64][2];
+  const int Bn = BN > 0 ? BN : BnRun;
+  constexpr int BA = BN > 0 ? BN : 16;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
   const int t3 = t % nt3, t2 = (t / nt3) % nt2, t1 = t / (nt3 * nt2);
   const int u1 = t1 * TS, u2 = t2 * TS, u3 = t3 * TS;
@@ -687,36 +765,52 @@ __global__ __launch_bounds__(64) void k_spread(int Bn, long M1, long M2, long M3
       const int s = tileList[base + threadIdx.x];
       const int4 m0 = ms[s];
       const typename V4t<Real>::T f = frac[s];
-      Real sh[16];
+      Real sh[BA];
       const int mstart[3] = {m0.x, m0.y, m0.z};
       const Real fr[3] = {f.x, f.y, f.z};
       const int ub[3] = {u1, u2, u3};
+#pragma unroll
       for (int d = 0; d < 3; d++) {
-        bsp_shifts<Real>(Bn, fr[d], sh);
+        if constexpr (BN > 0) bsp_shifts_c<Real, BN>(fr[d], sh);
+        else bsp_shifts<Real>(Bn, fr[d], sh);
+#pragma unroll
         for (int k = 0; k < TS; k++) {
           const int idx = ub[d] + k - mstart[d];
-          w[threadIdx.x][d * TS + k] = (idx >= 0 && idx < Bn) ? sh[idx] : (Real)0;
+          Real v = (Real)0;
+#pragma unroll
+          for (int j = 0; j < BA; j++)
+            if (j == idx && j < Bn) v = sh[j];
+          w[threadIdx.x][d * TS + k] = v;
         }
       }
       qs[threadIdx.x][0] = src[s].w;
       qs[threadIdx.x][1] = CX ? qi[s] : (Real)0;
     }
     __syncthreads();
+    // FP32 (or FP64) sums over the chunk, carried into FP64 per chunk
+    Real part[2 * TS];
+#pragma unroll
+    for (int k = 0; k < 2 * TS; k++) part[k] = 0;
     for (int c = 0; c < cnt; c++) {
       const Real w12 = w[c][li] * w[c][TS + lj];
+      Real w3[TS];
+#pragma unroll
+      for (int k = 0; k < TS; k++) w3[k] = w[c][2 * TS + k];
       const Real cr = qs[c][0] * w12;
       if (CX) {
         const Real ci = qs[c][1] * w12;
 #pragma unroll
         for (int k = 0; k < TS; k++) {
-          acc[2 * k] += (double)(cr * w[c][2 * TS + k]);
-          acc[2 * k + 1] += (double)(ci * w[c][2 * TS + k]);
+          part[2 * k] += cr * w3[k];
+          part[2 * k + 1] += ci * w3[k];
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < TS; k++) acc[k] += (double)(cr * w[c][2 * TS + k]);
+        for (int k = 0; k < TS; k++) part[k] += cr * w3[k];
       }
     }
+#pragma unroll
+    for (int k = 0; k < (CX ? 2 * TS : TS); k++) acc[k] += (double)part[k];
     __syncthreads();
   }
   const long p1 = u1 + li, p2 = u2 + lj;
@@ -743,10 +837,20 @@ void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int
 {
   (void)T;
   const dim3 g((unsigned)(nt[0] * nt[1] * nt[2]));
-#define NPME_SPREAD(R, CXV)                                                                     \
-  hipLaunchKernelGGL((k_spread<R, CXV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2], nt[0],   \
-                     nt[1], nt[2], tileStart, tileList, (const typename V4t<R>::T *)src,       \
+#define NPME_SPREAD_B(R, CXV, BNV)                                                              \
+  hipLaunchKernelGGL((k_spread<R, CXV, BNV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2],     \
+                     nt[0], nt[1], nt[2], tileStart, tileList, (const typename V4t<R>::T *)src, \
                      (const R *)qi, ms, (const typename V4t<R>::T *)frac, (R *)grid)
+#define NPME_SPREAD(R, CXV)                                                                     \
+  do {                                                                                          \
+    switch (Bn) {                                                                               \
+      case 4: NPME_SPREAD_B(R, CXV, 4); break;                                                  \
+      case 6: NPME_SPREAD_B(R, CXV, 6); break;                                                  \
+      case 8: NPME_SPREAD_B(R, CXV, 8); break;                                                  \
+      case 10: NPME_SPREAD_B(R, CXV, 10); break;                                                \
+      default: NPME_SPREAD_B(R, CXV, 0); break;                                                 \
+    }                                                                                           \
+  } while (0)
   if (fp64) {
     if (complexK) NPME_SPREAD(double, true);
     else NPME_SPREAD(double, false);
@@ -755,95 +859,153 @@ void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int
     else NPME_SPREAD(float, false);
   }
 #undef NPME_SPREAD
+#undef NPME_SPREAD_B
   NPME_HIP_CHECK(hipGetLastError());
 }
 
 // ------------------------------------------------------------------ interp
-template <typename Real, bool CX>
-__global__ __launch_bounds__(256) void k_interp(int Bn, long M1, long M2, long M3, double C1,
-                                                double C2, double C3, size_t n,
-                                                const int4 *__restrict__ ms,
-                                                const typename V4t<Real>::T *__restrict__ frac,
-                                                const Real *__restrict__ grid,
-                                                const double *__restrict__ Vdir,
-                                                const int *__restrict__ perm,
-                                                const double *__restrict__ q, double fsr,
-                                                double fsi, double *__restrict__ Vout)
+// B-spline interpolation of theta, one workgroup per T^3 tile of the compact
+// grid: the tile plus its (Bn-1)-point spline halo is staged in LDS once,
+// then every charge whose spline window starts in the tile (listed per tile
+// at set-up) is interpolated from LDS, combined with its direct sum and
+// self term, and written to its input position in FP64.
+template <typename Real, bool CX, int BN>
+__global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, long M2, long M3,
+                                                     int nt1, int nt2, int nt3, double C1,
+                                                     double C2, double C3,
+                                                     const int *__restrict__ tileStart,
+                                                     const int *__restrict__ tileList,
+                                                     const int4 *__restrict__ ms,
+                                                     const typename V4t<Real>::T *__restrict__ frac,
+                                                     const Real *__restrict__ grid,
+                                                     const double *__restrict__ Vdir,
+                                                     const int *__restrict__ perm,
+                                                     const double *__restrict__ q, double fsr,
+                                                     double fsi, double *__restrict__ Vout)
 {
-  const size_t s = (size_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int4 m0 = ms[s];
-  const typename V4t<Real>::T f = frac[s];
-  Real B[3][16], dB[3][16], sh[16];
-  const Real fr[3] = {f.x, f.y, f.z};
-  for (int d = 0; d < 3; d++) {
-    bsp_shifts<Real>(Bn, fr[d], B[d]);
-    bsp_shifts<Real>(Bn - 1, fr[d], sh);
-    for (int j = 0; j < Bn; j++) dB[d][j] = ((j < Bn - 1) ? sh[j] : (Real)0) - ((j >= 1) ? sh[j - 1] : (Real)0);
+  extern __shared__ char smem_raw[];
+  Real *th = reinterpret_cast<Real *>(smem_raw);
+  // BN > 0: spline order fixed at compile time (arrays stay in registers);
+  // BN == 0: generic order
+  const int Bn = BN > 0 ? BN : BnRun;
+  constexpr int BA = BN > 0 ? BN : 16;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int beg = tileStart[t], end = tileStart[t + 1];
+  if (beg == end) return;  // uniform per workgroup
+  const int t3 = t % nt3, t2 = (t / nt3) % nt2, t1 = t / (nt3 * nt2);
+  const int u1 = t1 * T, u2 = t2 * T, u3 = t3 * T;
+  const int E = T + Bn - 1;
+  const int e1 = (int)min((long)E, M1 - u1), e2 = (int)min((long)E, M2 - u2),
+            e3 = (int)min((long)E, M3 - u3);
+  constexpr int W = CX ? 2 : 1;
+  for (int idx = threadIdx.x; idx < e1 * e2 * e3; idx += blockDim.x) {
+    const int k = idx % e3, j = (idx / e3) % e2, i = idx / (e3 * e2);
+    const long g = (((long)(u1 + i) * M2) + (u2 + j)) * M3 + (u3 + k);
+    const int l = (i * E + j) * E + k;
+#pragma unroll
+    for (int w = 0; w < W; w++) th[W * l + w] = grid[W * g + w];
   }
-  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int j1 = 0; j1 < Bn; j1++) {
-    for (int j2 = 0; j2 < Bn; j2++) {
-      const long base = (((long)m0.x + j1) * M2 + (m0.y + j2)) * M3 + m0.z;
-      Real tb = 0, tdb = 0, tbi = 0, tdbi = 0;
-      for (int j3 = 0; j3 < Bn; j3++) {
-        if (CX) {
-          const Real tr = grid[2 * (base + j3)], ti = grid[2 * (base + j3) + 1];
-          tb += tr * B[2][j3];
-          tdb += tr * dB[2][j3];
-          tbi += ti * B[2][j3];
-          tdbi += ti * dB[2][j3];
-        } else {
-          const Real tv = grid[base + j3];
+  __syncthreads();
+  for (int c = beg + threadIdx.x; c < end; c += blockDim.x) {
+    const int s = tileList[c];
+    const int4 m0 = ms[s];
+    const typename V4t<Real>::T f = frac[s];
+    Real B[3][BA], dB[3][BA], sh[BA];
+    const Real fr[3] = {f.x, f.y, f.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      if constexpr (BN > 0) {
+        bsp_shifts_c<Real, BN>(fr[d], B[d]);
+        bsp_shifts_c<Real, BN - 1>(fr[d], sh);
+      } else {
+        bsp_shifts<Real>(Bn, fr[d], B[d]);
+        bsp_shifts<Real>(Bn - 1, fr[d], sh);
+      }
+#pragma unroll
+      for (int j = 0; j < BA; j++)
+        if (j < Bn) dB[d][j] = ((j < Bn - 1) ? sh[j] : (Real)0) - ((j >= 1) ? sh[j - 1] : (Real)0);
+    }
+    const int l1 = m0.x - u1, l2 = m0.y - u2, l3 = m0.z - u3;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int j1 = 0; j1 < BA; j1++) {
+      if (j1 >= Bn) break;
+#pragma unroll
+      for (int j2 = 0; j2 < BA; j2++) {
+        if (j2 >= Bn) break;
+        const Real *row = th + W * (((l1 + j1) * E + (l2 + j2)) * E + l3);
+        Real tb = 0, tdb = 0, tbi = 0, tdbi = 0;
+#pragma unroll
+        for (int j3 = 0; j3 < BA; j3++) {
+          if (j3 >= Bn) break;
+          const Real tv = row[W * j3];
           tb += tv * B[2][j3];
           tdb += tv * dB[2][j3];
+          if (CX) {
+            const Real ti = row[W * j3 + 1];
+            tbi += ti * B[2][j3];
+            tdbi += ti * dB[2][j3];
+          }
+        }
+        const double A1 = (double)(B[0][j1] * B[1][j2]);
+        const double A2 = (double)(dB[0][j1] * B[1][j2]);
+        const double A3 = (double)(B[0][j1] * dB[1][j2]);
+        v[0] += A1 * tb;
+        v[2] += A2 * tb;
+        v[4] += A3 * tb;
+        v[6] += A1 * tdb;
+        if (CX) {
+          v[1] += A1 * tbi;
+          v[3] += A2 * tbi;
+          v[5] += A3 * tbi;
+          v[7] += A1 * tdbi;
         }
       }
-      const double A1 = (double)(B[0][j1] * B[1][j2]);
-      const double A2 = (double)(dB[0][j1] * B[1][j2]);
-      const double A3 = (double)(B[0][j1] * dB[1][j2]);
-      v[0] += A1 * tb;
-      v[2] += A2 * tb;
-      v[4] += A3 * tb;
-      v[6] += A1 * tdb;
-      if (CX) {
-        v[1] += A1 * tbi;
-        v[3] += A2 * tbi;
-        v[5] += A3 * tbi;
-        v[7] += A1 * tdbi;
+    }
+    const int o = perm[s];
+    if (CX) {
+      const double qr = q[2 * (size_t)o], qim = q[2 * (size_t)o + 1];
+      const double Cc[4] = {1.0, C1, C2, C3};
+      double out[8];
+      for (int k = 0; k < 4; k++) {
+        out[2 * k] = v[2 * k] * Cc[k] + Vdir[8 * (size_t)s + 2 * k];
+        out[2 * k + 1] = v[2 * k + 1] * Cc[k] + Vdir[8 * (size_t)s + 2 * k + 1];
       }
+      out[0] -= fsr * qr - fsi * qim;
+      out[1] -= fsr * qim + fsi * qr;
+      for (int k = 0; k < 8; k++) Vout[8 * (size_t)o + k] = out[k];
+    } else {
+      Vout[4 * (size_t)o + 0] = v[0] + Vdir[4 * (size_t)s + 0] - fsr * q[o];
+      Vout[4 * (size_t)o + 1] = v[2] * C1 + Vdir[4 * (size_t)s + 1];
+      Vout[4 * (size_t)o + 2] = v[4] * C2 + Vdir[4 * (size_t)s + 2];
+      Vout[4 * (size_t)o + 3] = v[6] * C3 + Vdir[4 * (size_t)s + 3];
     }
-  }
-  const int o = perm[s];
-  if (CX) {
-    const double qr = q[2 * (size_t)o], qim = q[2 * (size_t)o + 1];
-    const double C[4] = {1.0, C1, C2, C3};
-    double out[8];
-    for (int k = 0; k < 4; k++) {
-      out[2 * k] = v[2 * k] * C[k] + Vdir[8 * s + 2 * k];
-      out[2 * k + 1] = v[2 * k + 1] * C[k] + Vdir[8 * s + 2 * k + 1];
-    }
-    out[0] -= fsr * qr - fsi * qim;
-    out[1] -= fsr * qim + fsi * qr;
-    for (int k = 0; k < 8; k++) Vout[8 * (size_t)o + k] = out[k];
-  } else {
-    Vout[4 * (size_t)o + 0] = v[0] + Vdir[4 * s + 0] - fsr * q[o];
-    Vout[4 * (size_t)o + 1] = v[2] * C1 + Vdir[4 * s + 1];
-    Vout[4 * (size_t)o + 2] = v[4] * C2 + Vdir[4 * s + 2];
-    Vout[4 * (size_t)o + 3] = v[6] * C3 + Vdir[4 * s + 3];
   }
 }
 
-void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3],
-                   size_t n, const int4 *ms, const void *frac, const void *grid,
-                   const double *Vdir, const int *perm, const double *q_in, const double fself[2],
-                   double *Vout, hipStream_t s)
+void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3], int T,
+                   const int nt[3], const int *tileStart, const int *tileList, const int4 *ms,
+                   const void *frac, const void *grid, const double *Vdir, const int *perm,
+                   const double *q_in, const double fself[2], double *Vout, hipStream_t s)
 {
-  const dim3 g((unsigned)((n + 255) / 256));
-#define NPME_INTERP(R, CXV)                                                                    \
-  hipLaunchKernelGGL((k_interp<R, CXV>), g, dim3(256), 0, s, (int)Bn, M[0], M[1], M[2], C[0],  \
-                     C[1], C[2], n, ms, (const typename V4t<R>::T *)frac, (const R *)grid,     \
-                     Vdir, perm, q_in, fself[0], fself[1], Vout)
+  const dim3 g((unsigned)(nt[0] * nt[1] * nt[2]));
+  const int E = T + (int)Bn - 1;
+  const size_t sh = (size_t)E * E * E * (fp64 ? 8 : 4) * (complexK ? 2 : 1);
+#define NPME_INTERP_B(R, CXV, BNV)                                                              \
+  hipLaunchKernelGGL((k_interp_tile<R, CXV, BNV>), g, dim3(256), sh, s, (int)Bn, T, M[0], M[1],   \
+                     M[2], nt[0], nt[1], nt[2], C[0], C[1], C[2], tileStart, tileList, ms,       \
+                     (const typename V4t<R>::T *)frac, (const R *)grid, Vdir, perm, q_in,        \
+                     fself[0], fself[1], Vout)
+#define NPME_INTERP(R, CXV)                                                                      \
+  do {                                                                                           \
+    switch (Bn) {                                                                                \
+      case 4: NPME_INTERP_B(R, CXV, 4); break;                                                   \
+      case 6: NPME_INTERP_B(R, CXV, 6); break;                                                   \
+      case 8: NPME_INTERP_B(R, CXV, 8); break;                                                   \
+      case 10: NPME_INTERP_B(R, CXV, 10); break;                                                 \
+      default: NPME_INTERP_B(R, CXV, 0); break;                                                  \
+    }                                                                                            \
+  } while (0)
   if (fp64) {
     if (complexK) NPME_INTERP(double, true);
     else NPME_INTERP(double, false);
@@ -852,6 +1014,7 @@ void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const dou
     else NPME_INTERP(float, false);
   }
 #undef NPME_INTERP
+#undef NPME_INTERP_B
   NPME_HIP_CHECK(hipGetLastError());
 }
 

@@ -29,29 +29,30 @@ static void kw_error(const std::string &m) { throw Error{NPME_ERR_KEYWORD, m}; }
 
 template <typename T> static T *dalloc(Handle &h, size_t count)
 {
-  void *p = nullptr;
   const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) {
+  void *p = dev_alloc(bytes);
+  if (!p) {
     std::ostringstream os;
-    os << "device allocation of " << bytes << " bytes failed: " << hipGetErrorString(e);
+    os << "device allocation of " << bytes << " bytes failed";
     throw Error{NPME_ERR_NOMEM, os.str()};
   }
-  h.deviceBytes += bytes;
+  h.deviceBytes += dev_block_bytes(bytes);
   return (T *)p;
 }
 
 void handle_free_device(Handle &h)
 {
-  void *ptrs[] = {h.d_perm, h.d_src, h.d_srcLo, h.d_blockBox, h.d_segStart, h.d_nlist,
+  void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
                   h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
+  // queued work of this handle may still read its buffers
+  if (h.stream) (void)hipStreamSynchronize(h.stream);
+  for (void *p : ptrs) dev_free(p);
   h.d_perm = nullptr;
-  h.d_src = h.d_qi = h.d_srcLo = h.d_blockBox = nullptr;
+  h.d_src = h.d_srcRes = h.d_qi = h.d_blockBox = nullptr;
   h.d_segStart = nullptr;
+  h.d_itileStart = h.d_itileList = nullptr;
   h.d_nlist = nullptr;
   h.d_sliceStart = nullptr;
   h.d_sliceLen = nullptr;
@@ -139,26 +140,18 @@ static double tsmooth(double x, double X, double Xmax, double a)
 // the entry code (nNeigh <= nCellClust1D) and the list fits a quarter of the
 // free device memory ('on' lifts the memory limit); otherwise the direct
 // sum walks the cell rows of each block (k_direct).
-static void build_neighbor_list(Handle &h, const Keywords &kw)
+static void build_neighbor_list(Handle &h, const std::string &mode)
 {
-  std::string mode = "auto";
-  kw.get("neighborList", mode);
-  if (mode == "off" || h.nNeigh > h.nCellClust1D || h.n >= (1u << 27)) return;
   const DirectParams dp = direct_params(h);
   const int nP = (int)h.nCellClust1D;
-  int *d_count = nullptr;
-  NPME_HIP_CHECK(hipMalloc(&d_count, h.n * sizeof(int)));
   std::vector<int> cnt(h.n);
-  try {
-    launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP, d_count,
-                 nullptr, nullptr, h.stream);
-    NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
+  {
+    DevBuf d_count(h.n * sizeof(int));
+    launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP,
+                 (int *)d_count.p, nullptr, nullptr, h.stream);
+    NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count.p, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
     NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  } catch (...) {
-    (void)hipFree(d_count);
-    throw;
   }
-  (void)hipFree(d_count);
   const size_t nSlices = (h.n + 63) / 64;
   std::vector<long> sliceStart(nSlices + 1, 0);
   std::vector<int> sliceLen(nSlices, 0);
@@ -183,6 +176,412 @@ static void build_neighbor_list(Handle &h, const Keywords &kw)
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   h.nlistEntries = entries;
   h.useNlist = true;
+}
+
+// ---------------------------------------------------------------- per-charge set-up
+static void *adopt(Handle &h, DevBuf &b)
+{
+  h.deviceBytes += dev_block_bytes(b.bytes);
+  return b.release();
+}
+
+static int fixed_shift(int maxOff)
+{
+  // positions lie in [0, 2^S] units of their cluster and offsets reach
+  // maxOff clusters, so every pair difference the kernels form stays
+  // within (maxOff + 1) 2^S <= 2^30
+  int S = 30;
+  while (((long)maxOff + 1) << S > (1L << 30)) S--;
+  if (S < 16) throw Error{NPME_ERR_UNSUPPORTED, "direct-sum cluster offsets too large"};
+  return S;
+}
+
+// Device set-up (geometry.hip); returns false when the neighbour list does
+// not fit the memory budget (the host set-up then builds the cell-row lists)
+static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
+                             const double *coord)
+{
+  const size_t n = h.n;
+  hipStream_t s = h.stream;
+  const bool f64 = h.fp64;
+  const size_t rs = f64 ? 8 : 4;
+  const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
+  h.d_coordS = dalloc<double>(h, 3 * n);
+  NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+  h.d_perm = dalloc<int>(h, n);
+  h.d_cell = dalloc<int4>(h, n);
+  h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
+  geometry_cells_device(g, n, h.d_coordS, h.d_perm, h.d_cell, h.d_cellStart, s);
+
+  // neighbour list: count, slice lengths and offsets, then fill
+  const DirectParams dp = direct_params(h);
+  const size_t nSlices = (n + 63) / 64;
+  size_t entries = 0;
+  h.d_sliceStart = dalloc<long>(h, nSlices + 1);
+  h.d_sliceLen = dalloc<int>(h, nSlices);
+  {
+    DevBuf cnt(n * sizeof(int));
+    launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, (int)h.nCellClust1D,
+                 (int *)cnt.p, nullptr, nullptr, s);
+    entries = nlist_slices_device(n, (const int *)cnt.p, h.d_sliceLen, h.d_sliceStart, s);
+  }
+  size_t freeB = 0, totalB = 0;
+  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  if (nlmode != "on" && entries * sizeof(unsigned) > freeB / 4) return false;
+  h.d_nlist = dalloc<unsigned>(h, entries);
+  NPME_HIP_CHECK(hipMemsetAsync(h.d_nlist, 0xff, std::max<size_t>(entries, 1) * sizeof(unsigned), s));
+  launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, (int)h.nCellClust1D,
+               nullptr, h.d_sliceStart, h.d_nlist, s);
+  h.nlistEntries = entries;
+  h.useNlist = true;
+
+  // positions, spline windows and tile lists
+  h.fxShift = fixed_shift(1);
+  g.fxShift = h.fxShift;
+  g.fxScale = std::ldexp(1.0, h.fxShift) / h.csA;
+  h.d_src = dalloc<char>(h, 4 * rs * n);
+  if (!f64) h.d_srcRes = dalloc<float4>(h, n);
+  h.d_frac = dalloc<char>(h, 4 * rs * n);
+  h.d_ms = dalloc<int4>(h, n);
+  DevBuf ts, tl, its, itl;
+  size_t m = 0;
+  geometry_charges_device(g, f64, n, h.d_coordS, h.d_perm, h.d_cell, h.d_src, h.d_srcRes, h.d_ms,
+                          h.d_frac, ts, tl, its, itl, m, s);
+  h.d_tileStart = (int *)adopt(h, ts);
+  h.d_tileList = (int *)adopt(h, tl);
+  h.d_itileStart = (int *)adopt(h, its);
+  h.d_itileList = (int *)adopt(h, itl);
+  h.nSegments = 0;
+  const int vw = h.complexK ? 8 : 4;
+  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
+  h.d_Vdir = dalloc<double>(h, vw * n);
+  h.d_V = dalloc<double>(h, vw * n);
+  h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  return true;
+}
+
+// Host set-up: the same arrays built on the host (FP64, stable sort), plus
+// the per-block source segments of the cell-row direct sum
+static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, bool nlPossible,
+                           const double *coord)
+{
+  const size_t n = h.n;
+  const unsigned long nP = g.nP;
+  const double csA = g.csA, csB = g.csB;
+  const double *mn = g.mn;
+  const unsigned long *nA = g.nA;
+  const double Rdir = h.ker.Rdir;
+  const unsigned long Nc = nP * nP * nP;
+  std::vector<unsigned long> key(n);
+  std::vector<int> gcell(3 * n);
+  for (size_t i = 0; i < n; i++) {
+    unsigned long a[3], b[3];
+    for (int d = 0; d < 3; d++) {
+      const double x1 = coord[3 * i + d] + (-mn[d]);
+      a[d] = coord2cell(x1, csA, nA[d]);
+      const double x2 = x1 - a[d] * csA;
+      b[d] = coord2cell(x2, csB, nP);
+      gcell[3 * i + d] = (int)(a[d] * nP + b[d]);
+    }
+    const unsigned long tA = (a[0] * nA[1] + a[1]) * nA[2] + a[2];
+    key[i] = tA * Nc + (b[0] * nP + b[1]) * nP + b[2];
+  }
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+
+  const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
+  // (start, count) of every cell of the dense cell grid in the sorted order;
+  // an empty cell starts where its key would be, so a run of consecutive
+  // cells of one cluster row is the range [start(first), start(last)+count)
+  std::vector<unsigned long> skey(n);
+  for (size_t s = 0; s < n; s++) skey[s] = key[perm[s]];
+  std::vector<int> cellSE(2 * (size_t)ncell, 0);
+  for (int g0 = 0; g0 < h.nb[0]; g0++)
+    for (int g1 = 0; g1 < h.nb[1]; g1++)
+      for (int g2 = 0; g2 < h.nb[2]; g2++) {
+        const unsigned long a0 = g0 / nP, a1 = g1 / nP, a2 = g2 / nP;
+        const unsigned long kc = ((a0 * nA[1] + a1) * nA[2] + a2) * Nc +
+                                 ((g0 % nP) * nP + (g1 % nP)) * nP + (g2 % nP);
+        auto lo = std::lower_bound(skey.begin(), skey.end(), kc);
+        auto hi = std::upper_bound(lo, skey.end(), kc);
+        const long c = ((long)g0 * h.nb[1] + g1) * h.nb[2] + g2;
+        cellSE[2 * c] = (int)(lo - skey.begin());
+        cellSE[2 * c + 1] = (int)(hi - lo);
+      }
+  std::vector<int4> cellOf(n);
+  std::vector<double> loc(4 * n);
+  for (size_t s = 0; s < n; s++) {
+    const int i = perm[s];
+    int gc[3];
+    for (int d = 0; d < 3; d++) {
+      gc[d] = gcell[3 * i + d];
+      // position relative to the low corner of the charge's cluster, as the
+      // reference forms it (x2 = x1 - n cellSizeA)
+      loc[4 * s + d] = (coord[3 * i + d] + (-mn[d])) - (double)(gc[d] / (int)nP) * csA;
+    }
+    loc[4 * s + 3] = 0;
+    cellOf[s] = make_int4(gc[0], gc[1], gc[2], 0);
+  }
+  // source segments of every block of 8 consecutive sorted targets: rows of
+  // cells within Rdir (slightly inflated, the kernel applies the exact
+  // cutoff) of the box spanned by the block's positions, split per cluster
+  const long nBlocks = ((long)n + 7) / 8;
+  std::vector<int> segStart(nBlocks + 1, 0);
+  int maxOff = 1;  // the neighbour list only reaches the adjacent clusters
+  std::vector<int4> segs;
+  segs.reserve((size_t)nBlocks * 48);
+  {
+    const double Rm = Rdir * (1.0 + 1.0e-9) + 1.0e-12;
+    for (long b = 0; b < nBlocks; b++) {
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      int alo[3] = {1 << 30, 1 << 30, 1 << 30}, ahi[3] = {-1, -1, -1};
+      for (size_t s = 8 * b; s < std::min(n, (size_t)(8 * b + 8)); s++) {
+        const int i = perm[s];
+        for (int d = 0; d < 3; d++) {
+          const double x = coord[3 * i + d] + (-mn[d]);
+          lo[d] = std::min(lo[d], x);
+          hi[d] = std::max(hi[d], x);
+          alo[d] = std::min(alo[d], gcell[3 * i + d] / (int)nP);
+          ahi[d] = std::max(ahi[d], gcell[3 * i + d] / (int)nP);
+        }
+      }
+      // largest cluster offset between a target of the block and a segment
+      auto note_off = [&](int d, int a) {
+        maxOff = std::max(maxOff, std::max(std::abs(a - alo[d]), std::abs(a - ahi[d])));
+      };
+      const int gx0 = std::max(0, (int)std::floor((lo[0] - Rm) / csB));
+      const int gx1 = std::min(h.nb[0] - 1, (int)std::floor((hi[0] + Rm) / csB));
+      const int gy0 = std::max(0, (int)std::floor((lo[1] - Rm) / csB));
+      const int gy1 = std::min(h.nb[1] - 1, (int)std::floor((hi[1] + Rm) / csB));
+      for (int gx = gx0; gx <= gx1; gx++) {
+        const double gapx = std::max(0.0, std::max(gx * csB - hi[0], lo[0] - (gx + 1) * csB));
+        if (gapx >= Rm) continue;
+        for (int gy = gy0; gy <= gy1; gy++) {
+          const double gapy = std::max(0.0, std::max(gy * csB - hi[1], lo[1] - (gy + 1) * csB));
+          const double rem = Rm * Rm - gapx * gapx - gapy * gapy;
+          if (rem <= 0) continue;
+          const double sr = std::sqrt(rem);
+          const int zlo = std::max(0, (int)std::floor((lo[2] - sr) / csB));
+          const int zhi = std::min(h.nb[2] - 1, (int)std::floor((hi[2] + sr) / csB));
+          const long row = ((long)gx * h.nb[1] + gy) * h.nb[2];
+          for (int az = zlo / (int)nP; az <= zhi / (int)nP; az++) {
+            const int z0 = std::max(zlo, az * (int)nP), z1 = std::min(zhi, az * (int)nP + (int)nP - 1);
+            const int sB = cellSE[2 * (row + z0)];
+            const int sE = cellSE[2 * (row + z1)] + cellSE[2 * (row + z1) + 1];
+            if (sE > sB) {
+              segs.push_back(make_int4(sB, sE, gx / (int)nP, ((gy / (int)nP) << 16) | az));
+              note_off(0, gx / (int)nP);
+              note_off(1, gy / (int)nP);
+              note_off(2, az);
+            }
+          }
+        }
+      }
+      if (segs.size() > 0x7fffffffUL) throw Error{NPME_ERR_NOMEM, "too many direct-sum segments"};
+      segStart[b + 1] = (int)segs.size();
+    }
+  }
+
+  // ---- spline windows (u = m + N/4) and per-tile lists
+  std::vector<int4> ms(n);
+  std::vector<double> frac(4 * n);
+  for (size_t s = 0; s < n; s++) {
+    const int i = perm[s];
+    int u0[3];
+    for (int d = 0; d < 3; d++) {
+      const double w = h.N[d] / h.L[d] * (coord[3 * i + d] - h.R[d]);
+      const long m0 = (w < 0) ? (long)(-w) + 1 : (long)(-w);
+      if (m0 < -h.N[d] / 4 || m0 + h.Bn - 1 > h.N[d] / 4 - 1)
+        throw Error{NPME_ERR_GRID, "spline window outside the compact grid"};
+      u0[d] = (int)(m0 + h.N[d] / 4);
+      frac[4 * s + d] = w + (double)m0;
+    }
+    frac[4 * s + 3] = 0;
+    ms[s] = make_int4(u0[0], u0[1], u0[2], 0);
+  }
+  const long ntiles = (long)h.nt[0] * h.nt[1] * h.nt[2];
+  std::vector<int> tcount(ntiles + 1, 0);
+  auto tile_range = [&](int u0, int &lo, int &hi) {
+    lo = u0 / 8;
+    hi = (u0 + (int)h.Bn - 1) / 8;
+  };
+  for (size_t s = 0; s < n; s++) {
+    int lo[3], hi[3];
+    tile_range(ms[s].x, lo[0], hi[0]);
+    tile_range(ms[s].y, lo[1], hi[1]);
+    tile_range(ms[s].z, lo[2], hi[2]);
+    for (int a = lo[0]; a <= hi[0]; a++)
+      for (int b = lo[1]; b <= hi[1]; b++)
+        for (int c = lo[2]; c <= hi[2]; c++) tcount[((long)a * h.nt[1] + b) * h.nt[2] + c + 1]++;
+  }
+  for (long t = 0; t < ntiles; t++) tcount[t + 1] += tcount[t];
+  std::vector<int> tlist(tcount[ntiles]);
+  {
+    std::vector<int> fill(tcount.begin(), tcount.end() - 1);
+    for (size_t s = 0; s < n; s++) {
+      int lo[3], hi[3];
+      tile_range(ms[s].x, lo[0], hi[0]);
+      tile_range(ms[s].y, lo[1], hi[1]);
+      tile_range(ms[s].z, lo[2], hi[2]);
+      for (int a = lo[0]; a <= hi[0]; a++)
+        for (int b = lo[1]; b <= hi[1]; b++)
+          for (int c = lo[2]; c <= hi[2]; c++) tlist[fill[((long)a * h.nt[1] + b) * h.nt[2] + c]++] = (int)s;
+    }
+  }
+  const long nitiles = (long)h.nti[0] * h.nti[1] * h.nti[2];
+  std::vector<int> icount(nitiles + 1, 0), ilist(n);
+  auto itile = [&](const int4 &u) {
+    return ((long)(u.x / h.TI) * h.nti[1] + u.y / h.TI) * h.nti[2] + u.z / h.TI;
+  };
+  for (size_t s = 0; s < n; s++) icount[itile(ms[s]) + 1]++;
+  for (long t = 0; t < nitiles; t++) icount[t + 1] += icount[t];
+  {
+    std::vector<int> fill(icount.begin(), icount.end() - 1);
+    for (size_t s = 0; s < n; s++) ilist[fill[itile(ms[s])]++] = (int)s;
+  }
+
+  // ---- uploads
+  const bool f64 = h.fp64;
+  const size_t rs = f64 ? 8 : 4;
+  auto up = [&](void *dst, const void *src, size_t bytes) {
+    NPME_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  };
+  h.d_perm = dalloc<int>(h, n);
+  up(h.d_perm, perm.data(), n * sizeof(int));
+  h.d_frac = dalloc<char>(h, 4 * rs * n);
+  if (f64) {
+    up(h.d_frac, frac.data(), 4 * n * sizeof(double));
+  } else {
+    std::vector<float> ff(frac.begin(), frac.end());
+    up(h.d_frac, ff.data(), 4 * n * sizeof(float));
+  }
+  h.d_segStart = dalloc<int>(h, segStart.size());
+  up(h.d_segStart, segStart.data(), segStart.size() * sizeof(int));
+  h.d_blockBox = dalloc<int4>(h, segs.size());
+  up(h.d_blockBox, segs.data(), segs.size() * sizeof(int4));
+  h.nSegments = segs.size();
+  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
+  h.d_cell = dalloc<int4>(h, n);
+  up(h.d_cell, cellOf.data(), n * sizeof(int4));
+  h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
+  up(h.d_cellStart, cellSE.data(), 2 * (size_t)ncell * sizeof(int));
+  h.d_ms = dalloc<int4>(h, n);
+  up(h.d_ms, ms.data(), n * sizeof(int4));
+  h.d_tileStart = dalloc<int>(h, ntiles + 1);
+  up(h.d_tileStart, tcount.data(), (ntiles + 1) * sizeof(int));
+  h.d_itileStart = dalloc<int>(h, nitiles + 1);
+  up(h.d_itileStart, icount.data(), (nitiles + 1) * sizeof(int));
+  h.d_itileList = dalloc<int>(h, n);
+  up(h.d_itileList, ilist.data(), n * sizeof(int));
+  h.d_tileList = dalloc<int>(h, tlist.size());
+  up(h.d_tileList, tlist.data(), tlist.size() * sizeof(int));
+  const int vw = h.complexK ? 8 : 4;
+  h.d_Vdir = dalloc<double>(h, vw * n);
+  h.d_V = dalloc<double>(h, vw * n);
+  h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
+  h.d_coordS = dalloc<double>(h, 3 * n);
+  up(h.d_coordS, coord, 3 * n * sizeof(double));
+
+  // ---- direct sum: the neighbour list when it fits, else the cell rows;
+  // the fixed-point scale depends on which of the two reads the positions
+  if (nlPossible) build_neighbor_list(h, nlmode);
+  h.fxShift = h.useNlist ? fixed_shift(1) : fixed_shift(maxOff);
+  h.d_src = dalloc<char>(h, 4 * rs * n);
+  if (f64) {
+    up(h.d_src, loc.data(), 4 * n * sizeof(double));
+  } else {
+    // int32 fixed point relative to the cluster corner (2^S units per
+    // cluster edge) in the x, y, z words; the w word receives the charge
+    // and the sub-unit residual of each coordinate
+    std::vector<int32_t> fx(4 * n, 0);
+    std::vector<float> fr(4 * n, 0.0f);
+    const double scale = std::ldexp(1.0, h.fxShift) / csA;
+    for (size_t s = 0; s < n; s++)
+      for (int d = 0; d < 3; d++) {
+        const double u = loc[4 * s + d] * scale;
+        const long v = std::lround(u);
+        if (v < -(1L << 20) || v > (1L << h.fxShift) + (1L << 20))
+          throw Error{NPME_ERR_ARG, "position outside its cluster"};
+        fx[4 * s + d] = (int32_t)v;
+        fr[4 * s + d] = (float)(u - (double)v);
+      }
+    up(h.d_src, fx.data(), 4 * n * sizeof(int32_t));
+    h.d_srcRes = dalloc<float4>(h, n);
+    up(h.d_srcRes, fr.data(), 4 * n * sizeof(float));
+  }
+}
+
+// FFT buffers, twiddles and the kernel multiplier G = F lam^2 / (N1 N2 N3)
+// (FP64 build, stored in the working precision)
+static void grid_setup(Handle &h)
+{
+  const bool f64 = h.fp64;
+  const size_t rs = f64 ? 8 : 4;
+  const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2];
+  const size_t cs = 2 * rs;  // complex element size
+  h.d_grid = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)(M1 * M2 * M3));
+  h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.K3));
+  h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.K3));
+  for (int d = 0; d < 3; d++) {
+    h.plan[d] = make_plan((int)h.N[d]);
+    h.d_tw[d] = upload_twiddles(h, (int)h.N[d], f64);
+  }
+
+  // kernel grid: real-space compact grid of f_LR with the smooth Fourier
+  // extension, then an even-extension DFT along each axis, all FP64
+  const long Mf[3] = {h.N[0] / 2 + 1, h.N[1] / 2 + 1, h.N[2] / 2 + 1};
+  const long totF = Mf[0] * Mf[1] * Mf[2];
+  std::vector<double> tab;
+  for (int d = 0; d < 3; d++) {
+    const double Xmax = h.X[d] + h.del[d], Xt = 2 * Xmax - h.X[d];
+    const double dx = 2 * Xmax / h.N[d];
+    std::vector<double> x1(Mf[d]), x2(Mf[d]), T1(Mf[d]), T2(Mf[d]);
+    for (long i = 0; i < Mf[d]; i++) {
+      x1[i] = i * dx;
+      const double eps = 1.0e-12 * Xmax;
+      if (x1[i] > Xmax + eps) throw Error{NPME_ERR_GRID, "kernel grid beyond the extension"};
+      if (x1[i] > h.X[d]) {
+        T1[i] = tsmooth(x1[i], h.X[d], Xt, h.ta[d]);
+        x2[i] = x1[i] - 2 * Xmax;
+        T2[i] = tsmooth(x2[i], h.X[d], Xt, h.ta[d]);
+      } else {
+        T1[i] = 1.0;
+        x2[i] = 1.0;
+        T2[i] = 0.0;
+      }
+    }
+    tab.insert(tab.end(), x1.begin(), x1.end());
+    tab.insert(tab.end(), x2.begin(), x2.end());
+    tab.insert(tab.end(), T1.begin(), T1.end());
+    tab.insert(tab.end(), T2.begin(), T2.end());
+  }
+  DevBuf d_tab(tab.size() * sizeof(double)), d_F(2 * totF * sizeof(double));
+  NPME_HIP_CHECK(hipMemcpy(d_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  launch_fgrid_real_space(h.ker, Mf, (const double *)d_tab.p, (double *)d_F.p, h.stream);
+  DevBuf tw64[3];
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> t = twiddles((int)h.N[d]);
+    tw64[d].reset(t.size() * sizeof(double));
+    NPME_HIP_CHECK(hipMemcpy(tw64[d].p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  // even-extension DFT along axis 3, 2, 1 of F[Mf0][Mf1][Mf2]
+  double *F = (double *)d_F.p;
+  fft_pass_even((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], tw64[2].p, F, h.stream);
+  fft_pass_even((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], tw64[1].p, F, h.stream);
+  fft_pass_even((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0].p, F, h.stream);
+  std::vector<double> lam;
+  for (int d = 0; d < 3; d++) {
+    std::vector<double> l(h.N[d]);
+    lambda2(h.N[d], h.Bn, l.data());
+    lam.insert(lam.end(), l.begin(), l.begin() + Mf[d]);
+  }
+  DevBuf d_lam(lam.size() * sizeof(double));
+  NPME_HIP_CHECK(hipMemcpy(d_lam.p, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+  h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)totF);
+  launch_make_G(f64, h.complexK, Mf, h.K3, F, (const double *)d_lam.p, h.d_G, h.stream);
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before the temporaries return to the pool
 }
 
 // ---------------------------------------------------------------- set-up
@@ -264,7 +663,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   pme_corr_box(h.X0, h.R0, h.N, h.del, h.Bn, h.X, h.R);
   for (int d = 0; d < 3; d++) h.L[d] = 2 * (h.X[d] + h.del[d]);
 
-  // ---- embedded cell partition and stable sort by embedded cell index
+  // ---- embedded cell partition: cells of Rdir/nNeigh inside clusters of
+  // nCellClust1D^3 cells, over the bounding box of the charges
+  GeoParams g{};
   const double csB = Rdir / h.nNeigh;
   const unsigned long nP = (unsigned long)h.nCellClust1D;
   const double csA = nP * csB;
@@ -275,284 +676,50 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
       mn[d] = std::min(mn[d], coord[3 * i + d]);
       mx[d] = std::max(mx[d], coord[3 * i + d]);
     }
-  unsigned long nA[3];
   for (int d = 0; d < 3; d++) {
-    nA[d] = (unsigned long)((mx[d] - mn[d]) / csA) + 1;
-    h.nb[d] = (int)(nA[d] * nP);
-    h.boxMin[d] = mn[d];
+    g.nA[d] = (unsigned long)((mx[d] - mn[d]) / csA) + 1;
+    h.nb[d] = (int)(g.nA[d] * nP);
+    h.boxMin[d] = g.mn[d] = mn[d];
+    g.nb[d] = h.nb[d];
+    g.NoverL[d] = h.N[d] / h.L[d];
+    g.R[d] = h.R[d];
+    g.N[d] = h.N[d];
   }
-  h.csB = csB;
-  const unsigned long Nc = nP * nP * nP;
-  std::vector<unsigned long> key(n);
-  std::vector<int> gcell(3 * n);
-  for (size_t i = 0; i < n; i++) {
-    unsigned long a[3], b[3];
-    for (int d = 0; d < 3; d++) {
-      const double x1 = coord[3 * i + d] + (-mn[d]);
-      a[d] = coord2cell(x1, csA, nA[d]);
-      const double x2 = x1 - a[d] * csA;
-      b[d] = coord2cell(x2, csB, nP);
-      gcell[3 * i + d] = (int)(a[d] * nP + b[d]);
-    }
-    const unsigned long tA = (a[0] * nA[1] + a[1]) * nA[2] + a[2];
-    key[i] = tA * Nc + (b[0] * nP + b[1]) * nP + b[2];
-  }
-  h.perm.resize(n);
-  std::iota(h.perm.begin(), h.perm.end(), 0);
-  std::stable_sort(h.perm.begin(), h.perm.end(), [&](int x, int y) { return key[x] < key[y]; });
-
-  const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
-  // (start, count) of every cell of the dense cell grid in the sorted order;
-  // an empty cell starts where its key would be, so a run of consecutive
-  // cells of one cluster row is the range [start(first), start(last)+count)
-  std::vector<unsigned long> skey(n);
-  for (size_t s = 0; s < n; s++) skey[s] = key[h.perm[s]];
-  std::vector<int> cellSE(2 * (size_t)ncell, 0);
-  for (int g0 = 0; g0 < h.nb[0]; g0++)
-    for (int g1 = 0; g1 < h.nb[1]; g1++)
-      for (int g2 = 0; g2 < h.nb[2]; g2++) {
-        const unsigned long a0 = g0 / nP, a1 = g1 / nP, a2 = g2 / nP;
-        const unsigned long kc = ((a0 * nA[1] + a1) * nA[2] + a2) * Nc +
-                                 ((g0 % nP) * nP + (g1 % nP)) * nP + (g2 % nP);
-        auto lo = std::lower_bound(skey.begin(), skey.end(), kc);
-        auto hi = std::upper_bound(lo, skey.end(), kc);
-        const long c = ((long)g0 * h.nb[1] + g1) * h.nb[2] + g2;
-        cellSE[2 * c] = (int)(lo - skey.begin());
-        cellSE[2 * c + 1] = (int)(hi - lo);
-      }
-  std::vector<int4> cellOf(n);
-  std::vector<double> loc(4 * n);
-  for (size_t s = 0; s < n; s++) {
-    const int i = h.perm[s];
-    int g[3];
-    for (int d = 0; d < 3; d++) {
-      g[d] = gcell[3 * i + d];
-      // position relative to the low corner of the charge's cluster, as the
-      // reference forms it (x2 = x1 - n cellSizeA)
-      loc[4 * s + d] = (coord[3 * i + d] + (-mn[d])) - (double)(g[d] / (int)nP) * csA;
-    }
-    loc[4 * s + 3] = 0;
-    cellOf[s] = make_int4(g[0], g[1], g[2], 0);
-  }
-  // source segments of every block of 8 consecutive sorted targets: rows of
-  // cells within Rdir (slightly inflated, the kernel applies the exact
-  // cutoff) of the box spanned by the block's positions, split per cluster
-  const long nBlocks = ((long)n + 7) / 8;
-  std::vector<int> segStart(nBlocks + 1, 0);
-  std::vector<int4> segs;
-  segs.reserve((size_t)nBlocks * 48);
+  h.csB = g.csB = csB;
+  h.csA = g.csA = csA;
+  g.nP = nP;
+  g.Bn = h.Bn;
+  // spreading tiles of 8^3 points; interpolation tiles of TI^3 points whose
+  // block plus halo fits 64 KB of LDS
+  for (int d = 0; d < 3; d++) h.nt[d] = g.nt[d] = (int)((h.M[d] + 7) / 8);
   {
-    const double Rm = Rdir * (1.0 + 1.0e-9) + 1.0e-12;
-    for (long b = 0; b < nBlocks; b++) {
-      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-      for (size_t s = 8 * b; s < std::min(n, (size_t)(8 * b + 8)); s++) {
-        const int i = h.perm[s];
-        for (int d = 0; d < 3; d++) {
-          const double x = coord[3 * i + d] + (-mn[d]);
-          lo[d] = std::min(lo[d], x);
-          hi[d] = std::max(hi[d], x);
-        }
-      }
-      const int gx0 = std::max(0, (int)std::floor((lo[0] - Rm) / csB));
-      const int gx1 = std::min(h.nb[0] - 1, (int)std::floor((hi[0] + Rm) / csB));
-      const int gy0 = std::max(0, (int)std::floor((lo[1] - Rm) / csB));
-      const int gy1 = std::min(h.nb[1] - 1, (int)std::floor((hi[1] + Rm) / csB));
-      for (int gx = gx0; gx <= gx1; gx++) {
-        const double gapx = std::max(0.0, std::max(gx * csB - hi[0], lo[0] - (gx + 1) * csB));
-        if (gapx >= Rm) continue;
-        for (int gy = gy0; gy <= gy1; gy++) {
-          const double gapy = std::max(0.0, std::max(gy * csB - hi[1], lo[1] - (gy + 1) * csB));
-          const double rem = Rm * Rm - gapx * gapx - gapy * gapy;
-          if (rem <= 0) continue;
-          const double sr = std::sqrt(rem);
-          const int zlo = std::max(0, (int)std::floor((lo[2] - sr) / csB));
-          const int zhi = std::min(h.nb[2] - 1, (int)std::floor((hi[2] + sr) / csB));
-          const long row = ((long)gx * h.nb[1] + gy) * h.nb[2];
-          for (int az = zlo / (int)nP; az <= zhi / (int)nP; az++) {
-            const int z0 = std::max(zlo, az * (int)nP), z1 = std::min(zhi, az * (int)nP + (int)nP - 1);
-            const int sB = cellSE[2 * (row + z0)];
-            const int sE = cellSE[2 * (row + z1)] + cellSE[2 * (row + z1) + 1];
-            if (sE > sB) segs.push_back(make_int4(sB, sE, gx / (int)nP, ((gy / (int)nP) << 16) | az));
-          }
-        }
-      }
-      if (segs.size() > 0x7fffffffUL) throw Error{NPME_ERR_NOMEM, "too many direct-sum segments"};
-      segStart[b + 1] = (int)segs.size();
-    }
+    const int es = (h.fp64 ? 8 : 4) * (h.complexK ? 2 : 1);
+    h.TI = 16;
+    while (h.TI > 4 && (long)(h.TI + h.Bn - 1) * (h.TI + h.Bn - 1) * (h.TI + h.Bn - 1) * es > 65536) h.TI /= 2;
   }
-  h.csA = csA;
+  g.TI = h.TI;
+  for (int d = 0; d < 3; d++) h.nti[d] = g.nti[d] = (int)((h.M[d] + h.TI - 1) / h.TI);
 
-  // ---- spline windows (u = m + N/4) and per-tile lists
-  std::vector<int4> ms(n);
-  std::vector<double> frac(4 * n);
-  for (size_t s = 0; s < n; s++) {
-    const int i = h.perm[s];
-    int u0[3];
-    for (int d = 0; d < 3; d++) {
-      const double w = h.N[d] / h.L[d] * (coord[3 * i + d] - h.R[d]);
-      const long m0 = (w < 0) ? (long)(-w) + 1 : (long)(-w);
-      if (m0 < -h.N[d] / 4 || m0 + h.Bn - 1 > h.N[d] / 4 - 1)
-        throw Error{NPME_ERR_GRID, "spline window outside the compact grid"};
-      u0[d] = (int)(m0 + h.N[d] / 4);
-      frac[4 * s + d] = w + (double)m0;
-    }
-    frac[4 * s + 3] = 0;
-    ms[s] = make_int4(u0[0], u0[1], u0[2], 0);
-  }
-  for (int d = 0; d < 3; d++) h.nt[d] = (int)((h.M[d] + 7) / 8);
-  const long ntiles = (long)h.nt[0] * h.nt[1] * h.nt[2];
-  std::vector<int> tcount(ntiles + 1, 0);
-  auto tile_range = [&](int u0, int &lo, int &hi) {
-    lo = u0 / 8;
-    hi = (u0 + (int)h.Bn - 1) / 8;
-  };
-  for (size_t s = 0; s < n; s++) {
-    int lo[3], hi[3];
-    tile_range(ms[s].x, lo[0], hi[0]);
-    tile_range(ms[s].y, lo[1], hi[1]);
-    tile_range(ms[s].z, lo[2], hi[2]);
-    for (int a = lo[0]; a <= hi[0]; a++)
-      for (int b = lo[1]; b <= hi[1]; b++)
-        for (int c = lo[2]; c <= hi[2]; c++) tcount[((long)a * h.nt[1] + b) * h.nt[2] + c + 1]++;
-  }
-  for (long t = 0; t < ntiles; t++) tcount[t + 1] += tcount[t];
-  std::vector<int> tlist(tcount[ntiles]);
-  {
-    std::vector<int> fill(tcount.begin(), tcount.end() - 1);
-    for (size_t s = 0; s < n; s++) {
-      int lo[3], hi[3];
-      tile_range(ms[s].x, lo[0], hi[0]);
-      tile_range(ms[s].y, lo[1], hi[1]);
-      tile_range(ms[s].z, lo[2], hi[2]);
-      for (int a = lo[0]; a <= hi[0]; a++)
-        for (int b = lo[1]; b <= hi[1]; b++)
-          for (int c = lo[2]; c <= hi[2]; c++) tlist[fill[((long)a * h.nt[1] + b) * h.nt[2] + c]++] = (int)s;
-    }
-  }
-
-  // ---- device state
+  // ---- per-charge state: on the device when the neighbour list is used
   NPME_HIP_CHECK(hipGetDevice(&h.device));
   if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   handle_free_device(h);
-  const bool f64 = h.fp64;
-  const size_t rs = f64 ? 8 : 4;
-  h.d_perm = dalloc<int>(h, n);
-  NPME_HIP_CHECK(hipMemcpy(h.d_perm, h.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
-  h.d_src = dalloc<char>(h, 4 * rs * n);
-  h.d_frac = dalloc<char>(h, 4 * rs * n);
-  if (f64) {
-    NPME_HIP_CHECK(hipMemcpy(h.d_src, loc.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
-    NPME_HIP_CHECK(hipMemcpy(h.d_frac, frac.data(), 4 * n * sizeof(double), hipMemcpyHostToDevice));
-  } else {
-    // FP32 hi part and the FP32 remainder: hi + lo carries ~48 bits
-    std::vector<float> lf(4 * n), lo(4 * n), ff(frac.begin(), frac.end());
-    for (size_t k = 0; k < 4 * n; k++) {
-      lf[k] = (float)loc[k];
-      lo[k] = (float)(loc[k] - (double)lf[k]);
-    }
-    NPME_HIP_CHECK(hipMemcpy(h.d_src, lf.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
-    h.d_srcLo = dalloc<float>(h, 4 * n);
-    NPME_HIP_CHECK(hipMemcpy(h.d_srcLo, lo.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
-    NPME_HIP_CHECK(hipMemcpy(h.d_frac, ff.data(), 4 * n * sizeof(float), hipMemcpyHostToDevice));
+  std::string smode = "device", nlmode = "auto";
+  kw.get("setup", smode);
+  kw.get("neighborList", nlmode);
+  if (smode != "device" && smode != "host") kw_error("setup must be 'device' or 'host'");
+  if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
+    kw_error("neighborList must be 'auto', 'on' or 'off'");
+  const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
+  bool done = false;
+  if (smode == "device" && nlPossible) done = particles_device(h, g, nlmode, coord);
+  if (!done) {
+    handle_free_device(h);
+    particles_host(h, g, nlmode, nlPossible, coord);
   }
-  h.d_segStart = dalloc<int>(h, segStart.size());
-  NPME_HIP_CHECK(hipMemcpy(h.d_segStart, segStart.data(), segStart.size() * sizeof(int), hipMemcpyHostToDevice));
-  h.d_blockBox = dalloc<int4>(h, segs.size());
-  NPME_HIP_CHECK(hipMemcpy(h.d_blockBox, segs.data(), segs.size() * sizeof(int4), hipMemcpyHostToDevice));
-  h.nSegments = segs.size();
-  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
-  h.d_cell = dalloc<int4>(h, n);
-  NPME_HIP_CHECK(hipMemcpy(h.d_cell, cellOf.data(), n * sizeof(int4), hipMemcpyHostToDevice));
-  h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
-  NPME_HIP_CHECK(hipMemcpy(h.d_cellStart, cellSE.data(), 2 * (size_t)ncell * sizeof(int), hipMemcpyHostToDevice));
-  h.d_ms = dalloc<int4>(h, n);
-  NPME_HIP_CHECK(hipMemcpy(h.d_ms, ms.data(), n * sizeof(int4), hipMemcpyHostToDevice));
-  h.d_tileStart = dalloc<int>(h, ntiles + 1);
-  NPME_HIP_CHECK(hipMemcpy(h.d_tileStart, tcount.data(), (ntiles + 1) * sizeof(int), hipMemcpyHostToDevice));
-  h.d_tileList = dalloc<int>(h, tlist.size());
-  NPME_HIP_CHECK(hipMemcpy(h.d_tileList, tlist.data(), tlist.size() * sizeof(int), hipMemcpyHostToDevice));
-  const int vw = h.complexK ? 8 : 4;
-  h.d_Vdir = dalloc<double>(h, vw * n);
-  h.d_V = dalloc<double>(h, vw * n);
-  h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
-  h.d_coordS = dalloc<double>(h, 3 * n);
-  NPME_HIP_CHECK(hipMemcpy(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-
-  const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2];
-  const size_t cs = 2 * rs;  // complex element size
-  h.d_grid = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)(M1 * M2 * M3));
-  h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.K3));
-  h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.K3));
-  for (int d = 0; d < 3; d++) {
-    h.plan[d] = make_plan((int)h.N[d]);
-    h.d_tw[d] = upload_twiddles(h, (int)h.N[d], f64);
-  }
-
-  // ---- kernel grid G (FP64 build, stored in the working precision)
-  const long Mf[3] = {h.N[0] / 2 + 1, h.N[1] / 2 + 1, h.N[2] / 2 + 1};
-  const long totF = Mf[0] * Mf[1] * Mf[2];
-  std::vector<double> tab;
-  for (int d = 0; d < 3; d++) {
-    const double Xmax = h.X[d] + h.del[d], Xt = 2 * Xmax - h.X[d];
-    const double dx = 2 * Xmax / h.N[d];
-    std::vector<double> x1(Mf[d]), x2(Mf[d]), T1(Mf[d]), T2(Mf[d]);
-    for (long i = 0; i < Mf[d]; i++) {
-      x1[i] = i * dx;
-      const double eps = 1.0e-12 * Xmax;
-      if (x1[i] > Xmax + eps) throw Error{NPME_ERR_GRID, "kernel grid beyond the extension"};
-      if (x1[i] > h.X[d]) {
-        T1[i] = tsmooth(x1[i], h.X[d], Xt, h.ta[d]);
-        x2[i] = x1[i] - 2 * Xmax;
-        T2[i] = tsmooth(x2[i], h.X[d], Xt, h.ta[d]);
-      } else {
-        T1[i] = 1.0;
-        x2[i] = 1.0;
-        T2[i] = 0.0;
-      }
-    }
-    tab.insert(tab.end(), x1.begin(), x1.end());
-    tab.insert(tab.end(), x2.begin(), x2.end());
-    tab.insert(tab.end(), T1.begin(), T1.end());
-    tab.insert(tab.end(), T2.begin(), T2.end());
-  }
-  double *d_tab = nullptr, *d_F = nullptr, *d_lam = nullptr, *d_tw64[3] = {nullptr, nullptr, nullptr};
-  try {
-    NPME_HIP_CHECK(hipMalloc(&d_tab, tab.size() * sizeof(double)));
-    NPME_HIP_CHECK(hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-    NPME_HIP_CHECK(hipMalloc(&d_F, 2 * totF * sizeof(double)));
-    launch_fgrid_real_space(h.ker, Mf, d_tab, d_F, h.stream);
-    for (int d = 0; d < 3; d++) {
-      std::vector<double> t = twiddles((int)h.N[d]);
-      NPME_HIP_CHECK(hipMalloc(&d_tw64[d], t.size() * sizeof(double)));
-      NPME_HIP_CHECK(hipMemcpy(d_tw64[d], t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    }
-    // even-extension DFT along axis 3, 2, 1 of F[Mf0][Mf1][Mf2]
-    fft_pass_even((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], d_tw64[2], d_F, h.stream);
-    fft_pass_even((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], d_tw64[1], d_F, h.stream);
-    fft_pass_even((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], d_tw64[0], d_F, h.stream);
-    std::vector<double> lam;
-    for (int d = 0; d < 3; d++) {
-      std::vector<double> l(h.N[d]);
-      lambda2(h.N[d], h.Bn, l.data());
-      lam.insert(lam.end(), l.begin(), l.begin() + Mf[d]);
-    }
-    NPME_HIP_CHECK(hipMalloc(&d_lam, lam.size() * sizeof(double)));
-    NPME_HIP_CHECK(hipMemcpy(d_lam, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
-    h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)totF);
-    launch_make_G(f64, h.complexK, Mf, h.K3, d_F, d_lam, h.d_G, h.stream);
-    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  } catch (...) {
-    (void)hipFree(d_tab);
-    (void)hipFree(d_F);
-    (void)hipFree(d_lam);
-    for (int d = 0; d < 3; d++) (void)hipFree(d_tw64[d]);
-    throw;
-  }
-  (void)hipFree(d_tab);
-  (void)hipFree(d_F);
-  (void)hipFree(d_lam);
-  for (int d = 0; d < 3; d++) (void)hipFree(d_tw64[d]);
-  build_neighbor_list(h, kw);
+  h.deviceSetup = done;
+  grid_setup(h);
 }
 
 // ---------------------------------------------------------------- CalcV
@@ -581,24 +748,20 @@ DirectParams direct_params(const Handle &h)
   dp.nNeigh = (int)h.nNeigh;
   dp.cs = h.csB;
   for (int d = 0; d < 3; d++) dp.nb[d] = h.nb[d];
+  dp.fxShift = h.fxShift;
+  dp.fxInv = std::ldexp(h.csA, -h.fxShift);
   return dp;
 }
 
 double handle_count_pairs(Handle &h)
 {
-  unsigned long long *d_tot = nullptr, tot = 0;
-  NPME_HIP_CHECK(hipMalloc(&d_tot, sizeof(unsigned long long)));
-  try {
-    NPME_HIP_CHECK(hipMemsetAsync(d_tot, 0, sizeof(unsigned long long), h.stream));
-    launch_count_pairs(direct_params(h), h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, d_tot,
-                       h.stream);
-    NPME_HIP_CHECK(hipMemcpyAsync(&tot, d_tot, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
-    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  } catch (...) {
-    (void)hipFree(d_tot);
-    throw;
-  }
-  (void)hipFree(d_tot);
+  unsigned long long tot = 0;
+  DevBuf d_tot(sizeof(unsigned long long));
+  NPME_HIP_CHECK(hipMemsetAsync(d_tot.p, 0, sizeof(unsigned long long), h.stream));
+  launch_count_pairs(direct_params(h), h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS,
+                     (unsigned long long *)d_tot.p, h.stream);
+  NPME_HIP_CHECK(hipMemcpyAsync(&tot, d_tot.p, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   return (double)tot;
 }
 
@@ -614,10 +777,10 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
   mark(h, st++);
   const DirectParams dp = direct_params(h);
   if (h.useNlist)
-    launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcLo, h.d_qi, h.d_sliceStart, h.d_sliceLen,
+    launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
                      h.d_nlist, h.csA, h.d_Vdir, s);
   else
-    launch_direct(f64, dp, h.n, h.d_src, h.d_srcLo, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
+    launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
                   (int)h.nCellClust1D, h.csA, h.d_Vdir, s);
   mark(h, st++);
   launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_src, h.d_qi,
@@ -657,8 +820,8 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     mark(h, st++);
   }
   const double C[3] = {N1 / h.L[0], N2 / h.L[1], N3 / h.L[2]};
-  launch_interp(f64, cx, h.Bn, h.M, C, h.n, h.d_ms, h.d_frac, h.d_grid, h.d_Vdir, h.d_perm, d_q,
-                h.fself, d_Vout, s);
+  launch_interp(f64, cx, h.Bn, h.M, C, h.TI, h.nti, h.d_itileStart, h.d_itileList, h.d_ms,
+                h.d_frac, h.d_grid, h.d_Vdir, h.d_perm, d_q, h.fself, d_Vout, s);
   mark(h, st++);
   NPME_HIP_CHECK(hipStreamSynchronize(s));
   if (h.profiling) {

@@ -141,3 +141,87 @@ def test_direct_sum_paths_agree(oracle, prec):
     Vl, Vc = hl.calcv(q), hc.calcv(q)
     m = oracle.compare_v(Vl, Vc)
     assert m["relV"] < (1e-13 if prec == "fp64" else 2e-7) and m["reldV"] < (1e-13 if prec == "fp64" else 1e-6), m
+
+
+def _blobs(n, nblob, box, sigma, seed):
+    rng = np.random.default_rng(seed)
+    cen = rng.uniform(0, box, (nblob, 3))
+    coord = cen[rng.integers(0, nblob, n)] + rng.normal(0, sigma, (n, 3))
+    q = rng.uniform(-1, 1, n)
+    return coord, q - q.mean()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", ["uniform", "blobs", "tiny", "ralpha_aniso", "helmholtz"])
+def test_device_setup_matches_host_setup(prec, case):
+    """The set-up on the device (cell keys, stable sort, lists) reproduces
+    the host set-up exactly: CalcV results are bit-identical."""
+    import npme_amd as N
+    extra, cx = {}, False
+    if case == "uniform":
+        coord, q = uniform_box(6000, 35.0)
+        kw = dict(funcType="Laplace", Rdir=7.0, BnOrder=6)
+    elif case == "blobs":
+        coord, q = _blobs(8000, 5, 40.0, 2.0, 21)
+        kw = dict(funcType="Laplace", Rdir=6.0, BnOrder=6)
+    elif case == "tiny":
+        coord, q = uniform_box(5, 6.0)
+        kw = dict(funcType="Laplace", Rdir=4.0, BnOrder=4, N1=32, N2=32, N3=32, n1=8, n2=8, n3=8)
+    elif case == "ralpha_aniso":
+        coord, q = uniform_box(3000, (20.0, 20.0, 60.0))
+        kw = dict(funcType="Ralpha", alpha=0.5, Rdir=6.0, BnOrder=6, nDeriv=6,
+                  N1=60, N2=60, N3=140, n1=15, n2=15, n3=14)
+    else:
+        coord, _ = uniform_box(3000, 24.0)
+        rng = np.random.default_rng(5)
+        q = rng.standard_normal(3000) + 1j * rng.standard_normal(3000)
+        kw = dict(funcType="Helmholtz", k0_r=0.7, k0_i=0.05, Rdir=5.0, BnOrder=8)
+        cx = True
+    print("case device-vs-host set-up", case, prec, flush=True)
+    out = {}
+    for mode in ("device", "host"):
+        k = _kw(calcType="pme", tol=1e-6, precision=prec, setup=mode, neighborList="on", **kw)
+        if cx:
+            assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeComplex=q), N.npme_last_error()
+            h = N.NPME_Interface_SetUpComplexKernel(k, len(q), coord)
+        else:
+            assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q), N.npme_last_error()
+            h = N.NPME_Interface_SetUpRealKernel(k, len(q), coord)
+        assert h is not None, N.npme_last_error()
+        assert ("set-up on the " + mode) in h.Print()
+        out[mode] = h.calcv(q)
+    assert np.array_equal(out["device"], out["host"])
+
+
+@pytest.mark.parametrize("setup", ["device", "host"])
+def test_mixed_close_pairs(setup):
+    """Pairs far closer than the fixed-point unit matters for: the mixed
+    direct sum must keep FP32 relative accuracy in r for them (checked
+    against the FP64 GPU path on the same grid)."""
+    import npme_amd as N
+    coord, q = uniform_box(3000, 20.0)
+    rng = np.random.default_rng(9)
+    seps = [1e-3, 3e-3, 1e-2, 3e-2, 0.1, 0.3]
+    for k, r in enumerate(seps):
+        u = rng.normal(size=3)
+        coord[2 * k + 1] = coord[2 * k] + r * u / np.linalg.norm(u)
+    print("case close pairs", setup, flush=True)
+    V = {}
+    for prec in ("fp64", "mixed"):
+        h = _real_setup(coord, q, prec, Rdir=5.0, setup=setup, neighborList="on")
+        V[prec] = h.calcv(q)
+    d = V["mixed"] - V["fp64"]
+    meanV = np.mean(np.abs(V["fp64"][:, 0]))
+    planted = np.arange(2 * len(seps))
+    rest = np.arange(2 * len(seps), len(q))
+    # a planted target's V is dominated by its close pair (up to 1e3 x the
+    # mean): its error is measured against its own |V|, the FP32 rounding
+    # level of that pair term; the other targets against the mean |V|
+    relOwn = np.abs(d[planted, 0]) / np.maximum(np.abs(V["fp64"][planted, 0]), meanV)
+    relRest = np.abs(d[rest, 0]) / meanV
+    relG = np.linalg.norm(d[planted, 1:], axis=1) / np.linalg.norm(V["fp64"][planted, 1:], axis=1)
+    print("planted relV", relOwn.max(), "planted relG", relG.max(), "other max relV", relRest.max())
+    # a position error of one fixed-point unit (2e-8 A) would give ~2e-5 in V
+    # and ~8e-5 in the gradient at the 1e-3 A pair
+    assert relOwn.max() < 2e-6 and relG.max() < 5e-6
+    assert relRest.max() < 1e-5
