@@ -42,6 +42,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-accuracy", action="store_true")
+    p.add_argument("--stream-requests", type=int, default=5000,
+                   help="requests of the mixed-size serving stream (0 skips it)")
+    p.add_argument("--stream-rate", type=float, default=200.0, help="Poisson arrival rate, 1/s")
+    p.add_argument("--slots", type=int, default=4, help="concurrent request slots of the server")
     p.add_argument("--calib-copy", action="store_true",
                    help="after the steps, copy a 256 MiB device buffer 3 times (a plain copy "
                         "kernel of known size, to calibrate profiler byte counters)")
@@ -99,6 +103,90 @@ def accuracy(h, q, V, n):
             "relV": float(np.sqrt(np.mean(d[:, 0] ** 2)) / magV),
             "reldV": float(np.sqrt(np.mean(d[:, 1:] ** 2)) / np.mean(np.abs(Ve[:, :3]))),
             "max_absV_over_meanV": float(np.max(np.abs(d[:, 0])) / magV)}
+
+
+def serving_stream(nreq, rate, slots, seed=2001):
+    """Mixed-size request stream through the serving loop: Poisson arrivals
+    at `rate`; N log-uniform in [500, 1e6]; positions uniform in a cube at
+    density 0.1; charges U[-1,1] (zero net) or complex N(0,1); kernel 70%
+    1/r, 20% r^alpha (alpha 0.5 or 1.5), 10% Helmholtz k0 = 0.3; tol 1e-4 or
+    1e-6 with equal probability; Rdir 8.  Latency = completion minus the
+    scheduled arrival, so a late submission counts against the server."""
+    import threading
+    import npme_amd as N
+    rng = np.random.default_rng(seed)
+    nmax = 1000000
+    base = rng.uniform(0.0, 1.0, (nmax, 3))
+    baseq = rng.uniform(-1.0, 1.0, nmax)
+    baseqc = rng.standard_normal(nmax) + 1j * rng.standard_normal(nmax)
+    sizes = np.exp(rng.uniform(np.log(500), np.log(nmax), nreq)).astype(np.int64)
+    u = rng.uniform(0, 1, nreq)
+    kinds = np.where(u < 0.7, 0, np.where(u < 0.9, 1, 2))
+    alphas = np.where(rng.uniform(0, 1, nreq) < 0.5, 0.5, 1.5)
+    tols = np.where(rng.uniform(0, 1, nreq) < 0.5, 1e-4, 1e-6)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, nreq)) + 0.5
+    S = N.NPME_Server(nSlots=slots)
+    tickets = [None] * nreq
+    infos = [None] * nreq
+    errors = []
+    ready = threading.Semaphore(0)
+
+    def consume():
+        for i in range(nreq):
+            ready.acquire()
+            try:
+                _, infos[i] = S.wait(tickets[i])
+            except Exception as e:  # recorded, reported in the JSON
+                errors.append(str(e))
+            if (i + 1) % 1000 == 0:
+                log("stream: %d/%d requests done" % (i + 1, nreq))
+
+    th = threading.Thread(target=consume)
+    th.start()
+    lag = 0.0
+    for i in range(nreq):
+        n = int(sizes[i])
+        L = (n / 0.1) ** (1.0 / 3.0)
+        coord = base[:n] * L
+        t = S.clock()
+        if t < arrive[i]:
+            time.sleep(arrive[i] - t)
+        lag = max(lag, S.clock() - arrive[i])
+        if kinds[i] == 2:
+            tickets[i] = S.submit("Helmholtz", coord, baseqc[:n], tol=tols[i], Rdir=8.0, k0=0.3,
+                                  tArrival=arrive[i])
+        else:
+            q = baseq[:n] - baseq[:n].mean()
+            ft = "Laplace" if kinds[i] == 0 else "Ralpha"
+            tickets[i] = S.submit(ft, coord, q, tol=tols[i], Rdir=8.0, alpha=alphas[i],
+                                  tArrival=arrive[i])
+        ready.release()
+    th.join()
+    S.close()
+    ok = [i for i in range(nreq) if infos[i] is not None]
+    lat = np.array([infos[i]["tEnd"] - infos[i]["tArrival"] for i in ok])
+    t_first = arrive[0]
+    t_last = max(infos[i]["tEnd"] for i in ok)
+    charges = int(sum(sizes[i] for i in ok))
+    by = {}
+    for k, name in enumerate(["laplace", "ralpha", "helmholtz"]):
+        sel = [j for j, i in enumerate(ok) if kinds[i] == k]
+        if sel:
+            by[name] = {"requests": len(sel), "p50_ms": float(np.percentile(lat[sel], 50) * 1e3),
+                        "p99_ms": float(np.percentile(lat[sel], 99) * 1e3)}
+    big = [j for j, i in enumerate(ok) if sizes[i] >= 500000]
+    return {"requests": nreq, "completed": len(ok), "errors": errors[:5],
+            "rate_per_s": rate, "slots": slots,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p90_ms": float(np.percentile(lat, 90) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "max_ms": float(lat.max() * 1e3),
+            "mean_ms": float(lat.mean() * 1e3),
+            "p99_ms_N_ge_500k": float(np.percentile(lat[big], 99) * 1e3) if big else None,
+            "charges_per_s": charges / (t_last - t_first), "total_charges": charges,
+            "max_submit_lag_ms": lag * 1e3, "by_kernel": by,
+            "workload": "Poisson %g req/s, N log-uniform [500, 1e6], density 0.1, 70%% 1/r / 20%% "
+                        "r^alpha (0.5|1.5) / 10%% Helmholtz k0=0.3, tol 1e-4|1e-6, Rdir 8, mixed "
+                        "precision, seed %d; latency includes set-up (RSEM grid, cell sort, lists, "
+                        "kernel grid), host<->device copies and CalcV" % (rate, seed)}
 
 
 def main():
@@ -191,6 +279,12 @@ def main():
             if line.startswith("traffic_bytes_per_launch"):
                 roof["traffic"] = float(line.split()[1])
 
+    latency = None
+    if a.stream_requests > 0:
+        log("serving stream: %d requests at %g/s on %d slots" % (a.stream_requests, a.stream_rate, a.slots))
+        latency = serving_stream(a.stream_requests, a.stream_rate, a.slots)
+        log("stream: p50 %.2f ms p99 %.2f ms" % (latency["p50_ms"], latency["p99_ms"]))
+
     cpu = None
     if not a.no_cpu_baseline:
         from oracle import oracle as O
@@ -230,7 +324,7 @@ def main():
         "stage_ms": dict(zip(STAGES, np.round(stages, 4).tolist())),
         "stages": stage_info,
         "fp64_gpu_path": fp64,
-        "latency": None,
+        "latency": latency,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -117,6 +117,49 @@ int npme_count_pairs(npme_handle *h, double *pairs);
 int npme_set_profiling(npme_handle *h, int on);
 int npme_get_stage_times(const npme_handle *h, double *ms, int n);
 
+/* ---- serving: a persistent loop of nSlots worker slots on the current
+ * GPU that evaluates independent CalcV requests (any size and kernel)
+ * concurrently.  A request is admitted first come first served when a slot
+ * is free and its device-memory estimate fits the budget (memGB; <= 0
+ * means 80% of the free device memory), runs NPME_AddMissingKeywords (RSEM
+ * grid for its tol), the device set-up and CalcV, and writes V.  coord,
+ * charge and V must stay valid until npme_server_wait returns for its
+ * ticket.  A result does not depend on the other requests in flight.  No
+ * reference counterpart: the reference evaluates one charge set per call
+ * (src/Interface.h:373-375). ---------------------------------------------- */
+typedef struct npme_server npme_server;
+typedef struct {
+  const char *funcType;   /* "Laplace", "Ralpha" or "Helmholtz"            */
+  double alpha;           /* r^alpha exponent (Ralpha)                     */
+  double k0_r, k0_i;      /* wave number (Helmholtz)                       */
+  double tol;             /* requested relative RMS error of V (RSEM)      */
+  double Rdir;            /* short-range cutoff                            */
+  long BnOrder;           /* B-spline order; 0 = the library default       */
+  int fp64;               /* 0 = mixed precision, 1 = FP64                 */
+  size_t nCharge;
+  const double *coord;    /* [nCharge][3]                                  */
+  const double *charge;   /* [nCharge] real or [nCharge][2] complex        */
+  double *V;              /* [nCharge][4] or [nCharge][4][2], output       */
+  double tArrival;        /* arrival time on the server clock, or < 0 for
+                             the time of submission                       */
+} npme_request;
+typedef struct {
+  int status;                              /* NPME_OK or an error code     */
+  double tArrival, tSubmit, tStart, tEnd;  /* server clock, seconds        */
+  long N[3];                               /* FFT grid used                */
+  double deviceGB;                         /* device footprint             */
+} npme_request_info;
+int npme_server_create(npme_server **s, int nSlots, double memGB);
+/* queues a request; returns at once with its ticket */
+int npme_server_submit(npme_server *s, const npme_request *r, long *ticket);
+/* blocks until the request is done; returns its status (each ticket is
+ * waited for once) */
+int npme_server_wait(npme_server *s, long ticket, npme_request_info *info);
+/* seconds since the server was created */
+double npme_server_clock(const npme_server *s);
+/* finishes the queued requests, then stops the slots */
+void npme_server_destroy(npme_server *s);
+
 const char *npme_last_error(void);
 
 #ifdef __cplusplus

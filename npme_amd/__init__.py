@@ -55,8 +55,28 @@ def lib():
         L.npme_set_profiling.argtypes = [p, i]
         L.npme_count_pairs.argtypes = [p, p]
         L.npme_get_stage_times.argtypes = [p, p, i]
+        L.npme_server_create.argtypes = [C.POINTER(p), i, d]
+        L.npme_server_submit.argtypes = [p, C.POINTER(_Request), C.POINTER(C.c_long)]
+        L.npme_server_wait.argtypes = [p, C.c_long, C.POINTER(_RequestInfo)]
+        L.npme_server_clock.argtypes = [p]
+        L.npme_server_clock.restype = d
+        L.npme_server_destroy.argtypes = [p]
         _lib = L
     return _lib
+
+
+class _Request(C.Structure):
+    _fields_ = [("funcType", C.c_char_p), ("alpha", C.c_double), ("k0_r", C.c_double),
+                ("k0_i", C.c_double), ("tol", C.c_double), ("Rdir", C.c_double),
+                ("BnOrder", C.c_long), ("fp64", C.c_int), ("nCharge", C.c_size_t),
+                ("coord", C.c_void_p), ("charge", C.c_void_p), ("V", C.c_void_p),
+                ("tArrival", C.c_double)]
+
+
+class _RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int), ("tArrival", C.c_double), ("tSubmit", C.c_double),
+                ("tStart", C.c_double), ("tEnd", C.c_double), ("N", C.c_long * 3),
+                ("deviceGB", C.c_double)]
 
 
 def npme_last_error():
@@ -293,3 +313,56 @@ def NPME_Interface_SetUpComplexKernel(keyword, nCharge, coord, useGenericKernel=
                                       printLog=False):
     obj = NPME_InterfaceComplex_Helmholtz_DM()
     return obj if obj.SetUp(keyword, nCharge, coord, printLog) else None
+
+
+# ----------------------------------------------------------------- serving
+class NPME_Server:
+    """Persistent request loop on the current GPU (npme_server_* in
+    include/npme_amd.h): nSlots requests run concurrently, each admitted
+    when a slot and its device-memory estimate are free.  No reference
+    counterpart."""
+
+    def __init__(self, nSlots=4, memGB=0.0):
+        self._s = C.c_void_p()
+        _check(lib().npme_server_create(C.byref(self._s), int(nSlots), float(memGB)), "server_create")
+        self._live = {}
+
+    def clock(self):
+        """Seconds since the server was created (the clock of tArrival)."""
+        return lib().npme_server_clock(self._s)
+
+    def submit(self, funcType, coord, charge, tol=1e-6, Rdir=8.0, BnOrder=0, alpha=0.0, k0=0.0,
+               fp64=False, tArrival=-1.0):
+        coord = _coord(coord)
+        cx = funcType == "Helmholtz"
+        q = np.ascontiguousarray(charge, dtype=np.complex128 if cx else np.float64)
+        n = len(q)
+        V = np.zeros((n, 4), np.complex128 if cx else np.float64)
+        ft = funcType.encode()
+        k0 = complex(k0)
+        r = _Request(ft, float(alpha), k0.real, k0.imag, float(tol), float(Rdir), int(BnOrder),
+                     int(bool(fp64)), n, coord.ctypes.data, q.ctypes.data, V.ctypes.data,
+                     float(tArrival))
+        t = C.c_long()
+        _check(lib().npme_server_submit(self._s, C.byref(r), C.byref(t)), "server_submit")
+        self._live[t.value] = (coord, q, V, ft)
+        return t.value
+
+    def wait(self, ticket):
+        """Blocks until the request is done: returns (V, info dict)."""
+        info = _RequestInfo()
+        rc = lib().npme_server_wait(self._s, int(ticket), C.byref(info))
+        _, _, V, _ = self._live.pop(ticket)
+        if rc != NPME_OK:
+            raise NPMEError(rc, "request %d" % ticket)
+        return V, {"tArrival": info.tArrival, "tSubmit": info.tSubmit, "tStart": info.tStart,
+                   "tEnd": info.tEnd, "grid": tuple(info.N), "deviceGB": info.deviceGB}
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().npme_server_destroy(self._s)
+            self._s = None
+            self._live.clear()
+
+    def __del__(self):
+        self.close()

@@ -13,7 +13,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnpme_amd.so")
 
 SOURCES = ["geometry.hip", "fft.hip", "particles.hip", "setup.cpp", "host_math.cpp", "capi.cpp",
-           "devmem.cpp"]
+           "devmem.cpp", "server.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NPME_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

@@ -19,29 +19,10 @@ struct npme_handle {
 namespace npme {
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
+std::string last_error() { return g_err; }
 }  // namespace npme
 
 using namespace npme;
-
-template <typename F> static int guarded(F &&f)
-{
-  try {
-    f();
-    return NPME_OK;
-  } catch (const Error &e) {
-    set_error(e.msg);
-    return e.code;
-  } catch (const DeviceError &e) {
-    set_error(e.msg);
-    return NPME_ERR_DEVICE;
-  } catch (const std::bad_alloc &) {
-    set_error("host allocation failed");
-    return NPME_ERR_NOMEM;
-  } catch (const std::exception &e) {
-    set_error(e.what());
-    return NPME_ERR_ARG;
-  }
-}
 
 extern "C" {
 
@@ -114,7 +95,18 @@ int npme_add_missing_keywords(npme_keywords *kwp, size_t nCharge, int isChargeRe
 {
   return guarded([&] {
     if (!kwp || !coord) throw Error{NPME_ERR_ARG, "npme_add_missing_keywords: NULL argument"};
-    Keywords &kw = kwp->kw;
+    add_missing_keywords(kwp->kw, nCharge, isChargeReal != 0, coord, chargeReal, chargeComplex);
+  });
+}
+
+}  // extern "C"
+
+namespace npme {
+
+void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const double *coord,
+                          const double *chargeReal, const double *chargeComplex)
+{
+  {
     std::string s, funcType, calcType, split;
     double Rdir, tol, k0r = 0, k0i = 0;
     long Bn;
@@ -207,8 +199,12 @@ int npme_add_missing_keywords(npme_keywords *kwp, size_t nCharge, int isChargeRe
         kw.set("nDeriv", (long)nDeriv);
       }
     }
-  });
+  }
 }
+
+}  // namespace npme
+
+extern "C" {
 
 // ---------------------------------------------------------------- handles
 int npme_create(npme_handle **out, const npme_keywords *kw, size_t nCharge, const double *coord)

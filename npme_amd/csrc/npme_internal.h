@@ -16,6 +16,7 @@ namespace npme {
 
 // ------------------------------------------------------------- errors
 void set_error(const std::string &msg);
+std::string last_error();  // the calling thread's last error message
 struct Status {
   int code;
   std::string msg;
@@ -36,6 +37,27 @@ struct DeviceError {
   std::string msg;
   explicit DeviceError(std::string m) : msg(std::move(m)) {}
 };
+
+// runs f, mapping exceptions to a status code and the thread's last error
+template <typename F> inline int guarded(F &&f)
+{
+  try {
+    f();
+    return NPME_OK;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const DeviceError &e) {
+    set_error(e.msg);
+    return NPME_ERR_DEVICE;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return NPME_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return NPME_ERR_ARG;
+  }
+}
 
 // ------------------------------------------------------------- kernels
 enum KernelKind { KIND_LAPLACE = 0, KIND_RALPHA = 1, KIND_HELMHOLTZ = 2 };
@@ -198,6 +220,9 @@ struct Handle {
 };
 
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord);
+// NPME_AddMissingKeywords: defaults, validation, RSEM grid (throws Error)
+void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const double *coord,
+                          const double *chargeReal, const double *chargeComplex);
 
 // geometry of the cell partition and the grids, shared by the host and the
 // device set-up (geometry.hip)

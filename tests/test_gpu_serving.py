@@ -1,0 +1,100 @@
+"""GPU tests of the serving loop (npme_server_*): requests of mixed kernels,
+sizes and tolerances run concurrently give the same V as each request run
+alone (bit-identical: the kernels are deterministic and requests share no
+state), meet their requested tolerance against the exact sum, and respect
+the device-memory admission budget."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [("Laplace", {}), ("Ralpha", {"alpha": 0.5}), ("Ralpha", {"alpha": 1.5}),
+         ("Helmholtz", {"k0": 0.3})]
+
+
+def _requests(sizes, seed=31):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i, n in enumerate(sizes):
+        ft, kw = KINDS[i % len(KINDS)]
+        tol = 1e-6 if (i // len(KINDS)) % 2 == 0 else 1e-4
+        L = (n / 0.1) ** (1.0 / 3.0)
+        coord = rng.uniform(0.0, L, (n, 3))
+        if ft == "Helmholtz":
+            q = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        else:
+            q = rng.uniform(-1.0, 1.0, n)
+            q -= q.mean()
+        reqs.append((ft, coord, q, tol, kw))
+    return reqs
+
+
+def _run(server, reqs):
+    ticks = [server.submit(ft, c, q, tol=tol, Rdir=8.0, **kw) for ft, c, q, tol, kw in reqs]
+    return [server.wait(t) for t in ticks]
+
+
+def test_server_mix_equals_alone():
+    import npme_amd as N
+    reqs = _requests([500, 900, 2000, 3500, 6000, 12000, 700, 2500, 5000, 1500, 8000, 30000])
+    print("case server mix: %d requests on 4 slots" % len(reqs), flush=True)
+    S = N.NPME_Server(nSlots=4)
+    mixed = _run(S, reqs)
+    S.close()
+    # the slots did run concurrently
+    ivals = sorted((inf["tStart"], inf["tEnd"]) for _, inf in mixed)
+    overlap = any(ivals[i + 1][0] < ivals[i][1] for i in range(len(ivals) - 1))
+    assert overlap
+    print("case server alone", flush=True)
+    S1 = N.NPME_Server(nSlots=1)
+    for (ft, c, q, tol, kw), (Vm, _) in zip(reqs, mixed):
+        (Va, _), = _run(S1, [(ft, c, q, tol, kw)])
+        assert np.array_equal(Va, Vm), ft
+    S1.close()
+
+
+def test_server_results_meet_tolerance(oracle):
+    import npme_amd as N
+    reqs = _requests([1200, 1500, 1800, 2000, 1300, 1600, 1900, 2200])
+    print("case server tolerance", flush=True)
+    S = N.NPME_Server(nSlots=4)
+    out = _run(S, reqs)
+    S.close()
+    for (ft, c, q, tol, kw), (V, inf) in zip(reqs, out):
+        if ft == "Laplace":
+            k = oracle.kernel(oracle.LAPLACE, 6, 8.0)
+        elif ft == "Ralpha":
+            k = oracle.kernel(oracle.RALPHA, 6, 8.0, alpha=kw["alpha"])
+        else:
+            k = oracle.kernel(oracle.HELMHOLTZ, 8, 8.0, k0=kw["k0"])
+        m = oracle.compare_v(oracle.exact(k, c, q), V)
+        print(ft, kw, tol, inf["grid"], m["relV"])
+        assert m["relV"] <= 2.0 * tol
+
+
+def test_server_memory_budget_serialises():
+    """With a budget below one request's estimate, requests run one at a
+    time (a request always runs when nothing else does)."""
+    import npme_amd as N
+    reqs = _requests([800, 1000, 1200, 1400])
+    print("case server budget", flush=True)
+    S = N.NPME_Server(nSlots=4, memGB=0.01)
+    out = _run(S, reqs)
+    S.close()
+    ivals = sorted((inf["tStart"], inf["tEnd"]) for _, inf in out)
+    assert all(ivals[i + 1][0] >= ivals[i][1] for i in range(len(ivals) - 1))
+
+
+def test_server_rejects_bad_requests():
+    import npme_amd as N
+    S = N.NPME_Server(nSlots=2)
+    c = np.random.default_rng(1).uniform(0, 10, (100, 3))
+    with pytest.raises(N.NPMEError):
+        S.submit("Yukawa", c, np.ones(100))
+    with pytest.raises(N.NPMEError):
+        S.submit("Laplace", c[:1], np.ones(1))
+    # a request that fails at set-up reports its error at wait()
+    t = S.submit("Laplace", c, np.ones(100), tol=1e-6, Rdir=8.0, BnOrder=7)
+    with pytest.raises(N.NPMEError):
+        S.wait(t)
+    S.close()
