@@ -21,6 +21,8 @@
 // row passes keep [c][n] with one element of padding per column.  Between
 // the read and the write barrier of a stage every thread holds its
 // butterflies in registers, so one LDS buffer suffices.
+#include <type_traits>
+
 #include "npme_internal.h"
 
 namespace npme {
@@ -219,9 +221,14 @@ __device__ __forceinline__ void stage(T *lds, ColRef col, int tj, int tpc, int N
   __syncthreads();
 }
 
+// radix classes of a plan: the kernels are compiled for the radices a plan
+// can contain, since every stage variant in the switch adds registers
+// (radix 8 only: ~80 VGPRs, 8/4/2: ~110, all six: ~170)
+enum { RC_ALL = 0, RC_POW2 = 1, RC_R8 = 2 };
+
 // all stages of the plan on every column of the workgroup; EMAX bounds the
 // elements one thread holds (N / tpc <= EMAX)
-template <typename T, int EMAX>
+template <typename T, int EMAX, int RC>
 __device__ void fft_columns(T *lds, ColRef col, int tj, int tpc, const PlanDev &p,
                             const T *__restrict__ tw)
 {
@@ -230,11 +237,21 @@ __device__ void fft_columns(T *lds, ColRef col, int tj, int tpc, const PlanDev &
     const int R = p.radix[s];
     switch (R) {
       case 8: stage<T, 8, (EMAX + 7) / 8>(lds, col, tj, tpc, p.N, Ns, tw); break;
-      case 4: stage<T, 4, (EMAX + 3) / 4>(lds, col, tj, tpc, p.N, Ns, tw); break;
-      case 2: stage<T, 2, (EMAX + 1) / 2>(lds, col, tj, tpc, p.N, Ns, tw); break;
-      case 3: stage<T, 3, (EMAX + 2) / 3>(lds, col, tj, tpc, p.N, Ns, tw); break;
-      case 5: stage<T, 5, (EMAX + 4) / 5>(lds, col, tj, tpc, p.N, Ns, tw); break;
-      case 7: stage<T, 7, (EMAX + 6) / 7>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 4:
+        if constexpr (RC != RC_R8) stage<T, 4, (EMAX + 3) / 4>(lds, col, tj, tpc, p.N, Ns, tw);
+        break;
+      case 2:
+        if constexpr (RC != RC_R8) stage<T, 2, (EMAX + 1) / 2>(lds, col, tj, tpc, p.N, Ns, tw);
+        break;
+      case 3:
+        if constexpr (RC == RC_ALL) stage<T, 3, (EMAX + 2) / 3>(lds, col, tj, tpc, p.N, Ns, tw);
+        break;
+      case 5:
+        if constexpr (RC == RC_ALL) stage<T, 5, (EMAX + 4) / 5>(lds, col, tj, tpc, p.N, Ns, tw);
+        break;
+      case 7:
+        if constexpr (RC == RC_ALL) stage<T, 7, (EMAX + 6) / 7>(lds, col, tj, tpc, p.N, Ns, tw);
+        break;
       default: break;
     }
     Ns *= R;
@@ -282,6 +299,44 @@ __device__ __forceinline__ Lane lane_rows(int lgB, int N)
   return L;
 }
 
+// loads of one column into LDS: all of a thread's (at most EMAX) elements
+// are fetched into registers first, so its global loads are in flight
+// together, then stored; get(n) returns element n (n < N)
+template <typename T, int EMAX, typename F>
+__device__ __forceinline__ void stage_in(T *lds, const Lane &L, int N, F &&get)
+{
+  T v[EMAX];
+#pragma unroll
+  for (int u = 0; u < EMAX; u++) {
+    const int n = L.tj + u * L.tpc;
+    if (n < N) v[u] = get(n);
+  }
+#pragma unroll
+  for (int u = 0; u < EMAX; u++) {
+    const int n = L.tj + u * L.tpc;
+    if (n < N) lds[L.col.at(n)] = v[u];
+  }
+}
+
+// twiddle table of the pass: copied into LDS behind the column data in FP32
+// (data <= 32 KB plus N <= 4096 entries stay within 64 KB; FP64 columns
+// already fill 64 KB), so the butterflies read it with ds_read instead of a
+// global load per twiddled element; the copy is ordered by the barrier that
+// follows stage_in
+template <typename Real> constexpr bool twl_lds() { return sizeof(Real) == 4; }
+
+template <typename T, bool TWL>
+__device__ __forceinline__ const T *twiddles_in(T *lds, int ldsElems, int N, const T *tw)
+{
+  if constexpr (TWL) {
+    T *tl = lds + ldsElems;
+    for (int i = threadIdx.x; i < N; i += NT) tl[i] = tw[i];
+    return tl;
+  } else {
+    return tw;
+  }
+}
+
 // ------------------------------------------------------------ pass kernels
 // generic complex pass.  Column col = o * nInner + i;
 // element n at o * outer + i * inner + n * pos (all element counts < 2^31).
@@ -291,7 +346,7 @@ struct CGeom {
   int inOuter, inInner, inPos, outOuter, outInner, outPos;
 };
 
-template <typename Real, bool ROWS>
+template <typename Real, bool ROWS, int RC>
 __global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const typename Cx<Real>::T *tw,
                                                     const typename Cx<Real>::T *__restrict__ in,
                                                     typename Cx<Real>::T *__restrict__ out, int lgB,
@@ -306,16 +361,17 @@ __global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const ty
   const bool valid = colg < g.nCols;
   const int o = valid ? colg / g.nInner : 0, i = valid ? colg - o * g.nInner : 0;
   const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
-  for (int n = L.tj; n < N; n += L.tpc) {
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (ROWS ? N + 1 : N), N, tw);
+  stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
     T v = mk<T>(0, 0);
     if (valid && n < g.Lin) {
       v = src[(long)n * g.inPos];
       if (backward) v = cconj(v);
     }
-    lds[L.col.at(n)] = v;
-  }
+    return v;
+  });
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
   T *dst = out + ((long)o * g.outOuter + (long)i * g.outInner);
   for (int n = L.tj; n < g.Lout; n += L.tpc) {
@@ -326,7 +382,7 @@ __global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const ty
 }
 
 // pass A: two real rows per complex column, R2C by the two-for-one split
-template <typename Real>
+template <typename Real, int RC>
 __global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, int nPairs, int inRow, int outRow,
                                                 PlanDev p, const typename Cx<Real>::T *tw,
                                                 const Real *__restrict__ in,
@@ -339,13 +395,12 @@ __global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, int nPairs, int 
   const int pr = (xcd_wg() << lgB) + L.c;
   const bool valid = pr < nPairs;
   const Real *ra = in + (long)(2 * pr) * inRow, *rb = ra + inRow;
-  for (int n = L.tj; n < N; n += L.tpc) {
-    T v = mk<T>(0, 0);
-    if (valid && n < Lin) v = mk<T>(ra[n], rb[n]);
-    lds[L.col.at(n)] = v;
-  }
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (N + 1), N, tw);
+  stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
+    return (valid && n < Lin) ? mk<T>(ra[n], rb[n]) : mk<T>(0, 0);
+  });
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
   const int K = N / 2 + 1;
   const Real h = (Real)0.5;
@@ -360,7 +415,7 @@ __global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, int nPairs, int 
 }
 
 // pass E: Hermitian half rows -> two real rows (backward transform)
-template <typename Real>
+template <typename Real, int RC>
 __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int inRow, int outRow,
                                                 PlanDev p, const typename Cx<Real>::T *tw,
                                                 const typename Cx<Real>::T *__restrict__ in,
@@ -374,7 +429,8 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
   const bool valid = pr < nPairs;
   const int h = N / 2;
   const T *ia = in + (long)(2 * pr) * inRow, *ib = ia + inRow;
-  for (int n = L.tj; n < N; n += L.tpc) {
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (N + 1), N, tw);
+  stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
     T v = mk<T>(0, 0);
     if (valid) {
       const int m = (n <= h) ? n : N - n;
@@ -383,10 +439,10 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
       if (n <= h) v = mk<T>(a.x - b.y, -(a.y + b.x));
       else v = mk<T>(a.x + b.y, a.y - b.x);
     }
-    lds[L.col.at(n)] = v;
-  }
+    return v;
+  });
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
   Real *oa = out + (long)(2 * pr) * outRow, *ob = oa + outRow;
   for (int n = L.tj; n < Lout; n += L.tpc) {
@@ -398,7 +454,7 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
 
 // pass C: forward along axis 1, X <- X G, backward, keep the first Lin
 // outputs.  Columns c = k2 * C3 + k3; G index (|k1|, |k2|, k3 or |k3|).
-template <typename Real, bool CG>
+template <typename Real, bool CG, int RC>
 __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int pos, int N2, int G2,
                                                  int C3, int N3fold, PlanDev p,
                                                  const typename Cx<Real>::T *tw, const void *Gv,
@@ -411,33 +467,41 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   const int colg = (xcd_wg() << lgB) + L.c;
   const bool valid = colg < nCols;
   T *col = data + (valid ? colg : 0);
-  for (int n = L.tj; n < N; n += L.tpc) {
-    T v = mk<T>(0, 0);
-    if (valid && n < Lin) v = col[(long)n * pos];
-    lds[L.col.at(n)] = v;
-  }
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * N, N, tw);
+  stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
+    return (valid && n < Lin) ? col[(long)n * pos] : mk<T>(0, 0);
+  });
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   // multiply by G and conjugate (for the backward transform)
   const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : C3;
   const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
   const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
   const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
   const int gbase = a2 * G3 + a3, gstep = G2 * G3;
-  for (int n = L.tj; n < N; n += L.tpc) {
-    const int a1 = (n <= N / 2) ? n : N - n;
-    T x = lds[L.col.at(n)];
-    if constexpr (CG) {
-      const T gg = reinterpret_cast<const T *>(Gv)[(long)a1 * gstep + gbase];
-      x = cconj(cmul(x, gg));
-    } else {
-      const Real gg = reinterpret_cast<const Real *>(Gv)[(long)a1 * gstep + gbase];
-      x = mk<T>(gg * x.x, -gg * x.y);
+  constexpr int E = emax<Real>();
+  using GT = typename std::conditional<CG, T, Real>::type;
+  GT gv[E];
+#pragma unroll
+  for (int u = 0; u < E; u++) {  // the G loads of the thread, in flight together
+    const int n = L.tj + u * L.tpc;
+    if (n < N) {
+      const int a1 = (n <= N / 2) ? n : N - n;
+      gv[u] = reinterpret_cast<const GT *>(Gv)[(long)a1 * gstep + gbase];
     }
-    lds[L.col.at(n)] = x;
+  }
+#pragma unroll
+  for (int u = 0; u < E; u++) {
+    const int n = L.tj + u * L.tpc;
+    if (n < N) {
+      T x = lds[L.col.at(n)];
+      if constexpr (CG) x = cconj(cmul(x, gv[u]));
+      else x = mk<T>(gv[u] * x.x, -gv[u] * x.y);
+      lds[L.col.at(n)] = x;
+    }
   }
   __syncthreads();
-  fft_columns<T, emax<Real>()>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
   for (int n = L.tj; n < Lin; n += L.tpc) col[(long)n * pos] = cconj(lds[L.col.at(n)]);
 }
@@ -456,13 +520,11 @@ __global__ __launch_bounds__(NT) void k_fft_even(int N, int nCols, int nInner, i
   const bool valid = colg < nCols;
   const int o = valid ? colg / nInner : 0, i = valid ? colg - o * nInner : 0;
   double2 *line = data + ((long)o * outer + (long)i * inner);
-  for (int n = L.tj; n < N; n += L.tpc) {
-    double2 v = mk<double2>(0, 0);
-    if (valid) v = line[(long)((n <= h) ? n : N - n) * pos];
-    lds[L.col.at(n)] = v;
-  }
+  stage_in<double2, 16>(lds, L, N, [&](int n) {
+    return valid ? line[(long)((n <= h) ? n : N - n) * pos] : mk<double2>(0, 0);
+  });
   __syncthreads();
-  fft_columns<double2, 16>(lds, L.col, L.tj, L.tpc, p, tw);
+  fft_columns<double2, 16, RC_ALL>(lds, L.col, L.tj, L.tpc, p, tw);
   if (!valid) return;
   for (int n = L.tj; n <= h; n += L.tpc) {
     double2 v = lds[L.col.at(n)];
@@ -501,6 +563,23 @@ static size_t lds_bytes(int N, int lgB, bool fp64, bool rows)
   return (size_t)(1 << lgB) * (rows ? N + 1 : N) * (fp64 ? 16 : 8);
 }
 
+// LDS of a pass: the columns, plus the twiddle table in FP32
+static size_t pass_lds(int N, int lgB, bool fp64, bool rows)
+{
+  return lds_bytes(N, lgB, fp64, rows) + (fp64 ? 0 : (size_t)N * 8);
+}
+
+static int radix_class(const FFTPlan1D &p)
+{
+  bool r8 = true, pow2 = true;
+  for (int i = 0; i < p.nst; i++) {
+    const int r = p.radix[i];
+    if (r != 8) r8 = false;
+    if (r != 8 && r != 4 && r != 2) pow2 = false;
+  }
+  return r8 ? RC_R8 : pow2 ? RC_POW2 : RC_ALL;
+}
+
 static void check_int(long v, const char *what)
 {
   if (v < 0 || v > 0x7fffffffL) throw Error{NPME_ERR_UNSUPPORTED, std::string(what) + " exceeds 2^31 elements"};
@@ -525,19 +604,27 @@ void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const
   c.outPos = (int)g.outPos;
   check_int(g.nCols * std::max<long>(1, g.N), "FFT pass");
   const dim3 grid((unsigned)((g.nCols + (1 << lgB) - 1) >> lgB));
-  const size_t sh = lds_bytes(g.N, lgB, fp64, rows);
-#define NPME_FFTC(R, ROWS)                                                                     \
-  hipLaunchKernelGGL((k_fft_complex<R, ROWS>), grid, dim3(NT), sh, s, c, to_dev(plan),         \
+  const int rc = radix_class(plan);
+  const size_t sh = pass_lds(g.N, lgB, fp64, rows);
+#define NPME_FFTC(R, ROWS, RCV)                                                                \
+  hipLaunchKernelGGL((k_fft_complex<R, ROWS, RCV>), grid, dim3(NT), sh, s, c, to_dev(plan),    \
                      (const typename Cx<R>::T *)tw, (const typename Cx<R>::T *)in,            \
                      (typename Cx<R>::T *)out, lgB, (int)backward)
+#define NPME_FFTC2(R, ROWS)                                                                    \
+  do {                                                                                         \
+    if (rc == RC_R8) NPME_FFTC(R, ROWS, RC_R8);                                                \
+    else if (rc == RC_POW2) NPME_FFTC(R, ROWS, RC_POW2);                                       \
+    else NPME_FFTC(R, ROWS, RC_ALL);                                                           \
+  } while (0)
   if (fp64) {
-    if (rows) NPME_FFTC(double, true);
-    else NPME_FFTC(double, false);
+    if (rows) NPME_FFTC2(double, true);
+    else NPME_FFTC2(double, false);
   } else {
-    if (rows) NPME_FFTC(float, true);
-    else NPME_FFTC(float, false);
+    if (rows) NPME_FFTC2(float, true);
+    else NPME_FFTC2(float, false);
   }
 #undef NPME_FFTC
+#undef NPME_FFTC2
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -549,15 +636,22 @@ void fft_pass_r2c_rows(bool fp64, int N, int Lin, long nRows, long inRowStride,
   const int nPairs = (int)(nRows / 2);
   const int lgB = lg_batch(N, fp64, true);
   const dim3 grid((unsigned)((nPairs + (1 << lgB) - 1) >> lgB));
-  const size_t sh = lds_bytes(N, lgB, fp64, true);
-  if (fp64)
-    hipLaunchKernelGGL(k_fft_r2c<double>, grid, dim3(NT), sh, s, N, Lin, nPairs, (int)inRowStride,
-                       (int)outRowStride, to_dev(plan), (const double2 *)tw, (const double *)in,
-                       (double2 *)out, lgB);
-  else
-    hipLaunchKernelGGL(k_fft_r2c<float>, grid, dim3(NT), sh, s, N, Lin, nPairs, (int)inRowStride,
-                       (int)outRowStride, to_dev(plan), (const float2 *)tw, (const float *)in,
-                       (float2 *)out, lgB);
+  const int rc = radix_class(plan);
+  const size_t sh = pass_lds(N, lgB, fp64, true);
+#define NPME_R2C(R, RCV)                                                                       \
+  hipLaunchKernelGGL((k_fft_r2c<R, RCV>), grid, dim3(NT), sh, s, N, Lin, nPairs, (int)inRowStride, \
+                     (int)outRowStride, to_dev(plan), (const typename Cx<R>::T *)tw,           \
+                     (const R *)in, (typename Cx<R>::T *)out, lgB)
+#define NPME_R2C2(R)                                                                           \
+  do {                                                                                         \
+    if (rc == RC_R8) NPME_R2C(R, RC_R8);                                                       \
+    else if (rc == RC_POW2) NPME_R2C(R, RC_POW2);                                              \
+    else NPME_R2C(R, RC_ALL);                                                                  \
+  } while (0)
+  if (fp64) NPME_R2C2(double);
+  else NPME_R2C2(float);
+#undef NPME_R2C
+#undef NPME_R2C2
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -569,15 +663,22 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
   const int nPairs = (int)(nRows / 2);
   const int lgB = lg_batch(N, fp64, true);
   const dim3 grid((unsigned)((nPairs + (1 << lgB) - 1) >> lgB));
-  const size_t sh = lds_bytes(N, lgB, fp64, true);
-  if (fp64)
-    hipLaunchKernelGGL(k_fft_c2r<double>, grid, dim3(NT), sh, s, N, Lout, nPairs, (int)inRowStride,
-                       (int)outRowStride, to_dev(plan), (const double2 *)tw, (const double2 *)in,
-                       (double *)out, lgB);
-  else
-    hipLaunchKernelGGL(k_fft_c2r<float>, grid, dim3(NT), sh, s, N, Lout, nPairs, (int)inRowStride,
-                       (int)outRowStride, to_dev(plan), (const float2 *)tw, (const float2 *)in,
-                       (float *)out, lgB);
+  const int rc = radix_class(plan);
+  const size_t sh = pass_lds(N, lgB, fp64, true);
+#define NPME_C2R(R, RCV)                                                                       \
+  hipLaunchKernelGGL((k_fft_c2r<R, RCV>), grid, dim3(NT), sh, s, N, Lout, nPairs, (int)inRowStride, \
+                     (int)outRowStride, to_dev(plan), (const typename Cx<R>::T *)tw,           \
+                     (const typename Cx<R>::T *)in, (R *)out, lgB)
+#define NPME_C2R2(R)                                                                           \
+  do {                                                                                         \
+    if (rc == RC_R8) NPME_C2R(R, RC_R8);                                                       \
+    else if (rc == RC_POW2) NPME_C2R(R, RC_POW2);                                              \
+    else NPME_C2R(R, RC_ALL);                                                                  \
+  } while (0)
+  if (fp64) NPME_C2R2(double);
+  else NPME_C2R2(float);
+#undef NPME_C2R
+#undef NPME_C2R2
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -590,20 +691,28 @@ void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, lon
   check_int(nCols * (long)Lin, "FFT pass");
   const int lgB = lg_batch(N, fp64, false);
   const dim3 grid((unsigned)((nCols + (1 << lgB) - 1) >> lgB));
-  const size_t sh = lds_bytes(N, lgB, fp64, false);
+  const int rc = radix_class(plan);
+  const size_t sh = pass_lds(N, lgB, fp64, false);
   const int N3fold = complexG ? (int)C3 : 0;
-#define NPME_CONV(R, CGV)                                                                      \
-  hipLaunchKernelGGL((k_fft_conv<R, CGV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,          \
+#define NPME_CONV(R, CGV, RCV)                                                                 \
+  hipLaunchKernelGGL((k_fft_conv<R, CGV, RCV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,     \
                      (int)pos_stride, N2, G2, (int)C3, N3fold, to_dev(plan),                    \
                      (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data, lgB)
+#define NPME_CONV2(R, CGV)                                                                     \
+  do {                                                                                         \
+    if (rc == RC_R8) NPME_CONV(R, CGV, RC_R8);                                                 \
+    else if (rc == RC_POW2) NPME_CONV(R, CGV, RC_POW2);                                        \
+    else NPME_CONV(R, CGV, RC_ALL);                                                            \
+  } while (0)
   if (fp64) {
-    if (complexG) NPME_CONV(double, true);
-    else NPME_CONV(double, false);
+    if (complexG) NPME_CONV2(double, true);
+    else NPME_CONV2(double, false);
   } else {
-    if (complexG) NPME_CONV(float, true);
-    else NPME_CONV(float, false);
+    if (complexG) NPME_CONV2(float, true);
+    else NPME_CONV2(float, false);
   }
 #undef NPME_CONV
+#undef NPME_CONV2
   NPME_HIP_CHECK(hipGetLastError());
 }
 

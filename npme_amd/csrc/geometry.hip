@@ -10,6 +10,7 @@
 #include <cstring>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocprim/device/device_scan.hpp>
 
 #include "npme_internal.h"
@@ -229,6 +230,38 @@ __global__ __launch_bounds__(256) void k_slice_len(size_t n, const int *__restri
   sliceWords[sl] = 64L * m;
 }
 
+__global__ __launch_bounds__(256) void k_sort_coords(size_t n, const int *__restrict__ perm,
+                                                     const double *__restrict__ coord,
+                                                     double4 *__restrict__ xs)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const size_t i = (size_t)perm[s];
+  xs[s] = make_double4(coord[3 * i], coord[3 * i + 1], coord[3 * i + 2], 0.0);
+}
+
+// per cell: count * (charges in the stencil around it), as double
+__global__ __launch_bounds__(256) void k_stencil_pairs(long ncell, int nb0, int nb1, int nb2,
+                                                       int nN, const int2 *__restrict__ cellSE,
+                                                       double *__restrict__ out)
+{
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int n0 = cellSE[c].y;
+  double sum = 0;
+  if (n0 > 0) {
+    const int g2 = (int)(c % nb2), g1 = (int)((c / nb2) % nb1), g0 = (int)(c / ((long)nb1 * nb2));
+    for (int dx = -nN; dx <= nN; dx++)
+      for (int dy = -nN; dy <= nN; dy++)
+        for (int dz = -nN; dz <= nN; dz++) {
+          const int a = g0 + dx, b = g1 + dy, e = g2 + dz;
+          if (a < 0 || b < 0 || e < 0 || a >= nb0 || b >= nb1 || e >= nb2) continue;
+          sum += (double)cellSE[((long)a * nb1 + b) * nb2 + e].y;
+        }
+  }
+  out[c] = (double)n0 * sum;
+}
+
 static dim3 grid_for(size_t n) { return dim3((unsigned)((n + 255) / 256)); }
 
 static int key_bits(unsigned long maxKey)
@@ -367,6 +400,32 @@ size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *
   NPME_HIP_CHECK(hipMemcpyAsync(&total, d_sliceStart + nSlices, sizeof(long), hipMemcpyDeviceToHost, s));
   NPME_HIP_CHECK(hipStreamSynchronize(s));
   return (size_t)total;
+}
+
+void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
+                          hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sort_coords, grid_for(n), dim3(256), 0, s, n, d_perm, d_coord, d_xs);
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellSE, hipStream_t s)
+{
+  const long ncell = (long)nb[0] * nb[1] * nb[2];
+  DevBuf per(ncell * sizeof(double)), tot(sizeof(double));
+  hipLaunchKernelGGL(k_stencil_pairs, grid_for(ncell), dim3(256), 0, s, ncell, nb[0], nb[1], nb[2],
+                     nNeigh, (const int2 *)d_cellSE, (double *)per.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  size_t bytes = 0;
+  NPME_HIP_CHECK(rocprim::reduce(nullptr, bytes, (const double *)per.p, (double *)tot.p, 0.0,
+                                 (size_t)ncell, rocprim::plus<double>(), s));
+  DevBuf tmp(bytes);
+  NPME_HIP_CHECK(rocprim::reduce(tmp.p, bytes, (const double *)per.p, (double *)tot.p, 0.0,
+                                 (size_t)ncell, rocprim::plus<double>(), s));
+  double v = 0;
+  NPME_HIP_CHECK(hipMemcpyAsync(&v, tot.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  return v;
 }
 
 }  // namespace npme

@@ -208,7 +208,8 @@ struct Handle {
   double *d_Vdir = nullptr;      // [n][4] (x2 for complex) direct sum, sorted order
   double *d_V = nullptr;         // [n][4] (x2) output staging, input order
   double *d_qin = nullptr;       // [n] (x2) input charges staging
-  double *d_coordS = nullptr;    // [n][3] sorted FP64 coordinates (exact sums)
+  double *d_coordS = nullptr;    // [n][3] FP64 input coordinates, input order (exact sums)
+  double4 *d_xs = nullptr;       // [n] FP64 coordinates in sorted order (pair search)
   FFTPlan1D plan[3];
   size_t deviceBytes = 0;
   bool deviceSetup = false;      // per-charge set-up ran on the device
@@ -249,6 +250,12 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
                              DevBuf &itileList, size_t &nTileEntries, hipStream_t s);
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s);
+// coordinates in sorted order: xs[s] = coord[perm[s]]
+void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
+                          hipStream_t s);
+// candidate pairs of the cell stencil: sum over cells of count * (counts of
+// the (2 nNeigh + 1)^3 cells around it)
+double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellSE, hipStream_t s);
 void handle_free_device(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout);
@@ -301,15 +308,15 @@ void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const do
 void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);
+unsigned nlist_pad();  // value of a padding entry of the neighbour list
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
-                  const int *perm, const double *coord, int nP, int *count, const long *sliceStart,
-                  unsigned *list, hipStream_t s);
+                  const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
+                  hipStream_t s);
 void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                       const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
                       double csA, double *Vdir, hipStream_t s);
 void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
-                        const int *perm, const double *coord, unsigned long long *total,
-                        hipStream_t s);
+                        const double4 *xs, unsigned long long *total, hipStream_t s);
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
                    const int nt[3], const int *tileStart, const int *tileList, const void *src,
                    const void *qi, const int4 *ms, const void *frac, void *grid, hipStream_t s);

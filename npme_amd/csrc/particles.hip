@@ -25,6 +25,7 @@ struct DirectDev {
   double a[62], b[62];    // DM coefficients (re, im), zero beyond Nder
   float af[62], bf[62];   // the same in FP32
   int nb0, nb1, nb2;
+  int fxShift;     // S (mixed form)
   int fxUnit;      // 2^S: one cluster edge in fixed-point units (mixed form)
   float fxInv;     // cluster edge / 2^S: length of one fixed-point unit
   float fxClose2;  // below this r^2 a pair adds the sub-unit residuals
@@ -104,7 +105,9 @@ void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const do
 template <typename Real> __device__ __forceinline__ Real rsqrt_r(Real x);
 __device__ __forceinline__ void sincos_r(float x, float *s, float *c) { sincosf(x, s, c); }
 __device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sincos(x, s, c); }
-template <> __device__ __forceinline__ float rsqrt_r<float>(float x) { return rsqrtf(x); }
+// v_rsq_f32 (1 ulp): r2 of a pair is never denormal, so the range fix-up
+// of rsqrtf is not needed
+template <> __device__ __forceinline__ float rsqrt_r<float>(float x) { return __builtin_amdgcn_rsqf(x); }
 template <> __device__ __forceinline__ double rsqrt_r<double>(double x) { return 1.0 / sqrt(x); }
 
 // Horner in r2 of a real coefficient set (stride-2 storage: re parts)
@@ -327,6 +330,7 @@ static DirectDev direct_dev(const DirectParams &p)
   d.nb0 = p.nb[0];
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
+  d.fxShift = p.fxShift;
   d.fxUnit = 1 << p.fxShift;
   d.fxInv = (float)p.fxInv;
   // pairs closer than 2^24 units: one unit exceeds FP32 relative accuracy
@@ -375,10 +379,14 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
 // sliceStart + k * 64 + l, so step k of a wavefront reads 256 contiguous
 // bytes.  An entry packs the source index (low 26 bits) with the offset of
 // the source's cluster from the target's cluster, dx+1, dy+1 and dz+1 in
-// two bits each (bits 26, 28, 30); padding entries are 0xffffffff.  The pairs are
+// two bits each (bits 26, 28, 30).  A padding entry is source 0 with the
+// code 3 on every axis, an offset of two clusters (at least one cluster edge
+// >= Rdir per axis): it is evaluated like any entry and fails r < Rdir, so
+// the loop needs no test for it.  The pairs are
 // found in FP64 with a cutoff inflated by 1e-9 relative; the direct-sum
 // kernel applies r < Rdir in its working precision.
-constexpr unsigned NL_PAD = 0xffffffffu;
+constexpr unsigned NL_PAD = 0xfc000000u;
+unsigned nlist_pad() { return NL_PAD; }
 constexpr int NL_SHIFT = 26;
 
 // walk the cells of the stencil of sorted target s in FP64 input
@@ -386,8 +394,7 @@ constexpr int NL_SHIFT = 26;
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4 *__restrict__ cell,
                                                const int2 *__restrict__ cellSE,
-                                               const int *__restrict__ perm,
-                                               const double *__restrict__ coord, int nP,
+                                               const double4 *__restrict__ xs, int nP,
                                                int *__restrict__ count,
                                                const long *__restrict__ sliceStart,
                                                unsigned *__restrict__ list)
@@ -395,8 +402,8 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
   const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int4 c = cell[s];
-  const long i = perm[s];
-  const double xi = coord[3 * i], yi = coord[3 * i + 1], zi = coord[3 * i + 2];
+  const double4 me = xs[s];
+  const double xi = me.x, yi = me.y, zi = me.z;
   const double R = p.Rdir * (1.0 + 1.0e-9);
   const double R2 = R * R;
   const int nN = p.nNeigh;
@@ -414,8 +421,8 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
                                          ((gz / nP - az + 1) << 4));
         for (int j = se.x; j < se.x + se.y; j++) {
           if ((size_t)j == s) continue;
-          const long o = perm[j];
-          const double rx = xi - coord[3 * o], ry = yi - coord[3 * o + 1], rz = zi - coord[3 * o + 2];
+          const double4 o = xs[j];
+          const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z;
           if (rx * rx + ry * ry + rz * rz < R2) {
             if (FILL) out[(long)k * 64] = (code << NL_SHIFT) | (unsigned)j;
             k++;
@@ -426,8 +433,8 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
 }
 
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
-                  const int *perm, const double *coord, int nP, int *count, const long *sliceStart,
-                  unsigned *list, hipStream_t s)
+                  const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
+                  hipStream_t s)
 {
   DirectDev d;
   d.Rdir = p.Rdir;
@@ -437,11 +444,11 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   d.nb2 = p.nb[2];
   const dim3 g((unsigned)((n + 255) / 256));
   if (fill)
-    hipLaunchKernelGGL(k_nlist<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, perm,
-                       coord, nP, count, sliceStart, list);
+    hipLaunchKernelGGL(k_nlist<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, xs,
+                       nP, count, sliceStart, list);
   else
-    hipLaunchKernelGGL(k_nlist<false>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, perm,
-                       coord, nP, count, sliceStart, list);
+    hipLaunchKernelGGL(k_nlist<false>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, xs,
+                       nP, count, sliceStart, list);
   NPME_HIP_CHECK(hipGetLastError());
 }
 
@@ -536,33 +543,34 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
   const unsigned *e = list + sliceStart[slice] + (s & 63);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // U entries per step: their source loads are independent, so each
-  // wavefront keeps U gathers in flight
+  // U entries per step.  Two buffer sets alternate (the loop body is the
+  // step unrolled twice), so while step i is evaluated the source gathers
+  // of step i+1 and the entries of step i+2 are in flight without register
+  // copies of loads in flight.  Lanes past n hold padding entries only.
   constexpr int U = 4;
-  for (int k0 = 0; k0 < len; k0 += U) {
-    unsigned ent[U];
-    V4 o[U];
-    Real oqi[U];
+  const unsigned ushift = (unsigned)p.fxShift;
+  auto load_ent = [&](int k0, unsigned *ent) {
 #pragma unroll
     for (int u = 0; u < U; u++) ent[u] = (k0 + u < len) ? e[(long)(k0 + u) * 64] : NL_PAD;
+  };
+  auto gather = [&](const unsigned *ent, V4 *o, Real *oqi) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      // padding entries load source 0 (always present) and are skipped below
-      const int j = (ent[u] == NL_PAD) ? 0 : (int)(ent[u] & ((1u << NL_SHIFT) - 1));
+      const int j = (int)(ent[u] & ((1u << NL_SHIFT) - 1));
       o[u] = src[j];
       if constexpr (CX) oqi[u] = qi[j];
     }
+  };
+  auto eval = [&](const unsigned *ent, const V4 *o, const Real *oqi) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (ent[u] == NL_PAD || !tvalid) continue;
       const unsigned en = ent[u];
-      const int cx = (int)((en >> NL_SHIFT) & 3), cy = (int)((en >> (NL_SHIFT + 2)) & 3),
-                cz = (int)(en >> (NL_SHIFT + 4));
       Real rx, ry, rz, r2;
       if constexpr (FIXP) {
-        const int dx = mx - cx * p.fxUnit - __float_as_int(o[u].x),
-                  dy = my - cy * p.fxUnit - __float_as_int(o[u].y),
-                  dz = mz - cz * p.fxUnit - __float_as_int(o[u].z);
+        // (me + unit) - o - code * unit, exact in int32
+        const int dx = (mx - __float_as_int(o[u].x)) - (int)(((en >> NL_SHIFT) & 3u) << ushift);
+        const int dy = (my - __float_as_int(o[u].y)) - (int)(((en >> (NL_SHIFT + 2)) & 3u) << ushift);
+        const int dz = (mz - __float_as_int(o[u].z)) - (int)((en >> (NL_SHIFT + 4)) << ushift);
         rx = (Real)dx * (Real)p.fxInv;
         ry = (Real)dy * (Real)p.fxInv;
         rz = (Real)dz * (Real)p.fxInv;
@@ -575,6 +583,8 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
           r2 = rx * rx + ry * ry + rz * rz;
         }
       } else {
+        const int cx = (int)((en >> NL_SHIFT) & 3), cy = (int)((en >> (NL_SHIFT + 2)) & 3),
+                  cz = (int)(en >> (NL_SHIFT + 4));
         rx = me.x - (o[u].x + (Real)(cx - 1) * cA);
         ry = me.y - (o[u].y + (Real)(cy - 1) * cA);
         rz = me.z - (o[u].z + (Real)(cz - 1) * cA);
@@ -582,6 +592,8 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
       }
       if (r2 < R2) pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o[u].w, CX ? oqi[u] : (Real)0, part);
     }
+  };
+  auto carry = [&](int k0) {
     if (((k0 + U) & 15) == 0) {  // FP64 carry every 16 entries
 #pragma unroll
       for (int c = 0; c < (CX ? 8 : 4); c++) {
@@ -589,6 +601,36 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
         part[c] = 0;
       }
     }
+  };
+  unsigned E[U], DA[U], DB[U];
+  V4 OA[U], OB[U];
+  Real QA[U], QB[U];
+  if (len > 0) {
+    load_ent(0, DA);
+    gather(DA, OA, QA);
+    load_ent(U, E);
+  }
+  for (int k0 = 0; k0 < len; k0 += 2 * U) {
+    // step k0 from (DA, OA); prefetch step k0+U into (DB, OB), entries k0+2U
+    const bool moreA = k0 + U < len;  // uniform per wavefront
+    if (moreA) {
+#pragma unroll
+      for (int u = 0; u < U; u++) DB[u] = E[u];
+      gather(DB, OB, QB);
+      load_ent(k0 + 2 * U, E);
+    }
+    eval(DA, OA, QA);
+    carry(k0);
+    if (!moreA) break;
+    // step k0+U from (DB, OB); prefetch step k0+2U into (DA, OA), entries k0+3U
+    if (k0 + 2 * U < len) {
+#pragma unroll
+      for (int u = 0; u < U; u++) DA[u] = E[u];
+      gather(DA, OA, QA);
+      load_ent(k0 + 3 * U, E);
+    }
+    eval(DB, OB, QB);
+    carry(k0 + U);
   }
   if (!tvalid) return;
 #pragma unroll
@@ -612,9 +654,15 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
   hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \
                      (const typename V4t<R>::T *)src, (const float4 *)res, (const R *)qi,      \
                      sliceStart, sliceLen, list, csA, Vdir)
+// Horner at the exact degree for Nder 4..8 (the coefficients above Nder
+// are zero, so a larger compile-time degree is also exact, only slower)
 #define NPME_DNL(R, K, HL)                                                                      \
   do {                                                                                          \
-    if (p.Nder <= 8) NPME_DNL_D(R, K, HL, 8);                                                   \
+    if (p.Nder <= 4) NPME_DNL_D(R, K, HL, 4);                                                   \
+    else if (p.Nder == 5) NPME_DNL_D(R, K, HL, 5);                                              \
+    else if (p.Nder == 6) NPME_DNL_D(R, K, HL, 6);                                              \
+    else if (p.Nder == 7) NPME_DNL_D(R, K, HL, 7);                                              \
+    else if (p.Nder == 8) NPME_DNL_D(R, K, HL, 8);                                              \
     else if (p.Nder <= 16) NPME_DNL_D(R, K, HL, 16);                                            \
     else NPME_DNL_D(R, K, HL, 30);                                                              \
   } while (0)
@@ -637,16 +685,14 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
 __global__ __launch_bounds__(256) void k_count_pairs(DirectDev p, size_t n,
                                                      const int4 *__restrict__ cell,
                                                      const int2 *__restrict__ cellSE,
-                                                     const int *__restrict__ perm,
-                                                     const double *__restrict__ coord,
+                                                     const double4 *__restrict__ xs,
                                                      unsigned long long *__restrict__ total)
 {
   const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long cnt = 0;
   if (s < n) {
     const int4 c = cell[s];
-    const long i = perm[s];
-    const double xi = coord[3 * i], yi = coord[3 * i + 1], zi = coord[3 * i + 2];
+    const double xi = xs[s].x, yi = xs[s].y, zi = xs[s].z;
     const double R2 = p.Rdir * p.Rdir;
     const int nN = p.nNeigh;
     for (int dx = -nN; dx <= nN; dx++)
@@ -657,8 +703,8 @@ __global__ __launch_bounds__(256) void k_count_pairs(DirectDev p, size_t n,
           const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
           for (int j = se.x; j < se.x + se.y; j++) {
             if ((size_t)j == s) continue;
-            const long o = perm[j];
-            const double rx = xi - coord[3 * o], ry = yi - coord[3 * o + 1], rz = zi - coord[3 * o + 2];
+            const double4 o = xs[j];
+            const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z;
             if (rx * rx + ry * ry + rz * rz < R2) cnt++;
           }
         }
@@ -669,8 +715,7 @@ __global__ __launch_bounds__(256) void k_count_pairs(DirectDev p, size_t n,
 }
 
 void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
-                        const int *perm, const double *coord, unsigned long long *total,
-                        hipStream_t s)
+                        const double4 *xs, unsigned long long *total, hipStream_t s)
 {
   DirectDev d;
   d.Rdir = p.Rdir;
@@ -679,7 +724,7 @@ void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
   hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, n, cell,
-                     (const int2 *)cellStart, perm, coord, total);
+                     (const int2 *)cellStart, xs, total);
   NPME_HIP_CHECK(hipGetLastError());
 }
 

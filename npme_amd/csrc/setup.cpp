@@ -45,7 +45,7 @@ void handle_free_device(Handle &h)
   void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
-                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS};
+                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs};
   // queued work of this handle may still read its buffers
   if (h.stream) (void)hipStreamSynchronize(h.stream);
   for (void *p : ptrs) dev_free(p);
@@ -66,6 +66,7 @@ void handle_free_device(Handle &h)
   h.d_grid = h.d_buf1 = h.d_buf2 = h.d_G = nullptr;
   h.d_tw[0] = h.d_tw[1] = h.d_tw[2] = nullptr;
   h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
+  h.d_xs = nullptr;
   h.deviceBytes = 0;
 }
 
@@ -140,14 +141,31 @@ static double tsmooth(double x, double X, double Xmax, double a)
 // the entry code (nNeigh <= nCellClust1D) and the list fits a quarter of the
 // free device memory ('on' lifts the memory limit); otherwise the direct
 // sum walks the cell rows of each block (k_direct).
+// Expected list size from the cell occupancy alone: the candidate pairs of
+// the cell stencil times the fraction of the stencil volume inside the
+// cutoff sphere.  When even half of it exceeds the budget (clustered charges
+// with thousands per cell), the exact count, a scan of every candidate
+// pair, is skipped.
+static bool nlist_clearly_too_big(Handle &h, const std::string &mode)
+{
+  if (mode == "on") return false;
+  const double cand = stencil_candidates_device(h.nb, (int)h.nNeigh, h.d_cellStart, h.stream);
+  const double side = (2.0 * h.nNeigh + 1.0) * h.csB, R = h.ker.Rdir;
+  const double est = cand * (4.18879020478639098 * R * R * R) / (side * side * side);
+  size_t freeB = 0, totalB = 0;
+  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  return 0.5 * est * sizeof(unsigned) > (double)(freeB / 4);
+}
+
 static void build_neighbor_list(Handle &h, const std::string &mode)
 {
+  if (nlist_clearly_too_big(h, mode)) return;
   const DirectParams dp = direct_params(h);
   const int nP = (int)h.nCellClust1D;
   std::vector<int> cnt(h.n);
   {
     DevBuf d_count(h.n * sizeof(int));
-    launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP,
+    launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP,
                  (int *)d_count.p, nullptr, nullptr, h.stream);
     NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count.p, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
     NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -170,9 +188,9 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   h.d_sliceLen = dalloc<int>(h, nSlices);
   NPME_HIP_CHECK(hipMemcpy(h.d_sliceStart, sliceStart.data(), (nSlices + 1) * sizeof(long), hipMemcpyHostToDevice));
   NPME_HIP_CHECK(hipMemcpy(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int), hipMemcpyHostToDevice));
-  NPME_HIP_CHECK(hipMemsetAsync(h.d_nlist, 0xff, entries * sizeof(unsigned), h.stream));
-  launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, nP, nullptr,
-               h.d_sliceStart, h.d_nlist, h.stream);
+  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(), entries, h.stream));
+  launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP, nullptr, h.d_sliceStart,
+               h.d_nlist, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   h.nlistEntries = entries;
   h.useNlist = true;
@@ -212,8 +230,11 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.d_cell = dalloc<int4>(h, n);
   h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
   geometry_cells_device(g, n, h.d_coordS, h.d_perm, h.d_cell, h.d_cellStart, s);
+  h.d_xs = dalloc<double4>(h, n);
+  sorted_coords_device(n, h.d_perm, h.d_coordS, h.d_xs, s);
 
   // neighbour list: count, slice lengths and offsets, then fill
+  if (nlist_clearly_too_big(h, nlmode)) return false;
   const DirectParams dp = direct_params(h);
   const size_t nSlices = (n + 63) / 64;
   size_t entries = 0;
@@ -221,17 +242,17 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.d_sliceLen = dalloc<int>(h, nSlices);
   {
     DevBuf cnt(n * sizeof(int));
-    launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, (int)h.nCellClust1D,
-                 (int *)cnt.p, nullptr, nullptr, s);
+    launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, (int *)cnt.p,
+                 nullptr, nullptr, s);
     entries = nlist_slices_device(n, (const int *)cnt.p, h.d_sliceLen, h.d_sliceStart, s);
   }
   size_t freeB = 0, totalB = 0;
   NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
   if (nlmode != "on" && entries * sizeof(unsigned) > freeB / 4) return false;
   h.d_nlist = dalloc<unsigned>(h, entries);
-  NPME_HIP_CHECK(hipMemsetAsync(h.d_nlist, 0xff, std::max<size_t>(entries, 1) * sizeof(unsigned), s));
-  launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS, (int)h.nCellClust1D,
-               nullptr, h.d_sliceStart, h.d_nlist, s);
+  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(), std::max<size_t>(entries, 1), s));
+  launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
+               h.d_sliceStart, h.d_nlist, s);
   h.nlistEntries = entries;
   h.useNlist = true;
 
@@ -483,6 +504,8 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
   h.d_coordS = dalloc<double>(h, 3 * n);
   up(h.d_coordS, coord, 3 * n * sizeof(double));
+  h.d_xs = dalloc<double4>(h, n);
+  sorted_coords_device(n, h.d_perm, h.d_coordS, h.d_xs, h.stream);
 
   // ---- direct sum: the neighbour list when it fits, else the cell rows;
   // the fixed-point scale depends on which of the two reads the positions
@@ -758,7 +781,7 @@ double handle_count_pairs(Handle &h)
   unsigned long long tot = 0;
   DevBuf d_tot(sizeof(unsigned long long));
   NPME_HIP_CHECK(hipMemsetAsync(d_tot.p, 0, sizeof(unsigned long long), h.stream));
-  launch_count_pairs(direct_params(h), h.n, h.d_cell, h.d_cellStart, h.d_perm, h.d_coordS,
+  launch_count_pairs(direct_params(h), h.n, h.d_cell, h.d_cellStart, h.d_xs,
                      (unsigned long long *)d_tot.p, h.stream);
   NPME_HIP_CHECK(hipMemcpyAsync(&tot, d_tot.p, sizeof(tot), hipMemcpyDeviceToHost, h.stream));
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));

@@ -1,0 +1,118 @@
+"""GPU runs of the full-size configurations beside the N = 1M metric, each
+checked against the exact FP64 direct sum (npme_calcv_exact_subset, the
+compareV metric) on a random subset of targets:
+
+  r^0.5   N = 200,000 in 100 x 100 x 400, Bn 6, Rdir 8, tol 1e-6 (the RSEM
+          table has no alpha = 0.5 row: grid of the alpha = 1 model)
+  Helmholtz k0 = 0.5, complex charges N(0,1), N = 500,000 in 120^3,
+          Rdir 10, Bn 8, tol 1e-6
+  1/r     N = 4,000,000 in 64 Gaussian blobs (sigma 5, centres in 400^3),
+          Bn 6, Rdir 8, tol 1e-6: thousands of charges per cell in the
+          blob cores, so the pairs (~1e11) cannot be listed and the
+          cell-row direct sum runs
+
+The mixed form must meet the requested tolerance 1e-6 for 1/r; for r^0.5
+the FP64 path must meet 1e-6 and the mixed form 2e-6 (its FP32 reciprocal
+sum, which dominates V for a growing kernel, rounds at ~1e-6), and 2e-6 for
+the FP32 complex Helmholtz path.  Each test prints its set-up and
+CalcV wall times (host arrays, including the PCIe copies)."""
+import time
+
+import numpy as np
+import pytest
+
+from conftest import uniform_box
+
+pytestmark = pytest.mark.gpu
+
+
+def _subset_error(h, q, V, ntarget, seed):
+    idx = np.sort(np.random.default_rng(seed).choice(len(q), ntarget, replace=False))
+    Ve = h.calcv_exact_subset(q, idx)
+    d = V[idx] - Ve
+    magV = np.mean(np.abs(Ve[:, 0]))
+    relV = float(np.sqrt(np.mean(np.abs(d[:, 0]) ** 2)) / magV)
+    reldV = float(np.sqrt(np.mean(np.abs(d[:, 1:]) ** 2)) / np.mean(np.abs(Ve[:, :3])))
+    return relV, reldV
+
+
+def _timed_setup(setup_fn):
+    t0 = time.perf_counter()
+    h = setup_fn()
+    return h, time.perf_counter() - t0
+
+
+def _timed_calcv(h, q, reps=3):
+    best, V = 1e9, None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        V = h.calcv(q)
+        best = min(best, time.perf_counter() - t0)
+    return V, best
+
+
+def test_config_ralpha_200k():
+    import npme_amd as N
+    coord, q = uniform_box(200000, (100.0, 100.0, 400.0), 2001, 2002)
+    print("case config r^0.5 200k", flush=True)
+    # grid of the alpha = 1 error model
+    k1 = N.NPME_KeywordInput(calcType="pme", funcType="Ralpha", alpha=1.0, Rdir=8.0, BnOrder=6,
+                             tol=1e-6)
+    assert N.NPME_AddMissingKeywords(k1, len(q), coord, chargeReal=q), N.npme_last_error()
+    grid = {key: k1.GetValue(key) for key in ("N1", "N2", "N3", "n1", "n2", "n3", "nDeriv")}
+    res = {}
+    for prec in ("fp64", "mixed"):
+        k = N.NPME_KeywordInput(calcType="pme", funcType="Ralpha", alpha=0.5, Rdir=8.0, BnOrder=6,
+                                tol=1e-6, precision=prec, **grid)
+        assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q), N.npme_last_error()
+        h, ts = _timed_setup(lambda: N.NPME_Interface_SetUpRealKernel(k, len(q), coord))
+        assert h is not None, N.npme_last_error()
+        V, tc = _timed_calcv(h, q)
+        res[prec] = _subset_error(h, q, V, 2000, 2003)
+        print("r^0.5 200k", prec, "grid", grid, "setup %.3f s calcv %.2f ms relV %.2e reldV %.2e"
+              % ((ts, tc * 1e3) + res[prec]))
+    # the grid meets the tolerance (FP64 path); the mixed form adds the FP32
+    # rounding of the far-field-dominated r^0.5 reciprocal sum (~1e-6)
+    assert res["fp64"][0] <= 1e-6
+    assert res["mixed"][0] <= 2e-6
+
+
+def test_config_helmholtz_500k():
+    import npme_amd as N
+    coord, _ = uniform_box(500000, 120.0, 3001, 3002)
+    rng = np.random.default_rng(3003)
+    qc = rng.standard_normal(500000) + 1j * rng.standard_normal(500000)
+    print("case config helmholtz 500k", flush=True)
+    k = N.NPME_KeywordInput(calcType="pme", funcType="Helmholtz", k0_r=0.5, k0_i=0.0, Rdir=10.0,
+                            BnOrder=8, tol=1e-6, precision="mixed")
+    assert N.NPME_AddMissingKeywords(k, len(qc), coord, chargeComplex=qc), N.npme_last_error()
+    h, ts = _timed_setup(lambda: N.NPME_Interface_SetUpComplexKernel(k, len(qc), coord))
+    assert h is not None, N.npme_last_error()
+    assert (h.Get_N1(), h.Get_N2(), h.Get_N3()) == (280, 280, 280)
+    V, tc = _timed_calcv(h, qc)
+    relV, reldV = _subset_error(h, qc, V, 2000, 3004)
+    print("helmholtz 500k grid 280^3 setup %.3f s calcv %.2f ms relV %.2e reldV %.2e"
+          % (ts, tc * 1e3, relV, reldV))
+    assert relV <= 2e-6
+
+
+def test_config_blobs_4m():
+    import npme_amd as N
+    rng = np.random.default_rng(4001)
+    n, nblob = 4000000, 64
+    cen = rng.uniform(0.0, 400.0, (nblob, 3))
+    coord = cen[rng.integers(0, nblob, n)] + rng.normal(0.0, 5.0, (n, 3))
+    q = rng.uniform(-1.0, 1.0, n)
+    q -= q.mean()
+    print("case config blobs 4M", flush=True)
+    k = N.NPME_KeywordInput(calcType="pme", funcType="Laplace", Rdir=8.0, BnOrder=6, tol=1e-6,
+                            precision="mixed")
+    assert N.NPME_AddMissingKeywords(k, n, coord, chargeReal=q), N.npme_last_error()
+    h, ts = _timed_setup(lambda: N.NPME_Interface_SetUpRealKernel(k, n, coord))
+    assert h is not None, N.npme_last_error()
+    print(h.Print(), flush=True)
+    V, tc = _timed_calcv(h, q, reps=1)
+    relV, reldV = _subset_error(h, q, V, 4000, 4002)
+    print("blobs 4M grid", (h.Get_N1(), h.Get_N2(), h.Get_N3()),
+          "setup %.3f s calcv %.2f ms relV %.2e reldV %.2e" % (ts, tc * 1e3, relV, reldV))
+    assert relV <= 1e-6
