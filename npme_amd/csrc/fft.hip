@@ -180,14 +180,15 @@ template <typename T, int R> __device__ __forceinline__ void dft_any(T *x)
 // element n of the thread's column: [c][n] (row layout, stride ld = N + 1)
 // or [n][c] (strided layout, B columns side by side)
 struct ColRef {
-  int base, step;  // lds index = base + n * step
-  __device__ __forceinline__ int at(int n) const { return base + n * step; }
+  int base, sh;  // lds index = base + n * 2^sh (the step is 1 or the column count)
+  __device__ __forceinline__ int at(int n) const { return base + (n << sh); }
 };
 
 // ------------------------------------------------------------ Stockham stage
 // butterfly j of a column reads x[j + r N/R], twiddles by w^(r k N/(Ns R)),
-// k = j mod Ns, and writes y[(j - k) R + k + r Ns].
-template <typename T, int R, int BPT>
+// k = j mod Ns, and writes y[(j - k) R + k + r Ns].  POW2: N and Ns are
+// powers of two, so the modulo is a mask (the generic form divides).
+template <typename T, int R, int BPT, bool POW2>
 __device__ __forceinline__ void stage(T *lds, ColRef col, int tj, int tpc, int N, int Ns,
                                       const T *__restrict__ tw)
 {
@@ -207,7 +208,7 @@ __device__ __forceinline__ void stage(T *lds, ColRef col, int tj, int tpc, int N
   for (int u = 0; u < BPT; u++) {
     const int j = tj + u * tpc;
     if (j < nb) {
-      const int k = (Ns == 1) ? 0 : j % Ns;
+      const int k = (Ns == 1) ? 0 : (POW2 ? (j & (Ns - 1)) : j % Ns);
       if (k != 0) {
 #pragma unroll
         for (int r = 1; r < R; r++) v[u][r] = cmul(v[u][r], tw[r * k * stride]);
@@ -232,25 +233,26 @@ template <typename T, int EMAX, int RC>
 __device__ void fft_columns(T *lds, ColRef col, int tj, int tpc, const PlanDev &p,
                             const T *__restrict__ tw)
 {
+  constexpr bool P2 = RC != RC_ALL;
   int Ns = 1;
   for (int s = 0; s < p.nst; s++) {
     const int R = p.radix[s];
     switch (R) {
-      case 8: stage<T, 8, (EMAX + 7) / 8>(lds, col, tj, tpc, p.N, Ns, tw); break;
+      case 8: stage<T, 8, (EMAX + 7) / 8, P2>(lds, col, tj, tpc, p.N, Ns, tw); break;
       case 4:
-        if constexpr (RC != RC_R8) stage<T, 4, (EMAX + 3) / 4>(lds, col, tj, tpc, p.N, Ns, tw);
+        if constexpr (RC != RC_R8) stage<T, 4, (EMAX + 3) / 4, P2>(lds, col, tj, tpc, p.N, Ns, tw);
         break;
       case 2:
-        if constexpr (RC != RC_R8) stage<T, 2, (EMAX + 1) / 2>(lds, col, tj, tpc, p.N, Ns, tw);
+        if constexpr (RC != RC_R8) stage<T, 2, (EMAX + 1) / 2, P2>(lds, col, tj, tpc, p.N, Ns, tw);
         break;
       case 3:
-        if constexpr (RC == RC_ALL) stage<T, 3, (EMAX + 2) / 3>(lds, col, tj, tpc, p.N, Ns, tw);
+        if constexpr (RC == RC_ALL) stage<T, 3, (EMAX + 2) / 3, P2>(lds, col, tj, tpc, p.N, Ns, tw);
         break;
       case 5:
-        if constexpr (RC == RC_ALL) stage<T, 5, (EMAX + 4) / 5>(lds, col, tj, tpc, p.N, Ns, tw);
+        if constexpr (RC == RC_ALL) stage<T, 5, (EMAX + 4) / 5, P2>(lds, col, tj, tpc, p.N, Ns, tw);
         break;
       case 7:
-        if constexpr (RC == RC_ALL) stage<T, 7, (EMAX + 6) / 7>(lds, col, tj, tpc, p.N, Ns, tw);
+        if constexpr (RC == RC_ALL) stage<T, 7, (EMAX + 6) / 7, P2>(lds, col, tj, tpc, p.N, Ns, tw);
         break;
       default: break;
     }
@@ -284,7 +286,7 @@ __device__ __forceinline__ Lane lane_strided(int lgB, int N)
   L.tj = threadIdx.x >> lgB;
   L.tpc = NT >> lgB;
   L.col.base = L.c;
-  L.col.step = B;
+  L.col.sh = lgB;
   (void)N;
   return L;
 }
@@ -295,7 +297,7 @@ __device__ __forceinline__ Lane lane_rows(int lgB, int N)
   L.c = threadIdx.x >> (8 - lgB);  // NT = 2^8
   L.tj = threadIdx.x & (L.tpc - 1);
   L.col.base = L.c * (N + 1);
-  L.col.step = 1;
+  L.col.sh = 0;
   return L;
 }
 
@@ -342,7 +344,7 @@ __device__ __forceinline__ const T *twiddles_in(T *lds, int ldsElems, int N, con
 // element n at o * outer + i * inner + n * pos (all element counts < 2^31).
 struct CGeom {
   int N, Lin, Lout;
-  int nCols, nInner;
+  int nCols, nInner, validInner;  // inner index >= validInner: padding column
   int inOuter, inInner, inPos, outOuter, outInner, outPos;
 };
 
@@ -358,8 +360,9 @@ __global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const ty
   const int N = g.N;
   const Lane L = ROWS ? lane_rows(lgB, N) : lane_strided(lgB, N);
   const int colg = (xcd_wg() << lgB) + L.c;
-  const bool valid = colg < g.nCols;
-  const int o = valid ? colg / g.nInner : 0, i = valid ? colg - o * g.nInner : 0;
+  const int o0 = colg / g.nInner;
+  const bool valid = colg < g.nCols && colg - o0 * g.nInner < g.validInner;
+  const int o = valid ? o0 : 0, i = valid ? colg - o0 * g.nInner : 0;
   const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (ROWS ? N + 1 : N), N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
@@ -456,7 +459,7 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
 // outputs.  Columns c = k2 * C3 + k3; G index (|k1|, |k2|, k3 or |k3|).
 template <typename Real, bool CG, int RC>
 __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int pos, int N2, int G2,
-                                                 int C3, int N3fold, PlanDev p,
+                                                 int C3, int Kv, int N3fold, PlanDev p,
                                                  const typename Cx<Real>::T *tw, const void *Gv,
                                                  typename Cx<Real>::T *__restrict__ data, int lgB)
 {
@@ -465,7 +468,8 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   T *lds = reinterpret_cast<T *>(smem_raw);
   const Lane L = lane_strided(lgB, N);
   const int colg = (xcd_wg() << lgB) + L.c;
-  const bool valid = colg < nCols;
+  // C3: row length of the data (padded); Kv: valid entries of a row
+  const bool valid = colg < nCols && colg % C3 < Kv;
   T *col = data + (valid ? colg : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * N, N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
@@ -474,7 +478,7 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   // multiply by G and conjugate (for the backward transform)
-  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : C3;
+  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : Kv;
   const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
   const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
   const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
@@ -504,6 +508,232 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
   for (int n = L.tj; n < Lin; n += L.tpc) col[(long)n * pos] = cconj(lds[L.col.at(n)]);
+}
+
+// ------------------------------------------------------------ N = 8^L passes
+// Strided passes (B, C, D) for N = 8^L (64, 512) with every index known at
+// compile time: 4096 / N columns per workgroup, each thread owns two
+// radix-8 butterflies of one column per stage.  The first stage reads its
+// eight inputs straight from global memory (rows j + r N/8; with HIN only
+// the first N/2 rows are non-zero, so four of them) and the last stage
+// writes straight to global memory, so LDS carries only the L-1 exchanges
+// between stages.  In the convolution the outputs of the forward transform's
+// last stage are exactly the inputs of the backward transform's first stage
+// for the same thread (rows j + r N/8), so the k-space multiply and the
+// conjugation happen in registers between the two transforms.
+template <typename T> __device__ __forceinline__ void dft8_half(T *x)
+{
+  // DFT8 of (x0, x1, x2, x3, 0, 0, 0, 0)
+  using S = decltype(T::x);
+  const S c = (S)0.70710678118654752440;
+  // even part DFT4(x0, x2, 0, 0) = (x0 + x2, x0 - i x2, x0 - x2, x0 + i x2)
+  const T e0 = cadd(x[0], x[2]), e2 = csub(x[0], x[2]);
+  const T e1 = cadd(x[0], mul_mi(x[2])), e3 = csub(x[0], mul_mi(x[2]));
+  const T o0 = cadd(x[1], x[3]), o2 = csub(x[1], x[3]);
+  const T o1 = cadd(x[1], mul_mi(x[3])), o3 = csub(x[1], mul_mi(x[3]));
+  const T t1 = mk<T>(c * (o1.x + o1.y), c * (o1.y - o1.x));   // * exp(-i pi/4)
+  const T t2 = mul_mi(o2);                                    // * -i
+  const T t3 = mk<T>(c * (o3.y - o3.x), -c * (o3.x + o3.y));  // * exp(-3i pi/4)
+  x[0] = cadd(e0, o0);
+  x[4] = csub(e0, o0);
+  x[1] = cadd(e1, t1);
+  x[5] = csub(e1, t1);
+  x[2] = cadd(e2, t2);
+  x[6] = csub(e2, t2);
+  x[3] = cadd(e3, t3);
+  x[7] = csub(e3, t3);
+}
+
+template <int L> struct P8 {
+  static constexpr int N = L == 2 ? 64 : 512;
+  static constexpr int NB = N / 8;          // butterflies per column and stage
+  static constexpr int LGB = L == 2 ? 6 : 3;  // columns per workgroup: 4096 / N
+  static constexpr int B = 1 << LGB;
+  static constexpr int TPC = NT / B;        // threads per column
+  static constexpr int BPT = NB / TPC;      // butterflies per thread
+};
+
+// middle stage NS of a column: read x[j + r NB], twiddle, DFT8, write
+// y[(j - k) 8 + k + r NS] (LDS index * B + c), with a barrier between the
+// reads and the writes
+template <typename T, int L, int NS>
+__device__ __forceinline__ void mid8(T *lds, int c, int tj, const T *tw)
+{
+  using P = P8<L>;
+  T v[P::BPT][8];
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[u][r] = lds[((j + r * P::NB) << P::LGB) + c];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC, k = j & (NS - 1);
+    constexpr int stride = P::N / (8 * NS);
+#pragma unroll
+    for (int r = 1; r < 8; r++) v[u][r] = cmul(v[u][r], tw[r * k * stride]);
+    DFT<T, 8>::run(v[u]);
+    const int o = (j - k) * 8 + k;
+#pragma unroll
+    for (int r = 0; r < 8; r++) lds[((o + r * NS) << P::LGB) + c] = v[u][r];
+  }
+  __syncthreads();
+}
+
+// last stage (NS = NB): k = j, outputs rows j + r NB, kept in registers
+template <typename T, int L>
+__device__ __forceinline__ void last8(T *lds, int c, int tj, const T *tw, T (*v)[8])
+{
+  using P = P8<L>;
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[u][r] = lds[((j + r * P::NB) << P::LGB) + c];
+  }
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 1; r < 8; r++) v[u][r] = cmul(v[u][r], tw[r * j]);
+    DFT<T, 8>::run(v[u]);
+  }
+}
+
+// first stage (NS = 1) from registers holding rows j + r NB: DFT8, write
+// y[8 j + r]
+template <typename T, int L, bool HALF>
+__device__ __forceinline__ void first8(T *lds, int c, int tj, T (*v)[8])
+{
+  using P = P8<L>;
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+    if constexpr (HALF) dft8_half(v[u]);
+    else DFT<T, 8>::run(v[u]);
+#pragma unroll
+    for (int r = 0; r < 8; r++) lds[((8 * j + r) << P::LGB) + c] = v[u][r];
+  }
+  __syncthreads();
+}
+
+// the transform of one column after its first stage: stages 1 .. L-1
+template <typename T, int L>
+__device__ __forceinline__ void rest8(T *lds, int c, int tj, const T *tw, T (*v)[8])
+{
+  if constexpr (L == 3) mid8<T, L, 8>(lds, c, tj, tw);
+  last8<T, L>(lds, c, tj, tw, v);
+}
+
+// passes B and D: column o * nInner + i, element n at o * outer + i * inner + n * pos
+template <typename Real, int L, bool HIN>
+__global__ __launch_bounds__(NT) void k_fft8_cols(CGeom g, const typename Cx<Real>::T *tw,
+                                                  const typename Cx<Real>::T *__restrict__ in,
+                                                  typename Cx<Real>::T *__restrict__ out,
+                                                  int backward)
+{
+  using T = typename Cx<Real>::T;
+  using P = P8<L>;
+  extern __shared__ char smem_raw[];
+  T *lds = reinterpret_cast<T *>(smem_raw);
+  const int c = threadIdx.x & (P::B - 1), tj = threadIdx.x >> P::LGB;
+  const int colg = (xcd_wg() << P::LGB) + c;
+  const int o0 = colg / g.nInner;
+  const bool valid = colg < g.nCols && colg - o0 * g.nInner < g.validInner;
+  const int o = valid ? o0 : 0, i = valid ? colg - o0 * g.nInner : 0;
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, P::B * P::N, P::N, tw);
+  const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
+  T v[P::BPT][8];
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      T x = mk<T>(0, 0);
+      const int n = j + r * P::NB;
+      if ((!HIN || r < 4) && valid && n < g.Lin) {
+        x = src[(long)n * g.inPos];
+        if (backward) x = cconj(x);
+      }
+      v[u][r] = x;
+    }
+  }
+  // the barrier of first8 also orders the twiddle copy before its use
+  first8<T, L, HIN>(lds, c, tj, v);
+  rest8<T, L>(lds, c, tj, twl, v);
+  if (!valid) return;
+  T *dst = out + ((long)o * g.outOuter + (long)i * g.outInner);
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int n = j + r * P::NB;
+      if (n < g.Lout) dst[(long)n * g.outPos] = backward ? cconj(v[u][r]) : v[u][r];
+    }
+  }
+}
+
+// pass C: forward along axis 1 (the first N/2 rows non-zero), times G,
+// backward, the first N/2 rows kept; columns c = k2 * C3 + k3, Kv valid k3
+template <typename Real, int L, bool CG>
+__global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, int G2, int C3, int Kv,
+                                                  int N3fold, const typename Cx<Real>::T *tw,
+                                                  const void *Gv,
+                                                  typename Cx<Real>::T *__restrict__ data)
+{
+  using T = typename Cx<Real>::T;
+  using P = P8<L>;
+  constexpr int N = P::N;
+  extern __shared__ char smem_raw[];
+  T *lds = reinterpret_cast<T *>(smem_raw);
+  const int c = threadIdx.x & (P::B - 1), tj = threadIdx.x >> P::LGB;
+  const int colg = (xcd_wg() << P::LGB) + c;
+  const bool valid = colg < nCols && colg % C3 < Kv;
+  T *col = data + (valid ? colg : 0);
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, P::B * N, N, tw);
+  // the G values of the thread's rows, fetched up front
+  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : Kv;
+  const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
+  const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
+  const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
+  const long gbase = (long)a2 * G3 + a3, gstep = (long)G2 * G3;
+  using GT = typename std::conditional<CG, T, Real>::type;
+  GT gv[P::BPT][8];
+  T v[P::BPT][8];
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int n = j + r * P::NB;
+      const int a1 = (n <= N / 2) ? n : N - n;
+      gv[u][r] = reinterpret_cast<const GT *>(Gv)[a1 * gstep + gbase];
+      v[u][r] = (r < 4 && valid) ? col[(long)n * pos] : mk<T>(0, 0);
+    }
+  }
+  first8<T, L, true>(lds, c, tj, v);
+  rest8<T, L>(lds, c, tj, twl, v);
+  // times G, conjugated for the backward transform (rows j + r NB)
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++)
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      if constexpr (CG) v[u][r] = cconj(cmul(v[u][r], gv[u][r]));
+      else v[u][r] = mk<T>(gv[u][r] * v[u][r].x, -gv[u][r] * v[u][r].y);
+    }
+  __syncthreads();  // every thread has read the last forward stage
+  first8<T, L, false>(lds, c, tj, v);
+  rest8<T, L>(lds, c, tj, twl, v);
+  if (!valid) return;
+#pragma unroll
+  for (int u = 0; u < P::BPT; u++) {
+    const int j = tj + u * P::TPC;
+#pragma unroll
+    for (int r = 0; r < 4; r++) col[(long)(j + r * P::NB) * pos] = cconj(v[u][r]);
+  }
 }
 
 // set-up: even extension of a compact line (length N/2+1) to length N,
@@ -569,6 +799,15 @@ static size_t pass_lds(int N, int lgB, bool fp64, bool rows)
   return lds_bytes(N, lgB, fp64, rows) + (fp64 ? 0 : (size_t)N * 8);
 }
 
+// L when the plan is L radix-8 stages of N = 8^L with L in {2, 3}, else 0
+static int pow8_level(const FFTPlan1D &p)
+{
+  if ((p.N != 64 && p.N != 512) || p.nst != (p.N == 64 ? 2 : 3)) return 0;
+  for (int i = 0; i < p.nst; i++)
+    if (p.radix[i] != 8) return 0;
+  return p.nst;
+}
+
 static int radix_class(const FFTPlan1D &p)
 {
   bool r8 = true, pow2 = true;
@@ -586,7 +825,7 @@ static void check_int(long v, const char *what)
 }
 
 void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const void *tw,
-                      const void *in, void *out, bool backward, hipStream_t s)
+                      const void *in, void *out, bool backward, hipStream_t s, bool pow8)
 {
   const bool rows = (g.inPos == 1 && g.outPos == 1);
   const int lgB = lg_batch(g.N, fp64, rows);
@@ -596,6 +835,7 @@ void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const
   c.Lout = g.Lout;
   c.nCols = (int)g.nCols;
   c.nInner = (int)g.nInner;
+  c.validInner = g.validInner < 0 ? (int)g.nInner : (int)g.validInner;
   c.inOuter = (int)g.inOuter;
   c.inInner = (int)g.inInner;
   c.inPos = (int)g.inPos;
@@ -604,6 +844,32 @@ void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const
   c.outPos = (int)g.outPos;
   check_int(g.nCols * std::max<long>(1, g.N), "FFT pass");
   const dim3 grid((unsigned)((g.nCols + (1 << lgB) - 1) >> lgB));
+  const int L8 = pow8 ? pow8_level(plan) : 0;
+  if (!rows && L8 > 0 && (g.Lin == g.N || 2 * g.Lin == g.N) && g.Lout <= g.N) {
+    // N = 8^L: the compile-time kernel (4096 / N columns per workgroup)
+    const size_t sh8 = pass_lds(g.N, L8 == 2 ? 6 : 3, fp64, false);
+    const bool half = 2 * g.Lin == g.N;
+#define NPME_F8(R, LV, H)                                                                      \
+  hipLaunchKernelGGL((k_fft8_cols<R, LV, H>), grid, dim3(NT), sh8, s, c,                       \
+                     (const typename Cx<R>::T *)tw, (const typename Cx<R>::T *)in,            \
+                     (typename Cx<R>::T *)out, (int)backward)
+#define NPME_F8L(R, LV)                                                                        \
+  do {                                                                                         \
+    if (half) NPME_F8(R, LV, true);                                                            \
+    else NPME_F8(R, LV, false);                                                                \
+  } while (0)
+    if (fp64) {
+      if (L8 == 2) NPME_F8L(double, 2);
+      else NPME_F8L(double, 3);
+    } else {
+      if (L8 == 2) NPME_F8L(float, 2);
+      else NPME_F8L(float, 3);
+    }
+#undef NPME_F8
+#undef NPME_F8L
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const int rc = radix_class(plan);
   const size_t sh = pass_lds(g.N, lgB, fp64, rows);
 #define NPME_FFTC(R, ROWS, RCV)                                                                \
@@ -683,20 +949,43 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
 }
 
 void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
-                       int N2, int G2, long C3, const FFTPlan1D &plan, const void *tw,
-                       const void *G, void *data, hipStream_t s)
+                       int N2, int G2, long C3, long Kv, const FFTPlan1D &plan, const void *tw,
+                       const void *G, void *data, hipStream_t s, bool pow8)
 {
-  // C3: number of axis-3 entries per axis-2 row in the data; for complex
-  // kernels the axis-3 index is folded (|k3|) against N3 = C3.
+  // C3: entries per axis-2 row in the data (padded), Kv of them valid; for
+  // complex kernels the axis-3 index is folded (|k3|) against N3 = Kv.
   check_int(nCols * (long)Lin, "FFT pass");
   const int lgB = lg_batch(N, fp64, false);
   const dim3 grid((unsigned)((nCols + (1 << lgB) - 1) >> lgB));
   const int rc = radix_class(plan);
   const size_t sh = pass_lds(N, lgB, fp64, false);
-  const int N3fold = complexG ? (int)C3 : 0;
+  const int N3fold = complexG ? (int)Kv : 0;
+  const int L8 = pow8 ? pow8_level(plan) : 0;
+  if (L8 > 0 && 2 * Lin == N) {
+#define NPME_C8(R, LV, CGV)                                                                    \
+  hipLaunchKernelGGL((k_fft8_conv<R, LV, CGV>), grid, dim3(NT), sh, s, (int)nCols,             \
+                     (int)pos_stride, N2, G2, (int)C3, (int)Kv, N3fold,                         \
+                     (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data)
+#define NPME_C8L(R, CGV)                                                                       \
+  do {                                                                                         \
+    if (L8 == 2) NPME_C8(R, 2, CGV);                                                           \
+    else NPME_C8(R, 3, CGV);                                                                   \
+  } while (0)
+    if (fp64) {
+      if (complexG) NPME_C8L(double, true);
+      else NPME_C8L(double, false);
+    } else {
+      if (complexG) NPME_C8L(float, true);
+      else NPME_C8L(float, false);
+    }
+#undef NPME_C8
+#undef NPME_C8L
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
 #define NPME_CONV(R, CGV, RCV)                                                                 \
   hipLaunchKernelGGL((k_fft_conv<R, CGV, RCV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,     \
-                     (int)pos_stride, N2, G2, (int)C3, N3fold, to_dev(plan),                    \
+                     (int)pos_stride, N2, G2, (int)C3, (int)Kv, N3fold, to_dev(plan),           \
                      (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data, lgB)
 #define NPME_CONV2(R, CGV)                                                                     \
   do {                                                                                         \

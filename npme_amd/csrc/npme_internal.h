@@ -160,6 +160,7 @@ struct Handle {
   bool fp64 = false;
   long Bn = 6;
   long N[3] = {0, 0, 0}, nblk[3] = {0, 0, 0}, M[3] = {0, 0, 0}, K3 = 0;
+  long Kp = 0;  // row length of the FFT buffers: K3 (or N3) rounded up to 8
   double del[3] = {0, 0, 0}, ta[3] = {4, 4, 4};
   long nNeigh = 2, nCellClust1D = 3;
   size_t n = 0;
@@ -201,8 +202,8 @@ struct Handle {
   int *d_itileStart = nullptr;   // [ntiles + 1] charges per interpolation tile
   int *d_itileList = nullptr;    // [n]
   void *d_grid = nullptr;        // Q / theta: M reals (real) or M complex
-  void *d_buf1 = nullptr;        // M1 x M2 x K3 complex (real) / M1 x M2 x N3 (complex)
-  void *d_buf2 = nullptr;        // M1 x N2 x K3 complex (real) / M1 x N2 x N3 (complex)
+  void *d_buf1 = nullptr;        // M1 x M2 x Kp complex (rows of K3 or N3 values)
+  void *d_buf2 = nullptr;        // M1 x N2 x Kp complex
   void *d_G = nullptr;           // k-space multiplier (N1/2+1)(N2/2+1)(K3 or N3/2+1)
   void *d_tw[3] = {nullptr, nullptr, nullptr};  // twiddle tables per axis length
   double *d_Vdir = nullptr;      // [n][4] (x2 for complex) direct sum, sorted order
@@ -213,6 +214,7 @@ struct Handle {
   FFTPlan1D plan[3];
   size_t deviceBytes = 0;
   bool deviceSetup = false;      // per-charge set-up ran on the device
+  bool fftPow8 = true;           // compile-time FFT kernels for N = 64, 512
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -270,9 +272,11 @@ struct PassGeom {
   long nInner;     // columns per outer group
   long inOuter, inInner, inPos;     // input strides (elements)
   long outOuter, outInner, outPos;  // output strides
+  long validInner = -1;  // columns with inner index >= this are padding (-1: none)
 };
+// pow8: the compile-time kernels may be used for N = 64 and 512 (radix-8 plans)
 void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const void *tw,
-                      const void *in, void *out, bool backward, hipStream_t s);
+                      const void *in, void *out, bool backward, hipStream_t s, bool pow8 = true);
 void fft_pass_r2c_rows(bool fp64, int N, int Lin, long nRows, long inRowStride,
                        long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
                        void *out, hipStream_t s);
@@ -281,8 +285,8 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
                        void *out, hipStream_t s);
 // middle pass: forward FFT along the axis, multiply by G, backward FFT
 void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
-                       int N2, int G2, long K3, const FFTPlan1D &plan, const void *tw,
-                       const void *G, void *data, hipStream_t s);
+                       int N2, int G2, long K3, long Kv, const FFTPlan1D &plan, const void *tw,
+                       const void *G, void *data, hipStream_t s, bool pow8 = true);
 // even-extension DFT of the compact kernel grid along one axis (set-up, FP64)
 void fft_pass_even(int N, long nCols, long nInner, long outer, long inner, long pos,
                    const FFTPlan1D &plan, const void *tw, void *data, hipStream_t s);

@@ -943,12 +943,30 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
   const int e1 = (int)min((long)E, M1 - u1), e2 = (int)min((long)E, M2 - u2),
             e3 = (int)min((long)E, M3 - u3);
   constexpr int W = CX ? 2 : 1;
-  for (int idx = threadIdx.x; idx < e1 * e2 * e3; idx += blockDim.x) {
-    const int k = idx % e3, j = (idx / e3) % e2, i = idx / (e3 * e2);
-    const long g = (((long)(u1 + i) * M2) + (u2 + j)) * M3 + (u3 + k);
-    const int l = (i * E + j) * E + k;
+  {
+    // (i, j, k) of element idx of the e1 x e2 x e3 block, advanced by the
+    // block size with carries instead of a division per element
+    const int nb = blockDim.x, tot = e1 * e2 * e3;
+    const int dk = nb % e3, dj = (nb / e3) % e2, di = nb / (e3 * e2);
+    int idx = threadIdx.x, k = idx % e3, j = (idx / e3) % e2, i = idx / (e3 * e2);
+    const Real *g0 = grid + W * (((long)u1 * M2 + u2) * M3 + u3);
+    for (; idx < tot; idx += nb) {
+      const long g = ((long)i * M2 + j) * M3 + k;
+      const int l = (i * E + j) * E + k;
 #pragma unroll
-    for (int w = 0; w < W; w++) th[W * l + w] = grid[W * g + w];
+      for (int w = 0; w < W; w++) th[W * l + w] = g0[W * g + w];
+      k += dk;
+      j += dj;
+      if (k >= e3) {
+        k -= e3;
+        j++;
+      }
+      i += di;
+      if (j >= e2) {
+        j -= e2;
+        i++;
+      }
+    }
   }
   __syncthreads();
   for (int c = beg + threadIdx.x; c < end; c += blockDim.x) {
@@ -971,10 +989,13 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
         if (j < Bn) dB[d][j] = ((j < Bn - 1) ? sh[j] : (Real)0) - ((j >= 1) ? sh[j - 1] : (Real)0);
     }
     const int l1 = m0.x - u1, l2 = m0.y - u2, l3 = m0.z - u3;
+    // per x-plane j1: sums over the (j2, j3) window in the working
+    // precision (36 terms), carried into FP64 once per plane
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int j1 = 0; j1 < BA; j1++) {
       if (j1 >= Bn) break;
+      Real p0 = 0, py = 0, pz = 0, p0i = 0, pyi = 0, pzi = 0;
 #pragma unroll
       for (int j2 = 0; j2 < BA; j2++) {
         if (j2 >= Bn) break;
@@ -992,19 +1013,25 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
             tdbi += ti * dB[2][j3];
           }
         }
-        const double A1 = (double)(B[0][j1] * B[1][j2]);
-        const double A2 = (double)(dB[0][j1] * B[1][j2]);
-        const double A3 = (double)(B[0][j1] * dB[1][j2]);
-        v[0] += A1 * tb;
-        v[2] += A2 * tb;
-        v[4] += A3 * tb;
-        v[6] += A1 * tdb;
+        p0 += B[1][j2] * tb;
+        py += dB[1][j2] * tb;
+        pz += B[1][j2] * tdb;
         if (CX) {
-          v[1] += A1 * tbi;
-          v[3] += A2 * tbi;
-          v[5] += A3 * tbi;
-          v[7] += A1 * tdbi;
+          p0i += B[1][j2] * tbi;
+          pyi += dB[1][j2] * tbi;
+          pzi += B[1][j2] * tdbi;
         }
+      }
+      const double b0 = (double)B[0][j1], d0 = (double)dB[0][j1];
+      v[0] += b0 * (double)p0;
+      v[2] += d0 * (double)p0;
+      v[4] += b0 * (double)py;
+      v[6] += b0 * (double)pz;
+      if (CX) {
+        v[1] += b0 * (double)p0i;
+        v[3] += d0 * (double)p0i;
+        v[5] += b0 * (double)pyi;
+        v[7] += b0 * (double)pzi;
       }
     }
     const int o = perm[s];

@@ -545,8 +545,8 @@ static void grid_setup(Handle &h)
   const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2];
   const size_t cs = 2 * rs;  // complex element size
   h.d_grid = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)(M1 * M2 * M3));
-  h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.K3));
-  h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.K3));
+  h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.Kp));
+  h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.Kp));
   for (int d = 0; d < 3; d++) {
     h.plan[d] = make_plan((int)h.N[d]);
     h.d_tw[d] = upload_twiddles(h, (int)h.N[d], f64);
@@ -637,6 +637,12 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (kw.get("useCompactF", useCompactF) && useCompactF != 1)
     throw Error{NPME_ERR_UNSUPPORTED, "useCompactF = 0 is not provided"};
   h.fp64 = kw.get("precision", prec) && prec == "fp64";
+  {
+    std::string fk = "auto";
+    kw.get("fftKernels", fk);
+    if (fk != "auto" && fk != "generic") kw_error("fftKernels must be 'auto' or 'generic'");
+    h.fftPow8 = fk == "auto";
+  }
   if (kw.get("precision", prec) && prec != "fp64" && prec != "mixed")
     kw_error("precision must be 'mixed' or 'fp64'");
 
@@ -680,6 +686,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     if (h.N[d] + 1 > 16 * 256)
       throw Error{NPME_ERR_UNSUPPORTED, "FFT length above 4095 is not provided"};
   h.K3 = h.complexK ? h.N[2] : h.N[2] / 2 + 1;
+  // rows of the FFT buffers padded to a multiple of 8 complex values, so a
+  // workgroup's 8 adjacent columns are one aligned 64-byte segment per row
+  h.Kp = (h.K3 + 7) / 8 * 8;
 
   // ---- geometry of the reciprocal sum
   box_dim_center(n, coord, h.X0, h.R0);
@@ -793,7 +802,7 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
   const bool f64 = h.fp64, cx = h.complexK;
   hipStream_t s = h.stream;
   const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2], N1 = h.N[0], N2 = h.N[1], N3 = h.N[2];
-  const long K3 = h.K3;
+  const long K3 = h.K3, Kp = h.Kp;
   int st = 0;
   mark(h, st++);
   launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
@@ -810,36 +819,40 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
                 h.d_ms, h.d_frac, h.d_grid, s);
   mark(h, st++);
   if (!cx) {
-    fft_pass_r2c_rows(f64, (int)N3, (int)M3, M1 * M2, M3, K3, h.plan[2], h.d_tw[2], h.d_grid,
+    fft_pass_r2c_rows(f64, (int)N3, (int)M3, M1 * M2, M3, Kp, h.plan[2], h.d_tw[2], h.d_grid,
                       h.d_buf1, s);
     mark(h, st++);
-    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * K3, K3, M2 * K3, 1, K3, N2 * K3, 1, K3};
-    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s);
+    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * Kp, Kp, M2 * Kp, 1, Kp, N2 * Kp, 1, Kp};
+    b.validInner = K3;
+    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s, h.fftPow8);
     mark(h, st++);
-    fft_pass_convolve(f64, false, (int)N1, (int)M1, N2 * K3, N2 * K3, (int)N2, (int)(N2 / 2 + 1),
-                      K3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s);
+    fft_pass_convolve(f64, false, (int)N1, (int)M1, N2 * Kp, N2 * Kp, (int)N2, (int)(N2 / 2 + 1),
+                      Kp, K3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8);
     mark(h, st++);
-    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * K3, K3, N2 * K3, 1, K3, M2 * K3, 1, K3};
-    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s);
+    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * Kp, Kp, N2 * Kp, 1, Kp, M2 * Kp, 1, Kp};
+    d.validInner = K3;
+    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s, h.fftPow8);
     mark(h, st++);
-    fft_pass_c2r_rows(f64, (int)N3, (int)M3, M1 * M2, K3, M3, h.plan[2], h.d_tw[2], h.d_buf1,
+    fft_pass_c2r_rows(f64, (int)N3, (int)M3, M1 * M2, Kp, M3, h.plan[2], h.d_tw[2], h.d_buf1,
                       h.d_grid, s);
     mark(h, st++);
   } else {
-    PassGeom a{(int)N3, (int)M3, (int)N3, M1 * M2, M1 * M2, 0, M3, 1, 0, N3, 1};
-    fft_pass_complex(f64, a, h.plan[2], h.d_tw[2], h.d_grid, h.d_buf1, false, s);
+    PassGeom a{(int)N3, (int)M3, (int)N3, M1 * M2, M1 * M2, 0, M3, 1, 0, Kp, 1};
+    fft_pass_complex(f64, a, h.plan[2], h.d_tw[2], h.d_grid, h.d_buf1, false, s, h.fftPow8);
     mark(h, st++);
-    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * N3, N3, M2 * N3, 1, N3, N2 * N3, 1, N3};
-    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s);
+    PassGeom b{(int)N2, (int)M2, (int)N2, M1 * Kp, Kp, M2 * Kp, 1, Kp, N2 * Kp, 1, Kp};
+    b.validInner = N3;
+    fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s, h.fftPow8);
     mark(h, st++);
-    fft_pass_convolve(f64, true, (int)N1, (int)M1, N2 * N3, N2 * N3, (int)N2, (int)(N2 / 2 + 1),
-                      N3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s);
+    fft_pass_convolve(f64, true, (int)N1, (int)M1, N2 * Kp, N2 * Kp, (int)N2, (int)(N2 / 2 + 1),
+                      Kp, N3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8);
     mark(h, st++);
-    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * N3, N3, N2 * N3, 1, N3, M2 * N3, 1, N3};
-    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s);
+    PassGeom d{(int)N2, (int)N2, (int)M2, M1 * Kp, Kp, N2 * Kp, 1, Kp, M2 * Kp, 1, Kp};
+    d.validInner = N3;
+    fft_pass_complex(f64, d, h.plan[1], h.d_tw[1], h.d_buf2, h.d_buf1, true, s, h.fftPow8);
     mark(h, st++);
-    PassGeom e{(int)N3, (int)N3, (int)M3, M1 * M2, M1 * M2, 0, N3, 1, 0, M3, 1};
-    fft_pass_complex(f64, e, h.plan[2], h.d_tw[2], h.d_buf1, h.d_grid, true, s);
+    PassGeom e{(int)N3, (int)N3, (int)M3, M1 * M2, M1 * M2, 0, Kp, 1, 0, M3, 1};
+    fft_pass_complex(f64, e, h.plan[2], h.d_tw[2], h.d_buf1, h.d_grid, true, s, h.fftPow8);
     mark(h, st++);
   }
   const double C[3] = {N1 / h.L[0], N2 / h.L[1], N3 / h.L[2]};

@@ -225,3 +225,36 @@ def test_mixed_close_pairs(setup):
     # and ~8e-5 in the gradient at the 1e-3 A pair
     assert relOwn.max() < 2e-6 and relG.max() < 5e-6
     assert relRest.max() < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fft_pow8_kernels_match_generic(prec):
+    """N = 512: the compile-time radix-8 FFT kernels give the same V as the
+    general Stockham kernels (keyword fftKernels generic)."""
+    coord, q = uniform_box(3000, 30.0)
+    grid = dict(N1=512, N2=512, N3=512, n1=16, n2=16, n3=16, nDeriv=6)
+    V = {}
+    for mode in ("auto", "generic"):
+        h = _real_setup(coord, q, prec, Rdir=6.0, fftKernels=mode, **grid)
+        V[mode] = h.calcv(q)
+    d = V["auto"] - V["generic"]
+    rel = np.sqrt(np.mean(d[:, 0] ** 2)) / np.mean(np.abs(V["generic"][:, 0]))
+    relg = np.sqrt(np.mean(d[:, 1:] ** 2)) / np.mean(np.abs(V["generic"][:, 1:]))
+    print(prec, rel, relg)
+    # FP64: the same transform to rounding; FP32: a different operation
+    # order (pruned first stage, loads straight into the butterflies)
+    assert rel < (1e-13 if prec == "fp64" else 2e-7) and relg < (1e-13 if prec == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_grid64_matches_oracle(oracle, prec):
+    """N = 64 per axis (the compile-time two-stage kernels) against the
+    oracle on the same grid."""
+    coord, q = uniform_box(2000, 20.0)
+    h = _real_setup(coord, q, prec, Rdir=6.0, N1=64, N2=64, N3=64, n1=8, n2=8, n3=8, nDeriv=6)
+    V = h.calcv(q)
+    k = oracle.kernel(oracle.LAPLACE, 6, 6.0)
+    Vo = oracle.calcv(k, (64, 64, 64), 6, coord, q)
+    mo = oracle.compare_v(Vo, V)
+    print(prec, mo)
+    assert mo["relV"] < (1e-11 if prec == "fp64" else 5e-7)
