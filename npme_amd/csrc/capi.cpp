@@ -375,6 +375,9 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  cells " << H.nb[0] << "x" << H.nb[1] << "x" << H.nb[2] << " of " << H.csB
      << "  device GB " << H.deviceBytes / 1e9 << "\n";
   os << "  per-charge set-up on the " << (H.deviceSetup ? "device" : "host") << "\n";
+  os << "  set-up ms: geometry " << H.setupMs[0] << ", free " << H.setupMs[1] << ", cells "
+     << H.setupMs[2] << ", list " << H.setupMs[3] << ", charges " << H.setupMs[4] << ", grid "
+     << H.setupMs[5] << "\n";
   if (H.useNlist)
     os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";
   else

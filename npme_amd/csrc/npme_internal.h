@@ -215,6 +215,10 @@ struct Handle {
   size_t deviceBytes = 0;
   bool deviceSetup = false;      // per-charge set-up ran on the device
   bool fftPow8 = true;           // compile-time FFT kernels for N = 64, 512
+  // wall times of the last set-up (ms): host geometry, previous state freed,
+  // cells and sort, neighbour list, per-charge arrays and tile lists, FFT
+  // buffers and kernel grid
+  double setupMs[6] = {0, 0, 0, 0, 0, 0};
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;

@@ -16,6 +16,7 @@
 // spline window is always a contiguous box and the zero padding of the
 // full grid is the tail [N/2, N) of every axis.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -26,6 +27,14 @@
 namespace npme {
 
 static void kw_error(const std::string &m) { throw Error{NPME_ERR_KEYWORD, m}; }
+
+static double ms_since(std::chrono::steady_clock::time_point &t)
+{
+  const auto now = std::chrono::steady_clock::now();
+  const double ms = std::chrono::duration<double, std::milli>(now - t).count();
+  t = now;
+  return ms;
+}
 
 template <typename T> static T *dalloc(Handle &h, size_t count)
 {
@@ -224,6 +233,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   const bool f64 = h.fp64;
   const size_t rs = f64 ? 8 : 4;
   const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
+  auto t0 = std::chrono::steady_clock::now();
   h.d_coordS = dalloc<double>(h, 3 * n);
   NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
   h.d_perm = dalloc<int>(h, n);
@@ -232,6 +242,8 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   geometry_cells_device(g, n, h.d_coordS, h.d_perm, h.d_cell, h.d_cellStart, s);
   h.d_xs = dalloc<double4>(h, n);
   sorted_coords_device(n, h.d_perm, h.d_coordS, h.d_xs, s);
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  h.setupMs[2] = ms_since(t0);
 
   // neighbour list: count, slice lengths and offsets, then fill
   if (nlist_clearly_too_big(h, nlmode)) return false;
@@ -255,6 +267,8 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
                h.d_sliceStart, h.d_nlist, s);
   h.nlistEntries = entries;
   h.useNlist = true;
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  h.setupMs[3] = ms_since(t0);
 
   // positions, spline windows and tile lists
   h.fxShift = fixed_shift(1);
@@ -279,6 +293,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.d_V = dalloc<double>(h, vw * n);
   h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
   NPME_HIP_CHECK(hipStreamSynchronize(s));
+  h.setupMs[4] = ms_since(t0);
   return true;
 }
 
@@ -610,6 +625,8 @@ static void grid_setup(Handle &h)
 // ---------------------------------------------------------------- set-up
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
 {
+  auto t0 = std::chrono::steady_clock::now();
+  for (double &v : h.setupMs) v = 0;
   h.kw = kw;
   h.n = n;
   if (n < 2) throw Error{NPME_ERR_ARG, "nCharge must be at least 2"};
@@ -735,8 +752,10 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   // ---- per-charge state: on the device when the neighbour list is used
   NPME_HIP_CHECK(hipGetDevice(&h.device));
   if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  h.setupMs[0] = ms_since(t0);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   handle_free_device(h);
+  h.setupMs[1] = ms_since(t0);
   std::string smode = "device", nlmode = "auto";
   kw.get("setup", smode);
   kw.get("neighborList", nlmode);
@@ -751,7 +770,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     particles_host(h, g, nlmode, nlPossible, coord);
   }
   h.deviceSetup = done;
+  ms_since(t0);
   grid_setup(h);
+  h.setupMs[5] = ms_since(t0);
 }
 
 // ---------------------------------------------------------------- CalcV
