@@ -30,6 +30,7 @@ extern "C" {
 #define NPME_ERR_DEVICE      4   /* HIP runtime error                      */
 #define NPME_ERR_ARG         5   /* bad argument (size mismatch, NULL ...) */
 #define NPME_ERR_UNSUPPORTED 6   /* combination not provided               */
+#define NPME_ERR_IO          7   /* file cannot be opened, read or parsed  */
 
 typedef struct npme_keywords npme_keywords;
 typedef struct npme_handle   npme_handle;
@@ -159,6 +160,28 @@ int npme_server_wait(npme_server *s, long ticket, npme_request_info *info);
 double npme_server_clock(const npme_server *s);
 /* finishes the queued requests, then stops the slots */
 void npme_server_destroy(npme_server *s);
+
+/* ---- on-disk formats of the command-line tools (src/ReadPrint.cpp) and
+ * the compareV metric.  Arrays returned through double ** are allocated by
+ * the library and released with npme_free. ------------------------------ */
+/* NPME_ReadSingleBoxChargeCoord (src/ReadPrint.cpp:282-440): coord [n][3],
+ * charge [n] or [n][2] (interleaved re, im) */
+int npme_read_coord_charge(const char *path, size_t *nCharge, int *isChargeReal,
+                           double **coord, double **charge);
+/* NPME_PrintSingleBoxChargeCoord (src/ReadPrint.cpp:234-280) */
+int npme_write_coord_charge(const char *path, size_t nCharge, int isChargeReal,
+                            const double *coord, const double *charge);
+/* NPME_PrintSingleBox_V (src/ReadPrint.cpp:487-560) */
+int npme_write_v(const char *path, size_t nCharge, int isChargeReal, const double *V,
+                 int highPrecision);
+/* NPME_ReadSingleBox_V (src/ReadPrint.cpp:565-739): V [n][4] or [n][4][2] */
+int npme_read_v(const char *path, size_t *nCharge, int *isChargeReal, double **V);
+void npme_free(void *p);
+/* NPME_CompareV1Arrays + NPME_CalcV1AvgVecMag (src/SupportFunctions.cpp:
+ * 88-305): out = {errorV, errordVdr, V_mag(Vref), dVdr_mag(Vref)}; the
+ * relative errors are out[0]/out[2] and out[1]/out[3] */
+int npme_compare_v(size_t nCharge, int isChargeReal, const double *V, const double *Vref,
+                   double out[4]);
 
 const char *npme_last_error(void);
 

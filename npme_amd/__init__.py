@@ -61,6 +61,13 @@ def lib():
         L.npme_server_clock.argtypes = [p]
         L.npme_server_clock.restype = d
         L.npme_server_destroy.argtypes = [p]
+        pp = C.POINTER(p)
+        L.npme_read_coord_charge.argtypes = [C.c_char_p, C.POINTER(s), C.POINTER(i), pp, pp]
+        L.npme_write_coord_charge.argtypes = [C.c_char_p, s, i, p, p]
+        L.npme_write_v.argtypes = [C.c_char_p, s, i, p, i]
+        L.npme_read_v.argtypes = [C.c_char_p, C.POINTER(s), C.POINTER(i), pp]
+        L.npme_free.argtypes = [p]
+        L.npme_compare_v.argtypes = [s, i, p, p, p]
         _lib = L
     return _lib
 
@@ -366,3 +373,71 @@ class NPME_Server:
 
     def __del__(self):
         self.close()
+
+
+# ----------------------------------------------------------------- files
+# The on-disk formats of the command-line tools (src/ReadPrint.cpp:234-739)
+# and the compareV metric (src/SupportFunctions.cpp:88-305), through the
+# library's readers and writers.
+def _take(ptr, count, dtype=np.float64):
+    a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_double)), shape=(count,)).copy()
+    lib().npme_free(ptr)
+    return a
+
+
+def NPME_ReadSingleBoxChargeCoord(filename):
+    """(coord[n][3], charge) of an @Coord / @ChargeReal|@ChargeComplex file;
+    charge is float64[n] or complex128[n]."""
+    L = lib()
+    n, real = C.c_size_t(), C.c_int()
+    pc, pq = C.c_void_p(), C.c_void_p()
+    _check(L.npme_read_coord_charge(os.fsencode(filename), C.byref(n), C.byref(real), C.byref(pc),
+                                    C.byref(pq)), "NPME_ReadSingleBoxChargeCoord")
+    coord = _take(pc, 3 * n.value).reshape(-1, 3)
+    q = _take(pq, (1 if real.value else 2) * n.value)
+    return coord, (q if real.value else q.view(np.complex128))
+
+
+def NPME_PrintSingleBoxChargeCoord(filename, coord, charge):
+    c = _coord(coord)
+    q = np.asarray(charge)
+    real = not np.iscomplexobj(q)
+    qd = np.ascontiguousarray(q, dtype=np.float64 if real else np.complex128)
+    if qd.shape[0] != c.shape[0]:
+        raise ValueError("coord and charge sizes differ")
+    _check(lib().npme_write_coord_charge(os.fsencode(filename), c.shape[0], int(real), _ptr(c),
+                                         _ptr(qd)), "NPME_PrintSingleBoxChargeCoord")
+
+
+def NPME_PrintSingleBox_V(filename, V, printHighPrecision=False):
+    """@Vreal / @Vcomplex file of V[n][4] (float64 or complex128)."""
+    real = not np.iscomplexobj(V)
+    v = np.ascontiguousarray(V, dtype=np.float64 if real else np.complex128).reshape(-1, 4)
+    _check(lib().npme_write_v(os.fsencode(filename), v.shape[0], int(real), _ptr(v),
+                              int(bool(printHighPrecision))), "NPME_PrintSingleBox_V")
+
+
+def NPME_ReadSingleBox_V(filename):
+    L = lib()
+    n, real = C.c_size_t(), C.c_int()
+    pv = C.c_void_p()
+    _check(L.npme_read_v(os.fsencode(filename), C.byref(n), C.byref(real), C.byref(pv)),
+           "NPME_ReadSingleBox_V")
+    v = _take(pv, (4 if real.value else 8) * n.value)
+    return v.reshape(-1, 4) if real.value else v.view(np.complex128).reshape(-1, 4)
+
+
+def NPME_CompareV1Arrays(V, Vref):
+    """compareV metric: dict with the absolute errors errorV, errordVdr and
+    the averages V_mag, dVdr_mag of Vref (relative error = ratio)."""
+    real = not (np.iscomplexobj(V) or np.iscomplexobj(Vref))
+    dt = np.float64 if real else np.complex128
+    a = np.ascontiguousarray(V, dtype=dt).reshape(-1, 4)
+    b = np.ascontiguousarray(Vref, dtype=dt).reshape(-1, 4)
+    if a.shape != b.shape:
+        raise ValueError("V and Vref sizes differ")
+    out = np.zeros(4)
+    _check(lib().npme_compare_v(a.shape[0], int(real), _ptr(a), _ptr(b), _ptr(out)),
+           "NPME_CompareV1Arrays")
+    return {"errorV": out[0], "errordVdr": out[1], "V_mag": out[2], "dVdr_mag": out[3],
+            "relV": out[0] / out[2], "reldV": out[1] / out[3]}

@@ -128,3 +128,22 @@ def test_cpp_dropin_header_host_calls(oracle, tmp_path):
     out = subprocess.run([exe, str(inp)], check=True, capture_output=True, text=True).stdout.split()
     N, nb, nd = oracle.rsem_static(q, coord, 6, -1.0, 1e-6, 8.0)
     assert [int(v) for v in out] == list(N) + list(nb) + [nd]
+
+
+def test_cpp_header_files_and_descriptors(tmp_path):
+    """The src/ReadPrint.h / SupportFunctions.h helpers and the
+    parameterised kernel descriptors of the C++ header, compiled with
+    -Wall -Wextra -Werror and run on the host (tests/cpp/readprint_host.cpp)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "npme_amd")
+    exe = str(tmp_path / "readprint_host")
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"),
+                    os.path.join(root, "tests", "cpp", "readprint_host.cpp"), "-L", libdir, "-lnpme_amd",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout, r.stderr)
