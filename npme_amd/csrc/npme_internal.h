@@ -316,7 +316,7 @@ void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const do
 void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);
-unsigned nlist_pad();  // value of a padding entry of the neighbour list
+unsigned nlist_pad(size_t sentinel);  // padding entry of the neighbour list (sentinel source = n)
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
                   hipStream_t s);

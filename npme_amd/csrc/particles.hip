@@ -50,6 +50,16 @@ __device__ __forceinline__ Real horner_fix(const Real *c, Real r2)
   for (int i = D - 1; i >= 0; i--) s = s * r2 + c[2 * i];
   return s;
 }
+// the same with the leading coefficient c[2D] passed in a register (a VGPR
+// copy made once per thread keeps the first FMA to one scalar operand)
+template <typename Real, int D>
+__device__ __forceinline__ Real horner_top(Real top, const Real *c, Real r2)
+{
+  Real s = top;
+#pragma unroll
+  for (int i = D - 1; i >= 0; i--) s = s * r2 + c[2 * i];
+  return s;
+}
 template <typename Real, int D>
 __device__ __forceinline__ void horner_fix_cx(const Real *c, Real r2, Real &re, Real &im)
 {
@@ -379,14 +389,15 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
 // sliceStart + k * 64 + l, so step k of a wavefront reads 256 contiguous
 // bytes.  An entry packs the source index (low 26 bits) with the offset of
 // the source's cluster from the target's cluster, dx+1, dy+1 and dz+1 in
-// two bits each (bits 26, 28, 30).  A padding entry is source 0 with the
-// code 3 on every axis, an offset of two clusters (at least one cluster edge
-// >= Rdir per axis): it is evaluated like any entry and fails r < Rdir, so
-// the loop needs no test for it.  The pairs are
-// found in FP64 with a cutoff inflated by 1e-9 relative; the direct-sum
-// kernel applies r < Rdir in its working precision.
+// two bits each (bits 26, 28, 30).  A padding entry is the sentinel source
+// n (position 0 in its frame, charge 0, stored after the n charges) with
+// the code 3 on every axis, an offset of two clusters: its distance is
+// finite and non-zero and its charge zero, so it adds exactly zero and the
+// loop needs no test for it.  The pairs are found in FP64 with a cutoff
+// inflated by 1e-9 relative and every entry is evaluated (the short-range
+// function vanishes at Rdir with its first Nder derivatives).
 constexpr unsigned NL_PAD = 0xfc000000u;
-unsigned nlist_pad() { return NL_PAD; }
+unsigned nlist_pad(size_t sentinel) { return NL_PAD | (unsigned)sentinel; }
 constexpr int NL_SHIFT = 26;
 
 // walk the cells of the stencil of sorted target s in FP64 input
@@ -455,14 +466,15 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
 // short-range DM pair term of one source (charge qr + i qim) at offset
 // (rx, ry, rz), r2 = |r|^2 < Rdir^2, added to part: V and grad (real), or
 // (re, im) of V, d/dx, d/dy, d/dz (complex)
+// topA, topB: the leading coefficients a[MAXD], b[MAXD] (real kernels)
 template <typename Real, int KIND, int MAXD>
 __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Real rz, Real r2,
-                                        Real qr, Real qim, Real *part)
+                                        Real qr, Real qim, Real *part, Real topA, Real topB)
 {
   if constexpr (KIND == KIND_LAPLACE) {
     const Real ri = rsqrt_r<Real>(r2);
-    const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-    const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+    const Real f0 = ri - horner_top<Real, MAXD>(topA, coef_a<Real>(p), r2);
+    const Real f1 = -ri * ri * ri - horner_top<Real, MAXD>(topB, coef_b<Real>(p), r2);
     const Real fq = f1 * qr;
     part[0] += f0 * qr;
     part[1] += rx * fq;
@@ -470,14 +482,16 @@ __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Re
     part[3] += rz * fq;
   } else if constexpr (KIND == KIND_RALPHA) {
     const Real fe = pow(r2, (Real)(0.5 * p.alpha));
-    const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-    const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
+    const Real f0 = fe - horner_top<Real, MAXD>(topA, coef_a<Real>(p), r2);
+    const Real f1 = (Real)p.alpha * fe / r2 - horner_top<Real, MAXD>(topB, coef_b<Real>(p), r2);
     const Real fq = f1 * qr;
     part[0] += f0 * qr;
     part[1] += rx * fq;
     part[2] += ry * fq;
     part[3] += rz * fq;
   } else {
+    (void)topA;
+    (void)topB;
     // exp(i k0 r)/r with the phase k0r r reduced modulo 2 pi in FP64
     const Real r = sqrt(r2);
     const Real ri = (Real)1 / r;
@@ -516,8 +530,8 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
                                                    const Real *__restrict__ qi,
                                                    const long *__restrict__ sliceStart,
                                                    const int *__restrict__ sliceLen,
-                                                   const unsigned *__restrict__ list, double csA,
-                                                   double *__restrict__ Vdir)
+                                                   const unsigned *__restrict__ list, unsigned pad,
+                                                   double csA, double *__restrict__ Vdir)
 {
   using V4 = typename V4t<Real>::T;
   constexpr bool CX = (KIND == KIND_HELMHOLTZ);
@@ -537,9 +551,12 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
     mz = __float_as_int(me.z) + p.fxUnit;
     mres = res[tt];
   }
-  const Real R2 = (Real)(p.Rdir * p.Rdir);
   const Real cA = (Real)csA;
-  const int len = sliceLen[slice];
+  Real topA = coef_a<Real>(p)[2 * MAXD], topB = coef_b<Real>(p)[2 * MAXD];
+  asm volatile("" : "+v"(topA), "+v"(topB));  // held in VGPRs for the pair loop
+  // the slice is the wavefront's: its length is uniform, so the loop and
+  // the bounds of the entry loads branch on scalars
+  const int len = __builtin_amdgcn_readfirstlane(sliceLen[slice]);
   const unsigned *e = list + sliceStart[slice] + (s & 63);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -551,7 +568,7 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
   const unsigned ushift = (unsigned)p.fxShift;
   auto load_ent = [&](int k0, unsigned *ent) {
 #pragma unroll
-    for (int u = 0; u < U; u++) ent[u] = (k0 + u < len) ? e[(long)(k0 + u) * 64] : NL_PAD;
+    for (int u = 0; u < U; u++) ent[u] = (k0 + u < len) ? e[(long)(k0 + u) * 64] : pad;
   };
   auto gather = [&](const unsigned *ent, V4 *o, Real *oqi) {
 #pragma unroll
@@ -590,7 +607,7 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
         rz = me.z - (o[u].z + (Real)(cz - 1) * cA);
         r2 = rx * rx + ry * ry + rz * rz;
       }
-      if (r2 < R2) pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o[u].w, CX ? oqi[u] : (Real)0, part);
+      pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o[u].w, CX ? oqi[u] : (Real)0, part, topA, topB);
     }
   };
   auto carry = [&](int k0) {
@@ -648,12 +665,13 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
                       const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
                       double csA, double *Vdir, hipStream_t s)
 {
+  const unsigned pad = nlist_pad(n);
   const DirectDev d = direct_dev(p);
   const dim3 grid((unsigned)((n + 255) / 256));
 #define NPME_DNL_D(R, K, HL, D)                                                                 \
   hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \
                      (const typename V4t<R>::T *)src, (const float4 *)res, (const R *)qi,      \
-                     sliceStart, sliceLen, list, csA, Vdir)
+                     sliceStart, sliceLen, list, pad, csA, Vdir)
 // Horner at the exact degree for Nder 4..8 (the coefficients above Nder
 // are zero, so a larger compile-time degree is also exact, only slower)
 #define NPME_DNL(R, K, HL)                                                                      \

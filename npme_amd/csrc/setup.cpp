@@ -197,7 +197,7 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   h.d_sliceLen = dalloc<int>(h, nSlices);
   NPME_HIP_CHECK(hipMemcpy(h.d_sliceStart, sliceStart.data(), (nSlices + 1) * sizeof(long), hipMemcpyHostToDevice));
   NPME_HIP_CHECK(hipMemcpy(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int), hipMemcpyHostToDevice));
-  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(), entries, h.stream));
+  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(h.n), entries, h.stream));
   launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP, nullptr, h.d_sliceStart,
                h.d_nlist, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -262,7 +262,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
   if (nlmode != "on" && entries * sizeof(unsigned) > freeB / 4) return false;
   h.d_nlist = dalloc<unsigned>(h, entries);
-  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(), std::max<size_t>(entries, 1), s));
+  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n), std::max<size_t>(entries, 1), s));
   launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
                h.d_sliceStart, h.d_nlist, s);
   h.nlistEntries = entries;
@@ -274,8 +274,13 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.fxShift = fixed_shift(1);
   g.fxShift = h.fxShift;
   g.fxScale = std::ldexp(1.0, h.fxShift) / h.csA;
-  h.d_src = dalloc<char>(h, 4 * rs * n);
-  if (!f64) h.d_srcRes = dalloc<float4>(h, n);
+  // [n + 1]: entry n is the neighbour list's zero-charge sentinel
+  h.d_src = dalloc<char>(h, 4 * rs * (n + 1));
+  NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_src + 4 * rs * n, 0, 4 * rs, s));
+  if (!f64) {
+    h.d_srcRes = dalloc<float4>(h, n + 1);
+    NPME_HIP_CHECK(hipMemsetAsync(static_cast<float4 *>(h.d_srcRes) + n, 0, sizeof(float4), s));
+  }
   h.d_frac = dalloc<char>(h, 4 * rs * n);
   h.d_ms = dalloc<int4>(h, n);
   DevBuf ts, tl, its, itl;
@@ -288,7 +293,10 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.d_itileList = (int *)adopt(h, itl);
   h.nSegments = 0;
   const int vw = h.complexK ? 8 : 4;
-  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
+  if (h.complexK) {
+    h.d_qi = dalloc<char>(h, rs * (n + 1));
+    NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_qi + rs * n, 0, rs, s));
+  }
   h.d_Vdir = dalloc<double>(h, vw * n);
   h.d_V = dalloc<double>(h, vw * n);
   h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
@@ -498,7 +506,10 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   h.d_blockBox = dalloc<int4>(h, segs.size());
   up(h.d_blockBox, segs.data(), segs.size() * sizeof(int4));
   h.nSegments = segs.size();
-  if (h.complexK) h.d_qi = dalloc<char>(h, rs * n);
+  if (h.complexK) {
+    h.d_qi = dalloc<char>(h, rs * (n + 1));  // entry n: the list's zero-charge sentinel
+    NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_qi + rs * n, 0, rs, h.stream));
+  }
   h.d_cell = dalloc<int4>(h, n);
   up(h.d_cell, cellOf.data(), n * sizeof(int4));
   h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
@@ -526,7 +537,9 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   // the fixed-point scale depends on which of the two reads the positions
   if (nlPossible) build_neighbor_list(h, nlmode);
   h.fxShift = h.useNlist ? fixed_shift(1) : fixed_shift(maxOff);
-  h.d_src = dalloc<char>(h, 4 * rs * n);
+  // [n + 1]: entry n is the neighbour list's zero-charge sentinel
+  h.d_src = dalloc<char>(h, 4 * rs * (n + 1));
+  NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_src + 4 * rs * n, 0, 4 * rs, h.stream));
   if (f64) {
     up(h.d_src, loc.data(), 4 * n * sizeof(double));
   } else {
@@ -546,7 +559,8 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
         fr[4 * s + d] = (float)(u - (double)v);
       }
     up(h.d_src, fx.data(), 4 * n * sizeof(int32_t));
-    h.d_srcRes = dalloc<float4>(h, n);
+    h.d_srcRes = dalloc<float4>(h, n + 1);
+    NPME_HIP_CHECK(hipMemsetAsync(static_cast<float4 *>(h.d_srcRes) + n, 0, sizeof(float4), h.stream));
     up(h.d_srcRes, fr.data(), 4 * n * sizeof(float));
   }
 }
