@@ -362,8 +362,9 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   if (!h || !buf) return NPME_ERR_ARG;
   const Handle &H = h->h;
   std::ostringstream os;
-  static const char *kn[3] = {"Laplace", "Ralpha", "Helmholtz"};
-  os << "NPME (MI355X) kernel " << kn[H.ker.kind] << " DerivMatch, precision "
+  static const char *kn[4] = {"Laplace", "Ralpha", "Helmholtz", "Laplace"};
+  os << "NPME (MI355X) kernel " << kn[H.ker.kind]
+     << (H.ker.kind == KIND_LAPLACE_ERF ? " LaplaceOrig" : " DerivMatch") << ", precision "
      << (H.fp64 ? "fp64" : "mixed") << "\n";
   os << "  nCharge " << H.n << "  Rdir " << H.ker.Rdir << "  Nder " << H.ker.Nder << "  BnOrder "
      << H.Bn << "\n";
@@ -372,6 +373,7 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  X Y Z " << H.X[0] << " " << H.X[1] << " " << H.X[2] << "  L " << H.L[0] << " "
      << H.L[1] << " " << H.L[2] << "\n";
   os << "  fself " << H.fself[0] << " " << H.fself[1] << "\n";
+  if (H.ker.kind == KIND_LAPLACE_ERF) os << "  beta " << H.ker.beta << "\n";
   os << "  cells " << H.nb[0] << "x" << H.nb[1] << "x" << H.nb[2] << " of " << H.csB
      << "  device GB " << H.deviceBytes / 1e9 << "\n";
   os << "  per-charge set-up on the " << (H.deviceSetup ? "device" : "host") << "\n";

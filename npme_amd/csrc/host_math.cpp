@@ -107,12 +107,51 @@ bool dm_kernel_init(DMKernel &k, int kind, double alpha, double k0r, double k0i,
   return true;
 }
 
+double rdir2beta(double Rdir, double tol)
+// x = beta Rdir solves x^2 + ln x + ln(sqrt(pi) tol Rdir) = 0, the
+// asymptotic form of erfc(x)/Rdir = tol; Newton from x = 0.1 to |F| < 1e-12
+// (src/KernelFunctionLaplace.cpp:41-72)
+{
+  const double C = std::log(std::sqrt(M_PI) * tol * Rdir);
+  double x = 0.1;
+  for (int it = 0; it < 1000; it++) {
+    const double F0 = x * x + std::log(x) + C, F1 = 2 * x + 1.0 / x;
+    if (std::fabs(F0) < 1e-12) return x / Rdir;
+    x -= F0 / F1;
+  }
+  throw Error{NPME_ERR_KEYWORD, "EwaldSplit LaplaceOrig: no beta for Rdir and tol"};
+}
+
+void erf_kernel_init(DMKernel &k, int Nder, double Rdir, double beta)
+{
+  k = DMKernel();
+  k.kind = KIND_LAPLACE_ERF;
+  k.alpha = -1.0;
+  k.Nder = Nder;
+  k.Rdir = Rdir;
+  k.beta = beta;
+}
+
 void dm_eval_lr(const DMKernel &k, double r2, double f0[2], double f1[2])
 // f_LR: even series inside Rdir, the full kernel outside
 // (src/KernelFunctionLaplace.cpp:394-437, KernelFunctionRalpha.cpp:343-385,
 // KernelFunctionHelmholtz.cpp:506-560)
 {
   const double r = std::sqrt(std::fabs(r2));
+  if (k.kind == KIND_LAPLACE_ERF) {
+    // erf(beta r)/r and (1/r) d/dr of it, by series below x = beta r = 1e-4
+    // (src/KernelFunctionLaplace.cpp:1087-1116)
+    const double x = k.beta * r, c = 2.0 / std::sqrt(M_PI);
+    if (x < 1e-4) {
+      f0[0] = k.beta * c * (1.0 - x * x / 3.0 + x * x * x * x / 10.0);
+      f1[0] = k.beta * k.beta * k.beta * c * (-2.0 / 3.0 + 0.4 * x * x);
+    } else {
+      f0[0] = std::erf(x) / r;
+      f1[0] = (k.beta * c * std::exp(-x * x) - f0[0]) / r2;
+    }
+    f0[1] = f1[1] = 0;
+    return;
+  }
   bool inside = (k.kind == KIND_RALPHA) ? !(r2 > k.Rdir * k.Rdir) : !(r > k.Rdir);
   cd v, d;
   if (inside) {

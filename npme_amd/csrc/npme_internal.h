@@ -60,7 +60,9 @@ template <typename F> inline int guarded(F &&f)
 }
 
 // ------------------------------------------------------------- kernels
-enum KernelKind { KIND_LAPLACE = 0, KIND_RALPHA = 1, KIND_HELMHOLTZ = 2 };
+// KIND_LAPLACE_ERF: 1/r with the Ewald split erf(beta r)/r + erfc(beta r)/r
+// (EwaldSplit = LaplaceOrig) instead of the derivative match
+enum KernelKind { KIND_LAPLACE = 0, KIND_RALPHA = 1, KIND_HELMHOLTZ = 2, KIND_LAPLACE_ERF = 3 };
 
 // Derivative-match split of a radial kernel (host, FP64).
 struct DMKernel {
@@ -71,9 +73,14 @@ struct DMKernel {
   double Rdir = 0;
   double a[2 * 31] = {0};  // (re, im) pairs
   double b[2 * 31] = {0};
+  double beta = 0;         // KIND_LAPLACE_ERF
 };
 bool dm_kernel_init(DMKernel &k, int kind, double alpha, double k0r, double k0i, int Nder,
                     double Rdir, std::string &err);
+// the Ewald split of 1/r (EwaldSplit = LaplaceOrig) with parameter beta
+void erf_kernel_init(DMKernel &k, int Nder, double Rdir, double beta);
+// beta with erfc(beta Rdir)/Rdir ~ tol (src/KernelFunctionLaplace.cpp:41-72)
+double rdir2beta(double Rdir, double tol);
 // f_LR at r2 (FP64): f0[2] (re, im)
 void dm_eval_lr(const DMKernel &k, double r2, double f0[2], double f1[2]);
 
@@ -301,6 +308,7 @@ struct DirectParams {
   int Nder;
   double Rdir;
   double alpha;
+  double beta;  // KIND_LAPLACE_ERF
   double k0r, k0i;
   double a[2 * 31], b[2 * 31];
   int nNeigh;

@@ -22,6 +22,7 @@ template <> struct V4t<double> { using T = double4; };
 struct DirectDev {
   int kind, Nder, nNeigh;
   double Rdir, alpha, k0r, k0i, cs;
+  double beta;     // Ewald split (KIND_LAPLACE_ERF)
   double a[62], b[62];    // DM coefficients (re, im), zero beyond Nder
   float af[62], bf[62];   // the same in FP32
   int nb0, nb1, nb2;
@@ -243,54 +244,9 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
             rz = me.z - (o.z + offz);
             r2 = rx * rx + ry * ry + rz * rz;
           }
-          if (r2 < R2) {
-            if constexpr (KIND == KIND_LAPLACE) {
-              const Real ri = rsqrt_r<Real>(r2);
-              const Real f0 = ri - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-              const Real f1 = -ri * ri * ri - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
-              const Real q = o.w, fq = f1 * q;
-              part[0] += f0 * q;
-              part[1] += rx * fq;
-              part[2] += ry * fq;
-              part[3] += rz * fq;
-            } else if constexpr (KIND == KIND_RALPHA) {
-              const Real fe = pow(r2, (Real)(0.5 * p.alpha));
-              const Real f0 = fe - horner_fix<Real, MAXD>(coef_a<Real>(p), r2);
-              const Real f1 = (Real)p.alpha * fe / r2 - horner_fix<Real, MAXD>(coef_b<Real>(p), r2);
-              const Real q = o.w, fq = f1 * q;
-              part[0] += f0 * q;
-              part[1] += rx * fq;
-              part[2] += ry * fq;
-              part[3] += rz * fq;
-            } else {
-              // exp(i k0 r)/r with the phase k0r r reduced modulo 2 pi in FP64
-              const Real r = sqrt(r2);
-              const Real ri = (Real)1 / r;
-              const double ph = p.k0r * (double)r;
-              const double red = ph - 6.283185307179586477 * rint(ph * 0.15915494309189533577);
-              Real sn, cn;
-              sincos_r((Real)red, &sn, &cn);
-              const Real damp = exp((Real)(-p.k0i) * r) * ri;
-              const Real fr = damp * cn, fi = damp * sn;
-              const Real k0r = (Real)p.k0r, k0i = (Real)p.k0i;
-              const Real ikr = -k0i - ri, iki = k0r;  // i k0 - 1/r
-              const Real f1r = (ikr * fr - iki * fi) * ri, f1i = (ikr * fi + iki * fr) * ri;
-              Real pr, pim, br, bi;
-              horner_fix_cx<Real, MAXD>(coef_a<Real>(p), r2, pr, pim);
-              horner_fix_cx<Real, MAXD>(coef_b<Real>(p), r2, br, bi);
-              const Real s0r = fr - pr, s0i = fi - pim, s1r = f1r - br, s1i = f1i - bi;
-              const Real qr = o.w, qim = qi[js];
-              part[0] += s0r * qr - s0i * qim;
-              part[1] += s0r * qim + s0i * qr;
-              const Real gr = s1r * qr - s1i * qim, gi = s1r * qim + s1i * qr;
-              part[2] += rx * gr;
-              part[3] += rx * gi;
-              part[4] += ry * gr;
-              part[5] += ry * gi;
-              part[6] += rz * gr;
-              part[7] += rz * gi;
-            }
-          }
+          if (r2 < R2)
+            pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o.w, CX ? qi[js] : (Real)0, part,
+                                      coef_a<Real>(p)[2 * MAXD], coef_b<Real>(p)[2 * MAXD]);
         }
         js = jn;
         o = on;
@@ -330,6 +286,7 @@ static DirectDev direct_dev(const DirectParams &p)
   d.k0r = p.k0r;
   d.k0i = p.k0i;
   d.cs = p.cs;
+  d.beta = p.beta;
   for (int i = 0; i < 62; i++) {
     const bool used = i < 2 * (p.Nder + 1);
     d.a[i] = used ? p.a[i] : 0.0;
@@ -367,12 +324,15 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
     else if (p.Nder <= 16) NPME_DIRECT_D(R, K, HL, 16);                                         \
     else NPME_DIRECT_D(R, K, HL, 30);                                                           \
   } while (0)
+  // the erfc split has no polynomial: one instance
   if (fp64) {
     if (p.kind == KIND_LAPLACE) NPME_DIRECT(double, KIND_LAPLACE, false);
+    else if (p.kind == KIND_LAPLACE_ERF) NPME_DIRECT_D(double, KIND_LAPLACE_ERF, false, 8);
     else if (p.kind == KIND_RALPHA) NPME_DIRECT(double, KIND_RALPHA, false);
     else NPME_DIRECT(double, KIND_HELMHOLTZ, false);
   } else {
     if (p.kind == KIND_LAPLACE) NPME_DIRECT(float, KIND_LAPLACE, true);
+    else if (p.kind == KIND_LAPLACE_ERF) NPME_DIRECT_D(float, KIND_LAPLACE_ERF, true, 8);
     else if (p.kind == KIND_RALPHA) NPME_DIRECT(float, KIND_RALPHA, true);
     else NPME_DIRECT(float, KIND_HELMHOLTZ, true);
   }
@@ -475,6 +435,21 @@ __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Re
     const Real ri = rsqrt_r<Real>(r2);
     const Real f0 = ri - horner_top<Real, MAXD>(topA, coef_a<Real>(p), r2);
     const Real f1 = -ri * ri * ri - horner_top<Real, MAXD>(topB, coef_b<Real>(p), r2);
+    const Real fq = f1 * qr;
+    part[0] += f0 * qr;
+    part[1] += rx * fq;
+    part[2] += ry * fq;
+    part[3] += rz * fq;
+  } else if constexpr (KIND == KIND_LAPLACE_ERF) {
+    // erfc(beta r)/r and (1/r) d/dr of it
+    // (src/KernelFunctionLaplace.cpp:1344-1373)
+    (void)topA;
+    (void)topB;
+    const Real ri = rsqrt_r<Real>(r2);
+    const Real b = (Real)p.beta;
+    const Real f0 = erfc(b * (r2 * ri)) * ri;
+    const Real g = (Real)(p.beta * 1.12837916709551257390) * exp(-(b * b) * r2);  // 2 beta/sqrt(pi)
+    const Real f1 = -(f0 + g) * ri * ri;
     const Real fq = f1 * qr;
     part[0] += f0 * qr;
     part[1] += rx * fq;
@@ -686,10 +661,12 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
   } while (0)
   if (fp64) {
     if (p.kind == KIND_LAPLACE) NPME_DNL(double, KIND_LAPLACE, false);
+    else if (p.kind == KIND_LAPLACE_ERF) NPME_DNL_D(double, KIND_LAPLACE_ERF, false, 4);
     else if (p.kind == KIND_RALPHA) NPME_DNL(double, KIND_RALPHA, false);
     else NPME_DNL(double, KIND_HELMHOLTZ, false);
   } else {
     if (p.kind == KIND_LAPLACE) NPME_DNL(float, KIND_LAPLACE, true);
+    else if (p.kind == KIND_LAPLACE_ERF) NPME_DNL_D(float, KIND_LAPLACE_ERF, true, 4);
     else if (p.kind == KIND_RALPHA) NPME_DNL(float, KIND_RALPHA, true);
     else NPME_DNL(float, KIND_HELMHOLTZ, true);
   }
@@ -1194,7 +1171,7 @@ void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
 {
   (void)perm;
   const dim3 g((unsigned)((nt + 255) / 256));
-  if (kind == KIND_LAPLACE)
+  if (kind == KIND_LAPLACE || kind == KIND_LAPLACE_ERF)  // the full kernel 1/r
     hipLaunchKernelGGL(k_exact<KIND_LAPLACE>, g, dim3(256), 0, s, alpha, k0r, k0i, n, coordS,
                        q_in, nt, idx, Vout);
   else if (kind == KIND_RALPHA)
@@ -1209,13 +1186,21 @@ void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
 // ------------------------------------------------------------------ F grid
 struct LRDev {
   int kind, Nder;
-  double Rdir, alpha, k0r, k0i;
+  double Rdir, alpha, k0r, k0i, beta;
   double a[62], b[62];
 };
 
 __device__ void lr_eval(const LRDev &k, double r2, double &fr, double &fi)
 {
   const double r = sqrt(fabs(r2));
+  if (k.kind == KIND_LAPLACE_ERF) {
+    // erf(beta r)/r, 2 beta/sqrt(pi) (1 - x^2/3) below x = beta r = 1e-8
+    // (src/KernelFunctionLaplace.cpp:1087-1116)
+    const double x = k.beta * r;
+    fr = x < 1e-8 ? k.beta * 1.12837916709551257390 * (1.0 - x * x / 3.0) : erf(x) / r;
+    fi = 0;
+    return;
+  }
   const bool inside = (k.kind == KIND_RALPHA) ? !(r2 > k.Rdir * k.Rdir) : !(r > k.Rdir);
   if (inside) {
     double sr = k.a[2 * k.Nder], si = k.a[2 * k.Nder + 1];
@@ -1278,6 +1263,7 @@ void launch_fgrid_real_space(const DMKernel &km, const long Mf[3], const double 
   k.alpha = km.alpha;
   k.k0r = km.k0r;
   k.k0i = km.k0i;
+  k.beta = km.beta;
   for (int i = 0; i < 62; i++) {
     k.a[i] = km.a[i];
     k.b[i] = km.b[i];

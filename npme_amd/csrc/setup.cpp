@@ -647,8 +647,10 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   std::string funcType, split, prec;
   if (!kw.get("funcType", funcType)) kw_error("missing keyword 'funcType'");
   if (!kw.get("EwaldSplit", split)) kw_error("missing keyword 'EwaldSplit'");
-  if (split != "DerivMatch")
-    throw Error{NPME_ERR_UNSUPPORTED, "EwaldSplit = " + split + " is not provided (DerivMatch only)"};
+  if (split != "DerivMatch" && split != "LaplaceOrig")
+    kw_error("EwaldSplit = " + split + " is undefined ('DerivMatch' or 'LaplaceOrig')");
+  if (split == "LaplaceOrig" && funcType != "Laplace")
+    kw_error("keywords for EwaldSplit and funcType are incompatible");
   double Rdir;
   int Nder;
   if (!kw.get("Rdir", Rdir)) kw_error("missing keyword 'Rdir'");
@@ -664,9 +666,14 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   }
   if (!kw.get("nNeigh", h.nNeigh)) kw_error("missing keyword 'nNeigh'");
   if (!kw.get("nCellClust1D", h.nCellClust1D)) kw_error("missing keyword 'nCellClust1D'");
+  // useCompactF 0 asks for the kernel grid F on the full N1 x N2 x N3 grid
+  // (src/RecSumGrid.cpp:293-379, src/RecSumV.cpp:583-637).  The built-in
+  // kernels are radial and the grid samples are symmetric, so the full F is
+  // the even extension of the compact one and the convolution is the same:
+  // both values run the compact F.
   long useCompactF = 1;
-  if (kw.get("useCompactF", useCompactF) && useCompactF != 1)
-    throw Error{NPME_ERR_UNSUPPORTED, "useCompactF = 0 is not provided"};
+  if (kw.get("useCompactF", useCompactF) && useCompactF != 0 && useCompactF != 1)
+    kw_error("useCompactF must be 0 or 1");
   h.fp64 = kw.get("precision", prec) && prec == "fp64";
   {
     std::string fk = "auto";
@@ -679,7 +686,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
 
   int kind;
   double alpha = -1.0, k0r = 0, k0i = 0;
-  if (funcType == "Laplace") kind = KIND_LAPLACE;
+  if (funcType == "Laplace") kind = split == "LaplaceOrig" ? KIND_LAPLACE_ERF : KIND_LAPLACE;
   else if (funcType == "Ralpha") {
     kind = KIND_RALPHA;
     if (!kw.get("alpha", alpha)) kw_error("funcType = Ralpha needs 'alpha'");
@@ -693,7 +700,15 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   }
   h.complexK = (kind == KIND_HELMHOLTZ);
   std::string err;
-  if (!dm_kernel_init(h.ker, kind, alpha, k0r, k0i, Nder, Rdir, err)) kw_error(err);
+  if (kind == KIND_LAPLACE_ERF) {
+    // NPME_InterfaceReal_Laplace_Original::SetUp (src/Interface.cpp:2152-2230)
+    double tol;
+    if (!kw.get("tol", tol)) kw_error("missing keyword 'tol'");
+    if (!(Rdir > 0) || !(tol > 0) || !(tol < 1)) kw_error("EwaldSplit LaplaceOrig needs Rdir > 0, 0 < tol < 1");
+    erf_kernel_init(h.ker, Nder, Rdir, rdir2beta(Rdir, tol));
+  } else if (!dm_kernel_init(h.ker, kind, alpha, k0r, k0i, Nder, Rdir, err)) {
+    kw_error(err);
+  }
   {
     double f1[2];
     dm_eval_lr(h.ker, 0.0, h.fself, f1);
@@ -808,6 +823,7 @@ DirectParams direct_params(const Handle &h)
   dp.Nder = h.ker.Nder;
   dp.Rdir = h.ker.Rdir;
   dp.alpha = h.ker.alpha;
+  dp.beta = h.ker.beta;
   dp.k0r = h.ker.k0r;
   dp.k0i = h.ker.k0i;
   std::memcpy(dp.a, h.ker.a, sizeof(dp.a));

@@ -129,6 +129,59 @@ int npo_kernel_init(npo_kernel *k, int kind, double alpha, double k0r,
   return 0;
 }
 
+/* ======================================================================
+ * Ewald split of 1/r (EwaldSplit = LaplaceOrig).
+ * beta: the root x of x^2 + ln x + ln(sqrt(pi) tol Rdir) = 0 (Newton from
+ * x = 0.1, |F| < 1e-12), beta = x / Rdir, so that erfc(beta Rdir)/Rdir is
+ * about tol (src/KernelFunctionLaplace.cpp:41-72).
+ * LR = erf(beta r)/r (2 beta/sqrt(pi) at r = 0), SR = erfc(beta r)/r for
+ * r <= Rdir, with f1 = (1/r) df/dr (src/KernelFunctionLaplace.cpp:1087-1116,
+ * :1344-1373).  The reference's direct sum also adds the erfc term of the
+ * pairs beyond Rdir inside the cluster pairs it keeps
+ * (src/PotentialLaplace.cpp:613-668 has no cutoff); here, as on the GPU,
+ * the sum is over r <= Rdir, which differs from it by O(tol).
+ * Pinned by: LR + SR = 1/r, LR(0) = 2 beta/sqrt(pi), f1 against a finite
+ * difference of f0, and the beta equation's residual (tests).
+ * ==================================================================== */
+double npo_rdir2beta(double Rdir, double tol)
+{
+  const double C = log(sqrt(PI) * tol * Rdir);
+  double x = 0.1;
+  for (int it = 0; it < 1000; it++) {
+    const double F0 = x * x + log(x) + C, F1 = 2 * x + 1.0 / x;
+    if (fabs(F0) < 1e-12) return x / Rdir;
+    x -= F0 / F1;
+  }
+  return -1.0;
+}
+
+int npo_kernel_init_erf(npo_kernel *k, double Rdir, double beta)
+{
+  memset(k, 0, sizeof(*k));
+  if (!(Rdir > 0) || !(beta > 0)) return -1;
+  k->kind = NPO_LAPLACE_ERF;
+  k->alpha = -1.0;
+  k->Nder = 2;
+  k->Rdir = Rdir;
+  k->beta = beta;
+  return 0;
+}
+
+/* erf(beta r)/r and (1/r) d/dr of it; the small-r limits by series */
+static void erf_over_r(double beta, double r, double *f0, double *f1)
+{
+  const double x = beta * r, c = 2.0 / sqrt(PI);
+  if (x < 1e-4) {
+    /* erf(x)/x = c (1 - x^2/3 + x^4/10); (1/x) d/dx = c (-2/3 + 2 x^2/5) */
+    *f0 = beta * c * (1.0 - x * x / 3.0 + x * x * x * x / 10.0);
+    *f1 = beta * beta * beta * c * (-2.0 / 3.0 + 0.4 * x * x);
+    return;
+  }
+  const double e = erf(x) / r;
+  *f0 = e;
+  *f1 = (beta * c * exp(-x * x) - e) / (r * r);
+}
+
 static cplx horner(const double *c, int deg, double r2)
 {
   cplx s = c[2 * deg] + I * c[2 * deg + 1];
@@ -170,6 +223,26 @@ void npo_kernel_eval(const npo_kernel *k, int part, double r2, double *f0o,
   cplx f0 = 0, f1 = 0;
   const double R = k->Rdir;
   const double r = sqrt(fabs(r2));
+  if (k->kind == NPO_LAPLACE_ERF) {
+    double l0, l1;
+    erf_over_r(k->beta, r, &l0, &l1);
+    if (part == NPO_FULL) {
+      f0o[0] = 1.0 / r;
+      f1o[0] = -1.0 / (r * r2);
+    } else if (part == NPO_LR) {
+      f0o[0] = l0;
+      f1o[0] = l1;
+    } else if (!(r > R)) {  /* erfc(beta r)/r inside the cutoff */
+      const double x = k->beta * r;
+      const double e = erfc(x) / r;
+      f0o[0] = e;
+      f1o[0] = -(k->beta * 2.0 / sqrt(PI) * exp(-x * x) + e) / (r * r);
+    } else {
+      f0o[0] = 0;
+      f1o[0] = 0;
+    }
+    return;
+  }
   int inside;
   if (k->kind == NPO_RALPHA) inside = !(r2 > R * R);
   else if (k->kind == NPO_HELMHOLTZ && part == NPO_SR) inside = (r < R);

@@ -26,6 +26,7 @@ extern "C" {
 #define NPO_LAPLACE   0   /* f = 1/r                         */
 #define NPO_RALPHA    1   /* f = r^alpha                     */
 #define NPO_HELMHOLTZ 2   /* f = exp(i k0 r)/r (complex)     */
+#define NPO_LAPLACE_ERF 3 /* f = 1/r with the Ewald split erf(beta r)/r + erfc(beta r)/r */
 
 /* kernel parts */
 #define NPO_FULL 0
@@ -40,6 +41,7 @@ typedef struct {
   double Rdir;       /* direct-space cutoff                           */
   double a[2*31];    /* DM even-series coefficients (complex: re,im)  */
   double b[2*31];    /* (1/r) d/dr coefficients                       */
+  double beta;       /* Ewald split parameter (NPO_LAPLACE_ERF)       */
 } npo_kernel;
 
 /* ---- B-splines (src/Bspline.cpp:16714, :16766) ------------------------ */
@@ -49,6 +51,11 @@ double npo_bspline_dw(int n, double w, double *dBdw);
 /* ---- DM split (src/FunctionDerivMatch.cpp:42-91, :226-298, :364-450) --- */
 int  npo_kernel_init(npo_kernel *k, int kind, double alpha, double k0r,
                      double k0i, int Nder, double Rdir);
+/* Ewald (erf/erfc) split of 1/r, EwaldSplit = LaplaceOrig
+ * (src/KernelFunctionLaplace.cpp:41-72, :1087-1116, :1344-1373):
+ * LR = erf(beta r)/r, SR = erfc(beta r)/r, beta from Rdir and tol. */
+double npo_rdir2beta(double Rdir, double tol);
+int  npo_kernel_init_erf(npo_kernel *k, double Rdir, double beta);
 /* evaluate one part of the kernel at r2: f0 and f1 = (1/r) df/dr.
  * Real kernels write f0[0], f1[0]; Helmholtz writes (re, im) pairs. */
 void npo_kernel_eval(const npo_kernel *k, int part, double r2,

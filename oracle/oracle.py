@@ -13,14 +13,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnpme_oracle.so")
 
-LAPLACE, RALPHA, HELMHOLTZ = 0, 1, 2
+LAPLACE, RALPHA, HELMHOLTZ, LAPLACE_ERF = 0, 1, 2, 3
 FULL, LR, SR = 0, 1, 2
 
 
 class Kernel(C.Structure):
     _fields_ = [("kind", C.c_int), ("alpha", C.c_double), ("k0r", C.c_double),
                 ("k0i", C.c_double), ("Nder", C.c_int), ("Rdir", C.c_double),
-                ("a", C.c_double * 62), ("b", C.c_double * 62)]
+                ("a", C.c_double * 62), ("b", C.c_double * 62), ("beta", C.c_double)]
 
     def coeffs(self):
         n = self.Nder + 1
@@ -56,6 +56,9 @@ def lib():
         L.npo_bspline_dw.restype = d
         L.npo_bspline_dw.argtypes = [i, d, C.POINTER(d)]
         L.npo_kernel_init.argtypes = [C.POINTER(Kernel), i, d, d, d, i, d]
+        L.npo_rdir2beta.argtypes = [d, d]
+        L.npo_rdir2beta.restype = d
+        L.npo_kernel_init_erf.argtypes = [C.POINTER(Kernel), d, d]
         L.npo_kernel_eval.argtypes = [C.POINTER(Kernel), i, d, p, p]
         L.npo_optimal_grid_size.restype = l
         L.npo_optimal_grid_size.argtypes = [l]
@@ -115,6 +118,18 @@ def kernel(kind, Nder, Rdir, alpha=-1.0, k0=0.0):
     rc = lib().npo_kernel_init(C.byref(k), int(kind), float(alpha), k0.real, k0.imag,
                                int(Nder), float(Rdir))
     if rc != 0:
+        raise ValueError("bad kernel parameters")
+    return k
+
+
+def rdir2beta(Rdir, tol):
+    return lib().npo_rdir2beta(float(Rdir), float(tol))
+
+
+def kernel_erf(Rdir, beta):
+    """1/r with the Ewald split (EwaldSplit = LaplaceOrig)"""
+    k = Kernel()
+    if lib().npo_kernel_init_erf(C.byref(k), float(Rdir), float(beta)) != 0:
         raise ValueError("bad kernel parameters")
     return k
 

@@ -1,5 +1,5 @@
 """GPU runs of the command-line tools on the reference's own test inputs
-(tests/golden/cli: the keyword files of test/01, 04, 05 and the
+(tests/golden/cli: the keyword files of test/01, 02, 04, 05 and the
 makeRandomBox inputs, with nCharge lowered to 4,000 so the exact sum stays
 short): makeRandomBox writes the box, npme / RalphaDM / helmholtzDM run
 calcType pme_exact, and compareV reads the two potential files.  Checked:
@@ -53,6 +53,9 @@ CASES = [
     ("RalphaDM", "RalphaDM_05.txt", "Real", 5e-6),
     ("helmholtzDM", "helmholtzDM_04.txt", "Complex", 5e-6),
     ("npme", "helmholtzDM_04.txt", "Complex", 5e-6),
+    # erf/erfc split on the grid RSEM picks for the derivative match: less
+    # accurate than DM there, as in the reference
+    ("npme", "laplaceOrig_06.txt", "Real", 2e-5),
 ]
 
 
@@ -63,7 +66,7 @@ def test_calc_tool_pme_exact(tmp_path, tool, kwfile, kind, bound):
     print("case cli", tool, kwfile, flush=True)
     crd = _box(tmp, kind)
     kw = os.path.join(tmp, kwfile)
-    _kw_file(os.path.join(GOLD, kwfile), kw)
+    _kw_file(os.path.join(GOLD, kwfile), kw, calcType="pme_exact")
     _run([os.path.join(BIN, tool), crd, kwfile], tmp)
     stem = crd.split(".")[0] + "_" + kwfile.split(".")[0]
     log = open(os.path.join(tmp, stem + ".log")).read()

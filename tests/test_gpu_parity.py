@@ -87,6 +87,30 @@ def test_ralpha_anisotropic(oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("compactF", [1, 0])
+def test_laplace_orig_split(oracle, prec, compactF):
+    """EwaldSplit LaplaceOrig (erf/erfc split, beta from Rdir and tol) on the
+    RSEM grid: against the oracle's erfc-split CalcV (same beta, same grid)
+    and against the exact sum, which it meets less well than the derivative
+    match on that grid (as in the reference).  useCompactF 0 gives the same
+    result as 1 (the full F of a radial kernel is the compact F extended)."""
+    coord, q = uniform_box(3000, 30.0)
+    h = _real_setup(coord, q, prec, Rdir=8.0, EwaldSplit="LaplaceOrig", useCompactF=compactF)
+    V = h.calcv(q)
+    assert "LaplaceOrig" in h.Print()
+    k = oracle.kernel_erf(8.0, oracle.rdir2beta(8.0, 1e-6))
+    Vo = oracle.calcv(k, _grid(h), 6, coord, q)
+    mo = oracle.compare_v(Vo, V)
+    me = oracle.compare_v(oracle.exact(oracle.kernel(oracle.LAPLACE, 6, 8.0), coord, q), V)
+    print(prec, compactF, "vs oracle", mo, "vs exact", me)
+    if prec == "fp64":
+        assert mo["relV"] < 1e-11 and mo["reldV"] < 1e-11
+    else:
+        assert mo["relV"] < 1e-6 and mo["reldV"] < 5e-6
+    assert me["relV"] < 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_helmholtz(oracle, prec):
     import npme_amd as N
     coord, _ = uniform_box(3000, 24.0)

@@ -253,3 +253,65 @@ def test_compare_v_metric(oracle):
     assert abs(m["magV"] - np.mean(np.abs(Vr[:, 0]))) < 1e-15
     # quirk of the reference: the "gradient magnitude" averages components 0..2
     assert abs(m["magdV"] - np.mean(np.abs(Vr[:, :3]))) < 1e-15
+
+
+# ------------------------------------------------------------ Ewald (erfc) split
+@pytest.mark.parametrize("R,tol", [(4.0, 1e-6), (6.0, 1e-6), (8.0, 1e-6), (6.0, 1e-4)])
+def test_erf_split_beta(oracle, R, tol):
+    """beta of EwaldSplit LaplaceOrig: erfc(beta R)/R is tol times the
+    asymptotic factor 1 - 1/(2x^2) + 3/(4x^4), x = beta R (scipy's erfc), and
+    beta R solves the defining equation x^2 + ln x + ln(sqrt(pi) tol R) = 0."""
+    from scipy.special import erfc
+    b = oracle.rdir2beta(R, tol)
+    x = b * R
+    assert abs(x * x + math.log(x) + math.log(math.sqrt(math.pi) * tol * R)) < 1e-11
+    ratio = erfc(x) / R / tol
+    assert 0.85 < ratio < 1.0
+    assert ratio == pytest.approx(1 - 1 / (2 * x * x) + 3 / (4 * x ** 4), abs=0.01)
+
+
+def test_erf_split_parts(oracle):
+    """LR + SR = 1/r inside Rdir (SR = 0 outside), LR(0) = 2 beta/sqrt(pi),
+    and f1 = (1/r) df/dr against central differences of f0, for both parts;
+    the values against scipy's erf / erfc."""
+    from scipy.special import erf, erfc
+    R, b = 6.0, oracle.rdir2beta(6.0, 1e-6)
+    k = oracle.kernel_erf(R, b)
+    assert oracle.kernel_eval(k, oracle.LR, 0.0)[0] == pytest.approx(2 * b / math.sqrt(math.pi), rel=1e-14)
+    for r in (1e-5, 0.3, 1.0, 2.5, 5.9, 7.0):
+        full = oracle.kernel_eval(k, oracle.FULL, r * r)[0]
+        lr, lr1 = oracle.kernel_eval(k, oracle.LR, r * r)
+        sr, sr1 = oracle.kernel_eval(k, oracle.SR, r * r)
+        assert lr == pytest.approx(erf(b * r) / r, rel=1e-12)
+        if r < R:
+            assert lr + sr == pytest.approx(full, rel=1e-12)
+            assert sr == pytest.approx(erfc(b * r) / r, rel=1e-12)
+        else:
+            assert sr == 0.0
+        h = 1e-5 * max(r, 1e-3)
+        if r > 1e-3:
+            for part, f1 in ((oracle.LR, lr1), (oracle.SR, sr1)):
+                if part == oracle.SR and r >= R:
+                    continue
+                d = (oracle.kernel_eval(k, part, (r + h) ** 2)[0]
+                     - oracle.kernel_eval(k, part, (r - h) ** 2)[0]) / (2 * h)
+                assert f1 == pytest.approx(d / r, rel=1e-6)
+
+
+def test_erf_split_whole_path_vs_exact(oracle):
+    """The oracle's CalcV with the erfc split reaches the split's truncation
+    floor (~tol) against the exact sum on a fine grid, and is worse than the
+    derivative match on the grid RSEM picks (the DM grid, as in the
+    reference's AddMissingKeywords)."""
+    coord, q = uniform_box(3000, 30.0)
+    R = 8.0
+    N, nb, nd = oracle.rsem_static(q, coord, 6, -1.0, 1e-6, R)
+    k = oracle.kernel_erf(R, oracle.rdir2beta(R, 1e-6))
+    kd = oracle.kernel(oracle.LAPLACE, nd, R)
+    Ve = oracle.exact(kd, coord, q)
+    fine = tuple(int(n * 3 // 2 // 4 * 4) for n in N)
+    e_fine = oracle.compare_v(Ve, oracle.calcv(k, fine, 6, coord, q))["relV"]
+    e_rsem = oracle.compare_v(Ve, oracle.calcv(k, N, 6, coord, q))["relV"]
+    e_dm = oracle.compare_v(Ve, oracle.calcv(kd, N, 6, coord, q))["relV"]
+    print("erf fine", e_fine, "erf rsem", e_rsem, "dm", e_dm)
+    assert e_fine < 2e-6 and e_dm < e_rsem < 1e-5
