@@ -202,6 +202,7 @@ def main():
     dev = torch.device("cuda:0")
     dq = torch.from_numpy(q).to(dev)
     dV = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()  # the library reads dq from its own streams
 
     h, t_setup = setup(coord, q, "mixed")
     Ngrid = (h.Get_N1(), h.Get_N2(), h.Get_N3())

@@ -95,9 +95,11 @@ int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget,
                             const long *idx, const double *charge);
 
 /* ---- device-resident CalcV: dCharge and dV are device pointers
- * (charges in input order, FP64; V [n][4] FP64 or [n][4][2]).  No host
- * traffic; returns when the result is complete.  No reference
- * counterpart: the same CalcV with inputs already in HBM. -------------- */
+ * (charges in input order, FP64; V [n][4] FP64 or [n][4][2]).  dCharge
+ * must be complete when the call is made (its producer synchronised): the
+ * library reads it from its own streams.  No host traffic; returns when
+ * the result is complete.  No reference counterpart: the same CalcV with
+ * inputs already in HBM. ------------------------------------------------ */
 int npme_calcv_device(npme_handle *h, double *dV, const double *dCharge);
 
 /* ---- parameters: Get_N1 ... Get_n3, Get_BnOrder (src/Interface.h:89-115),

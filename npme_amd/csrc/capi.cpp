@@ -217,7 +217,7 @@ int npme_create(npme_handle **out, const npme_keywords *kw, size_t nCharge, cons
       handle_setup(h->h, kw->kw, nCharge, coord);
     } catch (...) {
       handle_free_device(h->h);
-      if (h->h.stream) (void)hipStreamDestroy(h->h.stream);
+      handle_destroy_streams(h->h);
       throw;
     }
     *out = h.release();
@@ -238,8 +238,7 @@ void npme_destroy(npme_handle *h)
 {
   if (!h) return;
   handle_free_device(h->h);
-  for (hipEvent_t e : h->h.ev) (void)hipEventDestroy(e);
-  if (h->h.stream) (void)hipStreamDestroy(h->h.stream);
+  handle_destroy_streams(h->h);
   delete h;
 }
 
@@ -263,6 +262,8 @@ static void calcv_host(npme_handle *h, double *V, size_t n, const double *charge
   Handle &H = h->h;
   const size_t qn = (cx ? 2 : 1) * n, vn = (cx ? 8 : 4) * n;
   NPME_HIP_CHECK(hipMemcpyAsync(H.d_qin, charge, qn * sizeof(double), hipMemcpyHostToDevice, H.stream));
+  // CalcV reads the charges from both of its streams
+  NPME_HIP_CHECK(hipStreamSynchronize(H.stream));
   handle_calcv(H, H.d_qin, H.d_V);
   NPME_HIP_CHECK(hipMemcpyAsync(V, H.d_V, vn * sizeof(double), hipMemcpyDeviceToHost, H.stream));
   NPME_HIP_CHECK(hipStreamSynchronize(H.stream));

@@ -183,6 +183,9 @@ struct Handle {
   int nt[3] = {0, 0, 0};
   // device buffers
   hipStream_t stream = nullptr;
+  // the direct sum runs on stream2, concurrently with the reciprocal sum on
+  // stream; the host joins them (no cross-stream device waits)
+  hipStream_t stream2 = nullptr;
   int device = 0;
   int *d_perm = nullptr;         // [n]
   void *d_src = nullptr;         // [n] double4 (local xyz, q) or mixed: int32 fixed-point xyz + FP32 q
@@ -198,6 +201,8 @@ struct Handle {
   int *d_sliceLen = nullptr;
   size_t nlistEntries = 0;
   void *d_qi = nullptr;          // [n] imag part (float/double) for complex kernels
+  void *d_qs = nullptr;          // [n] sorted charges (re) for spreading, working precision
+  void *d_qsi = nullptr;         // [n] sorted charges (im), complex kernels
   int4 *d_cell = nullptr;        // [n] cell coords (x,y,z), pad
   int *d_cellStart = nullptr;    // [ncell + 1]
   int4 *d_ms = nullptr;          // [n] stencil start u0 per dim (u = m + N/4)
@@ -270,6 +275,8 @@ void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, do
 // the (2 nNeigh + 1)^3 cells around it)
 double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellSE, hipStream_t s);
 void handle_free_device(Handle &h);
+// destroys the handle's streams and timing events (after handle_free_device)
+void handle_destroy_streams(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout);
 
@@ -321,6 +328,8 @@ double handle_count_pairs(Handle &h);
 DirectParams direct_params(const Handle &h);
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s);
+void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
+                     void *qre, void *qim, hipStream_t s);
 void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);
@@ -334,8 +343,8 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
 void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                         const double4 *xs, unsigned long long *total, hipStream_t s);
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
-                   const int nt[3], const int *tileStart, const int *tileList, const void *src,
-                   const void *qi, const int4 *ms, const void *frac, void *grid, hipStream_t s);
+                   const int nt[3], const int *tileStart, const int *tileList, const void *qre,
+                   const void *qim, const int4 *ms, const void *frac, void *grid, hipStream_t s);
 void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const double C[3], int T,
                    const int nt[3], const int *tileStart, const int *tileList, const int4 *ms,
                    const void *frac, const void *grid, const double *Vdir, const int *perm,

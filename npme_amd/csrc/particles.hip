@@ -90,6 +90,40 @@ __global__ void k_gather(size_t n, const int *__restrict__ perm, const double *_
   }
 }
 
+// charges in sorted order for the reciprocal sum (re and im in separate
+// arrays), so spreading does not read the direct sum's packed sources
+template <typename Real, bool CX>
+__global__ void k_gather_q(size_t n, const int *__restrict__ perm, const double *__restrict__ q,
+                           Real *__restrict__ qre, Real *__restrict__ qim)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int o = perm[s];
+  if (CX) {
+    qre[s] = (Real)q[2 * (size_t)o];
+    qim[s] = (Real)q[2 * (size_t)o + 1];
+  } else {
+    qre[s] = (Real)q[o];
+  }
+}
+
+void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
+                     void *qre, void *qim, hipStream_t s)
+{
+  const dim3 grid((unsigned)((n + 255) / 256));
+#define NPME_GQ(R, CXV)                                                                        \
+  hipLaunchKernelGGL((k_gather_q<R, CXV>), grid, dim3(256), 0, s, n, perm, q_in, (R *)qre, (R *)qim)
+  if (fp64) {
+    if (complexK) NPME_GQ(double, true);
+    else NPME_GQ(double, false);
+  } else {
+    if (complexK) NPME_GQ(float, true);
+    else NPME_GQ(float, false);
+  }
+#undef NPME_GQ
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s)
 {
@@ -499,20 +533,19 @@ __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Re
 // a slice walk their lists in lockstep (lists of one slice have similar
 // lengths, so little of the wavefront idles)
 template <typename Real, int KIND, bool FIXP, int MAXD>
-__global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
-                                                   const typename V4t<Real>::T *__restrict__ src,
-                                                   const float4 *__restrict__ res,
-                                                   const Real *__restrict__ qi,
-                                                   const long *__restrict__ sliceStart,
-                                                   const int *__restrict__ sliceLen,
-                                                   const unsigned *__restrict__ list, unsigned pad,
-                                                   double csA, double *__restrict__ Vdir)
+__device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_t slice, int lane,
+                                             const typename V4t<Real>::T *__restrict__ src,
+                                             const float4 *__restrict__ res,
+                                             const Real *__restrict__ qi,
+                                             const long *__restrict__ sliceStart,
+                                             const int *__restrict__ sliceLen,
+                                             const unsigned *__restrict__ list, unsigned pad,
+                                             double csA, double *__restrict__ Vdir, Real topA,
+                                             Real topB)
 {
   using V4 = typename V4t<Real>::T;
   constexpr bool CX = (KIND == KIND_HELMHOLTZ);
-  const size_t s = (size_t)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const size_t slice = s >> 6;
-  if (slice * 64 >= n) return;  // uniform per wavefront
+  const size_t s = slice * 64 + lane;
   const bool tvalid = s < n;
   const size_t tt = tvalid ? s : slice * 64;
   const V4 me = src[tt];
@@ -527,8 +560,6 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
     mres = res[tt];
   }
   const Real cA = (Real)csA;
-  Real topA = coef_a<Real>(p)[2 * MAXD], topB = coef_b<Real>(p)[2 * MAXD];
-  asm volatile("" : "+v"(topA), "+v"(topB));  // held in VGPRs for the pair loop
   // the slice is the wavefront's: its length is uniform, so the loop and
   // the bounds of the entry loads branch on scalars
   const int len = __builtin_amdgcn_readfirstlane(sliceLen[slice]);
@@ -636,13 +667,44 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
   }
 }
 
+
+// Each wavefront walks a contiguous range of slices: one slice with the
+// grid of a wavefront per slice used here.  (Grids of a few workgroups per
+// CU, which leave wave slots to the reciprocal sum's kernels on the other
+// stream, measured slower at N = 1M.)
+template <typename Real, int KIND, bool FIXP, int MAXD>
+__global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
+                                                   const typename V4t<Real>::T *__restrict__ src,
+                                                   const float4 *__restrict__ res,
+                                                   const Real *__restrict__ qi,
+                                                   const long *__restrict__ sliceStart,
+                                                   const int *__restrict__ sliceLen,
+                                                   const unsigned *__restrict__ list, unsigned pad,
+                                                   double csA, double *__restrict__ Vdir)
+{
+  const int lane = threadIdx.x & 63;
+  const size_t nSlices = (n + 63) / 64;
+  const size_t w = (size_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  const size_t nw = (size_t)gridDim.x * 4;
+  const size_t per = nSlices / nw, extra = nSlices - per * nw;
+  const size_t beg = w * per + (w < extra ? w : extra);
+  const size_t end = beg + per + (w < extra ? 1 : 0);
+  Real topA = coef_a<Real>(p)[2 * MAXD], topB = coef_b<Real>(p)[2 * MAXD];
+  asm volatile("" : "+v"(topA), "+v"(topB));  // held in VGPRs for the pair loop
+  for (size_t sl = beg; sl < end; sl++)  // uniform per wavefront
+    direct_slice<Real, KIND, FIXP, MAXD>(p, n, sl, lane, src, res, qi, sliceStart, sliceLen,
+                                         list, pad, csA, Vdir, topA, topB);
+}
+
 void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                       const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
                       double csA, double *Vdir, hipStream_t s)
 {
   const unsigned pad = nlist_pad(n);
   const DirectDev d = direct_dev(p);
-  const dim3 grid((unsigned)((n + 255) / 256));
+  // a workgroup (4 wavefronts) per 4 slices
+  const long nwg = (long)((n + 255) / 256);
+  const dim3 grid((unsigned)std::max<long>(nwg, 1));
 #define NPME_DNL_D(R, K, HL, D)                                                                 \
   hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \
                      (const typename V4t<R>::T *)src, (const float4 *)res, (const R *)qi,      \
@@ -780,8 +842,8 @@ template <typename Real, bool CX, int BN>
 __global__ __launch_bounds__(64) void k_spread(int BnRun, long M1, long M2, long M3, int nt1, int nt2,
                                                int nt3, const int *__restrict__ tileStart,
                                                const int *__restrict__ tileList,
-                                               const typename V4t<Real>::T *__restrict__ src,
-                                               const Real *__restrict__ qi,
+                                               const Real *__restrict__ qre,
+                                               const Real *__restrict__ qim,
                                                const int4 *__restrict__ ms,
                                                const typename V4t<Real>::T *__restrict__ frac,
                                                Real *__restrict__ grid)
@@ -823,8 +885,8 @@ __global__ __launch_bounds__(64) void k_spread(int BnRun, long M1, long M2, long
           w[threadIdx.x][d * TS + k] = v;
         }
       }
-      qs[threadIdx.x][0] = src[s].w;
-      qs[threadIdx.x][1] = CX ? qi[s] : (Real)0;
+      qs[threadIdx.x][0] = qre[s];
+      qs[threadIdx.x][1] = CX ? qim[s] : (Real)0;
     }
     __syncthreads();
     // FP32 (or FP64) sums over the chunk, carried into FP64 per chunk
@@ -872,15 +934,15 @@ __global__ __launch_bounds__(64) void k_spread(int BnRun, long M1, long M2, long
 }
 
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
-                   const int nt[3], const int *tileStart, const int *tileList, const void *src,
-                   const void *qi, const int4 *ms, const void *frac, void *grid, hipStream_t s)
+                   const int nt[3], const int *tileStart, const int *tileList, const void *qre,
+                   const void *qim, const int4 *ms, const void *frac, void *grid, hipStream_t s)
 {
   (void)T;
   const dim3 g((unsigned)(nt[0] * nt[1] * nt[2]));
 #define NPME_SPREAD_B(R, CXV, BNV)                                                              \
   hipLaunchKernelGGL((k_spread<R, CXV, BNV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2],     \
-                     nt[0], nt[1], nt[2], tileStart, tileList, (const typename V4t<R>::T *)src, \
-                     (const R *)qi, ms, (const typename V4t<R>::T *)frac, (R *)grid)
+                     nt[0], nt[1], nt[2], tileStart, tileList, (const R *)qre, (const R *)qim,  \
+                     ms, (const typename V4t<R>::T *)frac, (R *)grid)
 #define NPME_SPREAD(R, CXV)                                                                     \
   do {                                                                                          \
     switch (Bn) {                                                                               \

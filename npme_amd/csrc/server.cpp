@@ -123,6 +123,7 @@ static void run_job(Handle &h, Job &j)
   const bool cx = h.complexK;
   const size_t qn = (cx ? 2 : 1) * j.n, vn = (cx ? 8 : 4) * j.n;
   NPME_HIP_CHECK(hipMemcpyAsync(h.d_qin, j.charge, qn * sizeof(double), hipMemcpyHostToDevice, h.stream));
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // CalcV reads the charges from both streams
   handle_calcv(h, h.d_qin, h.d_V);
   NPME_HIP_CHECK(hipMemcpyAsync(j.V, h.d_V, vn * sizeof(double), hipMemcpyDeviceToHost, h.stream));
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -169,8 +170,7 @@ static void worker_main(Server *S)
     S->cvWork.notify_all();
   }
   handle_free_device(h);
-  for (hipEvent_t e : h.ev) (void)hipEventDestroy(e);
-  if (h.stream) (void)hipStreamDestroy(h.stream);
+  handle_destroy_streams(h);
 }
 
 }  // namespace npme

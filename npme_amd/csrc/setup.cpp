@@ -54,12 +54,15 @@ void handle_free_device(Handle &h)
   void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
-                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs};
+                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs,
+                  h.d_qs, h.d_qsi};
   // queued work of this handle may still read its buffers
   if (h.stream) (void)hipStreamSynchronize(h.stream);
+  if (h.stream2) (void)hipStreamSynchronize(h.stream2);
   for (void *p : ptrs) dev_free(p);
   h.d_perm = nullptr;
   h.d_src = h.d_srcRes = h.d_qi = h.d_blockBox = nullptr;
+  h.d_qs = h.d_qsi = nullptr;
   h.d_segStart = nullptr;
   h.d_itileStart = h.d_itileList = nullptr;
   h.d_nlist = nullptr;
@@ -296,7 +299,9 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   if (h.complexK) {
     h.d_qi = dalloc<char>(h, rs * (n + 1));
     NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_qi + rs * n, 0, rs, s));
+    h.d_qsi = dalloc<char>(h, rs * n);
   }
+  h.d_qs = dalloc<char>(h, rs * n);
   h.d_Vdir = dalloc<double>(h, vw * n);
   h.d_V = dalloc<double>(h, vw * n);
   h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
@@ -509,7 +514,9 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   if (h.complexK) {
     h.d_qi = dalloc<char>(h, rs * (n + 1));  // entry n: the list's zero-charge sentinel
     NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_qi + rs * n, 0, rs, h.stream));
+    h.d_qsi = dalloc<char>(h, rs * n);
   }
+  h.d_qs = dalloc<char>(h, rs * n);
   h.d_cell = dalloc<int4>(h, n);
   up(h.d_cell, cellOf.data(), n * sizeof(int4));
   h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
@@ -781,6 +788,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   // ---- per-charge state: on the device when the neighbour list is used
   NPME_HIP_CHECK(hipGetDevice(&h.device));
   if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
   h.setupMs[0] = ms_since(t0);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   handle_free_device(h);
@@ -802,6 +810,15 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   ms_since(t0);
   grid_setup(h);
   h.setupMs[5] = ms_since(t0);
+}
+
+void handle_destroy_streams(Handle &h)
+{
+  for (hipEvent_t e : h.ev) (void)hipEventDestroy(e);
+  h.ev.clear();
+  if (h.stream2) (void)hipStreamDestroy(h.stream2);
+  if (h.stream) (void)hipStreamDestroy(h.stream);
+  h.stream = h.stream2 = nullptr;
 }
 
 // ---------------------------------------------------------------- CalcV
@@ -854,21 +871,44 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
   hipStream_t s = h.stream;
   const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2], N1 = h.N[0], N2 = h.N[1], N3 = h.N[2];
   const long K3 = h.K3, Kp = h.Kp;
+  // The direct sum (gather of the packed sources, pair sum into Vdir) runs
+  // on stream2 while the reciprocal sum runs on stream: both are queued at
+  // once and the GPU shares its CUs between them.  The two paths read
+  // disjoint inputs (each gathers the charges itself); the host waits for
+  // stream2 before queuing the interpolation, which reads Vdir, so no
+  // stream waits on another on the device.  With per-stage timing on,
+  // everything runs on stream in order.
+  const bool split = !h.profiling;
+  hipStream_t s2 = split ? h.stream2 : s;
+  const DirectParams dp = direct_params(h);
+  auto direct = [&] {
+    launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s2);
+    if (h.useNlist)
+      launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
+                       h.d_nlist, h.csA, h.d_Vdir, s2);
+    else
+      launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
+                    (int)h.nCellClust1D, h.csA, h.d_Vdir, s2);
+  };
+  if (split) direct();
   int st = 0;
   mark(h, st++);
-  launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
-  mark(h, st++);
-  const DirectParams dp = direct_params(h);
-  if (h.useNlist)
-    launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
-                     h.d_nlist, h.csA, h.d_Vdir, s);
-  else
-    launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
-                  (int)h.nCellClust1D, h.csA, h.d_Vdir, s);
-  mark(h, st++);
-  launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_src, h.d_qi,
+  if (!split) {
+    // the same kernels in order, timed one by one: gather, direct
+    launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
+    mark(h, st++);
+    if (h.useNlist)
+      launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
+                       h.d_nlist, h.csA, h.d_Vdir, s);
+    else
+      launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
+                    (int)h.nCellClust1D, h.csA, h.d_Vdir, s);
+    mark(h, st++);
+  }
+  launch_gather_q(f64, cx, h.n, h.d_perm, d_q, h.d_qs, h.d_qsi, s);
+  launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_qs, h.d_qsi,
                 h.d_ms, h.d_frac, h.d_grid, s);
-  mark(h, st++);
+  mark(h, st++);  // stage "spread" includes the charge gather of the reciprocal sum
   if (!cx) {
     fft_pass_r2c_rows(f64, (int)N3, (int)M3, M1 * M2, M3, Kp, h.plan[2], h.d_tw[2], h.d_grid,
                       h.d_buf1, s);
@@ -907,6 +947,7 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     mark(h, st++);
   }
   const double C[3] = {N1 / h.L[0], N2 / h.L[1], N3 / h.L[2]};
+  if (split) NPME_HIP_CHECK(hipStreamSynchronize(s2));  // Vdir is complete
   launch_interp(f64, cx, h.Bn, h.M, C, h.TI, h.nti, h.d_itileStart, h.d_itileList, h.d_ms,
                 h.d_frac, h.d_grid, h.d_Vdir, h.d_perm, d_q, h.fself, d_Vout, s);
   mark(h, st++);
