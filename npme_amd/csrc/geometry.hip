@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_charge_setup(size_t n, const int *__res
       u0[d] = (int)(m0 + g.N[d] / 4);
       fr[d] = w + (double)m0;
     }
-    cnt *= (u0[d] + (int)g.Bn - 1) / 8 - u0[d] / 8 + 1;
+    cnt *= (u0[d] + (int)g.Bn - 1) / g.ts[d] - u0[d] / g.ts[d] + 1;
   }
   ms[s] = make_int4(u0[0], u0[1], u0[2], 0);
   ntile[s] = cnt;
@@ -176,9 +176,9 @@ __global__ __launch_bounds__(256) void k_tile_emit(size_t n, const int4 *__restr
   const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int4 u = ms[s];
-  const int lo0 = u.x / 8, hi0 = (u.x + (int)g.Bn - 1) / 8;
-  const int lo1 = u.y / 8, hi1 = (u.y + (int)g.Bn - 1) / 8;
-  const int lo2 = u.z / 8, hi2 = (u.z + (int)g.Bn - 1) / 8;
+  const int lo0 = u.x / g.ts[0], hi0 = (u.x + (int)g.Bn - 1) / g.ts[0];
+  const int lo1 = u.y / g.ts[1], hi1 = (u.y + (int)g.Bn - 1) / g.ts[1];
+  const int lo2 = u.z / g.ts[2], hi2 = (u.z + (int)g.Bn - 1) / g.ts[2];
   int o = off[s];
   for (int a = lo0; a <= hi0; a++)
     for (int b = lo1; b <= hi1; b++)

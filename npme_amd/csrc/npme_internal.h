@@ -256,7 +256,8 @@ struct GeoParams {
   long Bn;
   double fxScale;          // mixed form: 2^fxShift / csA
   int fxShift;
-  int nt[3];               // 8^3 spreading tiles per axis
+  int ts[3];               // spreading tile edge per axis (16 x 8 x 8 or 8^3)
+  int nt[3];               // spreading tiles per axis
   int TI, nti[3];          // interpolation tiles
 };
 void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, int *d_perm,

@@ -933,12 +933,154 @@ __global__ __launch_bounds__(64) void k_spread(int BnRun, long M1, long M2, long
   }
 }
 
+// Matrix-core spreading (FP32, real charges).  A tile of 16 x 8 x 8 grid
+// points is one wavefront; over the charges c listed for the tile
+//   Q[i][(j, k)] = sum_c (q_c w1_c[i]) (w2_c[j] w3_c[k]),
+// a product of a 16 x C and a C x 64 matrix, which v_mfma_f32_16x16x4f32
+// computes four charges at a time in four 16-column blocks.  Operand
+// layout of that instruction: lane l gives A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16] in register r.
+// Block b holds the columns (j, k) = (2b + n / 8, n % 8), n = l % 16.
+// Each chunk of 64 charges stages its tile-restricted weights in LDS (one
+// row per charge, zero outside the spline window), then runs the products;
+// the FP32 block sums are carried into FP64 per chunk.
+typedef float npme_f4 __attribute__((ext_vector_type(4)));
+template <int BN>
+__global__ __launch_bounds__(64) void k_spread_mfma(int BnRun, long M1, long M2, long M3, int nt1,
+                                                    int nt2, int nt3,
+                                                    const int *__restrict__ tileStart,
+                                                    const int *__restrict__ tileList,
+                                                    const float *__restrict__ qre,
+                                                    const int4 *__restrict__ ms,
+                                                    const float4 *__restrict__ frac,
+                                                    float *__restrict__ grid)
+{
+  constexpr int TX = 16, TY = 8, TZ = 8;
+  // rows of 16 floats: the 16 lanes of an operand group read one row, the
+  // four groups of a step read four consecutive rows (no bank conflicts)
+  __shared__ __align__(16) float sA[64][TX];
+  __shared__ __align__(16) float sW2[64][TY];
+  __shared__ __align__(16) float sW3[64][TZ];
+  const int Bn = BN > 0 ? BN : BnRun;
+  constexpr int BA = BN > 0 ? BN : 16;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int t3 = t % nt3, t2 = (t / nt3) % nt2, t1 = t / (nt3 * nt2);
+  const int u1 = t1 * TX, u2 = t2 * TY, u3 = t3 * TZ;
+  const int l = threadIdx.x, n16 = l & 15, kq = l >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int b = 0; b < 4; b++)
+#pragma unroll
+    for (int r = 0; r < 4; r++) acc[b][r] = 0;
+  const int beg = tileStart[t], end = tileStart[t + 1];
+  // charge l of a chunk: index, window start, fractions, q; the next
+  // chunk's are loaded while the current chunk's products run
+  int4 cm = make_int4(0, 0, 0, 0);
+  float4 cf = make_float4(0.f, 0.f, 0.f, 0.f);
+  float cq = 0.f;
+  auto fetch = [&](int base, int4 &m0, float4 &f, float &q) {
+    if (base + l < end) {
+      const int c = tileList[base + l];
+      m0 = ms[c];
+      f = frac[c];
+      q = qre[c];
+    }
+  };
+  fetch(beg, cm, cf, cq);
+  for (int base = beg; base < end; base += 64) {
+    const int cnt = min(64, end - base);
+    // row l: zeros, then the spline window of charge l at its tile slots.
+    // The w2 row is stored as (w2[0], w2[2], w2[4], w2[6], w2[1], w2[3],
+    // w2[5], w2[7]): a lane's four B columns j = 2b + h (h = n16 / 8) are
+    // then one 16-byte read
+    {
+      const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < TX; k += 4) *reinterpret_cast<float4 *>(&sA[l][k]) = z4;
+#pragma unroll
+      for (int k = 0; k < TY; k += 4) *reinterpret_cast<float4 *>(&sW2[l][k]) = z4;
+#pragma unroll
+      for (int k = 0; k < TZ; k += 4) *reinterpret_cast<float4 *>(&sW3[l][k]) = z4;
+    }
+    if (l < cnt) {
+      float sh[BA];
+      if constexpr (BN > 0) bsp_shifts_c<float, BN>(cf.x, sh);
+      else bsp_shifts<float>(Bn, cf.x, sh);
+      int off = cm.x - u1;
+#pragma unroll
+      for (int j = 0; j < BA; j++)
+        if (j < Bn && off + j >= 0 && off + j < TX) sA[l][off + j] = cq * sh[j];
+      if constexpr (BN > 0) bsp_shifts_c<float, BN>(cf.y, sh);
+      else bsp_shifts<float>(Bn, cf.y, sh);
+      off = cm.y - u2;
+#pragma unroll
+      for (int j = 0; j < BA; j++) {
+        const int jj = off + j;
+        if (j < Bn && jj >= 0 && jj < TY) sW2[l][4 * (jj & 1) + (jj >> 1)] = sh[j];
+      }
+      if constexpr (BN > 0) bsp_shifts_c<float, BN>(cf.z, sh);
+      else bsp_shifts<float>(Bn, cf.z, sh);
+      off = cm.z - u3;
+#pragma unroll
+      for (int j = 0; j < BA; j++)
+        if (j < Bn && off + j >= 0 && off + j < TZ) sW3[l][off + j] = sh[j];
+    }
+    __syncthreads();
+    fetch(base + 64, cm, cf, cq);
+    npme_f4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = d0, d2 = d0, d3 = d0;
+    const int nstep = (cnt + 3) >> 2;  // rows past cnt are zero
+#pragma unroll 2
+    for (int st = 0; st < nstep; st++) {
+      const int c = 4 * st + kq;
+      const float av = sA[c][n16];
+      const float w3 = sW3[c][n16 & 7];
+      const float4 w2 = *reinterpret_cast<const float4 *>(&sW2[c][4 * (n16 >> 3)]);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, w2.x * w3, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, w2.y * w3, d1, 0, 0, 0);
+      d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, w2.z * w3, d2, 0, 0, 0);
+      d3 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, w2.w * w3, d3, 0, 0, 0);
+    }
+    const npme_f4 dd[4] = {d0, d1, d2, d3};
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) acc[b][r] += (double)dd[b][r];
+    __syncthreads();  // the rows are rewritten by the next chunk
+  }
+  const long p3 = u3 + (n16 & 7);
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const long p2 = u2 + 2 * b + (n16 >> 3);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const long p1 = u1 + 4 * kq + r;
+      if (p1 < M1 && p2 < M2 && p3 < M3) grid[(p1 * M2 + p2) * M3 + p3] = (float)acc[b][r];
+    }
+  }
+}
+
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
                    const int nt[3], const int *tileStart, const int *tileList, const void *qre,
                    const void *qim, const int4 *ms, const void *frac, void *grid, hipStream_t s)
 {
-  (void)T;
   const dim3 g((unsigned)(nt[0] * nt[1] * nt[2]));
+  if (!fp64 && !complexK && T[0] == 16 && T[1] == 8 && T[2] == 8) {
+#define NPME_SPREAD_M(BNV)                                                                      \
+  hipLaunchKernelGGL((k_spread_mfma<BNV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2], nt[0],  \
+                     nt[1], nt[2], tileStart, tileList, (const float *)qre, ms,                 \
+                     (const float4 *)frac, (float *)grid)
+    switch (Bn) {
+      case 4: NPME_SPREAD_M(4); break;
+      case 6: NPME_SPREAD_M(6); break;
+      case 8: NPME_SPREAD_M(8); break;
+      case 10: NPME_SPREAD_M(10); break;
+      default: NPME_SPREAD_M(0); break;
+    }
+#undef NPME_SPREAD_M
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (T[0] != 8 || T[1] != 8 || T[2] != 8) throw Error{NPME_ERR_UNSUPPORTED, "spreading tile shape"};
 #define NPME_SPREAD_B(R, CXV, BNV)                                                              \
   hipLaunchKernelGGL((k_spread<R, CXV, BNV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2],     \
                      nt[0], nt[1], nt[2], tileStart, tileList, (const R *)qre, (const R *)qim,  \

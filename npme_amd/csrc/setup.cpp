@@ -452,15 +452,15 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   }
   const long ntiles = (long)h.nt[0] * h.nt[1] * h.nt[2];
   std::vector<int> tcount(ntiles + 1, 0);
-  auto tile_range = [&](int u0, int &lo, int &hi) {
-    lo = u0 / 8;
-    hi = (u0 + (int)h.Bn - 1) / 8;
+  auto tile_range = [&](int u0, int d, int &lo, int &hi) {
+    lo = u0 / h.T[d];
+    hi = (u0 + (int)h.Bn - 1) / h.T[d];
   };
   for (size_t s = 0; s < n; s++) {
     int lo[3], hi[3];
-    tile_range(ms[s].x, lo[0], hi[0]);
-    tile_range(ms[s].y, lo[1], hi[1]);
-    tile_range(ms[s].z, lo[2], hi[2]);
+    tile_range(ms[s].x, 0, lo[0], hi[0]);
+    tile_range(ms[s].y, 1, lo[1], hi[1]);
+    tile_range(ms[s].z, 2, lo[2], hi[2]);
     for (int a = lo[0]; a <= hi[0]; a++)
       for (int b = lo[1]; b <= hi[1]; b++)
         for (int c = lo[2]; c <= hi[2]; c++) tcount[((long)a * h.nt[1] + b) * h.nt[2] + c + 1]++;
@@ -471,9 +471,9 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
     std::vector<int> fill(tcount.begin(), tcount.end() - 1);
     for (size_t s = 0; s < n; s++) {
       int lo[3], hi[3];
-      tile_range(ms[s].x, lo[0], hi[0]);
-      tile_range(ms[s].y, lo[1], hi[1]);
-      tile_range(ms[s].z, lo[2], hi[2]);
+      tile_range(ms[s].x, 0, lo[0], hi[0]);
+      tile_range(ms[s].y, 1, lo[1], hi[1]);
+      tile_range(ms[s].z, 2, lo[2], hi[2]);
       for (int a = lo[0]; a <= hi[0]; a++)
         for (int b = lo[1]; b <= hi[1]; b++)
           for (int c = lo[2]; c <= hi[2]; c++) tlist[fill[((long)a * h.nt[1] + b) * h.nt[2] + c]++] = (int)s;
@@ -774,9 +774,15 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   h.csA = g.csA = csA;
   g.nP = nP;
   g.Bn = h.Bn;
-  // spreading tiles of 8^3 points; interpolation tiles of TI^3 points whose
-  // block plus halo fits 64 KB of LDS
-  for (int d = 0; d < 3; d++) h.nt[d] = g.nt[d] = (int)((h.M[d] + 7) / 8);
+  // spreading tiles: 16 x 8 x 8 points for the FP32 real kernels (matrix-core
+  // spreading), else 8^3; interpolation tiles of TI^3 points whose block plus
+  // halo fits 64 KB of LDS
+  h.T[0] = (!h.fp64 && !h.complexK) ? 16 : 8;
+  h.T[1] = h.T[2] = 8;
+  for (int d = 0; d < 3; d++) {
+    g.ts[d] = h.T[d];
+    h.nt[d] = g.nt[d] = (int)((h.M[d] + h.T[d] - 1) / h.T[d]);
+  }
   {
     const int es = (h.fp64 ? 8 : 4) * (h.complexK ? 2 : 1);
     h.TI = 16;
