@@ -482,7 +482,10 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
   const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
   const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
-  const int gbase = a2 * G3 + a3, gstep = G2 * G3;
+  // G[a2][a3 / 8][a1][a3 % 8] with a1 in [0, N/2] (k_make_G)
+  const long gbase = (((long)a2 * ((G3 + 7) >> 3) + (a3 >> 3)) * (N / 2 + 1)) * 8 + (a3 & 7);
+  const int gstep = 8;
+  (void)G2;
   constexpr int E = emax<Real>();
   using GT = typename std::conditional<CG, T, Real>::type;
   GT gv[E];
@@ -699,7 +702,10 @@ __global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, in
   const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
   const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
   const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
-  const long gbase = (long)a2 * G3 + a3, gstep = (long)G2 * G3;
+  // G[a2][a3 / 8][a1][a3 % 8] with a1 in [0, N/2] (k_make_G)
+  const long gbase = (((long)a2 * ((G3 + 7) >> 3) + (a3 >> 3)) * (N / 2 + 1)) * 8 + (a3 & 7),
+             gstep = 8;
+  (void)G2;
   using GT = typename std::conditional<CG, T, Real>::type;
   GT gv[P::BPT][8];
   T v[P::BPT][8];

@@ -1478,7 +1478,8 @@ void launch_fgrid_real_space(const DMKernel &km, const long Mf[3], const double 
   NPME_HIP_CHECK(hipGetLastError());
 }
 
-// G = F lam1 lam2 lam3 / (N1 N2 N3) on the (N1/2+1)(N2/2+1)(N3/2+1) grid
+// G = F lam1 lam2 lam3 / (N1 N2 N3) on the (N1/2+1)(N2/2+1)(N3/2+1) grid,
+// stored in blocks of 8 along axis 3 (see k_make_G)
 template <typename Real, bool CX>
 __global__ void k_make_G(long Mf0, long Mf1, long Mf2, const double2 *__restrict__ F,
                          const double *__restrict__ lam, double scale, Real *__restrict__ G)
@@ -1488,11 +1489,15 @@ __global__ void k_make_G(long Mf0, long Mf1, long Mf2, const double2 *__restrict
   if (id >= tot) return;
   const long i3 = id % Mf2, i2 = (id / Mf2) % Mf1, i1 = id / (Mf2 * Mf1);
   const double l = lam[i1] * lam[Mf0 + i2] * lam[Mf0 + Mf1 + i3] * scale;
+  // G[i2][i3 / 8][i1][i3 % 8]: the x-pass workgroup of 8 consecutive i3
+  // reads one contiguous block of (N1/2+1) x 8 values
+  const long nb3 = (Mf2 + 7) / 8;
+  const long o = ((i2 * nb3 + (i3 >> 3)) * Mf0 + i1) * 8 + (i3 & 7);
   if (CX) {
-    G[2 * id] = (Real)(F[id].x * l);
-    G[2 * id + 1] = (Real)(F[id].y * l);
+    G[2 * o] = (Real)(F[id].x * l);
+    G[2 * o + 1] = (Real)(F[id].y * l);
   } else {
-    G[id] = (Real)(F[id].x * l);
+    G[o] = (Real)(F[id].x * l);
   }
 }
 

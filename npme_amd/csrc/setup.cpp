@@ -638,7 +638,9 @@ static void grid_setup(Handle &h)
   }
   DevBuf d_lam(lam.size() * sizeof(double));
   NPME_HIP_CHECK(hipMemcpy(d_lam.p, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
-  h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)totF);
+  // G in blocks of 8 along axis 3 (the last block padded)
+  const size_t totG = (size_t)Mf[0] * Mf[1] * ((Mf[2] + 7) / 8) * 8;
+  h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * totG);
   launch_make_G(f64, h.complexK, Mf, h.K3, F, (const double *)d_lam.p, h.d_G, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before the temporaries return to the pool
 }
