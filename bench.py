@@ -143,24 +143,41 @@ def serving_stream(nreq, rate, slots, seed=2001):
 
     th = threading.Thread(target=consume)
     th.start()
+
+    # the client prepares each request's arrays (inputs, and the output
+    # array with its pages touched) ahead of its arrival in a thread of its
+    # own, so building a large request does not delay the submission of the
+    # next ones
+    import queue
+    prep = queue.Queue(maxsize=16)
+
+    def produce():
+        for i in range(nreq):
+            n = int(sizes[i])
+            L = (n / 0.1) ** (1.0 / 3.0)
+            coord = base[:n] * L
+            q = baseqc[:n] if kinds[i] == 2 else baseq[:n] - baseq[:n].mean()
+            V = np.zeros((n, 4), np.complex128 if kinds[i] == 2 else np.float64)
+            prep.put((coord, q, V))
+
+    pt = threading.Thread(target=produce)
+    pt.start()
     lag = 0.0
     for i in range(nreq):
-        n = int(sizes[i])
-        L = (n / 0.1) ** (1.0 / 3.0)
-        coord = base[:n] * L
+        coord, q, V = prep.get()
         t = S.clock()
         if t < arrive[i]:
             time.sleep(arrive[i] - t)
         lag = max(lag, S.clock() - arrive[i])
         if kinds[i] == 2:
-            tickets[i] = S.submit("Helmholtz", coord, baseqc[:n], tol=tols[i], Rdir=8.0, k0=0.3,
-                                  tArrival=arrive[i])
+            tickets[i] = S.submit("Helmholtz", coord, q, tol=tols[i], Rdir=8.0, k0=0.3,
+                                  tArrival=arrive[i], V=V)
         else:
-            q = baseq[:n] - baseq[:n].mean()
             ft = "Laplace" if kinds[i] == 0 else "Ralpha"
             tickets[i] = S.submit(ft, coord, q, tol=tols[i], Rdir=8.0, alpha=alphas[i],
-                                  tArrival=arrive[i])
+                                  tArrival=arrive[i], V=V)
         ready.release()
+    pt.join()
     th.join()
     S.close()
     ok = [i for i in range(nreq) if infos[i] is not None]

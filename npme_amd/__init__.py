@@ -339,12 +339,19 @@ class NPME_Server:
         return lib().npme_server_clock(self._s)
 
     def submit(self, funcType, coord, charge, tol=1e-6, Rdir=8.0, BnOrder=0, alpha=0.0, k0=0.0,
-               fp64=False, tArrival=-1.0):
+               fp64=False, tArrival=-1.0, V=None):
+        """Queues a request; returns its ticket.  V: optional output array
+        [n][4] (float64, or complex128 for Helmholtz, C-contiguous) that
+        wait() returns filled; by default one is allocated."""
         coord = _coord(coord)
         cx = funcType == "Helmholtz"
         q = np.ascontiguousarray(charge, dtype=np.complex128 if cx else np.float64)
         n = len(q)
-        V = np.zeros((n, 4), np.complex128 if cx else np.float64)
+        dt = np.complex128 if cx else np.float64
+        if V is None:
+            V = np.empty((n, 4), dt)  # fully written by the server
+        elif V.shape != (n, 4) or V.dtype != dt or not V.flags.c_contiguous:
+            raise ValueError("V must be a C-contiguous [n][4] %s array" % np.dtype(dt).name)
         ft = funcType.encode()
         k0 = complex(k0)
         r = _Request(ft, float(alpha), k0.real, k0.imag, float(tol), float(Rdir), int(BnOrder),

@@ -226,6 +226,7 @@ __global__ __launch_bounds__(256) void k_slice_len(size_t n, const int *__restri
   int m = 0;
   const size_t e = (64 * sl + 64 < n) ? 64 * sl + 64 : n;
   for (size_t s = 64 * sl; s < e; s++) m = max(m, count[s]);
+  m = (m + 3) & ~3;  // the list stores groups of 4 entries per target
   sliceLen[sl] = m;
   sliceWords[sl] = 64L * m;
 }

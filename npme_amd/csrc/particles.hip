@@ -379,9 +379,11 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
 // The geometry is fixed between CalcV calls, so the pairs inside the cutoff
 // are found once at set-up and stored as a sliced-ELLPACK list: the targets
 // are cut into slices of 64 consecutive sorted charges (one wavefront), each
-// slice is padded to its longest list, and entry k of lane l sits at
-// sliceStart + k * 64 + l, so step k of a wavefront reads 256 contiguous
-// bytes.  An entry packs the source index (low 26 bits) with the offset of
+// slice is padded to its longest list rounded up to 4, and the entries go in
+// groups of 4 per lane: entry k of lane l sits at
+// sliceStart + (k / 4) * 256 + 4 l + k % 4, so group g of a wavefront is
+// 1 KB contiguous, read by one 16-byte load per lane (and written so by
+// the list build).  An entry packs the source index (low 26 bits) with the offset of
 // the source's cluster from the target's cluster, dx+1, dy+1 and dz+1 in
 // two bits each (bits 26, 28, 30).  A padding entry is the sentinel source
 // n (position 0 in its frame, charge 0, stored after the n charges) with
@@ -413,8 +415,9 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
   const double R2 = R * R;
   const int nN = p.nNeigh;
   const int ax = c.x / nP, ay = c.y / nP, az = c.z / nP;
-  unsigned *out = nullptr;
-  if (FILL) out = list + sliceStart[s >> 6] + (s & 63);
+  uint4 *out = nullptr;  // group g of this target at out[g * 64]
+  if (FILL) out = reinterpret_cast<uint4 *>(list + sliceStart[s >> 6]) + (s & 63);
+  uint4 grp = make_uint4(0, 0, 0, 0);
   int k = 0;
   for (int dx = -nN; dx <= nN; dx++)
     for (int dy = -nN; dy <= nN; dy++)
@@ -429,11 +432,27 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
           const double4 o = xs[j];
           const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z;
           if (rx * rx + ry * ry + rz * rz < R2) {
-            if (FILL) out[(long)k * 64] = (code << NL_SHIFT) | (unsigned)j;
+            if (FILL) {
+              const unsigned ent = (code << NL_SHIFT) | (unsigned)j;
+              const int e = k & 3;
+              grp.x = e == 0 ? ent : grp.x;
+              grp.y = e == 1 ? ent : grp.y;
+              grp.z = e == 2 ? ent : grp.z;
+              grp.w = ent;
+              if (e == 3) out[(long)(k >> 2) * 64] = grp;
+            }
             k++;
           }
         }
       }
+  if (FILL && (k & 3) != 0) {  // the last, partial group, padded
+    const unsigned pad = NL_PAD | (unsigned)n;
+    const int e = k & 3;
+    if (e <= 1) grp.y = pad;
+    if (e <= 2) grp.z = pad;
+    grp.w = pad;
+    out[(long)(k >> 2) * 64] = grp;
+  }
   if (!FILL) count[s] = k;
 }
 
@@ -563,7 +582,7 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   // the slice is the wavefront's: its length is uniform, so the loop and
   // the bounds of the entry loads branch on scalars
   const int len = __builtin_amdgcn_readfirstlane(sliceLen[slice]);
-  const unsigned *e = list + sliceStart[slice] + (s & 63);
+  const uint4 *e4 = reinterpret_cast<const uint4 *>(list + sliceStart[slice]) + (s & 63);
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // U entries per step.  Two buffer sets alternate (the loop body is the
@@ -572,9 +591,14 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   // copies of loads in flight.  Lanes past n hold padding entries only.
   constexpr int U = 4;
   const unsigned ushift = (unsigned)p.fxShift;
-  auto load_ent = [&](int k0, unsigned *ent) {
-#pragma unroll
-    for (int u = 0; u < U; u++) ent[u] = (k0 + u < len) ? e[(long)(k0 + u) * 64] : pad;
+  static_assert(U == 4, "one group of 4 list entries per step");
+  auto load_ent = [&](int k0, unsigned *ent) {  // k0 a multiple of 4, len too
+    uint4 g = make_uint4(pad, pad, pad, pad);
+    if (k0 < len) g = e4[(long)(k0 >> 2) * 64];
+    ent[0] = g.x;
+    ent[1] = g.y;
+    ent[2] = g.z;
+    ent[3] = g.w;
   };
   auto gather = [&](const unsigned *ent, V4 *o, Real *oqi) {
 #pragma unroll

@@ -188,6 +188,7 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   for (size_t sl = 0; sl < nSlices; sl++) {
     int m = 0;
     for (size_t s = 64 * sl; s < std::min(h.n, 64 * sl + 64); s++) m = std::max(m, cnt[s]);
+    m = (m + 3) & ~3;  // the list stores groups of 4 entries per target
     sliceLen[sl] = m;
     sliceStart[sl + 1] = sliceStart[sl] + 64L * m;
   }
