@@ -456,6 +456,107 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
   if (!FILL) count[s] = k;
 }
 
+// the same walk with a wavefront per target, for small charge sets (a
+// thread per target leaves most of the GPU idle below ~2e5 charges, and its
+// loop over ~1e3 candidates is latency-bound).  The wave first reads the
+// (start, count) of its stencil cells in parallel and scans the counts, so
+// the candidates of all cells form one sequence t = 0..T-1 (cell by cell in
+// the loop order of k_nlist, j ascending within a cell); it then tests 64
+// candidates at a time, each lane finding its cell by a binary search over
+// the scanned counts, and places the hits with a ballot.  The entries and
+// their order are those of k_nlist.  FILL writes single entries into a
+// list already filled with padding.  Stencils up to nNeigh = 3.
+#ifndef NPME_NLIST_WAVE_MAX
+#define NPME_NLIST_WAVE_MAX 196608
+#endif
+constexpr int NLW_NN = 3;
+constexpr int NLW_MAXC = (2 * NLW_NN + 1) * (2 * NLW_NN + 1) * (2 * NLW_NN + 1);
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
+                                                    const int4 *__restrict__ cell,
+                                                    const int2 *__restrict__ cellSE,
+                                                    const double4 *__restrict__ xs, int nP,
+                                                    int *__restrict__ count,
+                                                    const long *__restrict__ sliceStart,
+                                                    unsigned *__restrict__ list)
+{
+  __shared__ int shStart[4][NLW_MAXC];
+  __shared__ int shPre[4][NLW_MAXC];  // candidates before the cell
+  __shared__ unsigned char shCode[4][NLW_MAXC];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t s = (size_t)blockIdx.x * 4 + w;
+  const bool live = s < n;
+  const size_t sc = live ? s : n - 1;
+  const int4 c = cell[sc];
+  const double4 me = xs[sc];
+  const double R = p.Rdir * (1.0 + 1.0e-9);
+  const double R2 = R * R;
+  const int nN = p.nNeigh, W = 2 * nN + 1, M = W * W * W;
+  const int ax = c.x / nP, ay = c.y / nP, az = c.z / nP;
+  int T = 0;
+  for (int c0 = 0; c0 < M; c0 += 64) {
+    const int ci = c0 + lane;
+    int st = 0, cnt = 0;
+    unsigned code = 0;
+    if (ci < M) {
+      const int gx = c.x + ci / (W * W) - nN, gy = c.y + (ci / W) % W - nN, gz = c.z + ci % W - nN;
+      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2) {
+        const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+        st = se.x;
+        cnt = se.y;
+        code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4));
+      }
+    }
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
+    }
+    if (ci < M) {
+      shStart[w][ci] = st;
+      shPre[w][ci] = T + inc - cnt;
+      shCode[w][ci] = (unsigned char)code;
+    }
+    T += __shfl(inc, 63);
+  }
+  __syncthreads();
+  if (!live) return;  // after the only barrier
+  unsigned *out = nullptr;  // entry k of target s: out[256 (k / 4) + k % 4]
+  if (FILL) out = list + sliceStart[s >> 6] + 4 * (s & 63);
+  const unsigned long long below = (1ull << lane) - 1;
+  int k = 0;
+  for (int t0 = 0; t0 < T; t0 += 64) {
+    const int t = t0 + lane;
+    bool hit = false;
+    unsigned ent = 0;
+    if (t < T) {
+      // last cell whose first candidate is at or before t (empty cells
+      // share their start with the next cell, so it is the cell holding t)
+      int lo = 0, hi = M - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (shPre[w][mid] <= t) lo = mid;
+        else hi = mid - 1;
+      }
+      const int j = shStart[w][lo] + (t - shPre[w][lo]);
+      if ((size_t)j != s) {
+        const double4 o = xs[j];
+        const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
+        hit = rx * rx + ry * ry + rz * rz < R2;
+        ent = ((unsigned)shCode[w][lo] << NL_SHIFT) | (unsigned)j;
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    if (FILL && hit) {
+      const int pos = k + __popcll(m & below);
+      out[256L * (pos >> 2) + (pos & 3)] = ent;
+    }
+    k += __popcll(m);
+  }
+  if (!FILL && lane == 0) count[s] = k;
+}
+
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
                   hipStream_t s)
@@ -466,6 +567,17 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   d.nb0 = p.nb[0];
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
+  if (n <= NPME_NLIST_WAVE_MAX && p.nNeigh <= NLW_NN) {
+    const dim3 g((unsigned)((n + 3) / 4));  // 4 targets (wavefronts) per workgroup
+    if (fill)
+      hipLaunchKernelGGL(k_nlist_wave<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart,
+                         xs, nP, count, sliceStart, list);
+    else
+      hipLaunchKernelGGL(k_nlist_wave<false>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart,
+                         xs, nP, count, sliceStart, list);
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
   const dim3 g((unsigned)((n + 255) / 256));
   if (fill)
     hipLaunchKernelGGL(k_nlist<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart, xs,

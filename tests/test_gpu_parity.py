@@ -156,9 +156,13 @@ def test_calcv_rejects_wrong_size():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_direct_sum_paths_agree(oracle, prec):
-    """The neighbour-list and the cell-row direct sums give the same V."""
-    coord, q = uniform_box(6000, 35.0)
+@pytest.mark.parametrize("n,box", [(6000, 35.0), (220000, 130.0)])
+def test_direct_sum_paths_agree(oracle, prec, n, box):
+    """The neighbour-list and the cell-row direct sums give the same V.  The
+    two sizes build the list with the wavefront-per-target walk (small
+    sets) and the thread-per-target walk (from ~2e5 charges)."""
+    print("case direct paths", n, prec, flush=True)
+    coord, q = uniform_box(n, box)
     hl = _real_setup(coord, q, prec, Rdir=7.0, neighborList="on")
     hc = _real_setup(coord, q, prec, Rdir=7.0, neighborList="off")
     assert "neighbour list" in hl.Print() and "cell rows" in hc.Print()
