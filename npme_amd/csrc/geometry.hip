@@ -403,6 +403,77 @@ size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *
   return (size_t)total;
 }
 
+// per-axis minimum and maximum of the coordinates: one partial per
+// workgroup, then one workgroup over the partials.  Exact; a NaN
+// coordinate makes its axis' bounds NaN.
+__device__ __forceinline__ double nan_min(double a, double b) { return (a != a || a < b) ? a : b; }
+__device__ __forceinline__ double nan_max(double a, double b) { return (a != a || b < a) ? a : b; }
+
+constexpr int BOUNDS_WG = 1024;
+__global__ __launch_bounds__(256) void k_coord_bounds(size_t n, const double *__restrict__ coord,
+                                                      double *__restrict__ part)
+{
+  __shared__ double sh[6][256];
+  double v[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    for (int d = 0; d < 3; d++) {
+      const double x = coord[3 * i + d];
+      v[d] = nan_min(x, v[d]);
+      v[3 + d] = nan_max(x, v[3 + d]);
+    }
+  for (int d = 0; d < 6; d++) sh[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) {
+        sh[d][threadIdx.x] = nan_min(sh[d][threadIdx.x + w], sh[d][threadIdx.x]);
+        sh[3 + d][threadIdx.x] = nan_max(sh[3 + d][threadIdx.x + w], sh[3 + d][threadIdx.x]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_bounds_final(int np, const double *__restrict__ part,
+                                                      double *__restrict__ out)
+{
+  __shared__ double sh[6][256];
+  double v[6] = {HUGE_VAL, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  for (int b = threadIdx.x; b < np; b += blockDim.x)
+    for (int d = 0; d < 3; d++) {
+      v[d] = nan_min(part[6 * b + d], v[d]);
+      v[3 + d] = nan_max(part[6 * b + 3 + d], v[3 + d]);
+    }
+  for (int d = 0; d < 6; d++) sh[d][threadIdx.x] = v[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int d = 0; d < 3; d++) {
+        sh[d][threadIdx.x] = nan_min(sh[d][threadIdx.x + w], sh[d][threadIdx.x]);
+        sh[3 + d][threadIdx.x] = nan_max(sh[3 + d][threadIdx.x + w], sh[3 + d][threadIdx.x]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+void coord_bounds_device(size_t n, const double *d_coord, double mn[3], double mx[3], hipStream_t s)
+{
+  const int nb = (int)std::min<size_t>(BOUNDS_WG, (n + 255) / 256);
+  DevBuf part((size_t)6 * nb * sizeof(double)), out(6 * sizeof(double));
+  hipLaunchKernelGGL(k_coord_bounds, dim3(nb), dim3(256), 0, s, n, d_coord, (double *)part.p);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, nb, (const double *)part.p,
+                     (double *)out.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  double v[6];
+  NPME_HIP_CHECK(hipMemcpyAsync(v, out.p, sizeof(v), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  for (int d = 0; d < 3; d++) {
+    mn[d] = v[d];
+    mx[d] = v[3 + d];
+  }
+}
+
 void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
                           hipStream_t s)
 {

@@ -270,6 +270,8 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s);
 // coordinates in sorted order: xs[s] = coord[perm[s]]
+// exact per-axis bounds of the n positions on the device (NaN if any is NaN)
+void coord_bounds_device(size_t n, const double *d_coord, double mn[3], double mx[3], hipStream_t s);
 void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
                           hipStream_t s);
 // candidate pairs of the cell stencil: sum over cells of count * (counts of
@@ -322,6 +324,9 @@ struct DirectParams {
   int nNeigh;
   double cs;
   int nb[3];
+  double mn[3];   // low corner of the cell grid
+  double csA;     // cluster edge (nCellClust1D cells)
+  double cellTol; // bound on a charge's distance outside its cell's box
   int fxShift;    // mixed form: one cluster edge = 2^fxShift fixed-point units
   double fxInv;   // cluster edge / 2^fxShift
 };

@@ -26,6 +26,7 @@ struct DirectDev {
   double a[62], b[62];    // DM coefficients (re, im), zero beyond Nder
   float af[62], bf[62];   // the same in FP32
   int nb0, nb1, nb2;
+  double mn0, mn1, mn2, csA, cellTol;  // cell boxes (neighbour-list build)
   int fxShift;     // S (mixed form)
   int fxUnit;      // 2^S: one cluster edge in fixed-point units (mixed form)
   float fxInv;     // cluster edge / 2^S: length of one fixed-point unit
@@ -398,6 +399,31 @@ constexpr int NL_SHIFT = 26;
 
 // walk the cells of the stencil of sorted target s in FP64 input
 // coordinates; COUNT mode counts, FILL mode writes entries
+// where the cell (gx, gy, gz) lies for a target at (x, y, z): 0 no charge of
+// it is within the cutoff, 2 every charge is, 1 each must be tested.  The
+// cell's box is widened by cellTol (charges sit in their cell's box up to
+// the rounding of the cell search), and the two decisions keep a relative
+// margin of 1e-9 to the cutoff test, so they agree with it charge by charge.
+__device__ __forceinline__ int nl_cell_class(const DirectDev &p, int gx, int gy, int gz, int nP,
+                                             double x, double y, double z, double R2)
+{
+  double near2 = 0, far2 = 0;
+  const int g[3] = {gx, gy, gz};
+  const double mn[3] = {p.mn0, p.mn1, p.mn2}, pt[3] = {x, y, z};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double lo = mn[d] + (double)(g[d] / nP) * p.csA + (double)(g[d] % nP) * p.cs - p.cellTol;
+    const double hi = lo + p.cs + 2.0 * p.cellTol;
+    const double a = fmax(0.0, fmax(lo - pt[d], pt[d] - hi));
+    const double b = fmax(pt[d] - lo, hi - pt[d]);
+    near2 += a * a;
+    far2 += b * b;
+  }
+  if (near2 > R2 * (1.0 + 1.0e-9)) return 0;
+  if (far2 < p.Rdir * p.Rdir) return 2;
+  return 1;
+}
+
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4 *__restrict__ cell,
                                                const int2 *__restrict__ cellSE,
@@ -424,14 +450,24 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
       for (int dz = -nN; dz <= nN; dz++) {
         const int gx = c.x + dx, gy = c.y + dy, gz = c.z + dz;
         if (gx < 0 || gy < 0 || gz < 0 || gx >= p.nb0 || gy >= p.nb1 || gz >= p.nb2) continue;
+        const int cls = nl_cell_class(p, gx, gy, gz, nP, xi, yi, zi, R2);
+        if (cls == 0) continue;
         const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+        if (!FILL && cls == 2) {
+          k += se.y - ((size_t)se.x <= s && s < (size_t)(se.x + se.y) ? 1 : 0);
+          continue;
+        }
         const unsigned code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) |
                                          ((gz / nP - az + 1) << 4));
         for (int j = se.x; j < se.x + se.y; j++) {
           if ((size_t)j == s) continue;
-          const double4 o = xs[j];
-          const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z;
-          if (rx * rx + ry * ry + rz * rz < R2) {
+          bool in = cls == 2;
+          if (!in) {
+            const double4 o = xs[j];
+            const double rx = xi - o.x, ry = yi - o.y, rz = zi - o.z;
+            in = rx * rx + ry * ry + rz * rz < R2;
+          }
+          if (in) {
             if (FILL) {
               const unsigned ent = (code << NL_SHIFT) | (unsigned)j;
               const int e = k & 3;
@@ -467,9 +503,10 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
 // their order are those of k_nlist.  FILL writes single entries into a
 // list already filled with padding.  Stencils up to nNeigh = 3.
 #ifndef NPME_NLIST_WAVE_MAX
-#define NPME_NLIST_WAVE_MAX 196608
+#define NPME_NLIST_WAVE_MAX 32768
 #endif
 constexpr int NLW_NN = 3;
+constexpr unsigned NLW_ALLIN = 0x80u;  // stencil cell wholly within the cutoff
 constexpr int NLW_MAXC = (2 * NLW_NN + 1) * (2 * NLW_NN + 1) * (2 * NLW_NN + 1);
 
 template <bool FILL>
@@ -501,11 +538,14 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
     unsigned code = 0;
     if (ci < M) {
       const int gx = c.x + ci / (W * W) - nN, gy = c.y + (ci / W) % W - nN, gz = c.z + ci % W - nN;
-      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2) {
+      const int cls = (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2)
+                          ? nl_cell_class(p, gx, gy, gz, nP, me.x, me.y, me.z, R2) : 0;
+      if (cls) {
         const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
         st = se.x;
         cnt = se.y;
-        code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4));
+        code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4)) |
+               (cls == 2 ? NLW_ALLIN : 0u);
       }
     }
     int inc = cnt;
@@ -540,11 +580,16 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
         else hi = mid - 1;
       }
       const int j = shStart[w][lo] + (t - shPre[w][lo]);
+      const unsigned code = shCode[w][lo];
       if ((size_t)j != s) {
-        const double4 o = xs[j];
-        const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
-        hit = rx * rx + ry * ry + rz * rz < R2;
-        ent = ((unsigned)shCode[w][lo] << NL_SHIFT) | (unsigned)j;
+        if (code & NLW_ALLIN) {
+          hit = true;
+        } else {
+          const double4 o = xs[j];
+          const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
+          hit = rx * rx + ry * ry + rz * rz < R2;
+        }
+        ent = ((code & ~NLW_ALLIN) << NL_SHIFT) | (unsigned)j;
       }
     }
     const unsigned long long m = __ballot(hit);
@@ -557,6 +602,118 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
   if (!FILL && lane == 0) count[s] = k;
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// the same walk with a wavefront per cell, for large charge sets: the
+// targets of one cell share their stencil, so the wave loads the stencil's
+// candidates once, 64 at a time (the flattened sequence of k_nlist_wave),
+// and tests each chunk against the cell's targets in turn, one ballot per
+// target.  Lane i holds target i's position, list offset and entry count
+// (targets in batches of 64).  The entries and their order are those of
+// k_nlist.
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_nlist_cell(DirectDev p, long ncell,
+                                                    const int2 *__restrict__ cellSE,
+                                                    const double4 *__restrict__ xs, int nP,
+                                                    int *__restrict__ count,
+                                                    const long *__restrict__ sliceStart,
+                                                    unsigned *__restrict__ list)
+{
+  __shared__ int shStart[4][NLW_MAXC];
+  __shared__ int shPre[4][NLW_MAXC];
+  __shared__ unsigned char shCode[4][NLW_MAXC];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long c = (long)blockIdx.x * 4 + w;
+  const int2 home = c < ncell ? cellSE[c] : make_int2(0, 0);
+  const int hz = (int)(c % p.nb2), hy = (int)((c / p.nb2) % p.nb1), hx = (int)(c / ((long)p.nb1 * p.nb2));
+  const double R = p.Rdir * (1.0 + 1.0e-9);
+  const double R2 = R * R;
+  const int nN = p.nNeigh, W = 2 * nN + 1, M = W * W * W;
+  const int ax = hx / nP, ay = hy / nP, az = hz / nP;
+  int T = 0;
+  if (home.y > 0) {
+    for (int c0 = 0; c0 < M; c0 += 64) {
+      const int ci = c0 + lane;
+      int st = 0, cnt = 0;
+      unsigned code = 0;
+      if (ci < M) {
+        const int gx = hx + ci / (W * W) - nN, gy = hy + (ci / W) % W - nN, gz = hz + ci % W - nN;
+        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2) {
+          const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+          st = se.x;
+          cnt = se.y;
+          code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4));
+        }
+      }
+      int inc = cnt;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o);
+        if (lane >= o) inc += v;
+      }
+      if (ci < M) {
+        shStart[w][ci] = st;
+        shPre[w][ci] = T + inc - cnt;
+        shCode[w][ci] = (unsigned char)code;
+      }
+      T += __shfl(inc, 63);
+    }
+  }
+  __syncthreads();
+  if (home.y == 0) return;  // after the only barrier
+  const unsigned long long below = (1ull << lane) - 1;
+  for (int tb = 0; tb < home.y; tb += 64) {
+    const int mt = min(64, home.y - tb);
+    // lane i: target s = home.x + tb + i
+    const size_t si = (size_t)home.x + tb + min(lane, mt - 1);
+    const double4 tp = xs[si];
+    long obase = 0;
+    if (FILL) obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
+    int kreg = 0;
+    for (int t0 = 0; t0 < T; t0 += 64) {
+      const int t = t0 + lane;
+      int j = -1;
+      unsigned ent = 0;
+      double ox = 0, oy = 0, oz = 0;
+      if (t < T) {
+        int lo = 0, hi = M - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (shPre[w][mid] <= t) lo = mid;
+          else hi = mid - 1;
+        }
+        j = shStart[w][lo] + (t - shPre[w][lo]);
+        const double4 o = xs[j];
+        ox = o.x;
+        oy = o.y;
+        oz = o.z;
+        ent = ((unsigned)shCode[w][lo] << NL_SHIFT) | (unsigned)j;
+      }
+      for (int i = 0; i < mt; i++) {
+        const int s = home.x + tb + i;
+        const double rx = readlane_f64(tp.x, i) - ox, ry = readlane_f64(tp.y, i) - oy,
+                     rz = readlane_f64(tp.z, i) - oz;
+        const bool hit = j >= 0 && j != s && rx * rx + ry * ry + rz * rz < R2;
+        const unsigned long long m = __ballot(hit);
+        const int ki = __builtin_amdgcn_readlane(kreg, i);
+        if (FILL && hit) {
+          const long ob = (long)__builtin_amdgcn_readlane((unsigned)obase, i) |
+                          ((long)__builtin_amdgcn_readlane((unsigned)((unsigned long)obase >> 32), i) << 32);
+          const int pos = ki + __popcll(m & below);
+          list[ob + 256L * (pos >> 2) + (pos & 3)] = ent;
+        }
+        if (lane == i) kreg += __popcll(m);
+      }
+    }
+    if (!FILL && lane < mt) count[home.x + tb + lane] = kreg;
+  }
+}
+
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
                   hipStream_t s)
@@ -567,7 +724,30 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   d.nb0 = p.nb[0];
   d.nb1 = p.nb[1];
   d.nb2 = p.nb[2];
-  if (n <= NPME_NLIST_WAVE_MAX && p.nNeigh <= NLW_NN) {
+  d.cs = p.cs;
+  d.csA = p.csA;
+  d.mn0 = p.mn[0];
+  d.mn1 = p.mn[1];
+  d.mn2 = p.mn[2];
+  d.cellTol = p.cellTol;
+#ifndef NPME_NLIST_FORCE
+#define NPME_NLIST_FORCE 0
+#endif
+  const int pick = NPME_NLIST_FORCE ? NPME_NLIST_FORCE
+                   : p.nNeigh > NLW_NN ? 1 : n <= NPME_NLIST_WAVE_MAX ? 2 : 3;
+  if (pick == 3) {
+    const long ncell = (long)p.nb[0] * p.nb[1] * p.nb[2];
+    const dim3 g((unsigned)((ncell + 3) / 4));  // 4 cells (wavefronts) per workgroup
+    if (fill)
+      hipLaunchKernelGGL(k_nlist_cell<true>, g, dim3(256), 0, s, d, ncell, (const int2 *)cellStart, xs,
+                         nP, count, sliceStart, list);
+    else
+      hipLaunchKernelGGL(k_nlist_cell<false>, g, dim3(256), 0, s, d, ncell, (const int2 *)cellStart,
+                         xs, nP, count, sliceStart, list);
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (pick == 2) {
     const dim3 g((unsigned)((n + 3) / 4));  // 4 targets (wavefronts) per workgroup
     if (fill)
       hipLaunchKernelGGL(k_nlist_wave<true>, g, dim3(256), 0, s, d, n, cell, (const int2 *)cellStart,
@@ -1530,32 +1710,36 @@ struct LRDev {
   double a[62], b[62];
 };
 
-__device__ void lr_eval(const LRDev &k, double r2, double &fr, double &fi)
+template <int KIND>
+__device__ __forceinline__ void lr_eval(const LRDev &k, double r2, double &fr, double &fi)
 {
   const double r = sqrt(fabs(r2));
-  if (k.kind == KIND_LAPLACE_ERF) {
+  fi = 0;
+  if (KIND == KIND_LAPLACE_ERF) {
     // erf(beta r)/r, 2 beta/sqrt(pi) (1 - x^2/3) below x = beta r = 1e-8
     // (src/KernelFunctionLaplace.cpp:1087-1116)
     const double x = k.beta * r;
     fr = x < 1e-8 ? k.beta * 1.12837916709551257390 * (1.0 - x * x / 3.0) : erf(x) / r;
-    fi = 0;
     return;
   }
-  const bool inside = (k.kind == KIND_RALPHA) ? !(r2 > k.Rdir * k.Rdir) : !(r > k.Rdir);
+  const bool inside = (KIND == KIND_RALPHA) ? !(r2 > k.Rdir * k.Rdir) : !(r > k.Rdir);
   if (inside) {
-    double sr = k.a[2 * k.Nder], si = k.a[2 * k.Nder + 1];
-    for (int i = k.Nder - 1; i >= 0; i--) {
-      sr = sr * r2 + k.a[2 * i];
-      si = si * r2 + k.a[2 * i + 1];
+    double sr = k.a[2 * k.Nder];
+    if (KIND == KIND_HELMHOLTZ) {
+      double si = k.a[2 * k.Nder + 1];
+      for (int i = k.Nder - 1; i >= 0; i--) {
+        sr = sr * r2 + k.a[2 * i];
+        si = si * r2 + k.a[2 * i + 1];
+      }
+      fi = si;
+    } else {
+      for (int i = k.Nder - 1; i >= 0; i--) sr = sr * r2 + k.a[2 * i];
     }
     fr = sr;
-    fi = si;
-  } else if (k.kind == KIND_LAPLACE) {
+  } else if (KIND == KIND_LAPLACE) {
     fr = 1.0 / r;
-    fi = 0;
-  } else if (k.kind == KIND_RALPHA) {
+  } else if (KIND == KIND_RALPHA) {
     fr = pow(r2, 0.5 * k.alpha);
-    fi = 0;
   } else {
     double sn, cn;
     sincos(k.k0r * r, &sn, &cn);
@@ -1566,8 +1750,10 @@ __device__ void lr_eval(const LRDev &k, double r2, double &fr, double &fi)
 }
 
 // tab layout per axis d: [x1 | x2 | T1 | T2] each Mf[d] long, axes consecutive
-__global__ void k_fgrid(LRDev k, long Mf0, long Mf1, long Mf2, const double *__restrict__ tab,
-                        double2 *__restrict__ F)
+template <int KIND>
+__global__ __launch_bounds__(256) void k_fgrid(LRDev k, long Mf0, long Mf1, long Mf2,
+                                               const double *__restrict__ tab,
+                                               double2 *__restrict__ F)
 {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long tot = Mf0 * Mf1 * Mf2;
@@ -1578,12 +1764,15 @@ __global__ void k_fgrid(LRDev k, long Mf0, long Mf1, long Mf2, const double *__r
   const double Y[2] = {t1[i2], t1[Mf1 + i2]}, TY[2] = {t1[2 * Mf1 + i2], t1[3 * Mf1 + i2]};
   const double Z[2] = {t2[i3], t2[Mf2 + i3]}, TZ[2] = {t2[2 * Mf2 + i3], t2[3 * Mf2 + i3]};
   double sr = 0, si = 0;
+#pragma unroll
   for (int ax = 0; ax < 2; ax++)
+#pragma unroll
     for (int ay = 0; ay < 2; ay++)
+#pragma unroll
       for (int az = 0; az < 2; az++) {
         const double w = TX[ax] * TY[ay] * TZ[az];
         double fr, fi;
-        lr_eval(k, X[ax] * X[ax] + Y[ay] * Y[ay] + Z[az] * Z[az], fr, fi);
+        lr_eval<KIND>(k, X[ax] * X[ax] + Y[ay] * Y[ay] + Z[az] * Z[az], fr, fi);
         sr += w * fr;
         si += w * fi;
       }
@@ -1609,8 +1798,19 @@ void launch_fgrid_real_space(const DMKernel &km, const long Mf[3], const double 
     k.b[i] = km.b[i];
   }
   const long tot = Mf[0] * Mf[1] * Mf[2];
-  hipLaunchKernelGGL(k_fgrid, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, k, Mf[0],
-                     Mf[1], Mf[2], d_tab, (double2 *)F);
+  const dim3 g((unsigned)((tot + 255) / 256));
+  if (k.kind == KIND_LAPLACE)
+    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
+                       (double2 *)F);
+  else if (k.kind == KIND_LAPLACE_ERF)
+    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE_ERF>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
+                       (double2 *)F);
+  else if (k.kind == KIND_RALPHA)
+    hipLaunchKernelGGL(k_fgrid<KIND_RALPHA>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
+                       (double2 *)F);
+  else
+    hipLaunchKernelGGL(k_fgrid<KIND_HELMHOLTZ>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
+                       (double2 *)F);
   NPME_HIP_CHECK(hipGetLastError());
 }
 

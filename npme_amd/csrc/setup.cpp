@@ -229,8 +229,8 @@ static int fixed_shift(int maxOff)
 
 // Device set-up (geometry.hip); returns false when the neighbour list does
 // not fit the memory budget (the host set-up then builds the cell-row lists)
-static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
-                             const double *coord)
+// the positions are already on the device (h.d_coordS)
+static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
 {
   const size_t n = h.n;
   hipStream_t s = h.stream;
@@ -238,8 +238,6 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   const size_t rs = f64 ? 8 : 4;
   const long ncell = (long)h.nb[0] * h.nb[1] * h.nb[2];
   auto t0 = std::chrono::steady_clock::now();
-  h.d_coordS = dalloc<double>(h, 3 * n);
-  NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
   h.d_perm = dalloc<int>(h, n);
   h.d_cell = dalloc<int4>(h, n);
   h.d_cellStart = dalloc<int>(h, 2 * (size_t)ncell);
@@ -746,8 +744,54 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   // workgroup's 8 adjacent columns are one aligned 64-byte segment per row
   h.Kp = (h.K3 + 7) / 8 * 8;
 
-  // ---- geometry of the reciprocal sum
-  box_dim_center(n, coord, h.X0, h.R0);
+  // ---- streams; the device memory of an earlier set-up is released
+  NPME_HIP_CHECK(hipGetDevice(&h.device));
+  if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  handle_free_device(h);
+  std::string smode = "device", nlmode = "auto";
+  kw.get("setup", smode);
+  kw.get("neighborList", nlmode);
+  if (smode != "device" && smode != "host") kw_error("setup must be 'device' or 'host'");
+  if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
+    kw_error("neighborList must be 'auto', 'on' or 'off'");
+  const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
+  const bool devSetup = smode == "device" && nlPossible;
+  h.setupMs[1] = ms_since(t0);
+
+  // ---- bounds of the charges: on the device for the device set-up (the
+  // positions go there first), else on the host; both exact
+  double mn[3], mx[3];
+  if (devSetup) {
+    h.d_coordS = dalloc<double>(h, 3 * n);
+    NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                                  h.stream));
+    coord_bounds_device(n, h.d_coordS, mn, mx, h.stream);
+  } else {
+    bool finite = true;
+    for (int d = 0; d < 3; d++) mn[d] = mx[d] = coord[d];
+    for (size_t i = 0; i < n; i++)
+      for (int d = 0; d < 3; d++) {
+        const double x = coord[3 * i + d];
+        finite = finite && std::isfinite(x);
+        mn[d] = std::min(mn[d], x);
+        mx[d] = std::max(mx[d], x);
+      }
+    if (!finite) mn[0] = NAN;
+  }
+  for (int d = 0; d < 3; d++)
+    if (!std::isfinite(mn[d]) || !std::isfinite(mx[d]))
+      throw Error{NPME_ERR_ARG, "coordinates must be finite"};
+
+  // ---- geometry of the reciprocal sum: box size and centre as
+  // src/SupportFunctions.cpp NPME_FindBoxDimCenter (extremes seeded with
+  // -/+1e12)
+  for (int d = 0; d < 3; d++) {
+    const double hi = std::max(-1.0e12, mx[d]), lo = std::min(1.0e12, mn[d]);
+    h.X0[d] = hi - lo;
+    h.R0[d] = 0.5 * (hi + lo);
+  }
   pme_corr_box(h.X0, h.R0, h.N, h.del, h.Bn, h.X, h.R);
   for (int d = 0; d < 3; d++) h.L[d] = 2 * (h.X[d] + h.del[d]);
 
@@ -757,13 +801,6 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   const double csB = Rdir / h.nNeigh;
   const unsigned long nP = (unsigned long)h.nCellClust1D;
   const double csA = nP * csB;
-  double mn[3], mx[3];
-  for (int d = 0; d < 3; d++) mn[d] = mx[d] = coord[d];
-  for (size_t i = 1; i < n; i++)
-    for (int d = 0; d < 3; d++) {
-      mn[d] = std::min(mn[d], coord[3 * i + d]);
-      mx[d] = std::max(mx[d], coord[3 * i + d]);
-    }
   for (int d = 0; d < 3; d++) {
     g.nA[d] = (unsigned long)((mx[d] - mn[d]) / csA) + 1;
     h.nb[d] = (int)(g.nA[d] * nP);
@@ -795,22 +832,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   for (int d = 0; d < 3; d++) h.nti[d] = g.nti[d] = (int)((h.M[d] + h.TI - 1) / h.TI);
 
   // ---- per-charge state: on the device when the neighbour list is used
-  NPME_HIP_CHECK(hipGetDevice(&h.device));
-  if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
-  if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
   h.setupMs[0] = ms_since(t0);
-  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  handle_free_device(h);
-  h.setupMs[1] = ms_since(t0);
-  std::string smode = "device", nlmode = "auto";
-  kw.get("setup", smode);
-  kw.get("neighborList", nlmode);
-  if (smode != "device" && smode != "host") kw_error("setup must be 'device' or 'host'");
-  if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
-    kw_error("neighborList must be 'auto', 'on' or 'off'");
-  const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
   bool done = false;
-  if (smode == "device" && nlPossible) done = particles_device(h, g, nlmode, coord);
+  if (devSetup) done = particles_device(h, g, nlmode);
   if (!done) {
     handle_free_device(h);
     particles_host(h, g, nlmode, nlPossible, coord);
@@ -856,7 +880,16 @@ DirectParams direct_params(const Handle &h)
   std::memcpy(dp.b, h.ker.b, sizeof(dp.b));
   dp.nNeigh = (int)h.nNeigh;
   dp.cs = h.csB;
-  for (int d = 0; d < 3; d++) dp.nb[d] = h.nb[d];
+  dp.csA = h.csA;
+  // the cell of a charge is found from differences of coordinates of this
+  // magnitude, rounded, with an edge tolerance of 1e-14
+  double mag = 0;
+  for (int d = 0; d < 3; d++) {
+    dp.nb[d] = h.nb[d];
+    dp.mn[d] = h.boxMin[d];
+    mag = std::max(mag, std::fabs(h.boxMin[d]) + h.nb[d] * h.csB);
+  }
+  dp.cellTol = 1.0e-9 * h.csB + 1.0e-12 * mag + 1.0e-13;
   dp.fxShift = h.fxShift;
   dp.fxInv = std::ldexp(h.csA, -h.fxShift);
   return dp;
