@@ -245,6 +245,7 @@ void npme_destroy(npme_handle *h)
 static void check_call(npme_handle *h, size_t nCharge, bool wantComplex)
 {
   if (!h) throw Error{NPME_ERR_ARG, "NULL handle"};
+  if (!h->h.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
   if (nCharge != h->h.n) {
     std::ostringstream os;
     os << "nCharge = " << nCharge << " != " << h->h.n;
@@ -326,6 +327,7 @@ int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget, const lon
     if (!h || !V || !idx || !charge) throw Error{NPME_ERR_ARG, "npme_calcv_exact_subset: NULL argument"};
     std::lock_guard<std::mutex> lk(h->h.mtx);
     Handle &H = h->h;
+    if (!H.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
     for (size_t t = 0; t < nTarget; t++)
       if (idx[t] < 0 || (size_t)idx[t] >= H.n) throw Error{NPME_ERR_ARG, "target index out of range"};
     const bool cx = H.complexK;

@@ -160,6 +160,7 @@ struct Handle;
 // one CalcV pipeline instance (device buffers for one geometry)
 struct Handle {
   std::mutex mtx;
+  bool ready = false;  // the last set-up completed (device state valid)
   // configuration
   Keywords kw;
   DMKernel ker;

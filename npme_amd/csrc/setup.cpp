@@ -649,6 +649,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
 {
   auto t0 = std::chrono::steady_clock::now();
   for (double &v : h.setupMs) v = 0;
+  h.ready = false;
   h.kw = kw;
   h.n = n;
   if (n < 2) throw Error{NPME_ERR_ARG, "nCharge must be at least 2"};
@@ -843,6 +844,13 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   ms_since(t0);
   grid_setup(h);
   h.setupMs[5] = ms_since(t0);
+  h.ready = true;
+}
+
+// a failed SetUp / ResetCoord leaves no usable device state
+static void require_ready(const Handle &h)
+{
+  if (!h.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
 }
 
 void handle_destroy_streams(Handle &h)
@@ -897,6 +905,7 @@ DirectParams direct_params(const Handle &h)
 
 double handle_count_pairs(Handle &h)
 {
+  require_ready(h);
   unsigned long long tot = 0;
   DevBuf d_tot(sizeof(unsigned long long));
   NPME_HIP_CHECK(hipMemsetAsync(d_tot.p, 0, sizeof(unsigned long long), h.stream));
@@ -909,6 +918,7 @@ double handle_count_pairs(Handle &h)
 
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
 {
+  require_ready(h);
   const bool f64 = h.fp64, cx = h.complexK;
   hipStream_t s = h.stream;
   const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2], N1 = h.N[0], N2 = h.N[1], N3 = h.N[2];
@@ -1006,6 +1016,7 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
 
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout)
 {
+  require_ready(h);
   launch_exact(h.ker.kind, h.ker.alpha, h.ker.k0r, h.ker.k0i, h.n, h.d_coordS, h.d_perm, d_q, nt,
                d_idx, d_Vout, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));

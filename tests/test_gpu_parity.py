@@ -146,6 +146,27 @@ def test_linearity_and_reset_coord(oracle):
     np.testing.assert_allclose(h.calcv(q), V1, atol=1e-9)
 
 
+@pytest.mark.parametrize("mode", ["device", "host"])
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_setup_rejects_non_finite_coordinates(mode, bad):
+    import npme_amd as N
+    coord, q = uniform_box(3000, 25.0)
+    k = _kw(calcType="pme", funcType="Laplace", Rdir=5.0, BnOrder=6, tol=1e-6, setup=mode)
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q), N.npme_last_error()
+    c2 = coord.copy()
+    c2[1234, 1] = bad
+    assert N.NPME_Interface_SetUpRealKernel(k, len(q), c2) is None
+    assert "finite" in N.npme_last_error()
+    h = N.NPME_Interface_SetUpRealKernel(k, len(q), coord)
+    assert h is not None
+    assert not h.ResetCoord(len(q), c2) and "finite" in N.npme_last_error()
+    # the handle refuses work until a set-up succeeds
+    V = np.zeros((len(q), 4))
+    assert not h.CalcV(V, len(q), q, coord) and "no valid set-up" in N.npme_last_error()
+    assert h.ResetCoord(len(q), coord)
+    assert np.all(np.isfinite(h.calcv(q)))
+
+
 def test_calcv_rejects_wrong_size():
     import npme_amd as N
     coord, q = uniform_box(2000, 20.0)
