@@ -157,18 +157,19 @@ def serving_stream(nreq, rate, slots, seed=2001):
             L = (n / 0.1) ** (1.0 / 3.0)
             coord = base[:n] * L
             q = baseqc[:n] if kinds[i] == 2 else baseq[:n] - baseq[:n].mean()
-            V = np.zeros((n, 4), np.complex128 if kinds[i] == 2 else np.float64)
+            V = np.empty((n, 4), np.complex128 if kinds[i] == 2 else np.float64)
+            V.fill(0.0)  # pages touched: a serving client hands over a ready buffer
             prep.put((coord, q, V))
 
     pt = threading.Thread(target=produce)
     pt.start()
-    lag = 0.0
+    lags = np.zeros(nreq)
     for i in range(nreq):
         coord, q, V = prep.get()
         t = S.clock()
         if t < arrive[i]:
             time.sleep(arrive[i] - t)
-        lag = max(lag, S.clock() - arrive[i])
+        lags[i] = S.clock() - arrive[i]
         if kinds[i] == 2:
             tickets[i] = S.submit("Helmholtz", coord, q, tol=tols[i], Rdir=8.0, k0=0.3,
                                   tArrival=arrive[i], V=V)
@@ -192,6 +193,19 @@ def serving_stream(nreq, rate, slots, seed=2001):
             by[name] = {"requests": len(sel), "p50_ms": float(np.percentile(lat[sel], 50) * 1e3),
                         "p99_ms": float(np.percentile(lat[sel], 99) * 1e3)}
     big = [j for j, i in enumerate(ok) if sizes[i] >= 500000]
+
+    def phases(sel):
+        # mean wall time per phase (ms) of the selected requests; "wait" is
+        # the time from arrival to a slot taking the request
+        if not sel:
+            return None
+        out = {"requests": len(sel),
+               "wait": float(np.mean([(infos[ok[j]]["tStart"] - infos[ok[j]]["tArrival"]) * 1e3
+                                      for j in sel]))}
+        for k in ("keywords", "setup", "copy_in", "calcv", "copy_out"):
+            out[k] = float(np.mean([infos[ok[j]]["ms"][k] for j in sel]))
+        return out
+    tail = [j for j in range(len(ok)) if lat[j] >= np.percentile(lat, 99)]
     return {"requests": nreq, "completed": len(ok), "errors": errors[:5],
             "rate_per_s": rate, "slots": slots,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p90_ms": float(np.percentile(lat, 90) * 1e3),
@@ -199,7 +213,9 @@ def serving_stream(nreq, rate, slots, seed=2001):
             "mean_ms": float(lat.mean() * 1e3),
             "p99_ms_N_ge_500k": float(np.percentile(lat[big], 99) * 1e3) if big else None,
             "charges_per_s": charges / (t_last - t_first), "total_charges": charges,
-            "max_submit_lag_ms": lag * 1e3, "by_kernel": by,
+            "max_submit_lag_ms": float(lags.max() * 1e3),
+            "p99_submit_lag_ms": float(np.percentile(lags, 99) * 1e3), "by_kernel": by,
+            "phase_ms_N_ge_500k": phases(big), "phase_ms_slowest_1pct": phases(tail),
             "workload": "Poisson %g req/s, N log-uniform [500, 1e6], density 0.1, 70%% 1/r / 20%% "
                         "r^alpha (0.5|1.5) / 10%% Helmholtz k0=0.3, tol 1e-4|1e-6, Rdir 8, mixed "
                         "precision, seed %d; latency includes set-up (RSEM grid, cell sort, lists, "

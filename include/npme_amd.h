@@ -151,6 +151,10 @@ typedef struct {
   double tArrival, tSubmit, tStart, tEnd;  /* server clock, seconds        */
   long N[3];                               /* FFT grid used                */
   double deviceGB;                         /* device footprint             */
+  /* wall time of the request's phases on its slot, milliseconds: grid
+   * choice (RSEM keywords), set-up, charges to the device, CalcV, V to the
+   * host */
+  double msKeywords, msSetup, msCopyIn, msCalc, msCopyOut;
 } npme_request_info;
 int npme_server_create(npme_server **s, int nSlots, double memGB);
 /* queues a request; returns at once with its ticket */

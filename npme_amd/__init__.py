@@ -83,7 +83,8 @@ class _Request(C.Structure):
 class _RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("tArrival", C.c_double), ("tSubmit", C.c_double),
                 ("tStart", C.c_double), ("tEnd", C.c_double), ("N", C.c_long * 3),
-                ("deviceGB", C.c_double)]
+                ("deviceGB", C.c_double), ("msKeywords", C.c_double), ("msSetup", C.c_double),
+                ("msCopyIn", C.c_double), ("msCalc", C.c_double), ("msCopyOut", C.c_double)]
 
 
 def npme_last_error():
@@ -370,7 +371,10 @@ class NPME_Server:
         if rc != NPME_OK:
             raise NPMEError(rc, "request %d" % ticket)
         return V, {"tArrival": info.tArrival, "tSubmit": info.tSubmit, "tStart": info.tStart,
-                   "tEnd": info.tEnd, "grid": tuple(info.N), "deviceGB": info.deviceGB}
+                   "tEnd": info.tEnd, "grid": tuple(info.N), "deviceGB": info.deviceGB,
+                   "ms": {"keywords": info.msKeywords, "setup": info.msSetup,
+                          "copy_in": info.msCopyIn, "calcv": info.msCalc,
+                          "copy_out": info.msCopyOut}}
 
     def close(self):
         if getattr(self, "_s", None):

@@ -213,17 +213,43 @@ void fft_size_block(long Nideal, long nideal, long &N, long &nblk)
 // src/RecSumSupportFunctions.cpp:149-200 and :137-147, :276-296
 void box_dim_center(size_t n, const double *coord, double X0[3], double R0[3])
 {
-  double mx[3] = {-1e12, -1e12, -1e12}, mn[3] = {1e12, 1e12, 1e12};
-  for (size_t i = 0; i < n; i++)
+  // two charges (six values) per step: the six running extremes are
+  // independent chains.  Minima and maxima are exact, so the grouping does
+  // not change them.
+  double mx6[6], mn6[6];
+  for (int k = 0; k < 6; k++) {
+    mx6[k] = -1e12;
+    mn6[k] = 1e12;
+  }
+  size_t i = 0;
+  for (; i + 2 <= n; i += 2)
+    for (int k = 0; k < 6; k++) {
+      const double x = coord[3 * i + k];
+      mx6[k] = std::max(mx6[k], x);
+      mn6[k] = std::min(mn6[k], x);
+    }
+  for (; i < n; i++)
     for (int d = 0; d < 3; d++) {
-      double x = coord[3 * i + d];
-      mx[d] = std::max(mx[d], x);
-      mn[d] = std::min(mn[d], x);
+      mx6[d] = std::max(mx6[d], coord[3 * i + d]);
+      mn6[d] = std::min(mn6[d], coord[3 * i + d]);
     }
   for (int d = 0; d < 3; d++) {
-    X0[d] = mx[d] - mn[d];
-    R0[d] = 0.5 * (mx[d] + mn[d]);
+    const double mx = std::max(mx6[d], mx6[d + 3]), mn = std::min(mn6[d], mn6[d + 3]);
+    X0[d] = mx - mn;
+    R0[d] = 0.5 * (mx + mn);
   }
+}
+
+// sum of squares of m values in four interleaved partial sums, so the
+// loop is not bound by the latency of a single chain of additions
+static double sum_sq(size_t m, const double *v)
+{
+  double s[4] = {0, 0, 0, 0};
+  size_t i = 0;
+  for (; i + 4 <= m; i += 4)
+    for (int k = 0; k < 4; k++) s[k] += v[i + k] * v[i + k];
+  for (; i < m; i++) s[0] += v[i] * v[i];
+  return (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 void pme_corr_box(const double X0[3], const double R0[3], const long N[3], const double del[3],
@@ -266,13 +292,12 @@ static void find_u(double &u, double y0, const double a[5], double uMin, double 
   }
 }
 
-static void rsem_iterate(size_t n, const double *coord, double Q0, long Bn, double alphaExp,
-                         double tol, double Rdir, const double a[5], double uMin, double uMax,
-                         long N[3], long nblk[3])
+// X0, R0: box size and centre of the charges (box_dim_center)
+static void rsem_iterate(const double X0[3], const double R0[3], double Q0, long Bn,
+                         double alphaExp, double tol, double Rdir, const double a[5], double uMin,
+                         double uMax, long N[3], long nblk[3])
 {
   const double del = Rdir;
-  double X0[3], R0[3];
-  box_dim_center(n, coord, X0, R0);
   N[0] = N[1] = N[2] = 100;
   for (int it = 0; it < 10; it++) {
     double X[3], R[3], dl[3] = {del, del, del}, L[3];
@@ -298,9 +323,9 @@ static void rsem_iterate(size_t n, const double *coord, double Q0, long Bn, doub
 bool rsem_static(size_t n, const double *q, const double *coord, long Bn, double alpha, double tol,
                  double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder)
 {
-  double Q0 = 0;
-  for (size_t i = 0; i < n; i++) Q0 += q[i] * q[i];
-  Q0 = std::sqrt(Q0);
+  const double Q0 = std::sqrt(sum_sq(n, q));
+  double X0[3], R0[3];
+  box_dim_center(n, coord, X0, R0);
   bool found = false;
   long best = 0;
   for (int r = 0; r < NPME_RSEM_N_STATIC; r++) {
@@ -308,7 +333,7 @@ bool rsem_static(size_t n, const double *q, const double *coord, long Bn, double
     if ((long)row[0] != Bn || std::fabs(alpha - row[2]) >= 1e-6) continue;
     if (nDerivIn >= 0 && (int)row[1] != nDerivIn) continue;
     long Nt[3], nt[3];
-    rsem_iterate(n, coord, Q0, Bn, alpha, tol, Rdir, row + 5, row[3], row[4], Nt, nt);
+    rsem_iterate(X0, R0, Q0, Bn, alpha, tol, Rdir, row + 5, row[3], row[4], Nt, nt);
     long sz = Nt[0] * Nt[1] * Nt[2];
     if (!found || best > sz) {
       best = sz;
@@ -336,9 +361,7 @@ bool rsem_static_pred_error(size_t n, const double *q, const double *coord, long
     }
   }
   if (!row) return false;
-  double Q0 = 0;
-  for (size_t i = 0; i < n; i++) Q0 += q[i] * q[i];
-  Q0 = std::sqrt(Q0);
+  const double Q0 = std::sqrt(sum_sq(n, q));
   double X0[3], R0[3], X[3], R[3], dl[3] = {Rdir, Rdir, Rdir}, L[3];
   box_dim_center(n, coord, X0, R0);
   pme_corr_box(X0, R0, N, dl, Bn, X, R);
@@ -364,9 +387,9 @@ bool rsem_helmholtz(size_t n, const double *qc, const double *coord, long Bn, do
                     double k0i, double tol, double Rdir, int nDerivIn, long N[3], long nblk[3],
                     int &Nder, std::string &err)
 {
-  double Q0 = 0;
-  for (size_t i = 0; i < n; i++) Q0 += qc[2 * i] * qc[2 * i] + qc[2 * i + 1] * qc[2 * i + 1];
-  Q0 = std::sqrt(Q0);
+  const double Q0 = std::sqrt(sum_sq(2 * n, qc));
+  double X0[3], R0[3];
+  box_dim_center(n, coord, X0, R0);
   const double w = std::hypot(k0r, k0i) * Rdir;
   bool found = false;
   long best = 0;
@@ -389,7 +412,7 @@ bool rsem_helmholtz(size_t n, const double *qc, const double *coord, long Bn, do
       for (int q = 0; q < 5; q++) b[p] += row[6 + p * 5 + q] * wq[q];
     }
     long Nt[3], nt[3];
-    rsem_iterate(n, coord, Q0, Bn, -1.0, tol, Rdir, b, row[2], row[3], Nt, nt);
+    rsem_iterate(X0, R0, Q0, Bn, -1.0, tol, Rdir, b, row[2], row[3], Nt, nt);
     long sz = Nt[0] * Nt[1] * Nt[2];
     if (!found || best > sz) {
       best = sz;

@@ -44,6 +44,7 @@ struct Job {
   std::string err;
   long N[3] = {0, 0, 0};
   double deviceGB = 0;
+  double ms[5] = {0, 0, 0, 0, 0};  // keywords, set-up, copy in, CalcV, copy out
   bool done = false;
 };
 
@@ -115,18 +116,30 @@ static void run_job(Handle &h, Job &j)
     kw.set("k0_i", j.k0i);
   }
   kw.set("precision", std::string(j.fp64 ? "fp64" : "mixed"));
+  auto t = std::chrono::steady_clock::now();
+  auto lap = [&t](double &ms) {
+    const auto t1 = std::chrono::steady_clock::now();
+    ms = std::chrono::duration<double, std::milli>(t1 - t).count();
+    t = t1;
+  };
   add_keywords_for_request(kw, j);
+  lap(j.ms[0]);
   std::lock_guard<std::mutex> lk(h.mtx);
   handle_setup(h, kw, j.n, j.coord);
   for (int d = 0; d < 3; d++) j.N[d] = h.N[d];
   j.deviceGB = h.deviceBytes / 1e9;
+  lap(j.ms[1]);
   const bool cx = h.complexK;
   const size_t qn = (cx ? 2 : 1) * j.n, vn = (cx ? 8 : 4) * j.n;
   NPME_HIP_CHECK(hipMemcpyAsync(h.d_qin, j.charge, qn * sizeof(double), hipMemcpyHostToDevice, h.stream));
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // CalcV reads the charges from both streams
+  lap(j.ms[2]);
   handle_calcv(h, h.d_qin, h.d_V);
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  lap(j.ms[3]);
   NPME_HIP_CHECK(hipMemcpyAsync(j.V, h.d_V, vn * sizeof(double), hipMemcpyDeviceToHost, h.stream));
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  lap(j.ms[4]);
 }
 
 static void worker_main(Server *S)
@@ -266,6 +279,11 @@ int npme_server_wait(npme_server *srv, long ticket, npme_request_info *info)
     info->tEnd = j->tEnd;
     for (int d = 0; d < 3; d++) info->N[d] = j->N[d];
     info->deviceGB = j->deviceGB;
+    info->msKeywords = j->ms[0];
+    info->msSetup = j->ms[1];
+    info->msCopyIn = j->ms[2];
+    info->msCalc = j->ms[3];
+    info->msCopyOut = j->ms[4];
   }
   const int rc = j->status;
   if (rc != NPME_OK) set_error(j->err);
