@@ -118,7 +118,8 @@ class NPME_KeywordInput:
     """Keyword map (src/ReadPrint.h:28-102)."""
 
     def __init__(self, keywordFile=None, **kw):
-        self._h = C.c_void_p(lib().npme_keywords_create())
+        self._L = lib()  # kept for __del__, which may run during interpreter shutdown
+        self._h = C.c_void_p(self._L.npme_keywords_create())
         if keywordFile is not None:
             _check(lib().npme_keywords_read_file(self._h, keywordFile.encode()), "read keywords")
         for k, v in kw.items():
@@ -126,7 +127,7 @@ class NPME_KeywordInput:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().npme_keywords_destroy(self._h)
+            self._L.npme_keywords_destroy(self._h)
             self._h = None
 
     def AddKeyword(self, key, value):
@@ -174,6 +175,7 @@ class _InterfaceBase:
     _funcType = None
 
     def __init__(self):
+        self._L = lib()  # kept for __del__, which may run during interpreter shutdown
         self._h = None
         self.nCharge = 0
 
@@ -182,7 +184,7 @@ class _InterfaceBase:
 
     def _free(self):
         if getattr(self, "_h", None):
-            lib().npme_destroy(self._h)
+            self._L.npme_destroy(self._h)
             self._h = None
 
     def SetUp(self, keyword, nCharge, coord, printLog=False):
@@ -331,6 +333,7 @@ class NPME_Server:
     counterpart."""
 
     def __init__(self, nSlots=4, memGB=0.0):
+        self._L = lib()  # kept for __del__, which may run during interpreter shutdown
         self._s = C.c_void_p()
         _check(lib().npme_server_create(C.byref(self._s), int(nSlots), float(memGB)), "server_create")
         self._live = {}
@@ -378,7 +381,7 @@ class NPME_Server:
 
     def close(self):
         if getattr(self, "_s", None):
-            lib().npme_server_destroy(self._s)
+            self._L.npme_server_destroy(self._s)
             self._s = None
             self._live.clear()
 

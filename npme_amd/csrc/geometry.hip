@@ -388,6 +388,15 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
   }
 }
 
+long scan_words_device(const long *d_in, long *d_out, size_t m, hipStream_t s)
+{
+  exclusive_scan(d_in, d_out, m, s);
+  long total = 0;
+  NPME_HIP_CHECK(hipMemcpyAsync(&total, d_out + (m - 1), sizeof(long), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  return total;
+}
+
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s)
 {

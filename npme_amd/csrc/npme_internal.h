@@ -271,6 +271,9 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s);
 // coordinates in sorted order: xs[s] = coord[perm[s]]
+// exclusive prefix sums of m values into out[0..m-1]; returns out[m-1]
+// (the total when in[m-1] is 0)
+long scan_words_device(const long *d_in, long *d_out, size_t m, hipStream_t s);
 // exact per-axis bounds of the n positions on the device (NaN if any is NaN)
 void coord_bounds_device(size_t n, const double *d_coord, double mn[3], double mx[3], hipStream_t s);
 void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
@@ -341,9 +344,17 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);
 unsigned nlist_pad(size_t sentinel);  // padding entry of the neighbour list (sentinel source = n)
+// scratch of one list build, from its count pass to its fill pass: the
+// hit masks of the wavefront-per-cell walk and their per-cell offsets
+struct NlistWork {
+  DevBuf maskStart, masks;
+  bool masked = false;
+};
+// count pass (fill false: count[s] per target) or fill pass of the
+// neighbour list; both passes of one build take the same work
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
-                  hipStream_t s);
+                  NlistWork &work, hipStream_t s);
 void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                       const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
                       double csA, double *Vdir, hipStream_t s);

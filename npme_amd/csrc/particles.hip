@@ -610,20 +610,84 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l)
+{
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)v, l);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// stencil of the home cell (hx, hy, hz) for a wavefront: per stencil cell
+// (in the loop order of k_nlist) its first charge, the candidates before
+// it and its cluster-offset code, in LDS; returns the candidate count T
+__device__ __forceinline__ int nl_stencil_scan(const DirectDev &p, const int2 *__restrict__ cellSE,
+                                               int nP, int hx, int hy, int hz, int lane,
+                                               int *shStart, int *shPre, unsigned char *shCode)
+{
+  const int nN = p.nNeigh, W = 2 * nN + 1, M = W * W * W;
+  const int ax = hx / nP, ay = hy / nP, az = hz / nP;
+  int T = 0;
+  for (int c0 = 0; c0 < M; c0 += 64) {
+    const int ci = c0 + lane;
+    int st = 0, cnt = 0;
+    unsigned code = 0;
+    if (ci < M) {
+      const int gx = hx + ci / (W * W) - nN, gy = hy + (ci / W) % W - nN, gz = hz + ci % W - nN;
+      if (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2) {
+        const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+        st = se.x;
+        cnt = se.y;
+        code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4));
+      }
+    }
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(inc, o);
+      if (lane >= o) inc += v;
+    }
+    if (ci < M) {
+      shStart[ci] = st;
+      shPre[ci] = T + inc - cnt;
+      shCode[ci] = (unsigned char)code;
+    }
+    T += __shfl(inc, 63);
+  }
+  return T;
+}
+
+// candidate t of the scanned stencil: source index and list entry
+__device__ __forceinline__ int nl_candidate(int t, int M, const int *shStart, const int *shPre,
+                                            const unsigned char *shCode, unsigned &ent)
+{
+  // last cell whose first candidate is at or before t (empty cells share
+  // their start with the next cell, so it is the cell holding t)
+  int lo = 0, hi = M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (shPre[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  const int j = shStart[lo] + (t - shPre[lo]);
+  ent = ((unsigned)shCode[lo] << NL_SHIFT) | (unsigned)j;
+  return j;
+}
+
 // the same walk with a wavefront per cell, for large charge sets: the
 // targets of one cell share their stencil, so the wave loads the stencil's
 // candidates once, 64 at a time (the flattened sequence of k_nlist_wave),
 // and tests each chunk against the cell's targets in turn, one ballot per
 // target.  Lane i holds target i's position, list offset and entry count
-// (targets in batches of 64).  The entries and their order are those of
-// k_nlist.
-template <bool FILL>
-__global__ __launch_bounds__(256) void k_nlist_cell(DirectDev p, long ncell,
-                                                    const int2 *__restrict__ cellSE,
-                                                    const double4 *__restrict__ xs, int nP,
-                                                    int *__restrict__ count,
-                                                    const long *__restrict__ sliceStart,
-                                                    unsigned *__restrict__ list)
+// (targets in batches of 64).  This is the count pass; it keeps its
+// ballots, the hit mask of target tb + i for chunk k at
+// masks[maskStart[c] + k * (targets of c) + tb + i], and
+// k_nlist_cell_expand fills the list from them, so the entries and their
+// order are those of k_nlist.
+__global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncell,
+                                                          const int2 *__restrict__ cellSE,
+                                                          const double4 *__restrict__ xs, int nP,
+                                                          int *__restrict__ count,
+                                                          const long *__restrict__ maskStart,
+                                                          unsigned long long *__restrict__ masks)
 {
   __shared__ int shStart[4][NLW_MAXC];
   __shared__ int shPre[4][NLW_MAXC];
@@ -635,88 +699,125 @@ __global__ __launch_bounds__(256) void k_nlist_cell(DirectDev p, long ncell,
   const double R = p.Rdir * (1.0 + 1.0e-9);
   const double R2 = R * R;
   const int nN = p.nNeigh, W = 2 * nN + 1, M = W * W * W;
-  const int ax = hx / nP, ay = hy / nP, az = hz / nP;
   int T = 0;
-  if (home.y > 0) {
-    for (int c0 = 0; c0 < M; c0 += 64) {
-      const int ci = c0 + lane;
-      int st = 0, cnt = 0;
-      unsigned code = 0;
-      if (ci < M) {
-        const int gx = hx + ci / (W * W) - nN, gy = hy + (ci / W) % W - nN, gz = hz + ci % W - nN;
-        if (gx >= 0 && gy >= 0 && gz >= 0 && gx < p.nb0 && gy < p.nb1 && gz < p.nb2) {
-          const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
-          st = se.x;
-          cnt = se.y;
-          code = (unsigned)((gx / nP - ax + 1) | ((gy / nP - ay + 1) << 2) | ((gz / nP - az + 1) << 4));
-        }
-      }
-      int inc = cnt;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(inc, o);
-        if (lane >= o) inc += v;
-      }
-      if (ci < M) {
-        shStart[w][ci] = st;
-        shPre[w][ci] = T + inc - cnt;
-        shCode[w][ci] = (unsigned char)code;
-      }
-      T += __shfl(inc, 63);
-    }
-  }
+  if (home.y > 0) T = nl_stencil_scan(p, cellSE, nP, hx, hy, hz, lane, shStart[w], shPre[w], shCode[w]);
   __syncthreads();
   if (home.y == 0) return;  // after the only barrier
-  const unsigned long long below = (1ull << lane) - 1;
+  const long mbase = maskStart[c];
   for (int tb = 0; tb < home.y; tb += 64) {
     const int mt = min(64, home.y - tb);
     // lane i: target s = home.x + tb + i
     const size_t si = (size_t)home.x + tb + min(lane, mt - 1);
     const double4 tp = xs[si];
-    long obase = 0;
-    if (FILL) obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
     int kreg = 0;
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
       int j = -1;
-      unsigned ent = 0;
       double ox = 0, oy = 0, oz = 0;
       if (t < T) {
-        int lo = 0, hi = M - 1;
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (shPre[w][mid] <= t) lo = mid;
-          else hi = mid - 1;
-        }
-        j = shStart[w][lo] + (t - shPre[w][lo]);
+        unsigned ent;
+        j = nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
         const double4 o = xs[j];
         ox = o.x;
         oy = o.y;
         oz = o.z;
-        ent = ((unsigned)shCode[w][lo] << NL_SHIFT) | (unsigned)j;
       }
+      unsigned long long mym = 0;
       for (int i = 0; i < mt; i++) {
         const int s = home.x + tb + i;
         const double rx = readlane_f64(tp.x, i) - ox, ry = readlane_f64(tp.y, i) - oy,
                      rz = readlane_f64(tp.z, i) - oz;
         const bool hit = j >= 0 && j != s && rx * rx + ry * ry + rz * rz < R2;
         const unsigned long long m = __ballot(hit);
-        const int ki = __builtin_amdgcn_readlane(kreg, i);
-        if (FILL && hit) {
-          const long ob = (long)__builtin_amdgcn_readlane((unsigned)obase, i) |
-                          ((long)__builtin_amdgcn_readlane((unsigned)((unsigned long)obase >> 32), i) << 32);
-          const int pos = ki + __popcll(m & below);
+        if (lane == i) {
+          kreg += __popcll(m);
+          mym = m;
+        }
+      }
+      if (lane < mt) masks[mbase + (long)(t0 >> 6) * home.y + tb + lane] = mym;
+    }
+    if (lane < mt) count[home.x + tb + lane] = kreg;
+  }
+}
+
+// fill pass from the count pass's hit masks: the same candidate sequence,
+// without positions or distance tests
+__global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long ncell,
+                                                           const int2 *__restrict__ cellSE, int nP,
+                                                           const long *__restrict__ maskStart,
+                                                           const unsigned long long *__restrict__ masks,
+                                                           const long *__restrict__ sliceStart,
+                                                           unsigned *__restrict__ list)
+{
+  __shared__ int shStart[4][NLW_MAXC];
+  __shared__ int shPre[4][NLW_MAXC];
+  __shared__ unsigned char shCode[4][NLW_MAXC];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long c = (long)blockIdx.x * 4 + w;
+  const int2 home = c < ncell ? cellSE[c] : make_int2(0, 0);
+  const int hz = (int)(c % p.nb2), hy = (int)((c / p.nb2) % p.nb1), hx = (int)(c / ((long)p.nb1 * p.nb2));
+  const int nN = p.nNeigh, W = 2 * nN + 1, M = W * W * W;
+  int T = 0;
+  if (home.y > 0) T = nl_stencil_scan(p, cellSE, nP, hx, hy, hz, lane, shStart[w], shPre[w], shCode[w]);
+  __syncthreads();
+  if (home.y == 0) return;  // after the only barrier
+  const unsigned long long below = (1ull << lane) - 1;
+  const long mbase = maskStart[c];
+  for (int tb = 0; tb < home.y; tb += 64) {
+    const int mt = min(64, home.y - tb);
+    const size_t si = (size_t)home.x + tb + min(lane, mt - 1);
+    const long obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
+    int kreg = 0;
+    for (int t0 = 0; t0 < T; t0 += 64) {
+      const int t = t0 + lane;
+      unsigned ent = 0;
+      if (t < T) (void)nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
+      const unsigned long long mym = lane < mt ? masks[mbase + (long)(t0 >> 6) * home.y + tb + lane] : 0ull;
+      for (int i = 0; i < mt; i++) {
+        const unsigned long long m = readlane_u64(mym, i);
+        if (m == 0) continue;  // uniform
+        if ((m >> lane) & 1ull) {
+          const long ob = (long)readlane_u64((unsigned long long)obase, i);
+          const int pos = __builtin_amdgcn_readlane(kreg, i) + __popcll(m & below);
           list[ob + 256L * (pos >> 2) + (pos & 3)] = ent;
         }
         if (lane == i) kreg += __popcll(m);
       }
     }
-    if (!FILL && lane < mt) count[home.x + tb + lane] = kreg;
   }
+}
+
+// per cell: the hit masks the count pass keeps (targets x candidate
+// chunks); words[ncell] = 0 closes the scan
+__global__ __launch_bounds__(256) void k_nlist_cell_words(DirectDev p, long ncell,
+                                                          const int2 *__restrict__ cellSE,
+                                                          long *__restrict__ words)
+{
+  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > ncell) return;
+  if (c == ncell) {
+    words[c] = 0;
+    return;
+  }
+  const int2 home = cellSE[c];
+  long T = 0;
+  if (home.y > 0) {
+    const int hz = (int)(c % p.nb2), hy = (int)((c / p.nb2) % p.nb1), hx = (int)(c / ((long)p.nb1 * p.nb2));
+    const int nN = p.nNeigh;
+    for (int dx = -nN; dx <= nN; dx++)
+      for (int dy = -nN; dy <= nN; dy++)
+        for (int dz = -nN; dz <= nN; dz++) {
+          const int gx = hx + dx, gy = hy + dy, gz = hz + dz;
+          if (gx < 0 || gy < 0 || gz < 0 || gx >= p.nb0 || gy >= p.nb1 || gz >= p.nb2) continue;
+          T += cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz].y;
+        }
+  }
+  words[c] = (long)home.y * ((T + 63) / 64);
 }
 
 void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, const int *cellStart,
                   const double4 *xs, int nP, int *count, const long *sliceStart, unsigned *list,
-                  hipStream_t s)
+                  NlistWork &work, hipStream_t s)
 {
   DirectDev d;
   d.Rdir = p.Rdir;
@@ -738,12 +839,26 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   if (pick == 3) {
     const long ncell = (long)p.nb[0] * p.nb[1] * p.nb[2];
     const dim3 g((unsigned)((ncell + 3) / 4));  // 4 cells (wavefronts) per workgroup
-    if (fill)
-      hipLaunchKernelGGL(k_nlist_cell<true>, g, dim3(256), 0, s, d, ncell, (const int2 *)cellStart, xs,
-                         nP, count, sliceStart, list);
-    else
-      hipLaunchKernelGGL(k_nlist_cell<false>, g, dim3(256), 0, s, d, ncell, (const int2 *)cellStart,
-                         xs, nP, count, sliceStart, list);
+    const int2 *cse = (const int2 *)cellStart;
+    if (!fill) {
+      // hit masks of the count pass: about an eighth of the list's size
+      work.maskStart.reset((ncell + 1) * sizeof(long));
+      DevBuf words((ncell + 1) * sizeof(long));
+      hipLaunchKernelGGL(k_nlist_cell_words, dim3((unsigned)((ncell + 256) / 256)), dim3(256), 0, s, d,
+                         ncell, cse, (long *)words.p);
+      NPME_HIP_CHECK(hipGetLastError());
+      const long total = scan_words_device((const long *)words.p, (long *)work.maskStart.p,
+                                           (size_t)ncell + 1, s);
+      work.masks.reset(std::max<size_t>((size_t)total, 1) * 8);
+      work.masked = true;
+      hipLaunchKernelGGL(k_nlist_cell_count, g, dim3(256), 0, s, d, ncell, cse, xs, nP, count,
+                         (const long *)work.maskStart.p, (unsigned long long *)work.masks.p);
+    } else {
+      if (!work.masked) throw Error{NPME_ERR_ARG, "neighbour list: fill pass without its count pass"};
+      hipLaunchKernelGGL(k_nlist_cell_expand, g, dim3(256), 0, s, d, ncell, cse, nP,
+                         (const long *)work.maskStart.p, (const unsigned long long *)work.masks.p,
+                         sliceStart, list);
+    }
     NPME_HIP_CHECK(hipGetLastError());
     return;
   }

@@ -175,10 +175,11 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   const DirectParams dp = direct_params(h);
   const int nP = (int)h.nCellClust1D;
   std::vector<int> cnt(h.n);
+  NlistWork work;
   {
     DevBuf d_count(h.n * sizeof(int));
     launch_nlist(false, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP,
-                 (int *)d_count.p, nullptr, nullptr, h.stream);
+                 (int *)d_count.p, nullptr, nullptr, work, h.stream);
     NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count.p, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
     NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   }
@@ -203,7 +204,7 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   NPME_HIP_CHECK(hipMemcpy(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int), hipMemcpyHostToDevice));
   NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(h.n), entries, h.stream));
   launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP, nullptr, h.d_sliceStart,
-               h.d_nlist, h.stream);
+               h.d_nlist, work, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   h.nlistEntries = entries;
   h.useNlist = true;
@@ -254,10 +255,11 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
   size_t entries = 0;
   h.d_sliceStart = dalloc<long>(h, nSlices + 1);
   h.d_sliceLen = dalloc<int>(h, nSlices);
+  NlistWork work;
   {
     DevBuf cnt(n * sizeof(int));
     launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, (int *)cnt.p,
-                 nullptr, nullptr, s);
+                 nullptr, nullptr, work, s);
     entries = nlist_slices_device(n, (const int *)cnt.p, h.d_sliceLen, h.d_sliceStart, s);
   }
   size_t freeB = 0, totalB = 0;
@@ -266,7 +268,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
   h.d_nlist = dalloc<unsigned>(h, entries);
   NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n), std::max<size_t>(entries, 1), s));
   launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
-               h.d_sliceStart, h.d_nlist, s);
+               h.d_sliceStart, h.d_nlist, work, s);
   h.nlistEntries = entries;
   h.useNlist = true;
   NPME_HIP_CHECK(hipStreamSynchronize(s));
