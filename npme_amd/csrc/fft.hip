@@ -772,6 +772,45 @@ __global__ __launch_bounds__(NT) void k_fft_even(int N, int nCols, int nInner, i
   }
 }
 
+// set-up, real even lines (the kernels 1/r and r^alpha): lines 2c and
+// 2c + 1 of a pass share one complex transform z = a + i b.  The transform
+// of a real even line is real, so A = Re Z and B = Im Z; folded back to the
+// compact line as in k_fft_even, in place (FP64)
+__global__ __launch_bounds__(NT) void k_fft_even_pair(int N, int nCols, int nInner, int outer,
+                                                      int inner, int pos, PlanDev p,
+                                                      const double2 *tw, double *__restrict__ data,
+                                                      int lgB)
+{
+  extern __shared__ char smem_raw[];
+  double2 *lds = reinterpret_cast<double2 *>(smem_raw);
+  const int h = N / 2;
+  const Lane L = lane_strided(lgB, N);
+  const int pair = (xcd_wg() << lgB) + L.c;
+  const int ca = 2 * pair, cb = 2 * pair + 1;
+  const bool va = ca < nCols, vb = cb < nCols;
+  auto line_of = [&](int c) {
+    const int o = c / nInner, i = c - o * nInner;
+    return data + ((long)o * outer + (long)i * inner);
+  };
+  double *la = line_of(va ? ca : 0), *lb = line_of(vb ? cb : 0);
+  stage_in<double2, 16>(lds, L, N, [&](int n) {
+    const long k = (long)((n <= h) ? n : N - n) * pos;
+    return mk<double2>(va ? la[k] : 0.0, vb ? lb[k] : 0.0);
+  });
+  __syncthreads();
+  fft_columns<double2, 16, RC_ALL>(lds, L.col, L.tj, L.tpc, p, tw);
+  if (!va) return;
+  for (int n = L.tj; n <= h; n += L.tpc) {
+    double2 v = lds[L.col.at(n)];
+    if (n != 0 && n != h) {
+      const double2 w = lds[L.col.at(N - n)];
+      v = mk<double2>(0.5 * (v.x + w.x), 0.5 * (v.y + w.y));
+    }
+    la[(long)n * pos] = v.x;
+    if (vb) lb[(long)n * pos] = v.y;
+  }
+}
+
 // ------------------------------------------------------------ host side
 static PlanDev to_dev(const FFTPlan1D &p)
 {
@@ -1008,6 +1047,19 @@ void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, lon
   }
 #undef NPME_CONV
 #undef NPME_CONV2
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+void fft_pass_even_real(int N, long nCols, long nInner, long outer, long inner, long pos,
+                        const FFTPlan1D &plan, const void *tw, double *data, hipStream_t s)
+{
+  check_int(nCols * (long)(N / 2 + 1), "FFT pass");
+  const int lgB = lg_batch(N, true, false);
+  const long pairs = (nCols + 1) / 2;
+  const dim3 grid((unsigned)((pairs + (1 << lgB) - 1) >> lgB));
+  const size_t sh = lds_bytes(N, lgB, true, false);
+  hipLaunchKernelGGL(k_fft_even_pair, grid, dim3(NT), sh, s, N, (int)nCols, (int)nInner, (int)outer,
+                     (int)inner, (int)pos, to_dev(plan), (const double2 *)tw, data, lgB);
   NPME_HIP_CHECK(hipGetLastError());
 }
 

@@ -312,9 +312,12 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
 void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
                        int N2, int G2, long K3, long Kv, const FFTPlan1D &plan, const void *tw,
                        const void *G, void *data, hipStream_t s, bool pow8 = true);
-// even-extension DFT of the compact kernel grid along one axis (set-up, FP64)
+// even-extension DFT of the compact kernel grid along one axis (set-up,
+// FP64): complex lines, and real lines ([.] doubles) two per transform
 void fft_pass_even(int N, long nCols, long nInner, long outer, long inner, long pos,
                    const FFTPlan1D &plan, const void *tw, void *data, hipStream_t s);
+void fft_pass_even_real(int N, long nCols, long nInner, long outer, long inner, long pos,
+                        const FFTPlan1D &plan, const void *tw, double *data, hipStream_t s);
 
 // particle kernels (particles.hip)
 struct DirectParams {
@@ -370,6 +373,8 @@ void launch_interp(bool fp64, bool complexK, long Bn, const long M[3], const dou
 void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
                   const double *coordS, const int *perm, const double *q_in, size_t nt,
                   const long *idx, double *Vout, hipStream_t s);
+// F on the compact real-space grid: double per point for the real kernels,
+// double2 for Helmholtz
 void launch_fgrid_real_space(const DMKernel &k, const long Mf[3], const double *d_tab,
                              void *F, hipStream_t s);
 void launch_make_G(bool fp64, bool complexK, const long Mf[3], long K3, const double *F,

@@ -1868,7 +1868,7 @@ __device__ __forceinline__ void lr_eval(const LRDev &k, double r2, double &fr, d
 template <int KIND>
 __global__ __launch_bounds__(256) void k_fgrid(LRDev k, long Mf0, long Mf1, long Mf2,
                                                const double *__restrict__ tab,
-                                               double2 *__restrict__ F)
+                                               void *__restrict__ Fout)
 {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long tot = Mf0 * Mf1 * Mf2;
@@ -1885,16 +1885,23 @@ __global__ __launch_bounds__(256) void k_fgrid(LRDev k, long Mf0, long Mf1, long
     for (int ay = 0; ay < 2; ay++)
 #pragma unroll
       for (int az = 0; az < 2; az++) {
+        // outside the extension band of an axis its second weight is 0
+        // (with a finite dummy argument), so the term adds exactly zero
         const double w = TX[ax] * TY[ay] * TZ[az];
+        if (w == 0.0) continue;
         double fr, fi;
         lr_eval<KIND>(k, X[ax] * X[ax] + Y[ay] * Y[ay] + Z[az] * Z[az], fr, fi);
         sr += w * fr;
         si += w * fi;
       }
-  double2 o;
-  o.x = sr;
-  o.y = si;
-  F[id] = o;
+  if (KIND == KIND_HELMHOLTZ) {
+    double2 o;
+    o.x = sr;
+    o.y = si;
+    static_cast<double2 *>(Fout)[id] = o;
+  } else {
+    static_cast<double *>(Fout)[id] = sr;
+  }
 }
 
 void launch_fgrid_real_space(const DMKernel &km, const long Mf[3], const double *d_tab, void *F,
@@ -1915,24 +1922,20 @@ void launch_fgrid_real_space(const DMKernel &km, const long Mf[3], const double 
   const long tot = Mf[0] * Mf[1] * Mf[2];
   const dim3 g((unsigned)((tot + 255) / 256));
   if (k.kind == KIND_LAPLACE)
-    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
-                       (double2 *)F);
+    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab, F);
   else if (k.kind == KIND_LAPLACE_ERF)
-    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE_ERF>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
-                       (double2 *)F);
+    hipLaunchKernelGGL(k_fgrid<KIND_LAPLACE_ERF>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab, F);
   else if (k.kind == KIND_RALPHA)
-    hipLaunchKernelGGL(k_fgrid<KIND_RALPHA>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
-                       (double2 *)F);
+    hipLaunchKernelGGL(k_fgrid<KIND_RALPHA>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab, F);
   else
-    hipLaunchKernelGGL(k_fgrid<KIND_HELMHOLTZ>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab,
-                       (double2 *)F);
+    hipLaunchKernelGGL(k_fgrid<KIND_HELMHOLTZ>, g, dim3(256), 0, s, k, Mf[0], Mf[1], Mf[2], d_tab, F);
   NPME_HIP_CHECK(hipGetLastError());
 }
 
 // G = F lam1 lam2 lam3 / (N1 N2 N3) on the (N1/2+1)(N2/2+1)(N3/2+1) grid,
 // stored in blocks of 8 along axis 3 (see k_make_G)
 template <typename Real, bool CX>
-__global__ void k_make_G(long Mf0, long Mf1, long Mf2, const double2 *__restrict__ F,
+__global__ void k_make_G(long Mf0, long Mf1, long Mf2, const double *__restrict__ F,
                          const double *__restrict__ lam, double scale, Real *__restrict__ G)
 {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1944,11 +1947,11 @@ __global__ void k_make_G(long Mf0, long Mf1, long Mf2, const double2 *__restrict
   // reads one contiguous block of (N1/2+1) x 8 values
   const long nb3 = (Mf2 + 7) / 8;
   const long o = ((i2 * nb3 + (i3 >> 3)) * Mf0 + i1) * 8 + (i3 & 7);
-  if (CX) {
-    G[2 * o] = (Real)(F[id].x * l);
-    G[2 * o + 1] = (Real)(F[id].y * l);
-  } else {
-    G[o] = (Real)(F[id].x * l);
+  if (CX) {  // F: double2 per point
+    G[2 * o] = (Real)(F[2 * id] * l);
+    G[2 * o + 1] = (Real)(F[2 * id + 1] * l);
+  } else {   // F: double per point
+    G[o] = (Real)(F[id] * l);
   }
 }
 
@@ -1962,8 +1965,8 @@ void launch_make_G(bool fp64, bool complexK, const long Mf[3], long K3, const do
                               (double)(2 * (Mf[2] - 1)));
   const dim3 g((unsigned)((tot + 255) / 256));
 #define NPME_G(R, CXV)                                                                          \
-  hipLaunchKernelGGL((k_make_G<R, CXV>), g, dim3(256), 0, s, Mf[0], Mf[1], Mf[2],              \
-                     (const double2 *)F, lam, scale, (R *)G)
+  hipLaunchKernelGGL((k_make_G<R, CXV>), g, dim3(256), 0, s, Mf[0], Mf[1], Mf[2], F, lam,       \
+                     scale, (R *)G)
   if (fp64) {
     if (complexK) NPME_G(double, true);
     else NPME_G(double, false);

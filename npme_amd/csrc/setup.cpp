@@ -617,7 +617,8 @@ static void grid_setup(Handle &h)
     tab.insert(tab.end(), T1.begin(), T1.end());
     tab.insert(tab.end(), T2.begin(), T2.end());
   }
-  DevBuf d_tab(tab.size() * sizeof(double)), d_F(2 * totF * sizeof(double));
+  // F: double per point for the real kernels, double2 for Helmholtz
+  DevBuf d_tab(tab.size() * sizeof(double)), d_F((h.complexK ? 2 : 1) * totF * sizeof(double));
   NPME_HIP_CHECK(hipMemcpy(d_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   launch_fgrid_real_space(h.ker, Mf, (const double *)d_tab.p, (double *)d_F.p, h.stream);
   DevBuf tw64[3];
@@ -628,9 +629,15 @@ static void grid_setup(Handle &h)
   }
   // even-extension DFT along axis 3, 2, 1 of F[Mf0][Mf1][Mf2]
   double *F = (double *)d_F.p;
-  fft_pass_even((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], tw64[2].p, F, h.stream);
-  fft_pass_even((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], tw64[1].p, F, h.stream);
-  fft_pass_even((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0].p, F, h.stream);
+  auto pass = h.complexK ? fft_pass_even
+                         : [](int N, long nCols, long nInner, long outer, long inner, long pos,
+                              const FFTPlan1D &plan, const void *tw, void *data, hipStream_t st) {
+                             fft_pass_even_real(N, nCols, nInner, outer, inner, pos, plan, tw,
+                                                (double *)data, st);
+                           };
+  pass((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], tw64[2].p, F, h.stream);
+  pass((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], tw64[1].p, F, h.stream);
+  pass((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0].p, F, h.stream);
   std::vector<double> lam;
   for (int d = 0; d < 3; d++) {
     std::vector<double> l(h.N[d]);
