@@ -46,6 +46,8 @@ def parse():
                    help="requests of the mixed-size serving stream (0 skips it)")
     p.add_argument("--stream-rate", type=float, default=200.0, help="Poisson arrival rate, 1/s")
     p.add_argument("--slots", type=int, default=4, help="concurrent request slots of the server")
+    p.add_argument("--pageable-requests", action="store_true",
+                   help="serving client builds requests in pageable instead of page-locked memory")
     p.add_argument("--calib-copy", action="store_true",
                    help="after the steps, copy a 256 MiB device buffer 3 times (a plain copy "
                         "kernel of known size, to calibrate profiler byte counters)")
@@ -105,13 +107,19 @@ def accuracy(h, q, V, n):
             "max_absV_over_meanV": float(np.max(np.abs(d[:, 0])) / magV)}
 
 
-def serving_stream(nreq, rate, slots, seed=2001):
+def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
     """Mixed-size request stream through the serving loop: Poisson arrivals
     at `rate`; N log-uniform in [500, 1e6]; positions uniform in a cube at
     density 0.1; charges U[-1,1] (zero net) or complex N(0,1); kernel 70%
     1/r, 20% r^alpha (alpha 0.5 or 1.5), 10% Helmholtz k0 = 0.3; tol 1e-4 or
     1e-6 with equal probability; Rdir 8.  Latency = completion minus the
-    scheduled arrival, so a late submission counts against the server."""
+    scheduled arrival, so a late submission counts against the server.
+
+    The client builds each request in buffers from a pool of `npool`
+    max-size buffer sets, reused as requests complete: page-locked host
+    memory (N.pinned_empty) by default, as a serving front end registers
+    its request buffers, or ordinary pageable memory."""
+    import queue
     import threading
     import npme_amd as N
     rng = np.random.default_rng(seed)
@@ -125,6 +133,21 @@ def serving_stream(nreq, rate, slots, seed=2001):
     alphas = np.where(rng.uniform(0, 1, nreq) < 0.5, 0.5, 1.5)
     tols = np.where(rng.uniform(0, 1, nreq) < 0.5, 1e-4, 1e-6)
     arrive = np.cumsum(rng.exponential(1.0 / rate, nreq)) + 0.5
+    # buffer sets (coordinates, real charges, V as 8 doubles per charge),
+    # allocated before the server's clock starts
+    npool = 24
+    alloc = N.pinned_empty if pinned else (lambda shape, dtype=np.float64: np.zeros(shape, dtype))
+    pool = [(alloc((nmax, 3)), alloc(nmax), alloc(8 * nmax)) for _ in range(npool)]
+    for c, qr, v in pool:  # pages resident before the stream
+        c.fill(0.0)
+        qr.fill(0.0)
+        v.fill(0.0)
+    qcomplex = alloc(nmax, np.complex128)
+    qcomplex[:] = baseqc
+    free = queue.Queue()
+    for k in range(npool):
+        free.put(k)
+    slot_of = [0] * nreq
     S = N.NPME_Server(nSlots=slots)
     tickets = [None] * nreq
     infos = [None] * nreq
@@ -138,27 +161,34 @@ def serving_stream(nreq, rate, slots, seed=2001):
                 _, infos[i] = S.wait(tickets[i])
             except Exception as e:  # recorded, reported in the JSON
                 errors.append(str(e))
+            free.put(slot_of[i])
             if (i + 1) % 1000 == 0:
                 log("stream: %d/%d requests done" % (i + 1, nreq))
 
     th = threading.Thread(target=consume)
     th.start()
 
-    # the client prepares each request's arrays (inputs, and the output
-    # array with its pages touched) ahead of its arrival in a thread of its
-    # own, so building a large request does not delay the submission of the
-    # next ones
-    import queue
+    # the client prepares each request's arrays ahead of its arrival in a
+    # thread of its own, so building a large request does not delay the
+    # submission of the next ones
     prep = queue.Queue(maxsize=16)
 
     def produce():
         for i in range(nreq):
             n = int(sizes[i])
             L = (n / 0.1) ** (1.0 / 3.0)
-            coord = base[:n] * L
-            q = baseqc[:n] if kinds[i] == 2 else baseq[:n] - baseq[:n].mean()
-            V = np.empty((n, 4), np.complex128 if kinds[i] == 2 else np.float64)
-            V.fill(0.0)  # pages touched: a serving client hands over a ready buffer
+            k = free.get()
+            slot_of[i] = k
+            c, qr, v = pool[k]
+            coord = c[:n]
+            np.multiply(base[:n], L, out=coord)
+            if kinds[i] == 2:
+                q = qcomplex[:n]
+                V = v[:8 * n].view(np.complex128).reshape(n, 4)
+            else:
+                q = qr[:n]
+                np.subtract(baseq[:n], baseq[:n].mean(), out=q)
+                V = v[:4 * n].reshape(n, 4)
             prep.put((coord, q, V))
 
     pt = threading.Thread(target=produce)
@@ -214,12 +244,14 @@ def serving_stream(nreq, rate, slots, seed=2001):
             "p99_ms_N_ge_500k": float(np.percentile(lat[big], 99) * 1e3) if big else None,
             "charges_per_s": charges / (t_last - t_first), "total_charges": charges,
             "max_submit_lag_ms": float(lags.max() * 1e3),
-            "p99_submit_lag_ms": float(np.percentile(lags, 99) * 1e3), "by_kernel": by,
+            "p99_submit_lag_ms": float(np.percentile(lags, 99) * 1e3),
+            "max_submit_lag_request": int(np.argmax(lags)), "by_kernel": by,
             "phase_ms_N_ge_500k": phases(big), "phase_ms_slowest_1pct": phases(tail),
             "workload": "Poisson %g req/s, N log-uniform [500, 1e6], density 0.1, 70%% 1/r / 20%% "
                         "r^alpha (0.5|1.5) / 10%% Helmholtz k0=0.3, tol 1e-4|1e-6, Rdir 8, mixed "
                         "precision, seed %d; latency includes set-up (RSEM grid, cell sort, lists, "
-                        "kernel grid), host<->device copies and CalcV" % (rate, seed)}
+                        "kernel grid), host<->device copies and CalcV; request buffers %s"
+                        % (rate, seed, "page-locked (pool of 24)" if pinned else "pageable")}
 
 
 def main():
@@ -316,7 +348,8 @@ def main():
     latency = None
     if a.stream_requests > 0:
         log("serving stream: %d requests at %g/s on %d slots" % (a.stream_requests, a.stream_rate, a.slots))
-        latency = serving_stream(a.stream_requests, a.stream_rate, a.slots)
+        latency = serving_stream(a.stream_requests, a.stream_rate, a.slots,
+                                 pinned=not a.pageable_requests)
         log("stream: p50 %.2f ms p99 %.2f ms" % (latency["p50_ms"], latency["p99_ms"]))
 
     cpu = None

@@ -166,6 +166,12 @@ int npme_server_wait(npme_server *s, long ticket, npme_request_info *info);
 double npme_server_clock(const npme_server *s);
 /* finishes the queued requests, then stops the slots */
 void npme_server_destroy(npme_server *s);
+/* page-locked host memory for request buffers (coord, charge, V): the
+ * copies to and from the device then run at DMA speed.  No counterpart in
+ * the reference.  npme_host_free may wait for the device: free at shutdown,
+ * not per request. */
+int npme_host_alloc(size_t bytes, void **p);
+void npme_host_free(void *p);
 
 /* ---- on-disk formats of the command-line tools (src/ReadPrint.cpp) and
  * the compareV metric.  Arrays returned through double ** are allocated by

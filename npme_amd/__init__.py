@@ -61,6 +61,8 @@ def lib():
         L.npme_server_clock.argtypes = [p]
         L.npme_server_clock.restype = d
         L.npme_server_destroy.argtypes = [p]
+        L.npme_host_alloc.argtypes = [s, C.POINTER(p)]
+        L.npme_host_free.argtypes = [p]
         pp = C.POINTER(p)
         L.npme_read_coord_charge.argtypes = [C.c_char_p, C.POINTER(s), C.POINTER(i), pp, pp]
         L.npme_write_coord_charge.argtypes = [C.c_char_p, s, i, p, p]
@@ -387,6 +389,36 @@ class NPME_Server:
 
     def __del__(self):
         self.close()
+
+
+class _PinnedBlock:
+    """Owner of one page-locked host allocation (npme_host_alloc); freed
+    with the last array that views it."""
+
+    def __init__(self, nbytes):
+        self._L = lib()
+        self.ptr = C.c_void_p()
+        _check(self._L.npme_host_alloc(int(nbytes), C.byref(self.ptr)), "host_alloc")
+        self.nbytes = int(nbytes)
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self._L.npme_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float64):
+    """Uninitialised numpy array in page-locked host memory, for request
+    buffers whose device copies should run at DMA speed.  Allocate such
+    buffers once and reuse them: releasing page-locked memory may wait for
+    the device."""
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape))
+    blk = _PinnedBlock(count * dtype.itemsize)
+    raw = (C.c_char * max(blk.nbytes, 1)).from_address(blk.ptr.value)
+    raw._npme_owner = blk  # the buffer keeps its allocation alive
+    a = np.frombuffer(raw, dtype=dtype, count=count).reshape(shape)
+    return a
 
 
 # ----------------------------------------------------------------- files

@@ -14,6 +14,7 @@
 //
 // The result of a request does not depend on what else is running: every
 // kernel is deterministic and a request touches only its own buffers.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -292,6 +293,23 @@ int npme_server_wait(npme_server *srv, long ticket, npme_request_info *info)
 }
 
 double npme_server_clock(const npme_server *srv) { return srv ? srv->s.now() : 0.0; }
+
+int npme_host_alloc(size_t bytes, void **p)
+{
+  return guarded([&] {
+    if (!p) throw Error{NPME_ERR_ARG, "npme_host_alloc: NULL argument"};
+    *p = nullptr;
+    if (hipHostMalloc(p, std::max<size_t>(bytes, 1), hipHostMallocDefault) != hipSuccess) {
+      *p = nullptr;
+      throw Error{NPME_ERR_NOMEM, "npme_host_alloc: page-locked allocation failed"};
+    }
+  });
+}
+
+void npme_host_free(void *p)
+{
+  if (p) (void)hipHostFree(p);
+}
 
 void npme_server_destroy(npme_server *srv)
 {

@@ -98,3 +98,38 @@ def test_server_rejects_bad_requests():
     with pytest.raises(N.NPMEError):
         S.wait(t)
     S.close()
+
+
+def test_server_pinned_request_buffers():
+    """Requests built in page-locked buffers (pinned_empty) give the same V
+    as from ordinary arrays, including views into a larger pinned buffer
+    reused for a second request, and report their phase times."""
+    import npme_amd as N
+    reqs = _requests([3000, 20000, 5000, 40000], seed=37)
+    print("case server pinned buffers: %d requests" % len(reqs), flush=True)
+    S = N.NPME_Server(nSlots=2)
+    try:
+        ref = _run(S, reqs)
+        nmax = max(len(q) for _, _, q, _, _ in reqs)
+        cbuf, qbuf = N.pinned_empty((nmax, 3)), N.pinned_empty(nmax, np.complex128)
+        vbuf = N.pinned_empty(8 * nmax)
+        for (ft, c, q, tol, kw), (Vr, _) in zip(reqs, ref):
+            n = len(q)
+            coord = cbuf[:n]
+            coord[:] = c
+            if ft == "Helmholtz":
+                qp = qbuf[:n]
+                V = vbuf[:8 * n].view(np.complex128).reshape(n, 4)
+            else:
+                qp = qbuf.view(np.float64)[:n]
+                V = vbuf[:4 * n].reshape(n, 4)
+            qp[:] = q
+            V.fill(np.nan)
+            t = S.submit(ft, coord, qp, tol=tol, Rdir=8.0, V=V, **kw)
+            Vp, info = S.wait(t)
+            assert Vp is V or np.shares_memory(Vp, V)
+            assert np.array_equal(V, Vr), ft
+            assert set(info["ms"]) == {"keywords", "setup", "copy_in", "calcv", "copy_out"}
+            assert all(v >= 0 for v in info["ms"].values())
+    finally:
+        S.close()
