@@ -692,7 +692,10 @@ __global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncel
   __shared__ int shStart[4][NLW_MAXC];
   __shared__ int shPre[4][NLW_MAXC];
   __shared__ unsigned char shCode[4][NLW_MAXC];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave-uniform values live in scalar registers: the cell, its targets
+  // and their positions (scalar loads of xs at a uniform address)
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
   const long c = (long)blockIdx.x * 4 + w;
   const int2 home = c < ncell ? cellSE[c] : make_int2(0, 0);
   const int hz = (int)(c % p.nb2), hy = (int)((c / p.nb2) % p.nb1), hx = (int)(c / ((long)p.nb1 * p.nb2));
@@ -706,14 +709,14 @@ __global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncel
   const long mbase = maskStart[c];
   for (int tb = 0; tb < home.y; tb += 64) {
     const int mt = min(64, home.y - tb);
-    // lane i: target s = home.x + tb + i
-    const size_t si = (size_t)home.x + tb + min(lane, mt - 1);
-    const double4 tp = xs[si];
-    int kreg = 0;
+    const int s0 = home.x + tb;  // target i is s0 + i
+    const double4 *tpos = xs + s0;
+    int kreg = 0;  // lane i: entries of target s0 + i so far
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
+      // a lane past the candidates tests a point far outside every cutoff
       int j = -1;
-      double ox = 0, oy = 0, oz = 0;
+      double ox = 1.0e150, oy = 1.0e150, oz = 1.0e150;
       if (t < T) {
         unsigned ent;
         j = nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
@@ -724,24 +727,24 @@ __global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncel
       }
       unsigned long long mym = 0;
       for (int i = 0; i < mt; i++) {
-        const int s = home.x + tb + i;
-        const double rx = readlane_f64(tp.x, i) - ox, ry = readlane_f64(tp.y, i) - oy,
-                     rz = readlane_f64(tp.z, i) - oz;
-        const bool hit = j >= 0 && j != s && rx * rx + ry * ry + rz * rz < R2;
+        const double4 tp = tpos[i];
+        const double rx = tp.x - ox, ry = tp.y - oy, rz = tp.z - oz;
+        // & rather than &&: no branch around the distance test
+        const bool hit = (j != s0 + i) & (rx * rx + ry * ry + rz * rz < R2);
         const unsigned long long m = __ballot(hit);
-        if (lane == i) {
-          kreg += __popcll(m);
-          mym = m;
-        }
+        mym = lane == i ? m : mym;
       }
       if (lane < mt) masks[mbase + (long)(t0 >> 6) * home.y + tb + lane] = mym;
+      kreg += __popcll(mym);
     }
-    if (lane < mt) count[home.x + tb + lane] = kreg;
+    if (lane < mt) count[s0 + lane] = kreg;
   }
 }
 
 // fill pass from the count pass's hit masks: the same candidate sequence,
-// without positions or distance tests
+// without positions or distance tests.  Target i's mask and list offset are
+// wave-uniform (scalar loads); a hit lane's place is the target's entry
+// count plus the hits in lanes below it (mbcnt).
 __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long ncell,
                                                            const int2 *__restrict__ cellSE, int nP,
                                                            const long *__restrict__ maskStart,
@@ -752,7 +755,8 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
   __shared__ int shStart[4][NLW_MAXC];
   __shared__ int shPre[4][NLW_MAXC];
   __shared__ unsigned char shCode[4][NLW_MAXC];
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
   const long c = (long)blockIdx.x * 4 + w;
   const int2 home = c < ncell ? cellSE[c] : make_int2(0, 0);
   const int hz = (int)(c % p.nb2), hy = (int)((c / p.nb2) % p.nb1), hx = (int)(c / ((long)p.nb1 * p.nb2));
@@ -761,28 +765,30 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
   if (home.y > 0) T = nl_stencil_scan(p, cellSE, nP, hx, hy, hz, lane, shStart[w], shPre[w], shCode[w]);
   __syncthreads();
   if (home.y == 0) return;  // after the only barrier
-  const unsigned long long below = (1ull << lane) - 1;
   const long mbase = maskStart[c];
   for (int tb = 0; tb < home.y; tb += 64) {
     const int mt = min(64, home.y - tb);
-    const size_t si = (size_t)home.x + tb + min(lane, mt - 1);
-    const long obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
+    const int s0 = home.x + tb;
     int kreg = 0;
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
       unsigned ent = 0;
       if (t < T) (void)nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
-      const unsigned long long mym = lane < mt ? masks[mbase + (long)(t0 >> 6) * home.y + tb + lane] : 0ull;
+      const unsigned long long *mrow = masks + mbase + (long)(t0 >> 6) * home.y + tb;
+      const unsigned long long mym = lane < mt ? mrow[lane] : 0ull;
       for (int i = 0; i < mt; i++) {
-        const unsigned long long m = readlane_u64(mym, i);
+        const unsigned long long m = mrow[i];
         if (m == 0) continue;  // uniform
+        const int ki = __builtin_amdgcn_readlane(kreg, i);
         if ((m >> lane) & 1ull) {
-          const long ob = (long)readlane_u64((unsigned long long)obase, i);
-          const int pos = __builtin_amdgcn_readlane(kreg, i) + __popcll(m & below);
+          const int s = s0 + i;
+          const long ob = sliceStart[s >> 6] + 4 * (long)(s & 63);
+          const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)ki));
           list[ob + 256L * (pos >> 2) + (pos & 3)] = ent;
         }
-        if (lane == i) kreg += __popcll(m);
       }
+      kreg += __popcll(mym);
     }
   }
 }
