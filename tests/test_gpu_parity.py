@@ -177,15 +177,19 @@ def test_calcv_rejects_wrong_size():
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-@pytest.mark.parametrize("n,box", [(6000, 35.0), (220000, 130.0)])
-def test_direct_sum_paths_agree(oracle, prec, n, box):
+@pytest.mark.parametrize("n,box,nNeigh,nClust", [(6000, 35.0, 2, 3), (220000, 130.0, 2, 3),
+                                                 (6000, 35.0, 3, 3), (60000, 84.0, 3, 3),
+                                                 (6000, 35.0, 4, 4)])
+def test_direct_sum_paths_agree(oracle, prec, n, box, nNeigh, nClust):
     """The neighbour-list and the cell-row direct sums give the same V.  The
-    two sizes build the list with the wavefront-per-target walk (small
-    sets) and the thread-per-target walk (from ~2e5 charges)."""
-    print("case direct paths", n, prec, flush=True)
+    cases build the list with each walk: a wavefront per target (up to
+    32,768 charges), a wavefront per cell (above), both also with the
+    widest stencil they take (nNeigh 3), and a thread per target (nNeigh 4)."""
+    print("case direct paths", n, prec, nNeigh, flush=True)
     coord, q = uniform_box(n, box)
-    hl = _real_setup(coord, q, prec, Rdir=7.0, neighborList="on")
-    hc = _real_setup(coord, q, prec, Rdir=7.0, neighborList="off")
+    kw = dict(nNeigh=nNeigh, nCellClust1D=nClust)
+    hl = _real_setup(coord, q, prec, Rdir=7.0, neighborList="on", **kw)
+    hc = _real_setup(coord, q, prec, Rdir=7.0, neighborList="off", **kw)
     assert "neighbour list" in hl.Print() and "cell rows" in hc.Print()
     Vl, Vc = hl.calcv(q), hc.calcv(q)
     m = oracle.compare_v(Vl, Vc)
