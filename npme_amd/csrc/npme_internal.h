@@ -218,7 +218,8 @@ struct Handle {
   void *d_buf1 = nullptr;        // M1 x M2 x Kp complex (rows of K3 or N3 values)
   void *d_buf2 = nullptr;        // M1 x N2 x Kp complex
   void *d_G = nullptr;           // k-space multiplier (N1/2+1)(N2/2+1)(K3 or N3/2+1)
-  void *d_tw[3] = {nullptr, nullptr, nullptr};  // twiddle tables per axis length
+  void *d_tw[3] = {nullptr, nullptr, nullptr};  // twiddle tables per axis length (in d_tables)
+  void *d_tables = nullptr;  // the host-built tables of a set-up, one block
   double *d_Vdir = nullptr;      // [n][4] (x2 for complex) direct sum, sorted order
   double *d_V = nullptr;         // [n][4] (x2) output staging, input order
   double *d_qin = nullptr;       // [n] (x2) input charges staging

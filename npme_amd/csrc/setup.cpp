@@ -54,7 +54,7 @@ void handle_free_device(Handle &h)
   void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
-                  h.d_tw[0], h.d_tw[1], h.d_tw[2], h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs,
+                  h.d_tables, h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs,
                   h.d_qs, h.d_qsi};
   // queued work of this handle may still read its buffers
   if (h.stream) (void)hipStreamSynchronize(h.stream);
@@ -77,6 +77,7 @@ void handle_free_device(Handle &h)
   h.d_tileStart = h.d_tileList = nullptr;
   h.d_grid = h.d_buf1 = h.d_buf2 = h.d_G = nullptr;
   h.d_tw[0] = h.d_tw[1] = h.d_tw[2] = nullptr;
+  h.d_tables = nullptr;
   h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
   h.d_xs = nullptr;
   h.deviceBytes = 0;
@@ -108,20 +109,6 @@ static std::vector<double> twiddles(int N)
     t[2 * k + 1] = -std::sin(a);
   }
   return t;
-}
-
-static void *upload_twiddles(Handle &h, int N, bool fp64)
-{
-  std::vector<double> t = twiddles(N);
-  if (fp64) {
-    double *d = dalloc<double>(h, t.size());
-    NPME_HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-    return d;
-  }
-  std::vector<float> f(t.begin(), t.end());
-  float *d = dalloc<float>(h, f.size());
-  NPME_HIP_CHECK(hipMemcpy(d, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
-  return d;
 }
 
 // top-hat smoothing of the Fourier extension (src/RecSumGrid.cpp:1124-1231)
@@ -584,9 +571,27 @@ static void grid_setup(Handle &h)
   h.d_grid = dalloc<char>(h, (h.complexK ? cs : rs) * (size_t)(M1 * M2 * M3));
   h.d_buf1 = dalloc<char>(h, cs * (size_t)(M1 * M2 * h.Kp));
   h.d_buf2 = dalloc<char>(h, cs * (size_t)(M1 * h.N[1] * h.Kp));
+  // host-built tables (twiddles of the CalcV passes in the working
+  // precision, FP64 twiddles and the extension table of the kernel grid,
+  // lambda^2) go to the device as one block in one copy
+  std::vector<char> blob;
+  auto put = [&blob](const void *src, size_t bytes) {
+    const size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, src, bytes);
+    return off;
+  };
+  size_t offTw[3], offTw64[3];
   for (int d = 0; d < 3; d++) {
     h.plan[d] = make_plan((int)h.N[d]);
-    h.d_tw[d] = upload_twiddles(h, (int)h.N[d], f64);
+    const std::vector<double> t = twiddles((int)h.N[d]);
+    offTw64[d] = put(t.data(), t.size() * sizeof(double));
+    if (f64) {
+      offTw[d] = offTw64[d];
+    } else {
+      const std::vector<float> f(t.begin(), t.end());
+      offTw[d] = put(f.data(), f.size() * sizeof(float));
+    }
   }
 
   // kernel grid: real-space compact grid of f_LR with the smooth Fourier
@@ -618,15 +623,24 @@ static void grid_setup(Handle &h)
     tab.insert(tab.end(), T2.begin(), T2.end());
   }
   // F: double per point for the real kernels, double2 for Helmholtz
-  DevBuf d_tab(tab.size() * sizeof(double)), d_F((h.complexK ? 2 : 1) * totF * sizeof(double));
-  NPME_HIP_CHECK(hipMemcpy(d_tab.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  launch_fgrid_real_space(h.ker, Mf, (const double *)d_tab.p, (double *)d_F.p, h.stream);
-  DevBuf tw64[3];
+  const size_t offTab = put(tab.data(), tab.size() * sizeof(double));
+  std::vector<double> lam;
   for (int d = 0; d < 3; d++) {
-    std::vector<double> t = twiddles((int)h.N[d]);
-    tw64[d].reset(t.size() * sizeof(double));
-    NPME_HIP_CHECK(hipMemcpy(tw64[d].p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> l(h.N[d]);
+    lambda2(h.N[d], h.Bn, l.data());
+    lam.insert(lam.end(), l.begin(), l.begin() + Mf[d]);
   }
+  const size_t offLam = put(lam.data(), lam.size() * sizeof(double));
+  h.d_tables = dalloc<char>(h, blob.size());
+  NPME_HIP_CHECK(hipMemcpyAsync(h.d_tables, blob.data(), blob.size(), hipMemcpyHostToDevice, h.stream));
+  char *tb = static_cast<char *>(h.d_tables);
+  const void *tw64[3];
+  for (int d = 0; d < 3; d++) {
+    h.d_tw[d] = tb + offTw[d];
+    tw64[d] = tb + offTw64[d];
+  }
+  DevBuf d_F((h.complexK ? 2 : 1) * totF * sizeof(double));
+  launch_fgrid_real_space(h.ker, Mf, (const double *)(tb + offTab), (double *)d_F.p, h.stream);
   // even-extension DFT along axis 3, 2, 1 of F[Mf0][Mf1][Mf2]
   double *F = (double *)d_F.p;
   auto pass = h.complexK ? fft_pass_even
@@ -635,21 +649,13 @@ static void grid_setup(Handle &h)
                              fft_pass_even_real(N, nCols, nInner, outer, inner, pos, plan, tw,
                                                 (double *)data, st);
                            };
-  pass((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], tw64[2].p, F, h.stream);
-  pass((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], tw64[1].p, F, h.stream);
-  pass((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0].p, F, h.stream);
-  std::vector<double> lam;
-  for (int d = 0; d < 3; d++) {
-    std::vector<double> l(h.N[d]);
-    lambda2(h.N[d], h.Bn, l.data());
-    lam.insert(lam.end(), l.begin(), l.begin() + Mf[d]);
-  }
-  DevBuf d_lam(lam.size() * sizeof(double));
-  NPME_HIP_CHECK(hipMemcpy(d_lam.p, lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice));
+  pass((int)h.N[2], Mf[0] * Mf[1], Mf[0] * Mf[1], 0, Mf[2], 1, h.plan[2], tw64[2], F, h.stream);
+  pass((int)h.N[1], Mf[0] * Mf[2], Mf[2], Mf[1] * Mf[2], 1, Mf[2], h.plan[1], tw64[1], F, h.stream);
+  pass((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0], F, h.stream);
   // G in blocks of 8 along axis 3 (the last block padded)
   const size_t totG = (size_t)Mf[0] * Mf[1] * ((Mf[2] + 7) / 8) * 8;
   h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * totG);
-  launch_make_G(f64, h.complexK, Mf, h.K3, F, (const double *)d_lam.p, h.d_G, h.stream);
+  launch_make_G(f64, h.complexK, Mf, h.K3, F, (const double *)(tb + offLam), h.d_G, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before the temporaries return to the pool
 }
 
