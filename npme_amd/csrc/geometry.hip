@@ -8,6 +8,7 @@
 // src/PartitionEmbeddedBox.cpp:266-455, the spline window
 // src/RecSumSupportFunctions.h:123-149.
 #include <cstring>
+#include <deque>
 
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_reduce.hpp>
@@ -272,26 +273,47 @@ static int key_bits(unsigned long maxKey)
   return b;
 }
 
+// Temporary device storage of a phase, kept until the phase's final
+// stream synchronisation: the pool is shared with other streams, so memory
+// returns to it only when the work that uses it has finished.
+using Keep = std::deque<DevBuf>;
+static void keep_or_sync(DevBuf &tmp, Keep *keep, hipStream_t s)
+{
+  if (keep) {
+    keep->emplace_back();
+    keep->back().swap(tmp);
+  } else {
+    NPME_HIP_CHECK(hipStreamSynchronize(s));  // tmp is released on return
+  }
+}
+
 // stable sort of (key, value) pairs; temporary storage from the pool
 template <typename K>
 static void sort_pairs(const K *kin, K *kout, const int *vin, int *vout, size_t m, int bits,
-                       hipStream_t s)
+                       hipStream_t s, Keep *keep = nullptr)
 {
   size_t bytes = 0;
   NPME_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, m, 0, bits, s));
   DevBuf tmp(bytes);
   NPME_HIP_CHECK(rocprim::radix_sort_pairs(tmp.p, bytes, kin, kout, vin, vout, m, 0, bits, s));
-  NPME_HIP_CHECK(hipStreamSynchronize(s));  // tmp is released on return
+  keep_or_sync(tmp, keep, s);
 }
 
 template <typename T, typename S>
-static void exclusive_scan(const T *in, S *out, size_t m, hipStream_t s)
+static void exclusive_scan(const T *in, S *out, size_t m, hipStream_t s, Keep *keep = nullptr)
 {
   size_t bytes = 0;
   NPME_HIP_CHECK(rocprim::exclusive_scan(nullptr, bytes, in, out, (S)0, m, rocprim::plus<S>(), s));
   DevBuf tmp(bytes);
   NPME_HIP_CHECK(rocprim::exclusive_scan(tmp.p, bytes, in, out, (S)0, m, rocprim::plus<S>(), s));
-  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  keep_or_sync(tmp, keep, s);
+}
+
+static void throw_geo_err(int e)
+{
+  if (e & GEO_ERR_CELL) throw Error{NPME_ERR_ARG, "coordinate outside the cell box"};
+  if (e & GEO_ERR_WINDOW) throw Error{NPME_ERR_GRID, "spline window outside the compact grid"};
+  if (e & GEO_ERR_FIXED) throw Error{NPME_ERR_ARG, "position outside its cluster"};
 }
 
 static void check_err(const int *d_err, hipStream_t s)
@@ -313,11 +335,13 @@ void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, 
   hipLaunchKernelGGL(k_cell_keys, grid_for(n), dim3(256), 0, s, n, d_coord, g,
                      (unsigned long *)key.p, (int *)idx.p, (int4 *)gcell.p, (int *)err.p);
   NPME_HIP_CHECK(hipGetLastError());
-  check_err((const int *)err.p, s);
+  // a charge outside the box gets key 0 and cell 0, so the sort and the
+  // ranges stay in bounds; the error is raised after the phase
+  Keep keep;
   const unsigned long Nc = g.nP * g.nP * g.nP;
   const unsigned long maxKey = g.nA[0] * g.nA[1] * g.nA[2] * Nc;
   sort_pairs((const unsigned long *)key.p, (unsigned long *)skey.p, (const int *)idx.p, d_perm,
-             n, key_bits(maxKey), s);
+             n, key_bits(maxKey), s, &keep);
   const long ncell = (long)g.nb[0] * g.nb[1] * g.nb[2];
   hipLaunchKernelGGL(k_cell_ranges, grid_for(ncell), dim3(256), 0, s, ncell, g,
                      (const unsigned long *)skey.p, n, d_cellSE);
@@ -325,7 +349,7 @@ void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, 
   hipLaunchKernelGGL(k_gather_cells, grid_for(n), dim3(256), 0, s, n, (const int *)d_perm,
                      (const int4 *)gcell.p, d_cellOf);
   NPME_HIP_CHECK(hipGetLastError());
-  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  check_err((const int *)err.p, s);  // synchronises the stream
 }
 
 void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const double *d_coord,
@@ -345,14 +369,18 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
                        d_cellOf, g, d_src, (float4 *)d_srcRes, d_ms, d_frac, (int *)ntile.p,
                        (int *)err.p);
   NPME_HIP_CHECK(hipGetLastError());
-  check_err((const int *)err.p, s);
+  Keep keep;
 
-  // spreading lists: emit (tile, charge) in charge order, stable sort by tile
-  exclusive_scan((const int *)ntile.p, (int *)off.p, n, s);
-  int last[2] = {0, 0};
+  // spreading lists: emit (tile, charge) in charge order, stable sort by
+  // tile.  The error flag comes back with the list size, before any kernel
+  // uses the windows.
+  exclusive_scan((const int *)ntile.p, (int *)off.p, n, s, &keep);
+  int last[3] = {0, 0, 0};
   NPME_HIP_CHECK(hipMemcpyAsync(&last[0], (int *)off.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
   NPME_HIP_CHECK(hipMemcpyAsync(&last[1], (int *)ntile.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipMemcpyAsync(&last[2], err.p, sizeof(int), hipMemcpyDeviceToHost, s));
   NPME_HIP_CHECK(hipStreamSynchronize(s));
+  throw_geo_err(last[2]);
   const size_t m = (size_t)last[0] + (size_t)last[1];
   nTileEntries = m;
   {
@@ -363,12 +391,15 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
     NPME_HIP_CHECK(hipGetLastError());
     const long ntiles = (long)g.nt[0] * g.nt[1] * g.nt[2];
     sort_pairs((const unsigned *)tk.p, (unsigned *)sk.p, (const int *)tv.p, (int *)tileList.p, m,
-               key_bits((unsigned long)ntiles), s);
+               key_bits((unsigned long)ntiles), s, &keep);
     tileStart.reset((ntiles + 1) * sizeof(int));
     hipLaunchKernelGGL(k_starts, grid_for(ntiles + 1), dim3(256), 0, s, ntiles,
                        (const unsigned *)sk.p, m, (int *)tileStart.p);
     NPME_HIP_CHECK(hipGetLastError());
-    NPME_HIP_CHECK(hipStreamSynchronize(s));  // before the scratch returns to the pool
+    for (DevBuf *b : {&tk, &tv, &sk}) {
+      keep.emplace_back();
+      keep.back().swap(*b);
+    }
   }
   // interpolation lists: one tile per charge
   {
@@ -379,12 +410,12 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
     NPME_HIP_CHECK(hipGetLastError());
     const long nit = (long)g.nti[0] * g.nti[1] * g.nti[2];
     sort_pairs((const unsigned *)ik.p, (unsigned *)sk.p, (const int *)iv.p, (int *)itileList.p, n,
-               key_bits((unsigned long)nit), s);
+               key_bits((unsigned long)nit), s, &keep);
     itileStart.reset((nit + 1) * sizeof(int));
     hipLaunchKernelGGL(k_starts, grid_for(nit + 1), dim3(256), 0, s, nit, (const unsigned *)sk.p,
                        n, (int *)itileStart.p);
     NPME_HIP_CHECK(hipGetLastError());
-    NPME_HIP_CHECK(hipStreamSynchronize(s));
+    NPME_HIP_CHECK(hipStreamSynchronize(s));  // before the scratch returns to the pool
   }
 }
 

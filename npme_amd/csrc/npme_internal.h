@@ -8,6 +8,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/npme_amd.h"
@@ -131,6 +132,11 @@ struct DevBuf {
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { dev_free(p); }
+  void swap(DevBuf &o)
+  {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
   void reset(size_t b)
   {
     dev_free(p);
