@@ -383,6 +383,7 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  set-up ms: geometry " << H.setupMs[0] << ", free " << H.setupMs[1] << ", cells "
      << H.setupMs[2] << ", list " << H.setupMs[3] << ", charges " << H.setupMs[4] << ", grid "
      << H.setupMs[5] << "\n";
+  os << "  kernel grid G: " << (H.gReused ? "reused from the previous set-up" : "computed") << "\n";
   if (H.useNlist)
     os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";
   else

@@ -164,9 +164,33 @@ FFTPlan1D make_plan(int N);
 struct Handle;
 
 // one CalcV pipeline instance (device buffers for one geometry)
+// everything the kernel multiplier G depends on: the grid, the box extent
+// (not its centre), the smooth extension and the kernel
+struct GridKey {
+  long N[3] = {0, 0, 0};
+  double X[3] = {0, 0, 0}, del[3] = {0, 0, 0}, ta[3] = {0, 0, 0};
+  long Bn = 0;
+  bool fp64 = false, complexK = false;
+  int kind = -1, Nder = 0;
+  double alpha = 0, k0r = 0, k0i = 0, Rdir = 0, beta = 0;
+  bool operator==(const GridKey &o) const
+  {
+    for (int d = 0; d < 3; d++)
+      if (N[d] != o.N[d] || X[d] != o.X[d] || del[d] != o.del[d] || ta[d] != o.ta[d]) return false;
+    return Bn == o.Bn && fp64 == o.fp64 && complexK == o.complexK && kind == o.kind &&
+           Nder == o.Nder && alpha == o.alpha && k0r == o.k0r && k0i == o.k0i && Rdir == o.Rdir &&
+           beta == o.beta;
+  }
+};
+
 struct Handle {
   std::mutex mtx;
   bool ready = false;  // the last set-up completed (device state valid)
+  // G of the previous set-up, kept by a new set-up of the same handle and
+  // reused when its key matches (ResetCoord of a box of the same size)
+  GridKey gKey;
+  bool gValid = false, gReused = false;
+  size_t gBytes = 0;
   // configuration
   Keywords kw;
   DMKernel ker;
@@ -288,7 +312,9 @@ void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, do
 // candidate pairs of the cell stencil: sum over cells of count * (counts of
 // the (2 nNeigh + 1)^3 cells around it)
 double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellSE, hipStream_t s);
-void handle_free_device(Handle &h);
+// releases the device buffers of a set-up; keepG keeps G and its key for
+// the next set-up of the same handle
+void handle_free_device(Handle &h, bool keepG = false);
 // destroys the handle's streams and timing events (after handle_free_device)
 void handle_destroy_streams(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);

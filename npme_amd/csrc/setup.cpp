@@ -49,17 +49,23 @@ template <typename T> static T *dalloc(Handle &h, size_t count)
   return (T *)p;
 }
 
-void handle_free_device(Handle &h)
+void handle_free_device(Handle &h, bool keepG)
 {
   void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
-                  h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2, h.d_G,
+                  h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2,
                   h.d_tables, h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs,
                   h.d_qs, h.d_qsi};
   // queued work of this handle may still read its buffers
   if (h.stream) (void)hipStreamSynchronize(h.stream);
   if (h.stream2) (void)hipStreamSynchronize(h.stream2);
   for (void *p : ptrs) dev_free(p);
+  if (!keepG) {
+    dev_free(h.d_G);
+    h.d_G = nullptr;
+    h.gValid = false;
+    h.gBytes = 0;
+  }
   h.d_perm = nullptr;
   h.d_src = h.d_srcRes = h.d_qi = h.d_blockBox = nullptr;
   h.d_qs = h.d_qsi = nullptr;
@@ -75,12 +81,12 @@ void handle_free_device(Handle &h)
   h.d_ms = nullptr;
   h.d_frac = nullptr;
   h.d_tileStart = h.d_tileList = nullptr;
-  h.d_grid = h.d_buf1 = h.d_buf2 = h.d_G = nullptr;
+  h.d_grid = h.d_buf1 = h.d_buf2 = nullptr;
   h.d_tw[0] = h.d_tw[1] = h.d_tw[2] = nullptr;
   h.d_tables = nullptr;
   h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
   h.d_xs = nullptr;
-  h.deviceBytes = 0;
+  h.deviceBytes = h.d_G ? h.gBytes : 0;
 }
 
 // ---------------------------------------------------------------- cells
@@ -594,6 +600,39 @@ static void grid_setup(Handle &h)
     }
   }
 
+  // G of the previous set-up is reused when everything it depends on is
+  // unchanged (a ResetCoord that keeps the box size)
+  GridKey key;
+  for (int d = 0; d < 3; d++) {
+    key.N[d] = h.N[d];
+    key.X[d] = h.X[d];
+    key.del[d] = h.del[d];
+    key.ta[d] = h.ta[d];
+  }
+  key.Bn = h.Bn;
+  key.fp64 = f64;
+  key.complexK = h.complexK;
+  key.kind = h.ker.kind;
+  key.Nder = h.ker.Nder;
+  key.alpha = h.ker.alpha;
+  key.k0r = h.ker.k0r;
+  key.k0i = h.ker.k0i;
+  key.Rdir = h.ker.Rdir;
+  key.beta = h.ker.beta;
+  h.gReused = h.gValid && h.d_G && key == h.gKey;
+  if (h.gReused) {
+    h.d_tables = dalloc<char>(h, blob.size());
+    NPME_HIP_CHECK(hipMemcpyAsync(h.d_tables, blob.data(), blob.size(), hipMemcpyHostToDevice, h.stream));
+    for (int d = 0; d < 3; d++) h.d_tw[d] = static_cast<char *>(h.d_tables) + offTw[d];
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before blob is released
+    return;
+  }
+  dev_free(h.d_G);
+  h.d_G = nullptr;
+  h.deviceBytes -= h.gBytes;
+  h.gValid = false;
+  h.gBytes = 0;
+
   // kernel grid: real-space compact grid of f_LR with the smooth Fourier
   // extension, then an even-extension DFT along each axis, all FP64
   const long Mf[3] = {h.N[0] / 2 + 1, h.N[1] / 2 + 1, h.N[2] / 2 + 1};
@@ -654,9 +693,13 @@ static void grid_setup(Handle &h)
   pass((int)h.N[0], Mf[1] * Mf[2], Mf[1] * Mf[2], 0, 1, Mf[1] * Mf[2], h.plan[0], tw64[0], F, h.stream);
   // G in blocks of 8 along axis 3 (the last block padded)
   const size_t totG = (size_t)Mf[0] * Mf[1] * ((Mf[2] + 7) / 8) * 8;
+  const size_t before = h.deviceBytes;
   h.d_G = dalloc<char>(h, (h.complexK ? cs : rs) * totG);
+  h.gBytes = h.deviceBytes - before;
   launch_make_G(f64, h.complexK, Mf, h.K3, F, (const double *)(tb + offLam), h.d_G, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before the temporaries return to the pool
+  h.gKey = key;
+  h.gValid = true;
 }
 
 // ---------------------------------------------------------------- set-up
@@ -765,7 +808,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
   if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  handle_free_device(h);
+  handle_free_device(h, true);
   std::string smode = "device", nlmode = "auto";
   kw.get("setup", smode);
   kw.get("neighborList", nlmode);

@@ -167,6 +167,35 @@ def test_setup_rejects_non_finite_coordinates(mode, bad):
     assert np.all(np.isfinite(h.calcv(q)))
 
 
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_reset_coord_reuses_kernel_grid(prec):
+    """ResetCoord to a translated geometry keeps the box size, grid and
+    kernel, so G is reused; the result is bit-identical to a fresh set-up
+    of the translated charges.  A stretched geometry recomputes G."""
+    import npme_amd as N
+    coord, q = uniform_box(5000, 30.0)
+    # positions on a grid of 2^-20, so the translated box has exactly the
+    # same size (the shifted coordinates are exact)
+    coord = np.round(coord * 2.0 ** 20) / 2.0 ** 20
+    h = _real_setup(coord, q, prec, Rdir=6.0)
+    assert "kernel grid G: computed" in h.Print()
+    c2 = coord + np.array([5.0, -3.0, 2.5])
+    assert h.ResetCoord(len(q), c2)
+    assert "kernel grid G: reused" in h.Print()
+    V = h.calcv(q)
+    k = _kw(calcType="pme", funcType="Laplace", Rdir=6.0, BnOrder=6, tol=1e-6, precision=prec)
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+    fresh = N.NPME_Interface_SetUpRealKernel(k, len(q), c2)
+    assert fresh is not None and "kernel grid G: computed" in fresh.Print()
+    assert _grid(fresh) == _grid(h)
+    assert np.array_equal(V, fresh.calcv(q))
+    c3 = coord * np.array([1.0, 1.0, 1.05])
+    assert h.ResetCoord(len(q), c3)
+    assert "kernel grid G: computed" in h.Print()
+    fresh3 = N.NPME_Interface_SetUpRealKernel(k, len(q), c3)
+    assert np.array_equal(h.calcv(q), fresh3.calcv(q))
+
+
 def test_calcv_rejects_wrong_size():
     import npme_amd as N
     coord, q = uniform_box(2000, 20.0)
