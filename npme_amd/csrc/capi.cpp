@@ -104,7 +104,8 @@ int npme_add_missing_keywords(npme_keywords *kwp, size_t nCharge, int isChargeRe
 namespace npme {
 
 void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const double *coord,
-                          const double *chargeReal, const double *chargeComplex)
+                          const double *chargeReal, const double *chargeComplex,
+                          const ChargeStats *stats)
 {
   {
     std::string s, funcType, calcType, split;
@@ -167,18 +168,20 @@ void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const
       long N[3], nb[3];
       int nDeriv = -1, nd = 0;
       const bool haveNder = kw.get("nDeriv", nDeriv);
-      if (funcType == "Helmholtz") {
-        if (!chargeComplex) throw Error{NPME_ERR_ARG, "complex charges required for the grid choice"};
+      const bool cx = funcType == "Helmholtz";
+      if (cx && !chargeComplex) throw Error{NPME_ERR_ARG, "complex charges required for the grid choice"};
+      if (!cx && !chargeReal) throw Error{NPME_ERR_ARG, "real charges required for the grid choice"};
+      ChargeStats own;
+      if (!stats) charge_stats(nCharge, coord, cx ? chargeComplex : chargeReal, cx, own);
+      const ChargeStats &st = stats ? *stats : own;
+      if (cx) {
         std::string err;
-        if (!rsem_helmholtz(nCharge, chargeComplex, coord, Bn, k0r, k0i, tol, Rdir,
-                            haveNder ? nDeriv : -1, N, nb, nd, err))
+        if (!rsem_helmholtz(st, Bn, k0r, k0i, tol, Rdir, haveNder ? nDeriv : -1, N, nb, nd, err))
           throw Error{NPME_ERR_KEYWORD, err};
       } else {
-        if (!chargeReal) throw Error{NPME_ERR_ARG, "real charges required for the grid choice"};
         double alpha = -1.0;
         if (funcType == "Ralpha") kw.get("alpha", alpha);
-        if (!rsem_static(nCharge, chargeReal, coord, Bn, alpha, tol, Rdir, haveNder ? nDeriv : -1,
-                         N, nb, nd)) {
+        if (!rsem_static(st, Bn, alpha, tol, Rdir, haveNder ? nDeriv : -1, N, nb, nd)) {
           std::ostringstream os;
           os << "grid estimate: no error-model parameters for BnOrder = " << Bn
              << " alpha = " << alpha << (haveNder ? " nDeriv = " + std::to_string(nDeriv) : "");

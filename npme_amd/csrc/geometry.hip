@@ -497,16 +497,28 @@ __global__ __launch_bounds__(256) void k_bounds_final(int np, const double *__re
   if (threadIdx.x < 6) out[threadIdx.x] = sh[threadIdx.x][0];
 }
 
+size_t coord_bounds_scratch(size_t n)
+{
+  return (6 * std::min<size_t>(BOUNDS_WG, (n + 255) / 256) + 6) * sizeof(double);
+}
+
+void coord_bounds_launch(size_t n, const double *d_coord, double *d_scratch, hipStream_t s)
+{
+  const int nb = (int)std::min<size_t>(BOUNDS_WG, (n + 255) / 256);
+  hipLaunchKernelGGL(k_coord_bounds, dim3(nb), dim3(256), 0, s, n, d_coord, d_scratch);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, nb, (const double *)d_scratch,
+                     d_scratch + 6 * nb);
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
 void coord_bounds_device(size_t n, const double *d_coord, double mn[3], double mx[3], hipStream_t s)
 {
   const int nb = (int)std::min<size_t>(BOUNDS_WG, (n + 255) / 256);
-  DevBuf part((size_t)6 * nb * sizeof(double)), out(6 * sizeof(double));
-  hipLaunchKernelGGL(k_coord_bounds, dim3(nb), dim3(256), 0, s, n, d_coord, (double *)part.p);
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, nb, (const double *)part.p,
-                     (double *)out.p);
-  NPME_HIP_CHECK(hipGetLastError());
+  DevBuf scratch(coord_bounds_scratch(n));
+  coord_bounds_launch(n, d_coord, (double *)scratch.p, s);
+  const double *out = (const double *)scratch.p + 6 * nb;
   double v[6];
-  NPME_HIP_CHECK(hipMemcpyAsync(v, out.p, sizeof(v), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipMemcpyAsync(v, out, sizeof(v), hipMemcpyDeviceToHost, s));
   NPME_HIP_CHECK(hipStreamSynchronize(s));
   for (int d = 0; d < 3; d++) {
     mn[d] = v[d];

@@ -240,9 +240,18 @@ void box_dim_center(size_t n, const double *coord, double X0[3], double R0[3])
   }
 }
 
+void box_from_bounds(const double mn[3], const double mx[3], double X0[3], double R0[3])
+{
+  for (int d = 0; d < 3; d++) {
+    const double hi = std::max(-1.0e12, mx[d]), lo = std::min(1.0e12, mn[d]);
+    X0[d] = hi - lo;
+    R0[d] = 0.5 * (hi + lo);
+  }
+}
+
 // sum of squares of m values in four interleaved partial sums, so the
 // loop is not bound by the latency of a single chain of additions
-static double sum_sq(size_t m, const double *v)
+double sum_squares(size_t m, const double *v)
 {
   double s[4] = {0, 0, 0, 0};
   size_t i = 0;
@@ -320,12 +329,17 @@ static void rsem_iterate(const double X0[3], const double R0[3], double Q0, long
   for (int d = 0; d < 3; d++) fft_size_block(N[d], 20, N[d], nblk[d]);
 }
 
-bool rsem_static(size_t n, const double *q, const double *coord, long Bn, double alpha, double tol,
-                 double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder)
+void charge_stats(size_t n, const double *coord, const double *q, bool complexQ, ChargeStats &st)
 {
-  const double Q0 = std::sqrt(sum_sq(n, q));
-  double X0[3], R0[3];
-  box_dim_center(n, coord, X0, R0);
+  box_dim_center(n, coord, st.X0, st.R0);
+  st.Q0 = std::sqrt(sum_squares((complexQ ? 2 : 1) * n, q));
+}
+
+bool rsem_static(const ChargeStats &st, long Bn, double alpha, double tol, double Rdir,
+                 int nDerivIn, long N[3], long nblk[3], int &Nder)
+{
+  const double Q0 = st.Q0;
+  const double *X0 = st.X0, *R0 = st.R0;
   bool found = false;
   long best = 0;
   for (int r = 0; r < NPME_RSEM_N_STATIC; r++) {
@@ -361,7 +375,7 @@ bool rsem_static_pred_error(size_t n, const double *q, const double *coord, long
     }
   }
   if (!row) return false;
-  const double Q0 = std::sqrt(sum_sq(n, q));
+  const double Q0 = std::sqrt(sum_squares(n, q));
   double X0[3], R0[3], X[3], R[3], dl[3] = {Rdir, Rdir, Rdir}, L[3];
   box_dim_center(n, coord, X0, R0);
   pme_corr_box(X0, R0, N, dl, Bn, X, R);
@@ -383,13 +397,12 @@ bool rsem_static_pred_error(size_t n, const double *q, const double *coord, long
   return true;
 }
 
-bool rsem_helmholtz(size_t n, const double *qc, const double *coord, long Bn, double k0r,
-                    double k0i, double tol, double Rdir, int nDerivIn, long N[3], long nblk[3],
-                    int &Nder, std::string &err)
+bool rsem_helmholtz(const ChargeStats &st, long Bn, double k0r, double k0i, double tol,
+                    double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder,
+                    std::string &err)
 {
-  const double Q0 = std::sqrt(sum_sq(2 * n, qc));
-  double X0[3], R0[3];
-  box_dim_center(n, coord, X0, R0);
+  const double Q0 = st.Q0;
+  const double *X0 = st.X0, *R0 = st.R0;
   const double w = std::hypot(k0r, k0i) * Rdir;
   bool found = false;
   long best = 0;

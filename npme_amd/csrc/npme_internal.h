@@ -93,13 +93,23 @@ void fft_size_block(long Nideal, long nideal, long &N, long &nblk);
 void box_dim_center(size_t n, const double *coord, double X0[3], double R0[3]);
 void pme_corr_box(const double X0[3], const double R0[3], const long N[3], const double del[3],
                   long Bn, double X[3], double R[3]);
-bool rsem_static(size_t n, const double *q, const double *coord, long Bn, double alpha, double tol,
-                 double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder);
+// what the RSEM grid choice reads of the charges: box size and centre
+// (box_dim_center) and the norm of the charges, sqrt(sum |q|^2)
+struct ChargeStats {
+  double X0[3], R0[3];
+  double Q0;
+};
+void charge_stats(size_t n, const double *coord, const double *q, bool complexQ, ChargeStats &st);
+// box size and centre from exact per-axis bounds (box_dim_center's seeds)
+void box_from_bounds(const double mn[3], const double mx[3], double X0[3], double R0[3]);
+double sum_squares(size_t m, const double *v);
+bool rsem_static(const ChargeStats &st, long Bn, double alpha, double tol, double Rdir,
+                 int nDerivIn, long N[3], long nblk[3], int &Nder);
 bool rsem_static_pred_error(size_t n, const double *q, const double *coord, long Bn, double alpha,
                             int Nder, double Rdir, const long N[3], double &eps, bool &reliable);
-bool rsem_helmholtz(size_t n, const double *qc, const double *coord, long Bn, double k0r,
-                    double k0i, double tol, double Rdir, int nDerivIn, long N[3], long nblk[3],
-                    int &Nder, std::string &err);
+bool rsem_helmholtz(const ChargeStats &st, long Bn, double k0r, double k0i, double tol,
+                    double Rdir, int nDerivIn, long N[3], long nblk[3], int &Nder,
+                    std::string &err);
 void lambda2(long N, long Bn, double *lam2);
 
 // ------------------------------------------------------------- keywords
@@ -191,6 +201,12 @@ struct Handle {
   GridKey gKey;
   bool gValid = false, gReused = false;
   size_t gBytes = 0;
+  // positions copied to the device ahead of a set-up (handle_stage_coords):
+  // the set-up of the same n uses them and their bounds
+  bool coordStaged = false, stagedBoundsReady = false;
+  size_t stagedN = 0;
+  double stagedMn[3] = {0, 0, 0}, stagedMx[3] = {0, 0, 0};
+  DevBuf stageBuf;
   // configuration
   Keywords kw;
   DMKernel ker;
@@ -273,7 +289,8 @@ struct Handle {
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord);
 // NPME_AddMissingKeywords: defaults, validation, RSEM grid (throws Error)
 void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const double *coord,
-                          const double *chargeReal, const double *chargeComplex);
+                          const double *chargeReal, const double *chargeComplex,
+                          const ChargeStats *stats = nullptr);
 
 // geometry of the cell partition and the grids, shared by the host and the
 // device set-up (geometry.hip)
@@ -307,6 +324,10 @@ size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *
 long scan_words_device(const long *d_in, long *d_out, size_t m, hipStream_t s);
 // exact per-axis bounds of the n positions on the device (NaN if any is NaN)
 void coord_bounds_device(size_t n, const double *d_coord, double mn[3], double mx[3], hipStream_t s);
+// the same queued without waiting: scratch of coord_bounds_scratch(n)
+// bytes; the six results (minima, maxima) are its last six doubles
+size_t coord_bounds_scratch(size_t n);
+void coord_bounds_launch(size_t n, const double *d_coord, double *d_scratch, hipStream_t s);
 void sorted_coords_device(size_t n, const int *d_perm, const double *d_coord, double4 *d_xs,
                           hipStream_t s);
 // candidate pairs of the cell stencil: sum over cells of count * (counts of
@@ -315,6 +336,12 @@ double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellS
 // releases the device buffers of a set-up; keepG keeps G and its key for
 // the next set-up of the same handle
 void handle_free_device(Handle &h, bool keepG = false);
+// copies n positions to the device and queues their bounds, without
+// waiting, for the next handle_setup of the same positions (the caller
+// computes what else it needs meanwhile); releases the previous set-up
+void handle_stage_coords(Handle &h, size_t n, const double *coord);
+// waits for the staged positions' bounds
+void handle_staged_bounds(Handle &h, double mn[3], double mx[3]);
 // destroys the handle's streams and timing events (after handle_free_device)
 void handle_destroy_streams(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);

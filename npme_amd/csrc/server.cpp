@@ -81,25 +81,26 @@ static size_t estimate_bytes(size_t n, bool complexK, bool fp64)
 // model at the same tolerance.  For alpha = 0.5 and 1.5 (N = 4000, density
 // 0.1, Rdir 8) its grids reach relative RMS errors 100x below the requested
 // 1e-4 and 1e-6 against the exact sum, i.e. the substitute is conservative.
-static void add_keywords_for_request(Keywords &kw, const Job &j)
+static void add_keywords_for_request(Keywords &kw, const Job &j, const ChargeStats &st)
 {
   const bool cx = j.funcType == "Helmholtz";
   try {
-    add_missing_keywords(kw, j.n, !cx, j.coord, cx ? nullptr : j.charge, cx ? j.charge : nullptr);
+    add_missing_keywords(kw, j.n, !cx, j.coord, cx ? nullptr : j.charge, cx ? j.charge : nullptr,
+                         &st);
   } catch (const Error &) {
     if (j.funcType != "Ralpha") throw;
     std::string s;
     if (kw.get("N1", s)) throw;  // the grid was given: a genuine error
     Keywords k2 = kw;
     k2.set("alpha", 1.0);
-    add_missing_keywords(k2, j.n, true, j.coord, j.charge, nullptr);
+    add_missing_keywords(k2, j.n, true, j.coord, j.charge, nullptr, &st);
     const char *keys[] = {"N1", "N2", "N3", "n1", "n2", "n3", "nDeriv"};
     for (const char *k : keys) {
       long v;
       k2.get(k, v);
       kw.set(k, v);
     }
-    add_missing_keywords(kw, j.n, true, j.coord, j.charge, nullptr);
+    add_missing_keywords(kw, j.n, true, j.coord, j.charge, nullptr, &st);
   }
 }
 
@@ -123,9 +124,22 @@ static void run_job(Handle &h, Job &j)
     ms = std::chrono::duration<double, std::milli>(t1 - t).count();
     t = t1;
   };
-  add_keywords_for_request(kw, j);
-  lap(j.ms[0]);
+  // the positions go to the device first and their bounds are reduced
+  // there while the host sums the charges' squares; the grid choice then
+  // reads no position on the host, and the set-up uses the staged copy
   std::lock_guard<std::mutex> lk(h.mtx);
+  const bool cxq = j.funcType == "Helmholtz";
+  handle_stage_coords(h, j.n, j.coord);
+  ChargeStats st;
+  st.Q0 = std::sqrt(sum_squares((cxq ? 2 : 1) * j.n, j.charge));
+  double mn[3], mx[3];
+  handle_staged_bounds(h, mn, mx);
+  for (int d = 0; d < 3; d++)
+    if (!std::isfinite(mn[d]) || !std::isfinite(mx[d]))
+      throw Error{NPME_ERR_ARG, "coordinates must be finite"};
+  box_from_bounds(mn, mx, st.X0, st.R0);
+  add_keywords_for_request(kw, j, st);
+  lap(j.ms[0]);
   handle_setup(h, kw, j.n, j.coord);
   for (int d = 0; d < 3; d++) j.N[d] = h.N[d];
   j.deviceGB = h.deviceBytes / 1e9;

@@ -86,7 +86,52 @@ void handle_free_device(Handle &h, bool keepG)
   h.d_tables = nullptr;
   h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
   h.d_xs = nullptr;
+  h.coordStaged = h.stagedBoundsReady = false;
   h.deviceBytes = h.d_G ? h.gBytes : 0;
+}
+
+static void ensure_streams(Handle &h)
+{
+  NPME_HIP_CHECK(hipGetDevice(&h.device));
+  if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
+  if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
+}
+
+void handle_stage_coords(Handle &h, size_t n, const double *coord)
+{
+  if (n < 2 || !coord) throw Error{NPME_ERR_ARG, "staged positions: nCharge must be at least 2"};
+  ensure_streams(h);
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  handle_free_device(h, true);
+  h.ready = false;
+  h.d_coordS = dalloc<double>(h, 3 * n);
+  NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                                h.stream));
+  h.stageBuf.reset(coord_bounds_scratch(n));
+  coord_bounds_launch(n, h.d_coordS, (double *)h.stageBuf.p, h.stream);
+  h.stagedN = n;
+  h.coordStaged = true;
+  h.stagedBoundsReady = false;
+}
+
+void handle_staged_bounds(Handle &h, double mn[3], double mx[3])
+{
+  if (!h.coordStaged) throw Error{NPME_ERR_ARG, "no staged positions"};
+  if (!h.stagedBoundsReady) {
+    double v[6];
+    const double *out = (const double *)((const char *)h.stageBuf.p + h.stageBuf.bytes) - 6;
+    NPME_HIP_CHECK(hipMemcpyAsync(v, out, sizeof(v), hipMemcpyDeviceToHost, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+    for (int d = 0; d < 3; d++) {
+      h.stagedMn[d] = v[d];
+      h.stagedMx[d] = v[3 + d];
+    }
+    h.stagedBoundsReady = true;
+  }
+  for (int d = 0; d < 3; d++) {
+    mn[d] = h.stagedMn[d];
+    mx[d] = h.stagedMx[d];
+  }
 }
 
 // ---------------------------------------------------------------- cells
@@ -804,11 +849,8 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   h.Kp = (h.K3 + 7) / 8 * 8;
 
   // ---- streams; the device memory of an earlier set-up is released
-  NPME_HIP_CHECK(hipGetDevice(&h.device));
-  if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
-  if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
-  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
-  handle_free_device(h, true);
+  // (positions staged for this set-up are kept)
+  ensure_streams(h);
   std::string smode = "device", nlmode = "auto";
   kw.get("setup", smode);
   kw.get("neighborList", nlmode);
@@ -817,12 +859,20 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     kw_error("neighborList must be 'auto', 'on' or 'off'");
   const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
   const bool devSetup = smode == "device" && nlPossible;
+  const bool staged = devSetup && h.coordStaged && h.stagedN == n;
+  if (!staged) {
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+    handle_free_device(h, true);
+  }
   h.setupMs[1] = ms_since(t0);
 
   // ---- bounds of the charges: on the device for the device set-up (the
   // positions go there first), else on the host; both exact
   double mn[3], mx[3];
-  if (devSetup) {
+  if (staged) {
+    handle_staged_bounds(h, mn, mx);
+    h.coordStaged = h.stagedBoundsReady = false;
+  } else if (devSetup) {
     h.d_coordS = dalloc<double>(h, 3 * n);
     NPME_HIP_CHECK(hipMemcpyAsync(h.d_coordS, coord, 3 * n * sizeof(double), hipMemcpyHostToDevice,
                                   h.stream));
@@ -846,11 +896,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   // ---- geometry of the reciprocal sum: box size and centre as
   // src/SupportFunctions.cpp NPME_FindBoxDimCenter (extremes seeded with
   // -/+1e12)
-  for (int d = 0; d < 3; d++) {
-    const double hi = std::max(-1.0e12, mx[d]), lo = std::min(1.0e12, mn[d]);
-    h.X0[d] = hi - lo;
-    h.R0[d] = 0.5 * (hi + lo);
-  }
+  box_from_bounds(mn, mx, h.X0, h.R0);
   pme_corr_box(h.X0, h.R0, h.N, h.del, h.Bn, h.X, h.R);
   for (int d = 0; d < 3; d++) h.L[d] = 2 * (h.X[d] + h.del[d]);
 

@@ -133,3 +133,36 @@ def test_server_pinned_request_buffers():
             assert all(v >= 0 for v in info["ms"].values())
     finally:
         S.close()
+
+
+def test_server_request_equals_library_call():
+    """A served request (positions staged on the device, grid chosen from
+    their device bounds) gives bit for bit the V of the library's own
+    NPME_AddMissingKeywords + SetUp + CalcV with the same keywords."""
+    import npme_amd as N
+    rng = np.random.default_rng(41)
+    S = N.NPME_Server(nSlots=2)
+    try:
+        for n, ft, kw, tol in [(7000, "Laplace", {}, 1e-6), (15000, "Helmholtz", {"k0": 0.3}, 1e-4)]:
+            print("case server vs library", ft, n, flush=True)
+            L = (n / 0.1) ** (1.0 / 3.0)
+            coord = rng.uniform(0.0, L, (n, 3))
+            if ft == "Helmholtz":
+                q = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+            else:
+                q = rng.uniform(-1.0, 1.0, n)
+            V, info = S.wait(S.submit(ft, coord, q, tol=tol, Rdir=8.0, **kw))
+            k = N.NPME_KeywordInput(calcType="pme", funcType=ft, Rdir=8.0, tol=tol, precision="mixed")
+            if ft == "Helmholtz":
+                k.AddKeyword("k0_r", 0.3)
+                k.AddKeyword("k0_i", 0.0)
+                assert N.NPME_AddMissingKeywords(k, n, coord, chargeComplex=q)
+                h = N.NPME_Interface_SetUpComplexKernel(k, n, coord)
+            else:
+                assert N.NPME_AddMissingKeywords(k, n, coord, chargeReal=q)
+                h = N.NPME_Interface_SetUpRealKernel(k, n, coord)
+            assert h is not None, N.npme_last_error()
+            assert tuple(info["grid"]) == (h.Get_N1(), h.Get_N2(), h.Get_N3())
+            assert np.array_equal(V, h.calcv(q))
+    finally:
+        S.close()
