@@ -769,6 +769,10 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
   for (int tb = 0; tb < home.y; tb += 64) {
     const int mt = min(64, home.y - tb);
     const int s0 = home.x + tb;
+    // lane i: list offset of target s0 + i (entries of the target's group
+    // column start there), taken per target with readlane
+    const int si = s0 + min(lane, mt - 1);
+    const long obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
     int kreg = 0;
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
@@ -780,9 +784,8 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
         const unsigned long long m = mrow[i];
         if (m == 0) continue;  // uniform
         const int ki = __builtin_amdgcn_readlane(kreg, i);
+        const long ob = (long)readlane_u64((unsigned long long)obase, i);
         if ((m >> lane) & 1ull) {
-          const int s = s0 + i;
-          const long ob = sliceStart[s >> 6] + 4 * (long)(s & 63);
           const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)ki));
           list[ob + 256L * (pos >> 2) + (pos & 3)] = ent;
