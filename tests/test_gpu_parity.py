@@ -225,6 +225,20 @@ def test_direct_sum_paths_agree(oracle, prec, n, box, nNeigh, nClust):
     assert m["relV"] < (1e-13 if prec == "fp64" else 2e-7) and m["reldV"] < (1e-13 if prec == "fp64" else 1e-6), m
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_direct_sum_paths_agree_dense_cells(oracle, prec):
+    """Dense blobs above the small-set size: cells hold more than 64 charges,
+    so the wavefront-per-cell list walk takes its targets in several
+    batches; the list sum equals the cell-row sum."""
+    coord, q = _blobs(40000, 8, 60.0, 3.0, 43)
+    print("case direct paths dense cells", prec, flush=True)
+    hl = _real_setup(coord, q, prec, Rdir=5.0, neighborList="on")
+    hc = _real_setup(coord, q, prec, Rdir=5.0, neighborList="off")
+    assert "neighbour list" in hl.Print() and "cell rows" in hc.Print()
+    m = oracle.compare_v(hl.calcv(q), hc.calcv(q))
+    assert m["relV"] < (1e-13 if prec == "fp64" else 2e-7) and m["reldV"] < (1e-13 if prec == "fp64" else 1e-6), m
+
+
 def _blobs(n, nblob, box, sigma, seed):
     rng = np.random.default_rng(seed)
     cen = rng.uniform(0, box, (nblob, 3))
