@@ -1584,27 +1584,44 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
   constexpr int W = CX ? 2 : 1;
   {
     // (i, j, k) of element idx of the e1 x e2 x e3 block, advanced by the
-    // block size with carries instead of a division per element
+    // block size with carries instead of a division per element; 8
+    // elements per thread are loaded into registers before they are stored
+    // to LDS, so their global loads are in flight together
     const int nb = blockDim.x, tot = e1 * e2 * e3;
     const int dk = nb % e3, dj = (nb / e3) % e2, di = nb / (e3 * e2);
     int idx = threadIdx.x, k = idx % e3, j = (idx / e3) % e2, i = idx / (e3 * e2);
     const Real *g0 = grid + W * (((long)u1 * M2 + u2) * M3 + u3);
-    for (; idx < tot; idx += nb) {
-      const long g = ((long)i * M2 + j) * M3 + k;
-      const int l = (i * E + j) * E + k;
+    while (idx < tot) {
+      Real v[8][W];
+      int l[8];
 #pragma unroll
-      for (int w = 0; w < W; w++) th[W * l + w] = g0[W * g + w];
-      k += dk;
-      j += dj;
-      if (k >= e3) {
-        k -= e3;
-        j++;
+      for (int u = 0; u < 8; u++) {
+        l[u] = -1;
+        if (idx < tot) {
+          const long g = ((long)i * M2 + j) * M3 + k;
+          l[u] = (i * E + j) * E + k;
+#pragma unroll
+          for (int w = 0; w < W; w++) v[u][w] = g0[W * g + w];
+        }
+        idx += nb;
+        k += dk;
+        j += dj;
+        if (k >= e3) {
+          k -= e3;
+          j++;
+        }
+        i += di;
+        if (j >= e2) {
+          j -= e2;
+          i++;
+        }
       }
-      i += di;
-      if (j >= e2) {
-        j -= e2;
-        i++;
-      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (l[u] >= 0) {
+#pragma unroll
+          for (int w = 0; w < W; w++) th[W * l[u] + w] = v[u][w];
+        }
     }
   }
   __syncthreads();
