@@ -1629,6 +1629,14 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
     const int s = tileList[c];
     const int4 m0 = ms[s];
     const typename V4t<Real>::T f = frac[s];
+    // the combine's inputs are loaded before the interpolation, so their
+    // latency overlaps it
+    const int o = perm[s];
+    double vd[W * 4], qv[W];
+#pragma unroll
+    for (int k = 0; k < W * 4; k++) vd[k] = Vdir[W * 4 * (size_t)s + k];
+#pragma unroll
+    for (int k = 0; k < W; k++) qv[k] = q[W * (size_t)o + k];
     Real B[3][BA], dB[3][BA], sh[BA];
     const Real fr[3] = {f.x, f.y, f.z};
 #pragma unroll
@@ -1690,23 +1698,24 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
         v[7] += b0 * (double)pzi;
       }
     }
-    const int o = perm[s];
     if (CX) {
-      const double qr = q[2 * (size_t)o], qim = q[2 * (size_t)o + 1];
+      const double qr = qv[0], qim = qv[W - 1];
       const double Cc[4] = {1.0, C1, C2, C3};
       double out[8];
+#pragma unroll
       for (int k = 0; k < 4; k++) {
-        out[2 * k] = v[2 * k] * Cc[k] + Vdir[8 * (size_t)s + 2 * k];
-        out[2 * k + 1] = v[2 * k + 1] * Cc[k] + Vdir[8 * (size_t)s + 2 * k + 1];
+        out[2 * k] = v[2 * k] * Cc[k] + vd[2 * k];
+        out[2 * k + 1] = v[2 * k + 1] * Cc[k] + vd[(2 * k + 1) % (W * 4)];
       }
       out[0] -= fsr * qr - fsi * qim;
       out[1] -= fsr * qim + fsi * qr;
+#pragma unroll
       for (int k = 0; k < 8; k++) Vout[8 * (size_t)o + k] = out[k];
     } else {
-      Vout[4 * (size_t)o + 0] = v[0] + Vdir[4 * (size_t)s + 0] - fsr * q[o];
-      Vout[4 * (size_t)o + 1] = v[2] * C1 + Vdir[4 * (size_t)s + 1];
-      Vout[4 * (size_t)o + 2] = v[4] * C2 + Vdir[4 * (size_t)s + 2];
-      Vout[4 * (size_t)o + 3] = v[6] * C3 + Vdir[4 * (size_t)s + 3];
+      Vout[4 * (size_t)o + 0] = v[0] + vd[0] - fsr * qv[0];
+      Vout[4 * (size_t)o + 1] = v[2] * C1 + vd[1 % (W * 4)];
+      Vout[4 * (size_t)o + 2] = v[4] * C2 + vd[2 % (W * 4)];
+      Vout[4 * (size_t)o + 3] = v[6] * C3 + vd[3 % (W * 4)];
     }
   }
 }
