@@ -502,9 +502,10 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
 // the scanned counts, and places the hits with a ballot.  The entries and
 // their order are those of k_nlist.  FILL writes single entries into a
 // list already filled with padding.  Stencils up to nNeigh = 3.
-#ifndef NPME_NLIST_WAVE_MAX
-#define NPME_NLIST_WAVE_MAX 32768
-#endif
+
+// charge sets up to this size take the wavefront-per-target walk, larger
+// ones the wavefront-per-cell walk (measured crossover ~3e4 charges)
+constexpr size_t NLIST_WAVE_MAX = 32768;
 constexpr int NLW_NN = 3;
 constexpr unsigned NLW_ALLIN = 0x80u;  // stencil cell wholly within the cutoff
 constexpr int NLW_MAXC = (2 * NLW_NN + 1) * (2 * NLW_NN + 1) * (2 * NLW_NN + 1);
@@ -840,11 +841,9 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   d.mn1 = p.mn[1];
   d.mn2 = p.mn[2];
   d.cellTol = p.cellTol;
-#ifndef NPME_NLIST_FORCE
-#define NPME_NLIST_FORCE 0
-#endif
-  const int pick = NPME_NLIST_FORCE ? NPME_NLIST_FORCE
-                   : p.nNeigh > NLW_NN ? 1 : n <= NPME_NLIST_WAVE_MAX ? 2 : 3;
+  // 1: a thread per target (stencils wider than NLW_NN), 2: a wavefront per
+  // target (small sets), 3: a wavefront per cell
+  const int pick = p.nNeigh > NLW_NN ? 1 : n <= NLIST_WAVE_MAX ? 2 : 3;
   if (pick == 3) {
     const long ncell = (long)p.nb[0] * p.nb[1] * p.nb[2];
     const dim3 g((unsigned)((ncell + 3) / 4));  // 4 cells (wavefronts) per workgroup
