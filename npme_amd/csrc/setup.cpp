@@ -199,11 +199,16 @@ static double tsmooth(double x, double X, double Xmax, double a)
 static bool nlist_clearly_too_big(Handle &h, const std::string &mode)
 {
   if (mode == "on") return false;
+  size_t freeB = 0, totalB = 0;
+  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+  // no list exceeds 64 ceil(n / 64) slots per slice entry, n - 1 + 3
+  // entries per slot (padding to groups of 4): when even that fits, the
+  // estimate is not needed
+  const double n = (double)h.n;
+  if ((n + 64.0) * (n + 2.0) * sizeof(unsigned) <= (double)(freeB / 4)) return false;
   const double cand = stencil_candidates_device(h.nb, (int)h.nNeigh, h.d_cellStart, h.stream);
   const double side = (2.0 * h.nNeigh + 1.0) * h.csB, R = h.ker.Rdir;
   const double est = cand * (4.18879020478639098 * R * R * R) / (side * side * side);
-  size_t freeB = 0, totalB = 0;
-  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
   return 0.5 * est * sizeof(unsigned) > (double)(freeB / 4);
 }
 
