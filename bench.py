@@ -242,6 +242,7 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
             out[k] = float(np.mean([infos[ok[j]]["ms"][k] for j in sel]))
         return out
     tail = [j for j in range(len(ok)) if lat[j] >= np.percentile(lat, 99)]
+    small = [j for j, i in enumerate(ok) if sizes[i] < 30000]
     return {"requests": nreq, "completed": len(ok), "errors": errors[:5],
             "rate_per_s": rate, "slots": slots,
             "p50_ms": float(np.percentile(lat, 50) * 1e3), "p90_ms": float(np.percentile(lat, 90) * 1e3),
@@ -253,6 +254,7 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
             "p99_submit_lag_ms": float(np.percentile(lags, 99) * 1e3),
             "max_submit_lag_request": int(np.argmax(lags)), "by_kernel": by,
             "phase_ms_N_ge_500k": phases(big), "phase_ms_slowest_1pct": phases(tail),
+            "phase_ms_N_lt_30k": phases(small),
             "workload": "Poisson %g req/s, N log-uniform [500, 1e6], density 0.1, 70%% 1/r / 20%% "
                         "r^alpha (0.5|1.5) / 10%% Helmholtz k0=0.3, tol 1e-4|1e-6, Rdir 8, mixed "
                         "precision, seed %d; latency includes set-up (RSEM grid, cell sort, lists, "
