@@ -419,6 +419,130 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
   }
 }
 
+__global__ __launch_bounds__(256) void k_widen(long m, const int *__restrict__ in,
+                                               long *__restrict__ out)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = in[i];
+}
+
+__global__ __launch_bounds__(256) void k_narrow(long m, const long *__restrict__ in,
+                                                int *__restrict__ out)
+{
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = (int)in[i];
+}
+
+// Cell-row segments of the direct sum without a neighbour list (k_direct):
+// per block of 8 sorted targets, the contiguous source ranges of the
+// cluster-rows of cells its cutoff sphere can reach, with the same FP64
+// decisions (no FMA contraction) as the host build, so the segments are
+// identical; COUNT writes the number per block, FILL the segments
+// (sB, sE, ax, ay << 16 | az) at segStart[b], and the largest cluster
+// offset from a target's cluster goes to maxOff (atomic max).
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_cell_segments(long nBlocks, size_t n, GeoParams g,
+                                                       double Rm,
+                                                       const double4 *__restrict__ xs,
+                                                       const int4 *__restrict__ cellOf,
+                                                       const int2 *__restrict__ cellSE,
+                                                       int *__restrict__ count,
+                                                       const int *__restrict__ segStart,
+                                                       int4 *__restrict__ segs,
+                                                       int *__restrict__ maxOff)
+{
+#pragma clang fp contract(off)
+  const long b = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nBlocks) return;
+  const int nP = (int)g.nP;
+  const double csB = g.csB;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  int alo[3] = {1 << 30, 1 << 30, 1 << 30}, ahi[3] = {-1, -1, -1};
+  const size_t s1 = min(n, (size_t)(8 * b + 8));
+  for (size_t s = 8 * b; s < s1; s++) {
+    const double4 p = xs[s];
+    const int4 c = cellOf[s];
+    const double px[3] = {p.x, p.y, p.z};
+    const int cc[3] = {c.x, c.y, c.z};
+    for (int d = 0; d < 3; d++) {
+      const double x = px[d] + (-g.mn[d]);
+      lo[d] = fmin(lo[d], x);
+      hi[d] = fmax(hi[d], x);
+      alo[d] = min(alo[d], cc[d] / nP);
+      ahi[d] = max(ahi[d], cc[d] / nP);
+    }
+  }
+  int off = 1, k = 0;
+  const int start = FILL ? segStart[b] : 0;
+  const int gx0 = max(0, (int)floor((lo[0] - Rm) / csB));
+  const int gx1 = min(g.nb[0] - 1, (int)floor((hi[0] + Rm) / csB));
+  const int gy0 = max(0, (int)floor((lo[1] - Rm) / csB));
+  const int gy1 = min(g.nb[1] - 1, (int)floor((hi[1] + Rm) / csB));
+  for (int gx = gx0; gx <= gx1; gx++) {
+    const double gapx = fmax(0.0, fmax(gx * csB - hi[0], lo[0] - (gx + 1) * csB));
+    if (gapx >= Rm) continue;
+    for (int gy = gy0; gy <= gy1; gy++) {
+      const double gapy = fmax(0.0, fmax(gy * csB - hi[1], lo[1] - (gy + 1) * csB));
+      const double rem = Rm * Rm - gapx * gapx - gapy * gapy;
+      if (rem <= 0) continue;
+      const double sr = sqrt(rem);
+      const int zlo = max(0, (int)floor((lo[2] - sr) / csB));
+      const int zhi = min(g.nb[2] - 1, (int)floor((hi[2] + sr) / csB));
+      const long row = ((long)gx * g.nb[1] + gy) * g.nb[2];
+      for (int az = zlo / nP; az <= zhi / nP; az++) {
+        const int z0 = max(zlo, az * nP), z1 = min(zhi, az * nP + nP - 1);
+        const int sB = cellSE[row + z0].x;
+        const int sE = cellSE[row + z1].x + cellSE[row + z1].y;
+        if (sE > sB) {
+          if (FILL) segs[start + k] = make_int4(sB, sE, gx / nP, ((gy / nP) << 16) | az);
+          k++;
+          const int a[3] = {gx / nP, gy / nP, az};
+          for (int d = 0; d < 3; d++) off = max(off, max(abs(a[d] - alo[d]), abs(a[d] - ahi[d])));
+        }
+      }
+    }
+  }
+  if (!FILL) {
+    count[b] = k;
+    atomicMax(maxOff, off);
+  }
+}
+
+long cell_segments_device(const GeoParams &g, size_t n, double Rdir, const double4 *d_xs,
+                          const int4 *d_cellOf, const int *d_cellSE, int *d_segStart,
+                          DevBuf &segs, int &maxOff, hipStream_t s)
+{
+  const long nBlocks = ((long)n + 7) / 8;
+  const double Rm = Rdir * (1.0 + 1.0e-9) + 1.0e-12;
+  DevBuf cnt((nBlocks + 1) * sizeof(int)), off(sizeof(int));
+  const int one = 1;
+  NPME_HIP_CHECK(hipMemcpyAsync(off.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+  NPME_HIP_CHECK(hipMemsetAsync((int *)cnt.p + nBlocks, 0, sizeof(int), s));
+  const dim3 grid((unsigned)((nBlocks + 255) / 256));
+  hipLaunchKernelGGL(k_cell_segments<false>, grid, dim3(256), 0, s, nBlocks, n, g, Rm, d_xs, d_cellOf,
+                     (const int2 *)d_cellSE, (int *)cnt.p, (const int *)nullptr, (int4 *)nullptr,
+                     (int *)off.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  // segment counts in long, so an overflow of the int offsets is seen
+  DevBuf wide((nBlocks + 1) * sizeof(long)), start((nBlocks + 1) * sizeof(long));
+  hipLaunchKernelGGL(k_widen, grid_for(nBlocks + 1), dim3(256), 0, s, nBlocks + 1,
+                     (const int *)cnt.p, (long *)wide.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  const long total = scan_words_device((const long *)wide.p, (long *)start.p, nBlocks + 1, s);
+  NPME_HIP_CHECK(hipMemcpyAsync(&maxOff, off.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  if (total > 0x7fffffffL) throw Error{NPME_ERR_NOMEM, "too many direct-sum segments"};
+  hipLaunchKernelGGL(k_narrow, grid_for(nBlocks + 1), dim3(256), 0, s, nBlocks + 1,
+                     (const long *)start.p, d_segStart);
+  segs.reset(std::max<size_t>((size_t)total, 1) * sizeof(int4));
+  hipLaunchKernelGGL(k_cell_segments<true>, grid, dim3(256), 0, s, nBlocks, n, g, Rm, d_xs, d_cellOf,
+                     (const int2 *)d_cellSE, (int *)nullptr, (const int *)d_segStart, (int4 *)segs.p,
+                     (int *)nullptr);
+  NPME_HIP_CHECK(hipGetLastError());
+  NPME_HIP_CHECK(hipStreamSynchronize(s));  // before the scratch returns to the pool
+  return total;
+}
+
 long scan_words_device(const long *d_in, long *d_out, size_t m, hipStream_t s)
 {
   exclusive_scan(d_in, d_out, m, s);

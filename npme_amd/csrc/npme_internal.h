@@ -316,6 +316,12 @@ void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const doub
                              int4 *d_ms,
                              void *d_frac, DevBuf &tileStart, DevBuf &tileList, DevBuf &itileStart,
                              DevBuf &itileList, size_t &nTileEntries, hipStream_t s);
+// cell-row segments of the direct sum without a neighbour list, on the
+// device (segStart: nBlocks + 1 ints, nBlocks = ceil(n / 8)); returns their
+// number and the largest cluster offset (at least 1)
+long cell_segments_device(const GeoParams &g, size_t n, double Rdir, const double4 *d_xs,
+                          const int4 *d_cellOf, const int *d_cellSE, int *d_segStart,
+                          DevBuf &segs, int &maxOff, hipStream_t s);
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s);
 // coordinates in sorted order: xs[s] = coord[perm[s]]

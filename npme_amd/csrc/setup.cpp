@@ -274,7 +274,7 @@ static int fixed_shift(int maxOff)
 // Device set-up (geometry.hip); returns false when the neighbour list does
 // not fit the memory budget (the host set-up then builds the cell-row lists)
 // the positions are already on the device (h.d_coordS)
-static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
+static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode, bool nlPossible)
 {
   const size_t n = h.n;
   hipStream_t s = h.stream;
@@ -291,34 +291,53 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
   NPME_HIP_CHECK(hipStreamSynchronize(s));
   h.setupMs[2] = ms_since(t0);
 
-  // neighbour list: count, slice lengths and offsets, then fill
-  if (nlist_clearly_too_big(h, nlmode)) return false;
-  const DirectParams dp = direct_params(h);
-  const size_t nSlices = (n + 63) / 64;
-  size_t entries = 0;
-  h.d_sliceStart = dalloc<long>(h, nSlices + 1);
-  h.d_sliceLen = dalloc<int>(h, nSlices);
-  NlistWork work;
-  {
-    DevBuf cnt(n * sizeof(int));
-    launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, (int *)cnt.p,
-                 nullptr, nullptr, work, s);
-    entries = nlist_slices_device(n, (const int *)cnt.p, h.d_sliceLen, h.d_sliceStart, s);
+  // direct-sum structure: the neighbour list (count, slice lengths and
+  // offsets, then fill) when it is possible and fits, else the cell-row
+  // segments of k_direct
+  bool list = nlPossible && !nlist_clearly_too_big(h, nlmode);
+  if (list) {
+    const DirectParams dp = direct_params(h);
+    const size_t nSlices = (n + 63) / 64;
+    size_t entries = 0;
+    h.d_sliceStart = dalloc<long>(h, nSlices + 1);
+    h.d_sliceLen = dalloc<int>(h, nSlices);
+    NlistWork work;
+    {
+      DevBuf cnt(n * sizeof(int));
+      launch_nlist(false, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, (int *)cnt.p,
+                   nullptr, nullptr, work, s);
+      entries = nlist_slices_device(n, (const int *)cnt.p, h.d_sliceLen, h.d_sliceStart, s);
+    }
+    size_t freeB = 0, totalB = 0;
+    NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
+    if (nlmode != "on" && entries * sizeof(unsigned) > freeB / 4) {
+      list = false;
+    } else {
+      h.d_nlist = dalloc<unsigned>(h, entries);
+      NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n), std::max<size_t>(entries, 1), s));
+      launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
+                   h.d_sliceStart, h.d_nlist, work, s);
+      h.nlistEntries = entries;
+      h.useNlist = true;
+      NPME_HIP_CHECK(hipStreamSynchronize(s));
+    }
   }
-  size_t freeB = 0, totalB = 0;
-  NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-  if (nlmode != "on" && entries * sizeof(unsigned) > freeB / 4) return false;
-  h.d_nlist = dalloc<unsigned>(h, entries);
-  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n), std::max<size_t>(entries, 1), s));
-  launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
-               h.d_sliceStart, h.d_nlist, work, s);
-  h.nlistEntries = entries;
-  h.useNlist = true;
-  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  int maxOff = 1;
+  h.nSegments = 0;
+  if (!list) {
+    const long nBlocks = ((long)n + 7) / 8;
+    h.d_segStart = dalloc<int>(h, nBlocks + 1);
+    DevBuf segs;
+    h.nSegments = (size_t)cell_segments_device(g, n, h.ker.Rdir, h.d_xs, h.d_cell, h.d_cellStart,
+                                               h.d_segStart, segs, maxOff, s);
+    h.d_blockBox = (int4 *)adopt(h, segs);
+  }
   h.setupMs[3] = ms_since(t0);
 
-  // positions, spline windows and tile lists
-  h.fxShift = fixed_shift(1);
+  // positions, spline windows and tile lists; the fixed-point scale
+  // depends on how far the direct sum's sources reach (list: the adjacent
+  // clusters)
+  h.fxShift = list ? fixed_shift(1) : fixed_shift(maxOff);
   g.fxShift = h.fxShift;
   g.fxScale = std::ldexp(1.0, h.fxShift) / h.csA;
   // [n + 1]: entry n is the neighbour list's zero-charge sentinel
@@ -338,7 +357,6 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode)
   h.d_tileList = (int *)adopt(h, tl);
   h.d_itileStart = (int *)adopt(h, its);
   h.d_itileList = (int *)adopt(h, itl);
-  h.nSegments = 0;
   const int vw = h.complexK ? 8 : 4;
   if (h.complexK) {
     h.d_qi = dalloc<char>(h, rs * (n + 1));
@@ -863,7 +881,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
     kw_error("neighborList must be 'auto', 'on' or 'off'");
   const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
-  const bool devSetup = smode == "device" && nlPossible;
+  const bool devSetup = smode == "device";
   const bool staged = devSetup && h.coordStaged && h.stagedN == n;
   if (!staged) {
     NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -944,7 +962,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   // ---- per-charge state: on the device when the neighbour list is used
   h.setupMs[0] = ms_since(t0);
   bool done = false;
-  if (devSetup) done = particles_device(h, g, nlmode);
+  if (devSetup) done = particles_device(h, g, nlmode, nlPossible);
   if (!done) {
     handle_free_device(h);
     particles_host(h, g, nlmode, nlPossible, coord);

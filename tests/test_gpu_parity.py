@@ -247,11 +247,13 @@ def _blobs(n, nblob, box, sigma, seed):
     return coord, q - q.mean()
 
 
+@pytest.mark.parametrize("nl", ["on", "off"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", ["uniform", "blobs", "tiny", "ralpha_aniso", "helmholtz"])
-def test_device_setup_matches_host_setup(prec, case):
-    """The set-up on the device (cell keys, stable sort, lists) reproduces
-    the host set-up exactly: CalcV results are bit-identical."""
+def test_device_setup_matches_host_setup(prec, case, nl):
+    """The set-up on the device (cell keys, stable sort, neighbour list or
+    cell-row segments) reproduces the host set-up exactly: CalcV results
+    are bit-identical."""
     import npme_amd as N
     extra, cx = {}, False
     if case == "uniform":
@@ -276,7 +278,7 @@ def test_device_setup_matches_host_setup(prec, case):
     print("case device-vs-host set-up", case, prec, flush=True)
     out = {}
     for mode in ("device", "host"):
-        k = _kw(calcType="pme", tol=1e-6, precision=prec, setup=mode, neighborList="on", **kw)
+        k = _kw(calcType="pme", tol=1e-6, precision=prec, setup=mode, neighborList=nl, **kw)
         if cx:
             assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeComplex=q), N.npme_last_error()
             h = N.NPME_Interface_SetUpComplexKernel(k, len(q), coord)
@@ -286,7 +288,10 @@ def test_device_setup_matches_host_setup(prec, case):
         assert h is not None, N.npme_last_error()
         assert ("set-up on the " + mode) in h.Print()
         out[mode] = h.calcv(q)
+        out[mode + "_direct"] = [l for l in h.Print().splitlines() if "direct sum:" in l]
     assert np.array_equal(out["device"], out["host"])
+    # the same structure: list entries or cell-row segments
+    assert out["device_direct"] == out["host_direct"]
 
 
 @pytest.mark.parametrize("setup", ["device", "host"])
