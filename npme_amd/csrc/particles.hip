@@ -727,10 +727,21 @@ __global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncel
         oz = o.z;
       }
       unsigned long long mym = 0;
-      for (int i = 0; i < mt; i++) {
+      // two targets per step: their distance tests are independent
+      int i = 0;
+      for (; i + 1 < mt; i += 2) {
+        const double4 ta = tpos[i], tb2 = tpos[i + 1];
+        const double ax = ta.x - ox, ay = ta.y - oy, az = ta.z - oz;
+        const double bx = tb2.x - ox, by = tb2.y - oy, bz = tb2.z - oz;
+        // & rather than &&: no branch around the distance test
+        const bool ha = (j != s0 + i) & (ax * ax + ay * ay + az * az < R2);
+        const bool hb = (j != s0 + i + 1) & (bx * bx + by * by + bz * bz < R2);
+        const unsigned long long ma = __ballot(ha), mb = __ballot(hb);
+        mym = lane == i ? ma : (lane == i + 1 ? mb : mym);
+      }
+      if (i < mt) {
         const double4 tp = tpos[i];
         const double rx = tp.x - ox, ry = tp.y - oy, rz = tp.z - oz;
-        // & rather than &&: no branch around the distance test
         const bool hit = (j != s0 + i) & (rx * rx + ry * ry + rz * rz < R2);
         const unsigned long long m = __ballot(hit);
         mym = lane == i ? m : mym;
