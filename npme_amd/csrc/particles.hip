@@ -1509,6 +1509,106 @@ __global__ __launch_bounds__(64) void k_spread_mfma(int BnRun, long M1, long M2,
   }
 }
 
+// FP64 form of k_spread_mfma, on v_mfma_f64_16x16x4f64 (same operand
+// layout: lane l gives A[l % 16][l / 16] and B[l / 16][l % 16]; it holds
+// D[l / 16 + 4 r][l % 16], checked on the GPU against a host product).  The
+// products accumulate in FP64 in the matrix cores, so there is no carry.
+typedef double npme_d4 __attribute__((ext_vector_type(4)));
+template <int BN>
+__global__ __launch_bounds__(64) void k_spread_mfma64(int BnRun, long M1, long M2, long M3, int nt1,
+                                                      int nt2, int nt3,
+                                                      const int *__restrict__ tileStart,
+                                                      const int *__restrict__ tileList,
+                                                      const double *__restrict__ qre,
+                                                      const int4 *__restrict__ ms,
+                                                      const double4 *__restrict__ frac,
+                                                      double *__restrict__ grid)
+{
+  constexpr int TX = 16, TY = 8, TZ = 8;
+  __shared__ __align__(16) double sA[64][TX];
+  __shared__ __align__(16) double sW2[64][TY];
+  __shared__ __align__(16) double sW3[64][TZ];
+  const int Bn = BN > 0 ? BN : BnRun;
+  constexpr int BA = BN > 0 ? BN : 16;
+  const int t = xcd_remap(blockIdx.x, gridDim.x);
+  const int t3 = t % nt3, t2 = (t / nt3) % nt2, t1 = t / (nt3 * nt2);
+  const int u1 = t1 * TX, u2 = t2 * TY, u3 = t3 * TZ;
+  const int l = threadIdx.x, n16 = l & 15, kq = l >> 4;
+  const int beg = tileStart[t], end = tileStart[t + 1];
+  int4 cm = make_int4(0, 0, 0, 0);
+  double4 cf = make_double4(0, 0, 0, 0);
+  double cq = 0;
+  auto fetch = [&](int base, int4 &m0, double4 &f, double &q) {
+    if (base + l < end) {
+      const int c = tileList[base + l];
+      m0 = ms[c];
+      f = frac[c];
+      q = qre[c];
+    }
+  };
+  fetch(beg, cm, cf, cq);
+  npme_d4 d0 = {0, 0, 0, 0}, d1 = d0, d2 = d0, d3 = d0;
+  for (int base = beg; base < end; base += 64) {
+    const int cnt = min(64, end - base);
+    // row l as in k_spread_mfma: w2 stored (w2[0], w2[2], w2[4], w2[6],
+    // w2[1], w2[3], w2[5], w2[7])
+#pragma unroll
+    for (int k = 0; k < TX; k++) sA[l][k] = 0;
+#pragma unroll
+    for (int k = 0; k < TY; k++) sW2[l][k] = 0;
+#pragma unroll
+    for (int k = 0; k < TZ; k++) sW3[l][k] = 0;
+    if (l < cnt) {
+      double sh[BA];
+      if constexpr (BN > 0) bsp_shifts_c<double, BN>(cf.x, sh);
+      else bsp_shifts<double>(Bn, cf.x, sh);
+      int off = cm.x - u1;
+#pragma unroll
+      for (int j = 0; j < BA; j++)
+        if (j < Bn && off + j >= 0 && off + j < TX) sA[l][off + j] = cq * sh[j];
+      if constexpr (BN > 0) bsp_shifts_c<double, BN>(cf.y, sh);
+      else bsp_shifts<double>(Bn, cf.y, sh);
+      off = cm.y - u2;
+#pragma unroll
+      for (int j = 0; j < BA; j++) {
+        const int jj = off + j;
+        if (j < Bn && jj >= 0 && jj < TY) sW2[l][4 * (jj & 1) + (jj >> 1)] = sh[j];
+      }
+      if constexpr (BN > 0) bsp_shifts_c<double, BN>(cf.z, sh);
+      else bsp_shifts<double>(Bn, cf.z, sh);
+      off = cm.z - u3;
+#pragma unroll
+      for (int j = 0; j < BA; j++)
+        if (j < Bn && off + j >= 0 && off + j < TZ) sW3[l][off + j] = sh[j];
+    }
+    __syncthreads();
+    fetch(base + 64, cm, cf, cq);
+    const int nstep = (cnt + 3) >> 2;  // rows past cnt are zero
+    for (int st = 0; st < nstep; st++) {
+      const int c = 4 * st + kq;
+      const double av = sA[c][n16];
+      const double w3 = sW3[c][n16 & 7];
+      const double *w2 = &sW2[c][4 * (n16 >> 3)];
+      d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, w2[0] * w3, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, w2[1] * w3, d1, 0, 0, 0);
+      d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, w2[2] * w3, d2, 0, 0, 0);
+      d3 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, w2[3] * w3, d3, 0, 0, 0);
+    }
+    __syncthreads();  // the rows are rewritten by the next chunk
+  }
+  const npme_d4 dd[4] = {d0, d1, d2, d3};
+  const long p3 = u3 + (n16 & 7);
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const long p2 = u2 + 2 * b + (n16 >> 3);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const long p1 = u1 + kq + 4 * r;
+      if (p1 < M1 && p2 < M2 && p3 < M3) grid[(p1 * M2 + p2) * M3 + p3] = dd[b][r];
+    }
+  }
+}
+
 void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int T[3],
                    const int nt[3], const int *tileStart, const int *tileList, const void *qre,
                    const void *qim, const int4 *ms, const void *frac, void *grid, hipStream_t s)
@@ -1527,6 +1627,22 @@ void launch_spread(bool fp64, bool complexK, long Bn, const long M[3], const int
       default: NPME_SPREAD_M(0); break;
     }
 #undef NPME_SPREAD_M
+    NPME_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  if (fp64 && !complexK && T[0] == 16 && T[1] == 8 && T[2] == 8) {
+#define NPME_SPREAD_M64(BNV)                                                                    \
+  hipLaunchKernelGGL((k_spread_mfma64<BNV>), g, dim3(64), 0, s, (int)Bn, M[0], M[1], M[2], nt[0],\
+                     nt[1], nt[2], tileStart, tileList, (const double *)qre, ms,                \
+                     (const double4 *)frac, (double *)grid)
+    switch (Bn) {
+      case 4: NPME_SPREAD_M64(4); break;
+      case 6: NPME_SPREAD_M64(6); break;
+      case 8: NPME_SPREAD_M64(8); break;
+      case 10: NPME_SPREAD_M64(10); break;
+      default: NPME_SPREAD_M64(0); break;
+    }
+#undef NPME_SPREAD_M64
     NPME_HIP_CHECK(hipGetLastError());
     return;
   }

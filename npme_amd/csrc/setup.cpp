@@ -942,10 +942,10 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   h.csA = g.csA = csA;
   g.nP = nP;
   g.Bn = h.Bn;
-  // spreading tiles: 16 x 8 x 8 points for the FP32 real kernels (matrix-core
-  // spreading), else 8^3; interpolation tiles of TI^3 points whose block plus
-  // halo fits 64 KB of LDS
-  h.T[0] = (!h.fp64 && !h.complexK) ? 16 : 8;
+  // spreading tiles: 16 x 8 x 8 points for the real kernels (matrix-core
+  // spreading, FP32 and FP64), else 8^3; interpolation tiles of TI^3 points
+  // whose block plus halo fits 64 KB of LDS
+  h.T[0] = !h.complexK ? 16 : 8;
   h.T[1] = h.T[2] = 8;
   for (int d = 0; d < 3; d++) {
     g.ts[d] = h.T[d];
