@@ -862,6 +862,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.M[d] = N / 2;
   }
   if (h.Bn < 2 || h.Bn > 16) throw Error{NPME_ERR_UNSUPPORTED, "BnOrder must be in [2, 16]"};
+  // an odd order's interpolation factor |b(m)|^2 vanishes at m = N/2 (the
+  // reference documents BnOrder as an even integer)
+  if (h.Bn % 2 != 0) throw Error{NPME_ERR_UNSUPPORTED, "BnOrder must be even"};
   // one FFT column must fit a workgroup's LDS in FP64 (kernel-grid set-up)
   for (int d = 0; d < 3; d++)
     if (h.N[d] + 1 > 16 * 256)

@@ -70,6 +70,18 @@ def test_add_missing_keywords_explicit_grid_nderiv_default():
         assert k.GetValue("nDeriv") == str(nd)
 
 
+@pytest.mark.parametrize("Bn", [5, 7])
+def test_setup_rejects_odd_spline_order(Bn):
+    """An odd BnOrder is refused before any device work (its |b(m)|^2
+    vanishes at m = N/2), so this runs without a GPU."""
+    import npme_amd as N
+    coord, q = uniform_box(500, 10.0)
+    k = _kw(BnOrder=Bn, N1=64, N2=64, N3=64, n1=16, n2=16, n3=16, nDeriv=6)
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+    assert N.NPME_Interface_SetUpRealKernel(k, len(q), coord) is None
+    assert "even" in N.npme_last_error()
+
+
 @pytest.mark.parametrize("bad", [dict(calcType="fast"), dict(funcType="Yukawa"),
                                  dict(EwaldSplit="Other"), dict(funcType="Ralpha"),
                                  dict(funcType="Helmholtz", k0_r=1.0, k0_i=0.0)])

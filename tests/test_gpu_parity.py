@@ -59,6 +59,34 @@ def test_laplace_config1(oracle, prec):
     assert me["relV"] <= 1e-6
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("Bn", [4, 8, 10, 12, 14])
+def test_spline_orders_match_oracle(oracle, prec, Bn):
+    """Spreading and interpolation for each spline order the kernels are
+    compiled for (4, 6, 8, 10) and the generic order (12, 14), on a given
+    64^3 grid, against the oracle's reciprocal + direct sum.  The mixed
+    form's rounding grows with the order, roughly as Bn^2 (its window sums
+    have Bn^2 FP32 terms per plane): 4e-7 at order 4, 1.4e-6 at 12 and
+    2.9e-6 at 14 on this grid, so its bound is max(1e-6, 5e-8 Bn^2)."""
+    import npme_amd as N
+    coord, q = uniform_box(2500, 25.0)
+    print("case spline order", Bn, prec, flush=True)
+    k = _kw(calcType="pme", funcType="Laplace", Rdir=5.0, BnOrder=Bn, tol=1e-6, precision=prec,
+            N1=64, N2=64, N3=64, n1=8, n2=8, n3=8, nDeriv=6)
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q), N.npme_last_error()
+    h = N.NPME_Interface_SetUpRealKernel(k, len(q), coord)
+    assert h is not None, N.npme_last_error()
+    V = h.calcv(q)
+    ko = oracle.kernel(oracle.LAPLACE, h.Get_Nder(), 5.0)
+    mo = oracle.compare_v(oracle.calcv(ko, (64, 64, 64), Bn, coord, q), V)
+    print(Bn, prec, mo)
+    if prec == "fp64":
+        assert mo["relV"] < 1e-11 and mo["reldV"] < 1e-11
+    else:
+        bound = max(1e-6, 5e-8 * Bn * Bn)
+        assert mo["relV"] < bound and mo["reldV"] < 10 * bound
+
+
 def test_exact_matches_oracle(oracle):
     coord, q = uniform_box(3000, 20.0)
     h = _real_setup(coord, q, "mixed")
