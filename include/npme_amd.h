@@ -114,6 +114,20 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen);
  * src/Interface.cpp:1329-1334). ---------------------------------------- */
 int npme_count_pairs(npme_handle *h, double *pairs);
 
+/* ---- integer indexing of the last set-up, per charge in input order:
+ * clusterKey[i] and cellKey[i] are the embedded cluster and cell indices of
+ * NPME_PointIntoEmbeddedCell (src/PartitionEmbeddedBox.cpp:266-455, cell
+ * index of src/PartitionBox.cpp:311-351); mStart[i][3] is the first grid
+ * index m of the charge's spline window per axis
+ * (NPME_GetQpme_DFT_StartIndex, src/RecSumSupportFunctions.h:123-127, with
+ * wX = N1/L1 (x - R0x) of src/RecSumQ.cpp:106-108); order[s] is the input
+ * index of the s-th charge of the working order (a stable sort by cellKey).
+ * Any output may be NULL.  No reference counterpart (the reference keeps
+ * these inside the interface object); it exposes the set-up's integer
+ * decisions for bit-exact checks. ---------------------------------------- */
+int npme_get_indices(npme_handle *h, size_t nCharge, unsigned long long *clusterKey,
+                     unsigned long long *cellKey, long *mStart, long *order);
+
 /* ---- per-stage device times of the last CalcV when profiling is on:
  * ms[0] gather, [1] direct sum, [2] spread, [3..7] FFT passes A..E,
  * [8] interpolate+combine; returns the number of stages written. ------- */
@@ -194,6 +208,12 @@ void npme_free(void *p);
  * relative errors are out[0]/out[2] and out[1]/out[3] */
 int npme_compare_v(size_t nCharge, int isChargeReal, const double *V, const double *Vref,
                    double out[4]);
+
+/* ---- testing hook: the calling thread's k-th next device allocation of
+ * the library (k >= 0, counted from 0) fails as if the device were out of
+ * memory; k < 0 disables it.  No reference counterpart: it exercises the
+ * out-of-memory paths of set-up and CalcV. ----------------------------- */
+void npme_debug_fail_alloc(long k);
 
 const char *npme_last_error(void);
 

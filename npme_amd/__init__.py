@@ -55,6 +55,9 @@ def lib():
         L.npme_set_profiling.argtypes = [p, i]
         L.npme_count_pairs.argtypes = [p, p]
         L.npme_get_stage_times.argtypes = [p, p, i]
+        L.npme_get_indices.argtypes = [p, s, p, p, p, p]
+        L.npme_debug_fail_alloc.argtypes = [C.c_long]
+        L.npme_debug_fail_alloc.restype = None
         L.npme_server_create.argtypes = [C.POINTER(p), i, d]
         L.npme_server_submit.argtypes = [p, C.POINTER(_Request), C.POINTER(C.c_long)]
         L.npme_server_wait.argtypes = [p, C.c_long, C.POINTER(_RequestInfo)]
@@ -286,6 +289,20 @@ class _InterfaceBase:
         out = C.c_double()
         _check(lib().npme_count_pairs(self._h, C.byref(out)), "count_pairs")
         return int(out.value)
+
+    def get_indices(self):
+        """Integer indexing of the set-up, per charge in input order:
+        (clusterKey[n], cellKey[n], mStart[n][3], order[n]) -- the embedded
+        cluster and cell indices, the first grid index of each spline window
+        and the working (sorted) order."""
+        n = self.nCharge
+        ka = np.zeros(n, np.uint64)
+        kb = np.zeros(n, np.uint64)
+        ms = np.zeros((n, 3), np.int64)
+        order = np.zeros(n, np.int64)
+        _check(lib().npme_get_indices(self._h, n, _ptr(ka), _ptr(kb), _ptr(ms), _ptr(order)),
+               "get_indices")
+        return ka, kb, ms, order
 
     def set_profiling(self, on=True):
         lib().npme_set_profiling(self._h, int(on))

@@ -404,6 +404,47 @@ int npme_count_pairs(npme_handle *h, double *pairs)
   });
 }
 
+int npme_get_indices(npme_handle *h, size_t nCharge, unsigned long long *clusterKey,
+                     unsigned long long *cellKey, long *mStart, long *order)
+{
+  return guarded([&] {
+    if (!h) throw Error{NPME_ERR_ARG, "NULL handle"};
+    std::lock_guard<std::mutex> lk(h->h.mtx);
+    const Handle &H = h->h;
+    if (!H.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
+    if (nCharge != H.n) throw Error{NPME_ERR_ARG, "nCharge does not match the set-up"};
+    const size_t n = H.n;
+    std::vector<int> perm(n);
+    std::vector<int4> cell(n), ms(n);
+    NPME_HIP_CHECK(hipMemcpyAsync(perm.data(), H.d_perm, n * sizeof(int), hipMemcpyDeviceToHost, H.stream));
+    NPME_HIP_CHECK(hipMemcpyAsync(cell.data(), H.d_cell, n * sizeof(int4), hipMemcpyDeviceToHost, H.stream));
+    NPME_HIP_CHECK(hipMemcpyAsync(ms.data(), H.d_ms, n * sizeof(int4), hipMemcpyDeviceToHost, H.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(H.stream));
+    // the keys of src/PartitionEmbeddedBox.cpp:266-455 from the global cell
+    // coordinates g = a nP + b kept by the set-up (a: cluster, b: cell in
+    // the cluster)
+    const unsigned long nP = (unsigned long)H.nCellClust1D, Nc = nP * nP * nP;
+    const unsigned long nA[3] = {H.nb[0] / nP, H.nb[1] / nP, H.nb[2] / nP};
+    for (size_t s = 0; s < n; s++) {
+      const size_t i = (size_t)perm[s];
+      if (i >= n) throw Error{NPME_ERR_DEVICE, "set-up permutation out of range"};
+      const unsigned long g[3] = {(unsigned long)cell[s].x, (unsigned long)cell[s].y,
+                                  (unsigned long)cell[s].z};
+      const unsigned long tA = ((g[0] / nP) * nA[1] + g[1] / nP) * nA[2] + g[2] / nP;
+      if (clusterKey) clusterKey[i] = tA;
+      if (cellKey) cellKey[i] = tA * Nc + ((g[0] % nP) * nP + g[1] % nP) * nP + g[2] % nP;
+      if (mStart) {
+        mStart[3 * i + 0] = ms[s].x - H.N[0] / 4;
+        mStart[3 * i + 1] = ms[s].y - H.N[1] / 4;
+        mStart[3 * i + 2] = ms[s].z - H.N[2] / 4;
+      }
+      if (order) order[s] = (long)i;
+    }
+  });
+}
+
+void npme_debug_fail_alloc(long k) { dev_fail_after(k); }
+
 int npme_set_profiling(npme_handle *h, int on)
 {
   if (!h) return NPME_ERR_ARG;

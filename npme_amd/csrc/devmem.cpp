@@ -45,10 +45,15 @@ size_t size_class(size_t bytes)
   return units * unit;
 }
 
+thread_local long g_failAfter = -1;
+
 }  // namespace
+
+void dev_fail_after(long k) { g_failAfter = k < 0 ? -1 : k; }
 
 void *dev_alloc(size_t bytes)
 {
+  if (g_failAfter >= 0 && g_failAfter-- == 0) return nullptr;
   int dev = 0;
   NPME_HIP_CHECK(hipGetDevice(&dev));
   const size_t cls = size_class(std::max<size_t>(bytes, 1));

@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <string>
@@ -133,7 +134,14 @@ size_t dev_block_bytes(size_t bytes);
 void dev_trim();
 void dev_stats(size_t *inUse, size_t *cached);
 
-// RAII block from the pool
+// the calling thread's k-th next dev_alloc fails (k >= 0; -1 disables):
+// a hook for the tests of the error paths
+void dev_fail_after(long k);
+
+// RAII block from the pool.  Destroyed while an exception unwinds (a set-up
+// that failed part way), the block may still be in use by kernels queued
+// before the failure: the device is synchronised before the block goes
+// back to the pool, where another handle or server slot could receive it.
 struct DevBuf {
   void *p = nullptr;
   size_t bytes = 0;
@@ -141,7 +149,11 @@ struct DevBuf {
   explicit DevBuf(size_t b) { reset(b); }
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
-  ~DevBuf() { dev_free(p); }
+  ~DevBuf()
+  {
+    if (p && std::uncaught_exceptions() > 0) (void)hipDeviceSynchronize();
+    dev_free(p);
+  }
   void swap(DevBuf &o)
   {
     std::swap(p, o.p);

@@ -1236,48 +1236,116 @@ void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const
 }
 
 // ------------------------------------------------------------------ B-splines
-// M_n(t + j), j = 0..n-1 (Cox-de Boor over all shifts), n <= 16
-template <typename Real> __device__ __forceinline__ void bsp_shifts(int n, Real t, Real *m)
+// M_n(t + j), j = 0..n-1 (Cox-de Boor over all shifts), n <= 16.
+// The recursion always runs in FP64 and the weights are rounded once to the
+// output type.  Run in FP32, its rounding accumulates with the order (the
+// weights' sum drifts from 1 by several ulp per level), and that alone gave
+// the FP32 form relative errors in V of 5e-7 at order 4 up to 2.7e-6 at
+// order 14; rounded once, the weights contribute ~5e-8 at every order.
+// FP64 VALU operations issue at the FP32 rate on this chip (no packed FP32
+// here), so the cost is registers, not issue slots.
+__device__ __forceinline__ void bsp_rec(int n, double t, double *w)
 {
   if (n == 1) {
-    m[0] = (Real)1;
+    w[0] = 1.0;
     return;
   }
-  m[0] = t;
-  m[1] = (Real)1 - t;
+  w[0] = t;
+  w[1] = 1.0 - t;
   for (int k = 3; k <= n; k++) {
-    const Real inv = (Real)1 / (Real)(k - 1);
-    Real prev = (Real)0;  // M_{k-1}(t + j - 1) for j = 0
+    const double inv = 1.0 / (double)(k - 1);
+    double prev = 0.0;  // M_{k-1}(t + j - 1) for j = 0
     for (int j = 0; j < k; j++) {
-      const Real left = (j < k - 1) ? m[j] : (Real)0;
-      const Real val = ((t + j) * left + ((Real)k - t - j) * prev) * inv;
+      const double left = (j < k - 1) ? w[j] : 0.0;
+      const double val = ((t + j) * left + ((double)k - t - j) * prev) * inv;
       prev = left;
-      m[j] = val;
+      w[j] = val;
     }
   }
 }
 
-// the same with the order known at compile time (fully unrolled, registers)
-template <typename Real, int N> __device__ __forceinline__ void bsp_shifts_c(Real t, Real *m)
+template <typename Real> __device__ __forceinline__ void bsp_shifts(int n, double t, Real *m)
+{
+  double w[16];
+  bsp_rec(n, t, w);
+  for (int j = 0; j < n; j++) m[j] = (Real)w[j];
+}
+
+// the recursion with the order known at compile time (fully unrolled,
+// registers); w[0..N-1] = M_N(t + j)
+template <int N> __device__ __forceinline__ void bsp_rec_c(double t, double *w)
 {
   if constexpr (N == 1) {
-    m[0] = (Real)1;
+    w[0] = 1.0;
   } else {
-    m[0] = t;
-    m[1] = (Real)1 - t;
+    w[0] = t;
+    w[1] = 1.0 - t;
 #pragma unroll
     for (int k = 3; k <= N; k++) {
-      const Real inv = (Real)1 / (Real)(k - 1);
-      Real prev = (Real)0;
+      const double inv = 1.0 / (double)(k - 1);
+      double prev = 0.0;
 #pragma unroll
       for (int j = 0; j < k; j++) {
-        const Real left = (j < k - 1) ? m[j] : (Real)0;
-        const Real val = ((t + j) * left + ((Real)k - t - j) * prev) * inv;
+        const double left = (j < k - 1) ? w[j] : 0.0;
+        const double val = ((t + j) * left + ((double)k - t - j) * prev) * inv;
         prev = left;
-        m[j] = val;
+        w[j] = val;
       }
     }
   }
+}
+
+template <typename Real, int N> __device__ __forceinline__ void bsp_shifts_c(double t, Real *m)
+{
+  double w[N];
+  bsp_rec_c<N>(t, w);
+#pragma unroll
+  for (int j = 0; j < N; j++) m[j] = (Real)w[j];
+}
+
+// weights M_N(t + j) and gradient weights dM_N(t + j)/dt =
+// M_{N-1}(t + j) - M_{N-1}(t + j - 1) from one FP64 recursion (the order
+// N - 1 stage is the last but one level of the order N recursion), each
+// rounded once to the output type
+template <typename Real, int N>
+__device__ __forceinline__ void bsp_weights_grad_c(double t, Real *B, Real *dB)
+{
+  double w[N];
+  bsp_rec_c<N - 1>(t, w);
+#pragma unroll
+  for (int j = 0; j < N; j++)
+    dB[j] = (Real)(((j < N - 1) ? w[j] : 0.0) - ((j >= 1) ? w[j - 1] : 0.0));
+  {
+    // one more level of the recursion: order N - 1 -> N
+    const double inv = 1.0 / (double)(N - 1);
+    double prev = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+      const double left = (j < N - 1) ? w[j] : 0.0;
+      w[j] = ((t + j) * left + ((double)N - t - j) * prev) * inv;
+      prev = left;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < N; j++) B[j] = (Real)w[j];
+}
+
+// the same for an order known only at run time (n <= 16)
+template <typename Real>
+__device__ __forceinline__ void bsp_weights_grad(int n, double t, Real *B, Real *dB)
+{
+  double w[16];
+  bsp_rec(n - 1, t, w);
+  for (int j = 0; j < n; j++)
+    dB[j] = (Real)(((j < n - 1) ? w[j] : 0.0) - ((j >= 1) ? w[j - 1] : 0.0));
+  const double inv = 1.0 / (double)(n - 1);
+  double prev = 0.0;
+  for (int j = 0; j < n; j++) {
+    const double left = (j < n - 1) ? w[j] : 0.0;
+    w[j] = ((t + j) * left + ((double)n - t - j) * prev) * inv;
+    prev = left;
+  }
+  for (int j = 0; j < n; j++) B[j] = (Real)w[j];
 }
 
 // ------------------------------------------------------------------ spread
@@ -1763,20 +1831,12 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
     for (int k = 0; k < W * 4; k++) vd[k] = Vdir[W * 4 * (size_t)s + k];
 #pragma unroll
     for (int k = 0; k < W; k++) qv[k] = q[W * (size_t)o + k];
-    Real B[3][BA], dB[3][BA], sh[BA];
+    Real B[3][BA], dB[3][BA];
     const Real fr[3] = {f.x, f.y, f.z};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-      if constexpr (BN > 0) {
-        bsp_shifts_c<Real, BN>(fr[d], B[d]);
-        bsp_shifts_c<Real, BN - 1>(fr[d], sh);
-      } else {
-        bsp_shifts<Real>(Bn, fr[d], B[d]);
-        bsp_shifts<Real>(Bn - 1, fr[d], sh);
-      }
-#pragma unroll
-      for (int j = 0; j < BA; j++)
-        if (j < Bn) dB[d][j] = ((j < Bn - 1) ? sh[j] : (Real)0) - ((j >= 1) ? sh[j - 1] : (Real)0);
+      if constexpr (BN > 0) bsp_weights_grad_c<Real, BN>(fr[d], B[d], dB[d]);
+      else bsp_weights_grad<Real>(Bn, fr[d], B[d], dB[d]);
     }
     const int l1 = m0.x - u1, l2 = m0.y - u2, l3 = m0.z - u3;
     // per x-plane j1: sums over the (j2, j3) window in the working

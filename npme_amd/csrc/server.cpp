@@ -25,6 +25,7 @@
 #include <unordered_map>
 
 #include "npme_internal.h"
+#include "request_board.h"
 
 namespace npme {
 
@@ -46,21 +47,17 @@ struct Job {
   long N[3] = {0, 0, 0};
   double deviceGB = 0;
   double ms[5] = {0, 0, 0, 0, 0};  // keywords, set-up, copy in, CalcV, copy out
-  bool done = false;
 };
 
 }  // namespace
 
+// the admission state (reserved bytes, running requests) is guarded by the
+// board's mutex: it is read and changed only in the board's callbacks
 struct Server {
-  std::mutex mtx;
-  std::condition_variable cvWork, cvDone;
-  std::deque<Job *> queue;
-  std::unordered_map<long, Job *> jobs;
+  RequestBoard<Job> board;
   std::vector<std::thread> workers;
-  long nextTicket = 1;
   size_t budget = 0, reserved = 0;
   int running = 0;
-  bool stop = false;
   int device = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double now() const
@@ -162,40 +159,27 @@ static void worker_main(Server *S)
   (void)hipSetDevice(S->device);
   Handle h;
   for (;;) {
-    Job *j = nullptr;
-    {
-      std::unique_lock<std::mutex> lk(S->mtx);
-      // the head of the queue is admitted when its reservation fits the
-      // budget, or when nothing else runs (a request larger than the budget
-      // then runs alone)
-      S->cvWork.wait(lk, [&] {
-        if (S->stop) return true;
-        if (S->queue.empty()) return false;
-        const Job *hd = S->queue.front();
-        return S->running == 0 || S->reserved + hd->reserve <= S->budget;
-      });
-      if (S->stop && S->queue.empty()) break;
-      if (S->queue.empty()) continue;
-      j = S->queue.front();
-      S->queue.pop_front();
-      S->reserved += j->reserve;
-      S->running++;
-      j->tStart = S->now();
-    }
+    // the head of the queue is admitted when its reservation fits the
+    // budget, or when nothing else runs (a request larger than the budget
+    // then runs alone)
+    Job *j = S->board.take(
+        [S](const Job &hd) { return S->running == 0 || S->reserved + hd.reserve <= S->budget; },
+        [S](Job &jb) {
+          S->reserved += jb.reserve;
+          S->running++;
+          jb.tStart = S->now();
+        });
+    if (!j) break;
     std::string err;
     int rc = guarded([&] { run_job(h, *j); });
     if (rc != NPME_OK) err = last_error();
-    {
-      std::lock_guard<std::mutex> lk(S->mtx);
-      j->status = rc;
-      j->err = err;
-      j->tEnd = S->now();
-      j->done = true;
-      S->reserved -= j->reserve;
+    S->board.finish(j, [&](Job &jb) {
+      jb.status = rc;
+      jb.err = err;
+      jb.tEnd = S->now();
+      S->reserved -= jb.reserve;
       S->running--;
-    }
-    S->cvDone.notify_all();
-    S->cvWork.notify_all();
+    });
   }
   handle_free_device(h);
   handle_destroy_streams(h);
@@ -252,18 +236,9 @@ int npme_server_submit(npme_server *srv, const npme_request *r, long *ticket)
     j->charge = r->charge;
     j->V = r->V;
     j->reserve = estimate_bytes(j->n, ft == "Helmholtz", j->fp64);
-    {
-      std::lock_guard<std::mutex> lk(S.mtx);
-      if (S.stop) throw Error{NPME_ERR_ARG, "server is shutting down"};
-      j->tSubmit = S.now();
-      j->tArrive = r->tArrival >= 0 ? r->tArrival : j->tSubmit;
-      j->ticket = S.nextTicket++;
-      *ticket = j->ticket;
-      S.queue.push_back(j.get());
-      S.jobs[j->ticket] = j.get();
-      j.release();
-    }
-    S.cvWork.notify_one();
+    j->tSubmit = S.now();
+    j->tArrive = r->tArrival >= 0 ? r->tArrival : j->tSubmit;
+    if (!S.board.submit(std::move(j), *ticket)) throw Error{NPME_ERR_ARG, "server is shutting down"};
   });
 }
 
@@ -274,17 +249,15 @@ int npme_server_wait(npme_server *srv, long ticket, npme_request_info *info)
     return NPME_ERR_ARG;
   }
   Server &S = srv->s;
-  Job *j = nullptr;
-  {
-    std::unique_lock<std::mutex> lk(S.mtx);
-    auto it = S.jobs.find(ticket);
-    if (it == S.jobs.end()) {
-      set_error("npme_server_wait: unknown ticket");
-      return NPME_ERR_ARG;
-    }
-    j = it->second;
-    S.cvDone.wait(lk, [&] { return j->done; });
-    S.jobs.erase(it);
+  std::unique_ptr<Job> j;
+  const int w = S.board.wait(ticket, j);
+  if (w == -1) {
+    set_error("npme_server_wait: unknown ticket");
+    return NPME_ERR_ARG;
+  }
+  if (w == -2) {
+    set_error("npme_server_wait: the ticket is already being waited for");
+    return NPME_ERR_ARG;
   }
   if (info) {
     info->status = j->status;
@@ -302,7 +275,6 @@ int npme_server_wait(npme_server *srv, long ticket, npme_request_info *info)
   }
   const int rc = j->status;
   if (rc != NPME_OK) set_error(j->err);
-  delete j;
   return rc;
 }
 
@@ -329,14 +301,9 @@ void npme_server_destroy(npme_server *srv)
 {
   if (!srv) return;
   Server &S = srv->s;
-  {
-    std::lock_guard<std::mutex> lk(S.mtx);
-    S.stop = true;
-  }
-  S.cvWork.notify_all();
+  S.board.shutdown();  // the workers finish the queued requests, then stop
   for (auto &t : S.workers) t.join();
-  for (auto &kv : S.jobs) delete kv.second;
-  delete srv;
+  delete srv;  // the board releases requests that were never waited for
 }
 
 }  // extern "C"

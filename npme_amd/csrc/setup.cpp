@@ -955,9 +955,24 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.nt[d] = g.nt[d] = (int)((h.M[d] + h.T[d] - 1) / h.T[d]);
   }
   {
-    const int es = (h.fp64 ? 8 : 4) * (h.complexK ? 2 : 1);
+    // largest power-of-two tile edge (16 at most) whose theta block plus
+    // halo fits 64 KB; every accepted order fits at some edge >= 1
+    // (complex FP64 at order 16: edge 1, 16^3 points of 16 bytes)
+    const long es = (h.fp64 ? 8 : 4) * (h.complexK ? 2 : 1);
+    auto bytes = [&](int TI) {
+      const long E = TI + h.Bn - 1;
+      return E * E * E * es;
+    };
     h.TI = 16;
-    while (h.TI > 4 && (long)(h.TI + h.Bn - 1) * (h.TI + h.Bn - 1) * (h.TI + h.Bn - 1) * es > 65536) h.TI /= 2;
+    while (h.TI > 1 && bytes(h.TI) > 65536) h.TI /= 2;
+    int maxLds = 0;
+    NPME_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, h.device));
+    if (bytes(h.TI) > std::min<long>(65536, maxLds)) {
+      std::ostringstream os;
+      os << "interpolation tile of " << bytes(h.TI) << " bytes exceeds the LDS of a workgroup ("
+         << std::min<long>(65536, maxLds) << " bytes)";
+      throw Error{NPME_ERR_UNSUPPORTED, os.str()};
+    }
   }
   g.TI = h.TI;
   for (int d = 0; d < 3; d++) h.nti[d] = g.nti[d] = (int)((h.M[d] + h.TI - 1) / h.TI);

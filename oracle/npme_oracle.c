@@ -825,6 +825,22 @@ static long qstart(double w) { return (w < 0) ? (long) (-w) + 1 : (long) (-w); }
 static int qcheck(long ms, long N, long Bn) { return (ms >= -N / 4) && (ms + Bn - 1 <= N / 4 - 1); }
 static long dft2arr(long m, long M) { return (m < 0) ? m + M : m; }
 
+int npo_spline_start(size_t n, const double *coord, const long N[3],
+                     const double L[3], long Bn, const double R[3], long *mStart)
+/* window start of every charge per axis: wX = N1/L1*(x - R0x)
+ * (src/RecSumQ.cpp:106-108, :195-197) and mStart from
+ * NPME_GetQpme_DFT_StartIndex, checked by NPME_CheckQpme_DFT_StartIndex
+ * (src/RecSumSupportFunctions.h:123-149) */
+{
+  for (size_t i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      const double w = N[d] / L[d] * (coord[3 * i + d] - R[d]);
+      mStart[3 * i + d] = qstart(w);
+      if (!qcheck(mStart[3 * i + d], N[d], Bn)) return -1;
+    }
+  return 0;
+}
+
 int npo_spread_basic(size_t n, const double *q, int isComplex,
                      const double *coord, const long N[3], const double L[3],
                      long Bn, const double R[3], double *Qd)

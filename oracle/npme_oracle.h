@@ -93,6 +93,12 @@ int npo_cell_index(size_t n, const double *coord, double Rdir, long nNeigh,
                    long nCellClust1D, long nCellA[3], unsigned long *totA,
                    unsigned long *totB);
 
+/* first grid index m of every charge's spline window per axis,
+ * mStart[n][3] (src/RecSumQ.cpp:106-108, src/RecSumSupportFunctions.h:
+ * 123-149); returns -1 if a window leaves [-N/4, N/4-1] */
+int npo_spline_start(size_t n, const double *coord, const long N[3],
+                     const double L[3], long Bn, const double R[3], long *mStart);
+
 /* ---- FFT (MKL DFTI boundary, src/ExtLibrary.cpp:930-1033) ------------ */
 /* in-place complex FFT, no normalisation, sign = -1 forward, +1 backward */
 void npo_fft1d(long N, double *z, int sign);

@@ -74,6 +74,7 @@ def lib():
         L.npo_fgrid_compact.argtypes = [C.POINTER(Kernel), p, p, p, p, p]
         L.npo_lambda2.argtypes = [l, l, p]
         L.npo_spread_basic.argtypes = [sz, p, i, p, p, p, l, p, p]
+        L.npo_spline_start.argtypes = [sz, p, p, p, l, p, p]
         L.npo_theta_8parity.argtypes = [p, l, p, p, p]
         L.npo_interp_basic.argtypes = [sz, p, p, p, p, l, p, p]
         L.npo_direct_sr.argtypes = [C.POINTER(Kernel), sz, p, p, p]
@@ -258,6 +259,18 @@ def spread_basic(q, coord, N, L, Bn, R):
     if rc != 0:
         raise ValueError("spline window outside the compact grid")
     return Q
+
+
+def spline_start(coord, N, L, Bn, R):
+    """mStart[n][3]: first grid index of each charge's spline window"""
+    coord = _f64(coord)
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    ms = np.zeros((len(coord), 3), np.int64)
+    rc = lib().npo_spline_start(len(coord), _ptr(coord), _ptr(N), _ptr(_f64(L)), int(Bn),
+                                _ptr(_f64(R)), _ptr(ms))
+    if rc != 0:
+        raise ValueError("spline window outside the compact grid")
+    return ms
 
 
 def theta_8parity(N, Bn, F, Q):

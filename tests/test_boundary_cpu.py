@@ -159,3 +159,21 @@ def test_cpp_header_files_and_descriptors(tmp_path):
                     "-Wl,-rpath," + libdir, "-o", exe], check=True)
     r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.strip() == "ok", (r.returncode, r.stdout, r.stderr)
+
+
+def test_server_request_board_threads(tmp_path):
+    """The serving loop's ticket bookkeeping (npme_amd/csrc/request_board.h)
+    under threads, on the host: 20,000 requests submitted while four
+    threads wait for them and six workers finish them in random order; a
+    ticket is waited for once, a second waiter is refused, an unknown
+    ticket is refused (tests/cpp/request_board_host.cpp)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "request_board")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-pthread", "-I",
+                    os.path.join(root, "npme_amd", "csrc"),
+                    os.path.join(root, "tests", "cpp", "request_board_host.cpp"), "-o", exe],
+                   check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -170,6 +170,36 @@ def test_rsem_grid_choices(oracle):
         oracle.rsem_static(q, coord, 6, 0.5, 1e-6, 8.0)   # no alpha = 0.5 row
 
 
+@pytest.mark.parametrize("alpha,kind,Bn,Nder", [(-1.0, "LAPLACE", 6, 4), (-1.0, "LAPLACE", 6, 6),
+                                                 (-1.0, "LAPLACE", 8, 7), (1.0, "RALPHA", 6, 5),
+                                                 (1.0, "RALPHA", 8, 6)])
+def test_rsem_pred_error_tracks_measured(oracle, alpha, kind, Bn, Nder):
+    """The RSEM static error model (CalcPredError_V, src/EstimateParm.cpp:
+    421-558) predicts the absolute RMS error of V.  Over a sweep of grids
+    inside the model's reliable range of u = N Rdir / L, its prediction
+    must track the measured whole-path error of the oracle's CalcV against
+    the exact O(N^2) sum within a factor 2.5 either way, for 1/r and r^1;
+    and both must fall as the grid is refined.  A dropped factor (the
+    Rdir^alpha or (Rdir/L)^1.5 scale, the charge norm) or a wrong sign in
+    the fitted polynomial P(u) moves the ratio by orders of magnitude."""
+    n, Lb, Rdir = 3000, 28.0, 6.0
+    coord, q = uniform_box(n, Lb)
+    k = oracle.kernel(getattr(oracle, kind), Nder, Rdir, alpha=alpha)
+    Ve = oracle.exact(k, coord, q)
+    preds, meas = [], []
+    for Ng in (60, 72, 84, 96):
+        eps, reliable = oracle.rsem_static_pred_error(q, coord, Bn, alpha, Nder, Rdir, (Ng,) * 3)
+        assert reliable
+        m = oracle.compare_v(Ve, oracle.calcv(k, (Ng,) * 3, Bn, coord, q))
+        print(alpha, Bn, Nder, Ng, "predicted %.3e measured %.3e ratio %.2f" % (eps, m["errV"],
+                                                                                m["errV"] / eps))
+        assert 1 / 2.5 <= m["errV"] / eps <= 2.5
+        preds.append(eps)
+        meas.append(m["errV"])
+    assert all(a > b for a, b in zip(preds, preds[1:]))
+    assert all(a > b for a, b in zip(meas, meas[1:]))
+
+
 def test_rsem_helmholtz_grid_choice(oracle):
     n = 500000
     coord, _ = uniform_box(n, 120.0, 3001, 3002)
@@ -179,24 +209,65 @@ def test_rsem_helmholtz_grid_choice(oracle):
 
 
 # ------------------------------------------------------------ geometry
-def test_pme_corr_box_closed_form(oracle):
-    X0 = np.array([40.0, 75.0, 100.0])
-    R0 = np.array([1.0, -2.0, 3.0])
-    N = np.array([60, 80, 100])
-    dl = np.array([8.0, 8.0, 8.0])
-    X, R = oracle.pme_corr_box(X0, R0, N, dl, 6)
+def _mstart(w):
+    """NPME_GetQpme_DFT_StartIndex restated with numpy (C truncation)"""
+    return np.where(w < 0, np.trunc(-w) + 1, np.trunc(-w)).astype(np.int64)
+
+
+def _windows_fit(xmin, xmax, Xp, dl, N, Bn):
+    """do the spline windows of every charge in [xmin, xmax] fit the compact
+    grid [-N/4, N/4-1] when the half box is Xp (top-aligned: R + Xp/2 =
+    xmax)?  Brute force over 20,001 positions including both ends."""
+    R = xmax - 0.5 * Xp
+    L = 2.0 * (Xp + dl)
+    x = np.linspace(xmin, xmax, 20001)
+    m = _mstart(N / L * (x - R))
+    return m.min() >= -(N // 4) and m.max() + Bn - 1 <= N // 4 - 1
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_pme_corr_box_is_minimal(oracle, case):
+    """NPME_RecSumInterface_GetPMECorrBox (src/RecSumSupportFunctions.cpp:
+    137-147, 276-296) against brute force: with the returned box every
+    spline window of the charges fits [-N/4, N/4-1]; the box is not larger
+    than the smallest one that fits (found by bisection over the half box
+    with the same top alignment) by more than two grid spacings (the
+    formula keeps a margin of one spline order); it never shrinks below
+    the charges' extent X0; and its top edge is the charges' maximum."""
+    rng = np.random.default_rng(100 + case)
+    X0 = rng.uniform(5.0, 300.0, 3)
+    R0 = rng.uniform(-50.0, 50.0, 3)
+    Bn = int(rng.choice([4, 6, 8, 12]))
+    # coarse grids or narrow extensions, where the correction is active
+    # (with del = Rdir and grid spacings below del / (Bn + 1) the charges'
+    # own extent already fits and X = X0)
+    dl = np.full(3, float(rng.choice([0.5, 1.0, 2.0, 8.0])))
+    N = np.array([4 * int(rng.integers(Bn + 1, max(Bn + 2, int(X0[d] / 2)))) for d in range(3)])
+    X, R = oracle.pme_corr_box(X0, R0, N, dl, Bn)
+    active = 0
     for d in range(3):
-        a1 = 1 - 2 * 7 / N[d]
-        a2 = 1 - 4 * 7 / N[d]
-        exp = max(X0[d], X0[d] / a1 - dl[d] / 2 * a2 / a1)
-        assert abs(X[d] - exp) < 1e-12
-        assert abs(R[d] - (R0[d] + 0.5 * (X0[d] - X[d]))) < 1e-12
-        # the corrected box keeps every spline window in [-N/4, N/4-1]:
-        Lg = 2 * (X[d] + dl[d])
-        for x in (R0[d] - X0[d] / 2, R0[d] + X0[d] / 2):
-            w = N[d] / Lg * (x - R[d])
-            ms = int(-w) + 1 if w < 0 else int(-w)
-            assert ms >= -N[d] // 4 and ms + 6 - 1 <= N[d] // 4 - 1
+        xmin, xmax = R0[d] - X0[d] / 2, R0[d] + X0[d] / 2
+        assert X[d] >= X0[d]
+        assert abs((R[d] + X[d] / 2) - xmax) <= 1e-12 * (abs(xmax) + X[d])
+        assert _windows_fit(xmin, xmax, X[d], dl[d], N[d], Bn)
+        if X[d] == X0[d]:
+            continue  # the charges' own extent already fits: minimal
+        lo, hi = X0[d], X[d]
+        if _windows_fit(xmin, xmax, lo, dl[d], N[d], Bn):
+            Xmin = lo
+        else:
+            for _ in range(60):
+                mid = 0.5 * (lo + hi)
+                if _windows_fit(xmin, xmax, mid, dl[d], N[d], Bn):
+                    hi = mid
+                else:
+                    lo = mid
+            Xmin = hi
+        h = 2.0 * (X[d] + dl[d]) / N[d]
+        print(case, d, "N", N[d], "Bn", Bn, "X", X[d], "Xmin", Xmin, "slack/h", (X[d] - Xmin) / h)
+        assert 0.0 <= X[d] - Xmin <= 2.0 * h
+        active += 1
+    assert active >= 1
 
 
 def test_cell_index_containment(oracle):
