@@ -70,6 +70,66 @@ int npme_add_missing_keywords(npme_keywords *kw, size_t nCharge,
  *      accumulation) or fp64. ------------------------------------------ */
 int npme_create(npme_handle **h, const npme_keywords *kw, size_t nCharge,
                 const double *coord);
+
+/* ---- user-defined kernels: NPME_KfuncReal / NPME_KfuncComplex
+ * (src/KernelFunction.h:25-116) as C callbacks.  ctx is passed back
+ * unchanged.  The in-place conventions are the reference's:
+ *   value(ctx, n, x_f0, y, z): x_f0[i] holds x on entry, f(x, y, z) on
+ *     return (complex: x_f0_r holds x, then Re f; f0_i receives Im f);
+ *   grad(ctx, n, f0, x_fX, y_fY, z_fZ): f0 receives f; x_fX, y_fY, z_fZ
+ *     hold x, y, z on entry and df/dx, df/dy, df/dz on return (complex: the
+ *     real parts in place, the imaginary parts in fX_i, fY_i, fZ_i).
+ * The library may call a callback from several host threads at once (the
+ * reference calls Calc from OpenMP threads).  The callbacks are borrowed:
+ * they must stay valid until the handle is destroyed. */
+typedef struct {
+  void *ctx;
+  void (*value)(void *ctx, size_t n, double *x_f0, const double *y, const double *z);
+  void (*grad)(void *ctx, size_t n, double *f0, double *x_fX, double *y_fY, double *z_fZ);
+} npme_kfunc_real;
+typedef struct {
+  void *ctx;
+  void (*value)(void *ctx, size_t n, double *x_f0_r, double *f0_i, const double *y,
+                const double *z);
+  void (*grad)(void *ctx, size_t n, double *f0_r, double *f0_i, double *x_fX_r, double *fX_i,
+               double *y_fY_r, double *fY_i, double *z_fZ_r, double *fZ_i);
+} npme_kfunc_complex;
+
+/* NPME_InterfaceReal_GenFunc::SetUp (src/Interface.h:291-296,
+ * src/Interface.cpp:1606-1671) and NPME_InterfaceComplex_GenFunc::SetUp
+ * (src/Interface.h:563-650) with user kernels: func is the full kernel
+ * (CalcV_exact, evaluated on the host as the reference does), funcLR the
+ * smooth long-range part, sampled on the kernel grid on the host at every
+ * set-up (and fself = funcLR(0, 0, 0)), funcSR the short-range part, which
+ * must be radial: it is tabulated on the host as piecewise polynomials in r
+ * over [0, Rdir] and evaluated from that table by the device direct sum
+ * (npme_get_table_error reports the table's error).  The keywords must hold
+ * the grid and split (Rdir, BnOrder, N1-3, n1-3, del1-3, a1-3, nNeigh,
+ * nCellClust1D); funcType, EwaldSplit and nDeriv are not used. */
+int npme_create_genfunc_real(npme_handle **h, const npme_keywords *kw, size_t nCharge,
+                             const double *coord, const npme_kfunc_real *func,
+                             const npme_kfunc_real *funcLR, const npme_kfunc_real *funcSR);
+int npme_create_genfunc_complex(npme_handle **h, const npme_keywords *kw, size_t nCharge,
+                                const double *coord, const npme_kfunc_complex *func,
+                                const npme_kfunc_complex *funcLR,
+                                const npme_kfunc_complex *funcSR);
+/* the derivative-match split of a built-in kernel, as SetParm of
+ * NPME_Kfunc_{Laplace,Ralpha,Helmholtz}_{LR,SR}_DM computes it
+ * (src/FunctionDerivMatch.cpp:42-450): inside Rdir the long-range part is
+ * sum_k a_k r^2k and its (1/r) d/dr is sum_k b_k r^2k, k = 0..Nder; a and b
+ * are [Nder + 1][2] (re, im).  funcType "Laplace", "Ralpha" (alpha) or
+ * "Helmholtz" (k0_r, k0_i). */
+int npme_dm_coefficients(const char *funcType, double alpha, double k0_r, double k0_i,
+                         int Nder, double Rdir, double *a, double *b);
+/* NPME_EwaldSplitOrig_Rdir2Beta (src/KernelFunctionLaplace.cpp:41-72): the
+ * beta of the erf/erfc split of 1/r for a cutoff Rdir and tolerance tol */
+int npme_ewald_rdir2beta(double Rdir, double tol, double *beta);
+/* largest error of the short-range table of a user kernel against funcSR at
+ * check points between the interpolation nodes, relative to the largest
+ * |f| (err[0]) and |f'(r)/r| (err[1]) over the same octave of r; both 0 for
+ * the built-in kernels.  No reference counterpart. */
+int npme_get_table_error(const npme_handle *h, double err[2]);
+
 /* ::ResetCoord (src/Interface.h:186-187; src/Interface.cpp:1958-1987) */
 int npme_reset_coord(npme_handle *h, size_t nCharge, const double *coord);
 void npme_destroy(npme_handle *h);

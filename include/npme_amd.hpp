@@ -16,10 +16,11 @@
  *   npme->CalcV(&V[0], n, &q[0], &coord[0], 1, std::cout);
  *
  * The built-in kernels (Laplace, r^alpha, Helmholtz, each with its DM
- * split) are evaluated on the GPU.  The NPME_Kfunc* objects of
- * NPME_KernelList are descriptors of those kernels: a user-defined
- * NPME_KfuncReal / NPME_KfuncComplex subclass is not evaluated, and SetUp
- * returns false for it.  Failures return false and print the library's
+ * split) run natively on the GPU.  A user-defined NPME_KfuncReal /
+ * NPME_KfuncComplex subclass (the reference's plugin interface, pure
+ * virtual Calc and Print) runs through GenFunc::SetUp: its long-range part
+ * is sampled on the host, its radial short-range part tabulated for the
+ * device direct sum.  Failures return false and print the library's
  * message on the log stream, as the reference does.
  */
 #ifndef NPME_AMD_HPP
@@ -182,161 +183,597 @@ inline bool NPME_AddMissingKeywords(const char *keywordFile, NPME_KeywordInput &
                                  std::vector<double>(), chargeComplex, printLog, ofs_log);
 }
 
-// ------------------------------------------------------------ kernel descriptors
-// NPME_KfuncReal / NPME_KfuncComplex (src/KernelFunction.h:25-116) and the
-// built-in kernels of NPME_KernelList (src/Interface.h:757-771), with the
-// constructors and SetParm of src/KernelFunction{Laplace,Ralpha,Helmholtz}.h.
-// A descriptor records its parameters; SetUp passes those that were set
-// (alpha, k0, nDeriv, Rdir) to the library in place of the keywords'.
+// ------------------------------------------------------------ kernel plugins
+// NPME_KfuncReal / NPME_KfuncComplex (src/KernelFunction.h:25-116): the
+// plugin bases with the reference's pure virtual Calc and Print.  A user
+// subclass defines them; the library calls Calc on the host (possibly from
+// several threads at once, as the reference's OpenMP loops do, so Calc must
+// be safe to call concurrently and must not throw).
+//
+// The built-in kernels below (src/KernelFunction{Laplace,Ralpha,Helmholtz}.h)
+// evaluate on the host too, with the reference's formulas and cutoff
+// tests.  When GenFunc::SetUp receives the full, long-range and short-range
+// objects of one built-in family, the GPU runs its native kernels for them;
+// any other combination runs the user-kernel path (host-sampled long-range
+// grid, tabulated short-range part).
+struct NPME_BuiltinKernel {
+  enum Family { LAPLACE, RALPHA, HELMHOLTZ } family = LAPLACE;
+  enum Part { FULL, LR, SR } part = FULL;
+  double alpha = 0, k0_r = 0, k0_i = 0, Rdir = 0;
+  int Nder = 0;
+};
+
 class NPME_KfuncReal {
  public:
+  NPME_KfuncReal() {}
   virtual ~NPME_KfuncReal() {}
-  virtual bool builtin() const { return false; }
-  // parameters carried by a built-in descriptor: alpha, Nder, Rdir
-  bool hasAlpha = false, hasDM = false;
-  double alpha = 0, Rdir = 0;
-  int Nder = 0;
+  NPME_KfuncReal(const NPME_KfuncReal &) {}
+  NPME_KfuncReal &operator=(const NPME_KfuncReal &) { return *this; }
+
+  virtual void Print(std::ostream &os) const = 0;
+  // x_f0[N] holds x on entry and f(x, y, z) on return
+  virtual void Calc(const size_t N, double *x_f0, const double *y, const double *z) const = 0;
+  // f0[N] receives f; x_fX, y_fY, z_fZ hold x, y, z on entry and the
+  // gradient on return
+  virtual void Calc(const size_t N, double *f0, double *x_fX, double *y_fY,
+                    double *z_fZ) const = 0;
+  // the built-in kernel this object is, or nullptr (a user kernel)
+  virtual const NPME_BuiltinKernel *npme_builtin() const { return nullptr; }
 };
+
 class NPME_KfuncComplex {
  public:
+  NPME_KfuncComplex() {}
   virtual ~NPME_KfuncComplex() {}
-  virtual bool builtin() const { return false; }
-  bool hasK0 = false, hasDM = false;
-  double k0_r = 0, k0_i = 0, Rdir = 0;
-  int Nder = 0;
+  NPME_KfuncComplex(const NPME_KfuncComplex &) {}
+  NPME_KfuncComplex &operator=(const NPME_KfuncComplex &) { return *this; }
+
+  virtual void Print(std::ostream &os) const = 0;
+  virtual void Calc(const size_t N, double *x_f0_r, double *f0_i, const double *y,
+                    const double *z) const = 0;
+  virtual void Calc(const size_t N, double *f0_r, double *f0_i, double *x_fX_r, double *fX_i,
+                    double *y_fY_r, double *fY_i, double *z_fZ_r, double *fZ_i) const = 0;
+  virtual const NPME_BuiltinKernel *npme_builtin() const { return nullptr; }
 };
 
-// exact kernels: NPME_Kfunc_Laplace (src/KernelFunctionLaplace.h:38),
-// NPME_Kfunc_Ralpha(alpha) (src/KernelFunctionRalpha.h:34-37),
-// NPME_Kfunc_Helmholtz(k0) (src/KernelFunctionHelmholtz.h:34-37)
+namespace npme_detail {
+// radial kernel f(r) with g(r) = (1/r) df/dr (complex: re, im) applied to
+// the reference's in-place arrays
+template <typename F>
+inline void real_value(size_t N, double *x_f0, const double *y, const double *z, F &&fg)
+{
+  for (size_t i = 0; i < N; i++) {
+    const double r2 = x_f0[i] * x_f0[i] + y[i] * y[i] + z[i] * z[i];
+    double f, g;
+    fg(r2, f, g);
+    x_f0[i] = f;
+  }
+}
+template <typename F>
+inline void real_grad(size_t N, double *f0, double *x, double *y, double *z, F &&fg)
+{
+  for (size_t i = 0; i < N; i++) {
+    const double r2 = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
+    double f, g;
+    fg(r2, f, g);
+    f0[i] = f;
+    x[i] *= g;
+    y[i] *= g;
+    z[i] *= g;
+  }
+}
+template <typename F>
+inline void cx_value(size_t N, double *x_f0_r, double *f0_i, const double *y, const double *z,
+                     F &&fg)
+{
+  for (size_t i = 0; i < N; i++) {
+    const double r2 = x_f0_r[i] * x_f0_r[i] + y[i] * y[i] + z[i] * z[i];
+    double f[2], g[2];
+    fg(r2, f, g);
+    x_f0_r[i] = f[0];
+    f0_i[i] = f[1];
+  }
+}
+template <typename F>
+inline void cx_grad(size_t N, double *f0r, double *f0i, double *x, double *xi, double *y,
+                    double *yi, double *z, double *zi, F &&fg)
+{
+  for (size_t i = 0; i < N; i++) {
+    const double r2 = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
+    double f[2], g[2];
+    fg(r2, f, g);
+    f0r[i] = f[0];
+    f0i[i] = f[1];
+    const double X = x[i], Y = y[i], Z = z[i];
+    x[i] = X * g[0];
+    xi[i] = X * g[1];
+    y[i] = Y * g[0];
+    yi[i] = Y * g[1];
+    z[i] = Z * g[0];
+    zi[i] = Z * g[1];
+  }
+}
+// sum_k c_k r2^k, k = 0..n (c as (re, im) pairs)
+inline void horner2(const double *c, int n, double r2, double out[2])
+{
+  double re = c[2 * n], im = c[2 * n + 1];
+  for (int k = n - 1; k >= 0; k--) {
+    re = re * r2 + c[2 * k];
+    im = im * r2 + c[2 * k + 1];
+  }
+  out[0] = re;
+  out[1] = im;
+}
+// the DM coefficients of a built-in kernel through the library
+inline bool dm_fit(const char *ft, double alpha, double k0r, double k0i, int Nder, double Rdir,
+                   double *a, double *b, bool PRINT, std::ostream &os)
+{
+  const int rc = npme_dm_coefficients(ft, alpha, k0r, k0i, Nder, Rdir, a, b);
+  if (rc != NPME_OK && PRINT) os << "NPME error: " << npme_last_error() << "\n";
+  return rc == NPME_OK;
+}
+// 1/r, r^alpha, exp(i k0 r)/r with g = (1/r) df/dr
+// (src/KernelFunctionLaplace.cpp:126-150, src/KernelFunctionRalpha.cpp:57-82,
+// src/KernelFunctionHelmholtz.cpp:162-200)
+inline void laplace_fg(double r2, double &f, double &g)
+{
+  const double r = std::sqrt(r2);
+  f = 1.0 / r;
+  g = -f / r2;
+}
+inline void ralpha_fg(double alpha, double r2, double &f, double &g)
+{
+  f = std::pow(r2, 0.5 * alpha);
+  g = alpha * f / r2;
+}
+inline void helmholtz_fg(double k0r, double k0i, double r2, double f[2], double g[2])
+{
+  const double r = std::sqrt(r2), d = std::exp(-k0i * r) / r;
+  f[0] = d * std::cos(k0r * r);
+  f[1] = d * std::sin(k0r * r);
+  // (i k0 - 1/r) f / r
+  const double ar = -k0i - 1.0 / r, ai = k0r;
+  g[0] = (ar * f[0] - ai * f[1]) / r;
+  g[1] = (ar * f[1] + ai * f[0]) / r;
+}
+}  // namespace npme_detail
+
+// exact kernels: NPME_Kfunc_Laplace (src/KernelFunctionLaplace.h:34-78),
+// NPME_Kfunc_Ralpha(alpha) (src/KernelFunctionRalpha.h:34-90),
+// NPME_Kfunc_Helmholtz(k0) (src/KernelFunctionHelmholtz.h:34-100)
 class NPME_Kfunc_Laplace : public NPME_KfuncReal {
  public:
-  bool builtin() const override { return true; }
+  NPME_Kfunc_Laplace() { _d.family = NPME_BuiltinKernel::LAPLACE; }
+  void Print(std::ostream &os) const override { os << "NPME_Kfunc_Laplace: f = 1/r\n"; }
+  void Calc(const size_t N, double *x_f0, const double *y, const double *z) const override
+  {
+    npme_detail::real_value(N, x_f0, y, z, npme_detail::laplace_fg);
+  }
+  void Calc(const size_t N, double *f0, double *x, double *y, double *z) const override
+  {
+    npme_detail::real_grad(N, f0, x, y, z, npme_detail::laplace_fg);
+  }
+  const NPME_BuiltinKernel *npme_builtin() const override { return &_d; }
+
+ private:
+  NPME_BuiltinKernel _d;
 };
+
 class NPME_Kfunc_Ralpha : public NPME_KfuncReal {
  public:
-  NPME_Kfunc_Ralpha() {}
-  explicit NPME_Kfunc_Ralpha(double a) { SetParm(a); }
-  bool builtin() const override { return true; }
+  NPME_Kfunc_Ralpha() { _d.family = NPME_BuiltinKernel::RALPHA; }
+  explicit NPME_Kfunc_Ralpha(double a) : NPME_Kfunc_Ralpha() { SetParm(a); }
   bool SetParm(const double a)
   {
-    alpha = a;
-    hasAlpha = true;
+    _d.alpha = a;
     return true;
   }
+  void Print(std::ostream &os) const override { os << "NPME_Kfunc_Ralpha: alpha = " << _d.alpha << "\n"; }
+  void Calc(const size_t N, double *x_f0, const double *y, const double *z) const override
+  {
+    const double a = _d.alpha;
+    npme_detail::real_value(N, x_f0, y, z,
+                            [a](double r2, double &f, double &g) { npme_detail::ralpha_fg(a, r2, f, g); });
+  }
+  void Calc(const size_t N, double *f0, double *x, double *y, double *z) const override
+  {
+    const double a = _d.alpha;
+    npme_detail::real_grad(N, f0, x, y, z,
+                           [a](double r2, double &f, double &g) { npme_detail::ralpha_fg(a, r2, f, g); });
+  }
+  const NPME_BuiltinKernel *npme_builtin() const override { return &_d; }
+
+ private:
+  NPME_BuiltinKernel _d;
 };
+
 class NPME_Kfunc_Helmholtz : public NPME_KfuncComplex {
  public:
-  NPME_Kfunc_Helmholtz() {}
-  explicit NPME_Kfunc_Helmholtz(_Complex double k0) { SetParm(k0); }
-  bool builtin() const override { return true; }
+  NPME_Kfunc_Helmholtz() { _d.family = NPME_BuiltinKernel::HELMHOLTZ; }
+  explicit NPME_Kfunc_Helmholtz(_Complex double k0) : NPME_Kfunc_Helmholtz() { SetParm(k0); }
   bool SetParm(const _Complex double k0)
   {
-    k0_r = __real__ k0;
-    k0_i = __imag__ k0;
-    hasK0 = true;
+    _d.k0_r = __real__ k0;
+    _d.k0_i = __imag__ k0;
     return true;
+  }
+  void Print(std::ostream &os) const override
+  {
+    os << "NPME_Kfunc_Helmholtz: k0 = " << _d.k0_r << " + " << _d.k0_i << "i\n";
+  }
+  void Calc(const size_t N, double *x_f0_r, double *f0_i, const double *y,
+            const double *z) const override
+  {
+    const double kr = _d.k0_r, ki = _d.k0_i;
+    npme_detail::cx_value(N, x_f0_r, f0_i, y, z, [kr, ki](double r2, double *f, double *g) {
+      npme_detail::helmholtz_fg(kr, ki, r2, f, g);
+    });
+  }
+  void Calc(const size_t N, double *f0_r, double *f0_i, double *x, double *xi, double *y, double *yi,
+            double *z, double *zi) const override
+  {
+    const double kr = _d.k0_r, ki = _d.k0_i;
+    npme_detail::cx_grad(N, f0_r, f0_i, x, xi, y, yi, z, zi, [kr, ki](double r2, double *f, double *g) {
+      npme_detail::helmholtz_fg(kr, ki, r2, f, g);
+    });
+  }
+  const NPME_BuiltinKernel *npme_builtin() const override { return &_d; }
+
+ private:
+  NPME_BuiltinKernel _d;
+};
+
+// the derivative-match split parts: inside Rdir the long-range part is the
+// even polynomial sum a_k r^2k (gradient factor sum b_k r^2k) and the
+// short-range part the kernel minus it; outside, LR is the kernel and SR is
+// zero.  The inside tests are the reference's per family: Laplace r <= Rdir
+// (src/KernelFunctionLaplace.cpp:394-437,762-803), r^alpha r2 <= Rdir^2
+// (src/KernelFunctionRalpha.cpp:343-385,718-758), Helmholtz LR r <= Rdir and
+// SR r < Rdir (src/KernelFunctionHelmholtz.cpp:506-560,962-1029).
+class NPME_DMSplitBase {
+ protected:
+  NPME_BuiltinKernel _d;
+  double _a[2 * 31] = {0}, _b[2 * 31] = {0};
+  bool fit(const char *ft, bool PRINT, std::ostream &os)
+  {
+    return npme_detail::dm_fit(ft, _d.alpha, _d.k0_r, _d.k0_i, _d.Nder, _d.Rdir, _a, _b, PRINT, os);
+  }
+  bool inside(double r2) const
+  {
+    const double R = _d.Rdir;
+    if (_d.family == NPME_BuiltinKernel::RALPHA) return !(r2 > R * R);
+    if (_d.family == NPME_BuiltinKernel::HELMHOLTZ && _d.part == NPME_BuiltinKernel::SR)
+      return std::sqrt(r2) < R;
+    return !(std::sqrt(r2) > R);
+  }
+  void full(double r2, double f[2], double g[2]) const
+  {
+    f[1] = g[1] = 0;
+    if (_d.family == NPME_BuiltinKernel::LAPLACE) npme_detail::laplace_fg(r2, f[0], g[0]);
+    else if (_d.family == NPME_BuiltinKernel::RALPHA) npme_detail::ralpha_fg(_d.alpha, r2, f[0], g[0]);
+    else npme_detail::helmholtz_fg(_d.k0_r, _d.k0_i, r2, f, g);
+  }
+  void eval(double r2, double f[2], double g[2]) const
+  {
+    const bool in = inside(r2);
+    if (_d.part == NPME_BuiltinKernel::LR) {
+      if (in) {
+        npme_detail::horner2(_a, _d.Nder, r2, f);
+        npme_detail::horner2(_b, _d.Nder, r2, g);
+      } else {
+        full(r2, f, g);
+      }
+    } else {
+      f[0] = f[1] = g[0] = g[1] = 0;
+      if (in) {
+        double pa[2], pb[2];
+        full(r2, f, g);
+        npme_detail::horner2(_a, _d.Nder, r2, pa);
+        npme_detail::horner2(_b, _d.Nder, r2, pb);
+        f[0] -= pa[0];
+        f[1] -= pa[1];
+        g[0] -= pb[0];
+        g[1] -= pb[1];
+      }
+    }
   }
 };
 
-// DM split kernels: (Nder, Rdir) for Laplace (src/KernelFunctionLaplace.h:
-// 87-116,155-183), (alpha, Nder, Rdir) for r^alpha
-// (src/KernelFunctionRalpha.h:94-126,169-200), (k0, Nder, Rdir) for
-// Helmholtz (src/KernelFunctionHelmholtz.h:102-133,182-212)
-#define NPME_AMD_LAPLACE_DM(name)                                                     \
-  class name : public NPME_KfuncReal {                                                \
-   public:                                                                            \
-    name() {}                                                                         \
-    name(int nd, double rd) { SetParm(nd, rd, false, std::cout); }                    \
-    bool builtin() const override { return true; }                                    \
-    bool SetParm(const int nd, const double rd, bool, std::ostream &)                 \
-    {                                                                                 \
-      Nder = nd;                                                                      \
-      Rdir = rd;                                                                      \
-      hasDM = true;                                                                   \
-      return true;                                                                    \
-    }                                                                                 \
+#define NPME_AMD_REAL_SPLIT(name, FAM, PART)                                                   \
+  class name : public NPME_KfuncReal, private NPME_DMSplitBase {                                \
+   public:                                                                                      \
+    name()                                                                                      \
+    {                                                                                           \
+      _d.family = NPME_BuiltinKernel::FAM;                                                      \
+      _d.part = NPME_BuiltinKernel::PART;                                                       \
+      _d.alpha = -1.0;                                                                          \
+    }                                                                                           \
+    void Print(std::ostream &os) const override                                                 \
+    {                                                                                           \
+      os << #name << ": Nder = " << _d.Nder << " Rdir = " << _d.Rdir << "\n";                   \
+    }                                                                                           \
+    void Calc(const size_t N, double *x_f0, const double *y, const double *z) const override    \
+    {                                                                                           \
+      npme_detail::real_value(N, x_f0, y, z, [this](double r2, double &f, double &g) {          \
+        double F[2], G[2];                                                                      \
+        eval(r2, F, G);                                                                         \
+        f = F[0];                                                                               \
+        g = G[0];                                                                               \
+      });                                                                                       \
+    }                                                                                           \
+    void Calc(const size_t N, double *f0, double *x, double *y, double *z) const override      \
+    {                                                                                           \
+      npme_detail::real_grad(N, f0, x, y, z, [this](double r2, double &f, double &g) {          \
+        double F[2], G[2];                                                                      \
+        eval(r2, F, G);                                                                         \
+        f = F[0];                                                                               \
+        g = G[0];                                                                               \
+      });                                                                                       \
+    }                                                                                           \
+    const NPME_BuiltinKernel *npme_builtin() const override { return &_d; }
+// Laplace DM parts: SetParm(Nder, Rdir) (src/KernelFunctionLaplace.h:80-212)
+#define NPME_AMD_LAPLACE_DM(name, PART)                                                         \
+  NPME_AMD_REAL_SPLIT(name, LAPLACE, PART)                                                      \
+    name(int nd, double rd) : name() { SetParm(nd, rd, false, std::cout); }                     \
+    bool SetParm(const int nd, const double rd, bool PRINT, std::ostream &os)                   \
+    {                                                                                           \
+      _d.Nder = nd;                                                                             \
+      _d.Rdir = rd;                                                                             \
+      return fit("Laplace", PRINT, os);                                                         \
+    }                                                                                           \
   };
-#define NPME_AMD_RALPHA_DM(name)                                                      \
-  class name : public NPME_KfuncReal {                                                \
-   public:                                                                            \
-    name() {}                                                                         \
-    name(double a, int nd, double rd) { SetParm(a, nd, rd, false, std::cout); }       \
-    bool builtin() const override { return true; }                                    \
-    bool SetParm(const double a, const int nd, const double rd, bool, std::ostream &) \
-    {                                                                                 \
-      alpha = a;                                                                      \
-      Nder = nd;                                                                      \
-      Rdir = rd;                                                                      \
-      hasAlpha = hasDM = true;                                                        \
-      return true;                                                                    \
-    }                                                                                 \
+// r^alpha DM parts: SetParm(alpha, Nder, Rdir) (src/KernelFunctionRalpha.h:94-200)
+#define NPME_AMD_RALPHA_DM(name, PART)                                                          \
+  NPME_AMD_REAL_SPLIT(name, RALPHA, PART)                                                       \
+    name(double a, int nd, double rd) : name() { SetParm(a, nd, rd, false, std::cout); }        \
+    bool SetParm(const double a, const int nd, const double rd, bool PRINT, std::ostream &os)   \
+    {                                                                                           \
+      _d.alpha = a;                                                                             \
+      _d.Nder = nd;                                                                             \
+      _d.Rdir = rd;                                                                             \
+      return fit("Ralpha", PRINT, os);                                                          \
+    }                                                                                           \
   };
-#define NPME_AMD_HELMHOLTZ_DM(name)                                                   \
-  class name : public NPME_KfuncComplex {                                             \
-   public:                                                                            \
-    name() {}                                                                         \
-    name(_Complex double k0, int nd, double rd) { SetParm(k0, nd, rd, false, std::cout); } \
-    bool builtin() const override { return true; }                                    \
-    bool SetParm(const _Complex double k0, const int nd, const double rd, bool,       \
-                 std::ostream &)                                                      \
-    {                                                                                 \
-      k0_r = __real__ k0;                                                             \
-      k0_i = __imag__ k0;                                                             \
-      Nder = nd;                                                                      \
-      Rdir = rd;                                                                      \
-      hasK0 = hasDM = true;                                                           \
-      return true;                                                                    \
-    }                                                                                 \
-  };
-NPME_AMD_LAPLACE_DM(NPME_Kfunc_Laplace_LR_DM)
-NPME_AMD_LAPLACE_DM(NPME_Kfunc_Laplace_SR_DM)
-NPME_AMD_RALPHA_DM(NPME_Kfunc_Ralpha_LR_DM)
-NPME_AMD_RALPHA_DM(NPME_Kfunc_Ralpha_SR_DM)
-NPME_AMD_HELMHOLTZ_DM(NPME_Kfunc_Helmholtz_LR_DM)
-NPME_AMD_HELMHOLTZ_DM(NPME_Kfunc_Helmholtz_SR_DM)
+NPME_AMD_LAPLACE_DM(NPME_Kfunc_Laplace_LR_DM, LR)
+NPME_AMD_LAPLACE_DM(NPME_Kfunc_Laplace_SR_DM, SR)
+NPME_AMD_RALPHA_DM(NPME_Kfunc_Ralpha_LR_DM, LR)
+NPME_AMD_RALPHA_DM(NPME_Kfunc_Ralpha_SR_DM, SR)
 #undef NPME_AMD_LAPLACE_DM
 #undef NPME_AMD_RALPHA_DM
+#undef NPME_AMD_REAL_SPLIT
+
+// Helmholtz DM parts: SetParm(k0, Nder, Rdir)
+// (src/KernelFunctionHelmholtz.h:102-212)
+#define NPME_AMD_HELMHOLTZ_DM(name, PART)                                                       \
+  class name : public NPME_KfuncComplex, private NPME_DMSplitBase {                             \
+   public:                                                                                      \
+    name()                                                                                      \
+    {                                                                                           \
+      _d.family = NPME_BuiltinKernel::HELMHOLTZ;                                                \
+      _d.part = NPME_BuiltinKernel::PART;                                                       \
+    }                                                                                           \
+    name(_Complex double k0, int nd, double rd) : name() { SetParm(k0, nd, rd, false, std::cout); } \
+    bool SetParm(const _Complex double k0, const int nd, const double rd, bool PRINT,           \
+                 std::ostream &os)                                                              \
+    {                                                                                           \
+      _d.k0_r = __real__ k0;                                                                    \
+      _d.k0_i = __imag__ k0;                                                                    \
+      _d.Nder = nd;                                                                             \
+      _d.Rdir = rd;                                                                             \
+      return fit("Helmholtz", PRINT, os);                                                       \
+    }                                                                                           \
+    void Print(std::ostream &os) const override                                                 \
+    {                                                                                           \
+      os << #name << ": k0 = " << _d.k0_r << " + " << _d.k0_i << "i Nder = " << _d.Nder        \
+         << " Rdir = " << _d.Rdir << "\n";                                                      \
+    }                                                                                           \
+    void Calc(const size_t N, double *x_f0_r, double *f0_i, const double *y,                    \
+              const double *z) const override                                                   \
+    {                                                                                           \
+      npme_detail::cx_value(N, x_f0_r, f0_i, y, z,                                              \
+                            [this](double r2, double *f, double *g) { eval(r2, f, g); });       \
+    }                                                                                           \
+    void Calc(const size_t N, double *f0_r, double *f0_i, double *x, double *xi, double *y,     \
+              double *yi, double *z, double *zi) const override                                 \
+    {                                                                                           \
+      npme_detail::cx_grad(N, f0_r, f0_i, x, xi, y, yi, z, zi,                                  \
+                           [this](double r2, double *f, double *g) { eval(r2, f, g); });        \
+    }                                                                                           \
+    const NPME_BuiltinKernel *npme_builtin() const override { return &_d; }                     \
+  };
+NPME_AMD_HELMHOLTZ_DM(NPME_Kfunc_Helmholtz_LR_DM, LR)
+NPME_AMD_HELMHOLTZ_DM(NPME_Kfunc_Helmholtz_SR_DM, SR)
 #undef NPME_AMD_HELMHOLTZ_DM
 
-// the parameters a set of descriptors carries, written over the keywords
-// (a later descriptor wins; the short-range one is applied last)
-inline NPME_KeywordInput npme_apply_kernel_params(const NPME_KeywordInput &keyword,
-                                                  const NPME_KfuncReal *const *f, int nf)
-{
-  NPME_KeywordInput k = keyword;
-  for (int i = 0; i < nf; i++) {
-    if (!f[i]) continue;
-    if (f[i]->hasAlpha) k.AddKeyword("alpha", f[i]->alpha);
-    if (f[i]->hasDM) {
-      k.AddKeyword("nDeriv", f[i]->Nder);
-      k.AddKeyword("Rdir", f[i]->Rdir);
-    }
+// the erf/erfc split of 1/r, SetParm(beta): LR = erf(beta r)/r and
+// SR = erfc(beta r)/r with no cutoff of its own
+// (src/KernelFunctionLaplace.h:214-320, src/KernelFunctionLaplace.cpp:
+// 1087-1116, 1344-1373).  They run on the GPU as user kernels (tabulated
+// short-range part) when passed to GenFunc::SetUp.
+class NPME_Kfunc_Laplace_LR_Original : public NPME_KfuncReal {
+ public:
+  NPME_Kfunc_Laplace_LR_Original() {}
+  explicit NPME_Kfunc_Laplace_LR_Original(double beta) { SetParm(beta); }
+  void SetParm(const double beta) { _beta = beta; }
+  void Print(std::ostream &os) const override { os << "NPME_Kfunc_Laplace_LR_Original: beta = " << _beta << "\n"; }
+  void Calc(const size_t N, double *x_f0, const double *y, const double *z) const override
+  {
+    const double b = _beta;
+    npme_detail::real_value(N, x_f0, y, z, [b](double r2, double &f, double &g) { fg(b, r2, f, g); });
   }
-  return k;
+  void Calc(const size_t N, double *f0, double *x, double *y, double *z) const override
+  {
+    const double b = _beta;
+    npme_detail::real_grad(N, f0, x, y, z, [b](double r2, double &f, double &g) { fg(b, r2, f, g); });
+  }
+  // erf(x)/r with x = beta r, and (1/r) d/dr, by series below x = 1e-4
+  static void fg(double beta, double r2, double &f, double &g)
+  {
+    const double r = std::sqrt(r2), x = beta * r, c = 2.0 / std::sqrt(M_PI);
+    if (x < 1e-4) {
+      f = beta * c * (1.0 - x * x / 3.0 + x * x * x * x / 10.0);
+      g = beta * beta * beta * c * (-2.0 / 3.0 + 0.4 * x * x);
+      return;
+    }
+    f = std::erf(x) / r;
+    g = (beta * c * std::exp(-x * x) - f) / r2;
+  }
+
+ private:
+  double _beta = 0;
+};
+class NPME_Kfunc_Laplace_SR_Original : public NPME_KfuncReal {
+ public:
+  NPME_Kfunc_Laplace_SR_Original() {}
+  explicit NPME_Kfunc_Laplace_SR_Original(double beta) { SetParm(beta); }
+  void SetParm(const double beta) { _beta = beta; }
+  void Print(std::ostream &os) const override { os << "NPME_Kfunc_Laplace_SR_Original: beta = " << _beta << "\n"; }
+  void Calc(const size_t N, double *x_f0, const double *y, const double *z) const override
+  {
+    const double b = _beta;
+    npme_detail::real_value(N, x_f0, y, z, [b](double r2, double &f, double &g) { fg(b, r2, f, g); });
+  }
+  void Calc(const size_t N, double *f0, double *x, double *y, double *z) const override
+  {
+    const double b = _beta;
+    npme_detail::real_grad(N, f0, x, y, z, [b](double r2, double &f, double &g) { fg(b, r2, f, g); });
+  }
+  static void fg(double beta, double r2, double &f, double &g)
+  {
+    const double r = std::sqrt(r2), x = beta * r;
+    f = std::erfc(x) / r;
+    g = -(beta * 2.0 / std::sqrt(M_PI) * std::exp(-x * x) + f) / r2;
+  }
+
+ private:
+  double _beta = 0;
+};
+
+// NPME_KernelFuncCheck (src/KernelFunction.h:118-131,
+// src/KernelFunction.cpp:36-372): the two forms of Calc agree (rms 1e-12)
+// and the analytic gradient matches central differences with h = 1e-5
+// (rms 1e-6) at N random points in (Xmin, Xmax)^3.  vecOption is accepted
+// and ignored (there are no AVX forms to compare).
+inline void npme_check_rms(double &error, size_t N, const double *a, const double *b)
+{
+  error = 0;
+  for (size_t i = 0; i < N; i++) error += (a[i] - b[i]) * (a[i] - b[i]);
+  error = std::sqrt(error / (double)N);
 }
-inline NPME_KeywordInput npme_apply_kernel_params(const NPME_KeywordInput &keyword,
-                                                  const NPME_KfuncComplex *const *f, int nf)
+inline bool NPME_KernelFuncCheck(NPME_KfuncReal &func, const size_t N, const char *funcName,
+                                 const double Xmin, const double Xmax, int vecOption, bool PRINT,
+                                 bool PRINT_ALL, std::ostream &os)
 {
-  NPME_KeywordInput k = keyword;
-  for (int i = 0; i < nf; i++) {
-    if (!f[i]) continue;
-    if (f[i]->hasK0) {
-      k.AddKeyword("k0_r", f[i]->k0_r);
-      k.AddKeyword("k0_i", f[i]->k0_i);
+  (void)vecOption;
+  const double tol = 1.0e-12, tolNume = 1.0e-6, h = 1.0e-5;
+  std::vector<double> x(N), y(N), z(N), f0(N), fX(N), fY(N), fZ(N), fRef(N), fp(N), fm(N), num(N);
+  for (size_t i = 0; i < N; i++) {
+    x[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+    y[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+    z[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+  }
+  fRef = x;
+  func.Calc(N, fRef.data(), y.data(), z.data());
+  fX = x;
+  fY = y;
+  fZ = z;
+  func.Calc(N, f0.data(), fX.data(), fY.data(), fZ.data());
+  double err;
+  npme_check_rms(err, N, fRef.data(), f0.data());
+  if (PRINT_ALL) os << "NPME_KernelFuncCheck for " << funcName << ": f0 error " << err << "\n";
+  if (err > tol) {
+    if (PRINT) os << "Error in NPME_KernelFuncCheck: error in f0 = " << err << " > " << tol << "\n";
+    return false;
+  }
+  const std::vector<double> *an[3] = {&fX, &fY, &fZ};
+  for (int p = 0; p < 3; p++) {
+    std::vector<double> xp = x, yp = y, zp = z, xm = x, ym = y, zm = z;
+    std::vector<double> &vp = p == 0 ? xp : p == 1 ? yp : zp, &vm = p == 0 ? xm : p == 1 ? ym : zm;
+    for (size_t i = 0; i < N; i++) {
+      vp[i] += h;
+      vm[i] -= h;
     }
-    if (f[i]->hasDM) {
-      k.AddKeyword("nDeriv", f[i]->Nder);
-      k.AddKeyword("Rdir", f[i]->Rdir);
+    func.Calc(N, xp.data(), yp.data(), zp.data());
+    func.Calc(N, xm.data(), ym.data(), zm.data());
+    for (size_t i = 0; i < N; i++) num[i] = (xp[i] - xm[i]) / (2 * h);
+    npme_check_rms(err, N, an[p]->data(), num.data());
+    if (PRINT_ALL) os << "  d/d" << "xyz"[p] << " numerical derivative error " << err << "\n";
+    if (err > tolNume) {
+      if (PRINT)
+        os << "Error in NPME_KernelFuncCheck: numerical derivative error in f" << "XYZ"[p] << " = "
+           << err << " > " << tolNume << "\n";
+      return false;
     }
   }
-  return k;
+  return true;
+}
+inline bool NPME_KernelFuncCheck(NPME_KfuncComplex &func, const size_t N, const char *funcName,
+                                 const double Xmin, const double Xmax, int vecOption, bool PRINT,
+                                 bool PRINT_ALL, std::ostream &os)
+{
+  (void)vecOption;
+  const double tol = 1.0e-12, tolNume = 1.0e-6, h = 1.0e-5;
+  std::vector<double> x(N), y(N), z(N);
+  for (size_t i = 0; i < N; i++) {
+    x[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+    y[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+    z[i] = Xmin + (Xmax - Xmin) * (std::rand() / (double)RAND_MAX);
+  }
+  std::vector<double> vr = x, vi(N), f0r(N), f0i(N), g[6];
+  func.Calc(N, vr.data(), vi.data(), y.data(), z.data());
+  g[0] = x;
+  g[2] = y;
+  g[4] = z;
+  g[1].resize(N);
+  g[3].resize(N);
+  g[5].resize(N);
+  func.Calc(N, f0r.data(), f0i.data(), g[0].data(), g[1].data(), g[2].data(), g[3].data(),
+            g[4].data(), g[5].data());
+  double er, ei;
+  npme_check_rms(er, N, vr.data(), f0r.data());
+  npme_check_rms(ei, N, vi.data(), f0i.data());
+  const double err = std::sqrt(er * er + ei * ei);
+  if (PRINT_ALL) os << "NPME_KernelFuncCheck for " << funcName << ": f0 error " << err << "\n";
+  if (err > tol) {
+    if (PRINT) os << "Error in NPME_KernelFuncCheck: error in f0 = " << err << " > " << tol << "\n";
+    return false;
+  }
+  for (int p = 0; p < 3; p++) {
+    std::vector<double> xp = x, yp = y, zp = z, xm = x, ym = y, zm = z, ip(N), im(N);
+    std::vector<double> &vp = p == 0 ? xp : p == 1 ? yp : zp, &vm = p == 0 ? xm : p == 1 ? ym : zm;
+    for (size_t i = 0; i < N; i++) {
+      vp[i] += h;
+      vm[i] -= h;
+    }
+    func.Calc(N, xp.data(), ip.data(), yp.data(), zp.data());
+    func.Calc(N, xm.data(), im.data(), ym.data(), zm.data());
+    std::vector<double> nr(N), ni(N);
+    for (size_t i = 0; i < N; i++) {
+      nr[i] = (xp[i] - xm[i]) / (2 * h);
+      ni[i] = (ip[i] - im[i]) / (2 * h);
+    }
+    double e1, e2;
+    npme_check_rms(e1, N, g[2 * p].data(), nr.data());
+    npme_check_rms(e2, N, g[2 * p + 1].data(), ni.data());
+    const double e = std::sqrt(e1 * e1 + e2 * e2);
+    if (PRINT_ALL) os << "  d/d" << "xyz"[p] << " numerical derivative error " << e << "\n";
+    if (e > tolNume) {
+      if (PRINT)
+        os << "Error in NPME_KernelFuncCheck: numerical derivative error in f" << "XYZ"[p] << " = " << e
+           << " > " << tolNume << "\n";
+      return false;
+    }
+  }
+  return true;
 }
 
+// the kernel objects of src/Interface.h:757-771
 struct NPME_KernelList {
   NPME_Kfunc_Laplace funcLaplace;
   NPME_Kfunc_Laplace_LR_DM funcLaplace_LR_DM;
   NPME_Kfunc_Laplace_SR_DM funcLaplace_SR_DM;
+  NPME_Kfunc_Laplace_LR_Original funcLaplace_LR_Orig;
+  NPME_Kfunc_Laplace_SR_Original funcLaplace_SR_Orig;
   NPME_Kfunc_Ralpha funcRalpha;
   NPME_Kfunc_Ralpha_LR_DM funcRalpha_LR_DM;
   NPME_Kfunc_Ralpha_SR_DM funcRalpha_SR_DM;
@@ -345,28 +782,73 @@ struct NPME_KernelList {
   NPME_Kfunc_Helmholtz_SR_DM funcHelmholtz_SR_DM;
 };
 
-// NPME_Interface_SelectKernelPtr (src/Interface.h:775-788): the kernel
-// parameters (alpha, k0, Rdir, nDeriv) stay in the keywords
+// NPME_Interface_SelectKernelPtr (src/Interface.cpp:545-733): picks the
+// kernels of the keywords' funcType and EwaldSplit from the list and sets
+// their parameters from the keywords (nDeriv and Rdir for DerivMatch, beta
+// from Rdir and tol for LaplaceOrig, alpha, k0)
 inline bool NPME_Interface_SelectKernelPtr(NPME_KfuncReal *&func, NPME_KfuncReal *&func_LR,
                                            NPME_KfuncReal *&func_SR, NPME_KernelList &kl,
                                            const NPME_KeywordInput &keyword, bool printLog,
                                            std::ostream &os)
 {
-  std::string ft;
-  keyword.GetValue("funcType", ft);
+  func = func_LR = func_SR = nullptr;
+  std::string ft, split;
+  int nd = 0;
+  double Rdir = 0, tol = 0, beta = 0, alpha = 0;
+  if (!keyword.GetValue("funcType", ft) || !keyword.GetValue("EwaldSplit", split)) {
+    if (printLog) os << "NPME error: funcType and EwaldSplit must be defined\n";
+    return false;
+  }
+  if (!keyword.GetValue("Rdir", Rdir)) {
+    if (printLog) os << "NPME error: 'Rdir' keyword is missing\n";
+    return false;
+  }
+  if (split == "DerivMatch") {
+    if (!keyword.GetValue("nDeriv", nd)) {
+      if (printLog) os << "NPME error: 'nDeriv' keyword is missing\n";
+      return false;
+    }
+  } else if (split == "LaplaceOrig") {
+    if (!keyword.GetValue("tol", tol) || npme_ewald_rdir2beta(Rdir, tol, &beta) != NPME_OK) {
+      if (printLog) os << "NPME error: LaplaceOrig needs Rdir and tol\n";
+      return false;
+    }
+  } else {
+    if (printLog) os << "NPME error: EwaldSplit = " << split << " is not defined\n";
+    return false;
+  }
   if (ft == "Laplace") {
     func = &kl.funcLaplace;
-    func_LR = &kl.funcLaplace_LR_DM;
-    func_SR = &kl.funcLaplace_SR_DM;
+    if (split == "DerivMatch") {
+      if (!kl.funcLaplace_LR_DM.SetParm(nd, Rdir, printLog, os) ||
+          !kl.funcLaplace_SR_DM.SetParm(nd, Rdir, printLog, os))
+        return false;
+      func_LR = &kl.funcLaplace_LR_DM;
+      func_SR = &kl.funcLaplace_SR_DM;
+    } else {
+      kl.funcLaplace_LR_Orig.SetParm(beta);
+      kl.funcLaplace_SR_Orig.SetParm(beta);
+      func_LR = &kl.funcLaplace_LR_Orig;
+      func_SR = &kl.funcLaplace_SR_Orig;
+    }
     return true;
   }
-  if (ft == "Ralpha") {
+  if (ft == "Ralpha" && split == "DerivMatch") {
+    if (!keyword.GetValue("alpha", alpha)) {
+      if (printLog) os << "NPME error: 'alpha' keyword is missing\n";
+      return false;
+    }
+    kl.funcRalpha.SetParm(alpha);
+    if (!kl.funcRalpha_LR_DM.SetParm(alpha, nd, Rdir, printLog, os) ||
+        !kl.funcRalpha_SR_DM.SetParm(alpha, nd, Rdir, printLog, os))
+      return false;
     func = &kl.funcRalpha;
     func_LR = &kl.funcRalpha_LR_DM;
     func_SR = &kl.funcRalpha_SR_DM;
     return true;
   }
-  if (printLog) os << "NPME error: funcType '" << ft << "' is not a real kernel\n";
+  if (printLog) os << "NPME error: funcType '" << ft << "' with EwaldSplit '" << split
+                   << "' is not a real kernel\n";
   return false;
 }
 inline bool NPME_Interface_SelectKernelPtr(NPME_KfuncComplex *&func, NPME_KfuncComplex *&func_LR,
@@ -374,17 +856,65 @@ inline bool NPME_Interface_SelectKernelPtr(NPME_KfuncComplex *&func, NPME_KfuncC
                                            const NPME_KeywordInput &keyword, bool printLog,
                                            std::ostream &os)
 {
-  std::string ft;
+  func = func_LR = func_SR = nullptr;
+  std::string ft, split;
+  int nd = 0;
+  double Rdir = 0, k0r = 0, k0i = 0;
   keyword.GetValue("funcType", ft);
-  if (ft == "Helmholtz") {
-    func = &kl.funcHelmholtz;
-    func_LR = &kl.funcHelmholtz_LR_DM;
-    func_SR = &kl.funcHelmholtz_SR_DM;
-    return true;
+  keyword.GetValue("EwaldSplit", split);
+  if (ft != "Helmholtz" || split != "DerivMatch") {
+    if (printLog) os << "NPME error: funcType '" << ft << "' with EwaldSplit '" << split
+                     << "' is not a complex kernel\n";
+    return false;
   }
-  if (printLog) os << "NPME error: funcType '" << ft << "' is not a complex kernel\n";
-  return false;
+  if (!keyword.GetValue("nDeriv", nd) || !keyword.GetValue("Rdir", Rdir) ||
+      !keyword.GetValue("k0_r", k0r) || !keyword.GetValue("k0_i", k0i)) {
+    if (printLog) os << "NPME error: Helmholtz needs nDeriv, Rdir, k0_r, k0_i\n";
+    return false;
+  }
+  _Complex double k0;
+  __real__ k0 = k0r;
+  __imag__ k0 = k0i;
+  kl.funcHelmholtz.SetParm(k0);
+  if (!kl.funcHelmholtz_LR_DM.SetParm(k0, nd, Rdir, printLog, os) ||
+      !kl.funcHelmholtz_SR_DM.SetParm(k0, nd, Rdir, printLog, os))
+    return false;
+  func = &kl.funcHelmholtz;
+  func_LR = &kl.funcHelmholtz_LR_DM;
+  func_SR = &kl.funcHelmholtz_SR_DM;
+  return true;
 }
+
+namespace npme_detail {
+// C callbacks over the virtual Calc of a plugin object (ctx)
+inline void tramp_value_r(void *ctx, size_t n, double *x_f0, const double *y, const double *z)
+{
+  static_cast<const NPME_KfuncReal *>(ctx)->Calc(n, x_f0, y, z);
+}
+inline void tramp_grad_r(void *ctx, size_t n, double *f0, double *x, double *y, double *z)
+{
+  static_cast<const NPME_KfuncReal *>(ctx)->Calc(n, f0, x, y, z);
+}
+inline void tramp_value_c(void *ctx, size_t n, double *x_f0_r, double *f0_i, const double *y,
+                          const double *z)
+{
+  static_cast<const NPME_KfuncComplex *>(ctx)->Calc(n, x_f0_r, f0_i, y, z);
+}
+inline void tramp_grad_c(void *ctx, size_t n, double *f0r, double *f0i, double *x, double *xi,
+                         double *y, double *yi, double *z, double *zi)
+{
+  static_cast<const NPME_KfuncComplex *>(ctx)->Calc(n, f0r, f0i, x, xi, y, yi, z, zi);
+}
+// the three objects are the full, long-range and short-range parts of one
+// built-in family (then the GPU's native kernels run)
+template <typename K> inline bool one_family(const K *f, const K *lr, const K *sr)
+{
+  const NPME_BuiltinKernel *a = f->npme_builtin(), *b = lr->npme_builtin(), *c = sr->npme_builtin();
+  return a && b && c && a->family == b->family && b->family == c->family &&
+         a->part == NPME_BuiltinKernel::FULL && b->part == NPME_BuiltinKernel::LR &&
+         c->part == NPME_BuiltinKernel::SR;
+}
+}  // namespace npme_detail
 
 // ------------------------------------------------------------ interfaces
 // NPME_InterfaceBase getters (src/Interface.h:55-115)
@@ -500,22 +1030,50 @@ class NPME_InterfaceReal : public NPME_InterfaceBase {
   virtual void Print(std::ostream &ofs_log) { print(ofs_log); }
 };
 
-// NPME_InterfaceReal_GenFunc::SetUp (src/Interface.h:291-296): func, funcLR
-// and funcSR must be the built-in descriptors of NPME_KernelList
+// NPME_InterfaceReal_GenFunc::SetUp (src/Interface.h:291-296,
+// src/Interface.cpp:1606-1671).  The objects are borrowed and must outlive
+// this one.  The full, long-range and short-range objects of one built-in
+// family run the GPU's native kernels with their parameters; any other
+// objects run as user kernels: f_LR sampled on the host on the kernel grid,
+// f_SR (which must be radial) tabulated for the device direct sum, and
+// CalcV_exact evaluating func on the host.
 class NPME_InterfaceReal_GenFunc : public NPME_InterfaceReal {
  public:
   bool SetUp(const NPME_KeywordInput &keyword, const size_t nCharge, const double *coord,
              NPME_KfuncReal *func, NPME_KfuncReal *funcLR, NPME_KfuncReal *funcSR,
              bool printLog, std::ostream &ofs_log)
   {
-    if (!func || !funcLR || !funcSR || !func->builtin() || !funcLR->builtin() ||
-        !funcSR->builtin()) {
-      if (printLog) ofs_log << "NPME error: user-defined kernels are not provided on the GPU\n";
+    const char *miss = !func ? "func" : !funcLR ? "funcLR" : !funcSR ? "funcSR" : nullptr;
+    if (miss) {
+      if (printLog) ofs_log << "Error in NPME_InterfaceReal_GenFunc::SetUp\n  " << miss << " == NULL\n";
       return false;
     }
-    const NPME_KfuncReal *f[3] = {func, funcLR, funcSR};
-    return setup(npme_apply_kernel_params(keyword, f, 3), nCharge, coord, printLog, ofs_log);
+    if (npme_detail::one_family<NPME_KfuncReal>(func, funcLR, funcSR)) {
+      const NPME_BuiltinKernel &d = *funcLR->npme_builtin();
+      NPME_KeywordInput k = keyword;
+      k.AddKeyword("funcType", d.family == NPME_BuiltinKernel::RALPHA ? "Ralpha" : "Laplace");
+      k.AddKeyword("EwaldSplit", "DerivMatch");
+      if (d.family == NPME_BuiltinKernel::RALPHA) k.AddKeyword("alpha", d.alpha);
+      k.AddKeyword("nDeriv", d.Nder);
+      k.AddKeyword("Rdir", d.Rdir);
+      return setup(k, nCharge, coord, printLog, ofs_log);
+    }
+    npme_kfunc_real c[3];
+    NPME_KfuncReal *f[3] = {func, funcLR, funcSR};
+    for (int i = 0; i < 3; i++) c[i] = {f[i], npme_detail::tramp_value_r, npme_detail::tramp_grad_r};
+    npme_destroy(_h);
+    _h = nullptr;
+    _n = 0;
+    const int rc = npme_create_genfunc_real(&_h, keyword.c_handle(), nCharge, coord, &c[0], &c[1], &c[2]);
+    if (rc != NPME_OK) {
+      _h = nullptr;
+      return npme_report(rc, printLog, ofs_log);
+    }
+    _n = nCharge;
+    return true;
   }
+  // errors of the short-range table of user kernels (0 for built-in ones)
+  void TableError(double err[2]) const { npme_get_table_error(_h, err); }
 };
 
 // NPME_InterfaceReal_Laplace_DM::SetUp (src/Interface.h:393-395)
@@ -525,6 +1083,21 @@ class NPME_InterfaceReal_Laplace_DM : public NPME_InterfaceReal {
              bool printLog, std::ostream &ofs_log)
   {
     return setup(keyword, nCharge, coord, printLog, ofs_log);
+  }
+};
+
+// NPME_InterfaceReal_Laplace_Original::SetUp (src/Interface.h:413-505,
+// src/Interface.cpp:2152-2230): 1/r with the erf/erfc split, beta from Rdir
+// and tol (EwaldSplit LaplaceOrig)
+class NPME_InterfaceReal_Laplace_Original : public NPME_InterfaceReal {
+ public:
+  bool SetUp(const NPME_KeywordInput &keyword, const size_t nCharge, const double *coord,
+             bool printLog, std::ostream &ofs_log)
+  {
+    NPME_KeywordInput k = keyword;
+    k.AddKeyword("funcType", "Laplace");
+    k.AddKeyword("EwaldSplit", "LaplaceOrig");
+    return setup(k, nCharge, coord, printLog, ofs_log);
   }
 };
 
@@ -568,6 +1141,49 @@ class NPME_InterfaceComplex_Helmholtz_DM : public NPME_InterfaceComplex {
   }
 };
 
+// NPME_InterfaceComplex_GenFunc::SetUp (src/Interface.h:563-650): as the
+// real GenFunc, for complex kernels (the built-in Helmholtz family runs
+// natively)
+class NPME_InterfaceComplex_GenFunc : public NPME_InterfaceComplex {
+ public:
+  bool SetUp(const NPME_KeywordInput &keyword, const size_t nCharge, const double *coord,
+             NPME_KfuncComplex *func, NPME_KfuncComplex *funcLR, NPME_KfuncComplex *funcSR,
+             bool printLog, std::ostream &ofs_log)
+  {
+    const char *miss = !func ? "func" : !funcLR ? "funcLR" : !funcSR ? "funcSR" : nullptr;
+    if (miss) {
+      if (printLog) ofs_log << "Error in NPME_InterfaceComplex_GenFunc::SetUp\n  " << miss << " == NULL\n";
+      return false;
+    }
+    if (npme_detail::one_family<NPME_KfuncComplex>(func, funcLR, funcSR)) {
+      const NPME_BuiltinKernel &d = *funcLR->npme_builtin();
+      NPME_KeywordInput k = keyword;
+      k.AddKeyword("funcType", "Helmholtz");
+      k.AddKeyword("EwaldSplit", "DerivMatch");
+      k.AddKeyword("k0_r", d.k0_r);
+      k.AddKeyword("k0_i", d.k0_i);
+      k.AddKeyword("nDeriv", d.Nder);
+      k.AddKeyword("Rdir", d.Rdir);
+      return setup(k, nCharge, coord, printLog, ofs_log);
+    }
+    npme_kfunc_complex c[3];
+    NPME_KfuncComplex *f[3] = {func, funcLR, funcSR};
+    for (int i = 0; i < 3; i++) c[i] = {f[i], npme_detail::tramp_value_c, npme_detail::tramp_grad_c};
+    npme_destroy(_h);
+    _h = nullptr;
+    _n = 0;
+    const int rc =
+        npme_create_genfunc_complex(&_h, keyword.c_handle(), nCharge, coord, &c[0], &c[1], &c[2]);
+    if (rc != NPME_OK) {
+      _h = nullptr;
+      return npme_report(rc, printLog, ofs_log);
+    }
+    _n = nCharge;
+    return true;
+  }
+  void TableError(double err[2]) const { npme_get_table_error(_h, err); }
+};
+
 // NPME_Interface_SetUpRealKernel / SetUpComplexKernel
 // (src/Interface.h:800-829): allocates with new; the caller deletes
 inline bool NPME_Interface_SetUpRealKernel(NPME_InterfaceReal *&npme, NPME_KfuncReal *func,
@@ -577,20 +1193,43 @@ inline bool NPME_Interface_SetUpRealKernel(NPME_InterfaceReal *&npme, NPME_Kfunc
                                            bool useGenericKernel, bool printLog,
                                            std::ostream &ofs_log)
 {
+  // src/Interface.cpp:884-1060: Laplace + DerivMatch -> Laplace_DM,
+  // Laplace + LaplaceOrig -> Laplace_Original, Ralpha + DerivMatch and
+  // useGenericKernel -> GenFunc with the given kernel objects
   npme = nullptr;
-  std::string ft;
-  keyword.GetValue("funcType", ft);
-  if (ft == "Laplace" && !useGenericKernel) {
-    auto *p = new NPME_InterfaceReal_Laplace_DM;
-    if (!p->SetUp(keyword, nCharge, coord, printLog, ofs_log)) {
-      delete p;
-      return false;
-    }
-    npme = p;
-    return true;
+  const char *miss = !func ? "func" : !funcLR ? "funcLR" : !funcSR ? "funcSR" : nullptr;
+  if (miss) {
+    if (printLog) ofs_log << "Error in NPME_Interface_SetUpRealKernel\n  " << miss << " == NULL\n";
+    return false;
   }
-  auto *p = new NPME_InterfaceReal_GenFunc;
-  if (!p->SetUp(keyword, nCharge, coord, func, funcLR, funcSR, printLog, ofs_log)) {
+  std::string ft, split;
+  if (!keyword.GetValue("funcType", ft) || !keyword.GetValue("EwaldSplit", split)) {
+    if (printLog) ofs_log << "Error in NPME_Interface_SetUpRealKernel\n  funcType and EwaldSplit must be defined\n";
+    return false;
+  }
+  const bool known = (ft == "Laplace" && (split == "DerivMatch" || split == "LaplaceOrig")) ||
+                     (ft == "Ralpha" && split == "DerivMatch");
+  if (!known) {
+    if (printLog) ofs_log << "Error in NPME_Interface_SetUpRealKernel\n  funcType = " << ft
+                          << " and EwaldSplit = " << split << " is not provided\n";
+    return false;
+  }
+  NPME_InterfaceReal *p = nullptr;
+  bool ok;
+  if (ft == "Laplace" && !useGenericKernel && split == "DerivMatch") {
+    auto *q = new NPME_InterfaceReal_Laplace_DM;
+    ok = q->SetUp(keyword, nCharge, coord, printLog, ofs_log);
+    p = q;
+  } else if (ft == "Laplace" && !useGenericKernel) {
+    auto *q = new NPME_InterfaceReal_Laplace_Original;
+    ok = q->SetUp(keyword, nCharge, coord, printLog, ofs_log);
+    p = q;
+  } else {
+    auto *q = new NPME_InterfaceReal_GenFunc;
+    ok = q->SetUp(keyword, nCharge, coord, func, funcLR, funcSR, printLog, ofs_log);
+    p = q;
+  }
+  if (!ok) {
     delete p;
     return false;
   }
@@ -605,16 +1244,34 @@ inline bool NPME_Interface_SetUpComplexKernel(NPME_InterfaceComplex *&npme,
                                               bool useGenericKernel, bool printLog,
                                               std::ostream &ofs_log)
 {
-  (void)useGenericKernel;
+  // src/Interface.cpp:1061-1200: Helmholtz + DerivMatch -> Helmholtz_DM, or
+  // GenFunc with the given kernel objects when useGenericKernel
   npme = nullptr;
-  if (!func || !funcLR || !funcSR || !func->builtin() || !funcLR->builtin() ||
-      !funcSR->builtin()) {
-    if (printLog) ofs_log << "NPME error: user-defined kernels are not provided on the GPU\n";
+  const char *miss = !func ? "func" : !funcLR ? "funcLR" : !funcSR ? "funcSR" : nullptr;
+  if (miss) {
+    if (printLog) ofs_log << "Error in NPME_Interface_SetUpComplexKernel\n  " << miss << " == NULL\n";
     return false;
   }
-  auto *p = new NPME_InterfaceComplex_Helmholtz_DM;
-  const NPME_KfuncComplex *f[3] = {func, funcLR, funcSR};
-  if (!p->SetUp(npme_apply_kernel_params(keyword, f, 3), nCharge, coord, printLog, ofs_log)) {
+  std::string ft, split;
+  keyword.GetValue("funcType", ft);
+  keyword.GetValue("EwaldSplit", split);
+  if (ft != "Helmholtz" || split != "DerivMatch") {
+    if (printLog) ofs_log << "Error in NPME_Interface_SetUpComplexKernel\n  funcType = " << ft
+                          << " and EwaldSplit = " << split << " is not provided\n";
+    return false;
+  }
+  NPME_InterfaceComplex *p = nullptr;
+  bool ok;
+  if (!useGenericKernel) {
+    auto *q = new NPME_InterfaceComplex_Helmholtz_DM;
+    ok = q->SetUp(keyword, nCharge, coord, printLog, ofs_log);
+    p = q;
+  } else {
+    auto *q = new NPME_InterfaceComplex_GenFunc;
+    ok = q->SetUp(keyword, nCharge, coord, func, funcLR, funcSR, printLog, ofs_log);
+    p = q;
+  }
+  if (!ok) {
     delete p;
     return false;
   }

@@ -54,6 +54,9 @@ def lib():
         L.npme_print.argtypes = [p, C.c_char_p, s]
         L.npme_set_profiling.argtypes = [p, i]
         L.npme_count_pairs.argtypes = [p, p]
+        L.npme_create_genfunc_real.argtypes = [C.POINTER(p), p, s, p, p, p, p]
+        L.npme_create_genfunc_complex.argtypes = [C.POINTER(p), p, s, p, p, p, p]
+        L.npme_get_table_error.argtypes = [p, p]
         L.npme_get_stage_times.argtypes = [p, p, i]
         L.npme_get_indices.argtypes = [p, s, p, p, p, p]
         L.npme_debug_fail_alloc.argtypes = [C.c_long]
@@ -317,9 +320,130 @@ class NPME_InterfaceReal_Laplace_DM(_InterfaceBase):
     _funcType = ("Laplace",)
 
 
-class NPME_InterfaceReal_GenFunc(_InterfaceBase):
-    """Generic real kernel; built-in r^alpha (funcType Ralpha) and Laplace."""
+# ----------------------------------------------------------------- user kernels
+_PD = C.POINTER(C.c_double)
+_VAL_R = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, _PD, _PD, _PD)
+_GRAD_R = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, _PD, _PD, _PD, _PD)
+_VAL_C = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, _PD, _PD, _PD, _PD)
+_GRAD_C = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, _PD, _PD, _PD, _PD, _PD, _PD, _PD, _PD)
+
+
+class _KfuncRealC(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("value", _VAL_R), ("grad", _GRAD_R)]
+
+
+class _KfuncComplexC(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("value", _VAL_C), ("grad", _GRAD_C)]
+
+
+def _callback(obj):
+    """A ctypes callback running obj.Calc(N, *arrays) on views of the
+    library's arrays.  An exception in Calc is kept on obj (raised again by
+    SetUp) and the outputs are set to NaN, which the library refuses."""
+    def fn(ctx, n, *ptrs):
+        N = int(n)
+        if N == 0:
+            return
+        arrs = [np.ctypeslib.as_array(p, (N,)) for p in ptrs]
+        try:
+            obj.Calc(N, *arrs)
+        except BaseException as e:  # noqa: BLE001 -- reported through SetUp
+            obj._calc_error = e
+            for a in arrs:
+                a[:] = np.nan
+    return fn
+
+
+class NPME_KfuncReal:
+    """Base of a user-defined real kernel (src/KernelFunction.h:25-60).
+    A subclass defines Calc for both of the reference's forms, told apart by
+    the number of arrays (numpy float64 views of length N, changed in place):
+      Calc(N, x_f0, y, z):          x_f0 holds x, and f(x, y, z) on return
+      Calc(N, f0, x_fX, y_fY, z_fZ): f0 receives f; x_fX, y_fY, z_fZ hold
+                                     x, y, z and the gradient on return
+    and Print(os).  Calc may be called with several batches at once from the
+    library's threads (one at a time, under the interpreter lock)."""
+    _cstruct = _KfuncRealC
+
+    def Calc(self, N, *arrays):
+        raise NotImplementedError("NPME_KfuncReal.Calc")
+
+    def Print(self, os=None):
+        pass
+
+    def _c(self):
+        if getattr(self, "_cfun", None) is None:
+            self._cfun = self._cstruct(None, _VAL_R(_callback(self)), _GRAD_R(_callback(self)))
+        return self._cfun
+
+
+class NPME_KfuncComplex(NPME_KfuncReal):
+    """Base of a user-defined complex kernel (src/KernelFunction.h:62-116):
+      Calc(N, x_f0_r, f0_i, y, z)
+      Calc(N, f0_r, f0_i, x_fX_r, fX_i, y_fY_r, fY_i, z_fZ_r, fZ_i)"""
+    _cstruct = _KfuncComplexC
+
+    def _c(self):
+        if getattr(self, "_cfun", None) is None:
+            self._cfun = self._cstruct(None, _VAL_C(_callback(self)), _GRAD_C(_callback(self)))
+        return self._cfun
+
+
+class _GenFuncBase(_InterfaceBase):
+    def SetUp(self, keyword, nCharge, coord, func=None, funcLR=None, funcSR=None,
+              printLog=False):
+        """SetUp(keyword, nCharge, coord, func, funcLR, funcSR)
+        (src/Interface.h:291-296, :624-629).  With user kernels (subclasses
+        of NPME_KfuncReal / NPME_KfuncComplex) the grid keywords must be set
+        (N1-3, n1-3, BnOrder, Rdir ...).  Without them the built-in kernel
+        of the keywords' funcType is used."""
+        if isinstance(func, bool):  # SetUp(keyword, nCharge, coord, printLog)
+            func, printLog = None, func
+        if func is None and funcLR is None and funcSR is None:
+            return _InterfaceBase.SetUp(self, keyword, nCharge, coord, printLog)
+        fs = (func, funcLR, funcSR)
+        if any(f is None for f in fs):
+            return False
+        self._free()
+        coord = _coord(coord)
+        for f in fs:
+            f._calc_error = None
+        cs = [f._c() for f in fs]
+        h = C.c_void_p()
+        create = lib().npme_create_genfunc_complex if self._complex else lib().npme_create_genfunc_real
+        rc = create(C.byref(h), keyword._h, int(nCharge), _ptr(coord),
+                    *[C.cast(C.pointer(c), C.c_void_p) for c in cs])
+        for f in fs:
+            if f._calc_error is not None:
+                err, f._calc_error = f._calc_error, None
+                if rc == NPME_OK:
+                    lib().npme_destroy(h)
+                raise err
+        if rc != NPME_OK:
+            return False
+        self._h = h
+        self._kfuncs = fs  # the callbacks must outlive the handle
+        self.nCharge = int(nCharge)
+        return True
+
+    def table_error(self):
+        """(f, f'/r) errors of the short-range table of user kernels."""
+        e = np.zeros(2)
+        _check(lib().npme_get_table_error(self._h, _ptr(e)), "table_error")
+        return float(e[0]), float(e[1])
+
+
+class NPME_InterfaceReal_GenFunc(_GenFuncBase):
+    """Generic real kernel: user kernels, or the built-in r^alpha (funcType
+    Ralpha) and Laplace."""
     _funcType = ("Ralpha", "Laplace")
+
+
+class NPME_InterfaceComplex_GenFunc(_GenFuncBase):
+    """Generic complex kernel (src/Interface.h:563-650): user kernels, or the
+    built-in Helmholtz."""
+    _complex = True
+    _funcType = ("Helmholtz",)
 
 
 class NPME_InterfaceComplex_Helmholtz_DM(_InterfaceBase):
@@ -335,13 +459,13 @@ def NPME_Interface_SetUpRealKernel(keyword, nCharge, coord, useGenericKernel=Fal
     cls = NPME_InterfaceReal_Laplace_DM if (ft == "Laplace" and not useGenericKernel) \
         else NPME_InterfaceReal_GenFunc
     obj = cls()
-    return obj if obj.SetUp(keyword, nCharge, coord, printLog) else None
+    return obj if obj.SetUp(keyword, nCharge, coord, printLog=printLog) else None
 
 
 def NPME_Interface_SetUpComplexKernel(keyword, nCharge, coord, useGenericKernel=False,
                                       printLog=False):
     obj = NPME_InterfaceComplex_Helmholtz_DM()
-    return obj if obj.SetUp(keyword, nCharge, coord, printLog) else None
+    return obj if obj.SetUp(keyword, nCharge, coord, printLog=printLog) else None
 
 
 # ----------------------------------------------------------------- serving

@@ -227,6 +227,107 @@ int npme_create(npme_handle **out, const npme_keywords *kw, size_t nCharge, cons
   });
 }
 
+static int create_user(npme_handle **out, const npme_keywords *kw, size_t nCharge,
+                       const double *coord, const UserKernels &u)
+{
+  return guarded([&] {
+    if (!out || !kw || !coord) throw Error{NPME_ERR_ARG, "npme_create_genfunc: NULL argument"};
+    *out = nullptr;
+    std::unique_ptr<npme_handle> h(new npme_handle());
+    h->h.userK = true;
+    h->h.user = u;
+    try {
+      handle_setup(h->h, kw->kw, nCharge, coord);
+    } catch (...) {
+      handle_free_device(h->h);
+      handle_destroy_streams(h->h);
+      throw;
+    }
+    *out = h.release();
+  });
+}
+
+int npme_create_genfunc_real(npme_handle **out, const npme_keywords *kw, size_t nCharge,
+                             const double *coord, const npme_kfunc_real *func,
+                             const npme_kfunc_real *funcLR, const npme_kfunc_real *funcSR)
+{
+  const npme_kfunc_real *f[3] = {func, funcLR, funcSR};
+  UserKernels u;
+  u.complex = false;
+  for (int i = 0; i < 3; i++) {
+    // src/Interface.cpp:1620-1638: each of func, funcLR, funcSR is required
+    if (!f[i] || !f[i]->value || !f[i]->grad) {
+      set_error(std::string("npme_create_genfunc_real: ") + (i == 0 ? "func" : i == 1 ? "funcLR" : "funcSR") +
+                " == NULL");
+      if (out) *out = nullptr;
+      return NPME_ERR_ARG;
+    }
+    u.r[i] = *f[i];
+  }
+  return create_user(out, kw, nCharge, coord, u);
+}
+
+int npme_create_genfunc_complex(npme_handle **out, const npme_keywords *kw, size_t nCharge,
+                                const double *coord, const npme_kfunc_complex *func,
+                                const npme_kfunc_complex *funcLR,
+                                const npme_kfunc_complex *funcSR)
+{
+  const npme_kfunc_complex *f[3] = {func, funcLR, funcSR};
+  UserKernels u;
+  u.complex = true;
+  for (int i = 0; i < 3; i++) {
+    if (!f[i] || !f[i]->value || !f[i]->grad) {
+      set_error(std::string("npme_create_genfunc_complex: ") +
+                (i == 0 ? "func" : i == 1 ? "funcLR" : "funcSR") + " == NULL");
+      if (out) *out = nullptr;
+      return NPME_ERR_ARG;
+    }
+    u.c[i] = *f[i];
+  }
+  return create_user(out, kw, nCharge, coord, u);
+}
+
+int npme_dm_coefficients(const char *funcType, double alpha, double k0_r, double k0_i, int Nder,
+                         double Rdir, double *a, double *b)
+{
+  return guarded([&] {
+    if (!funcType || !a || !b) throw Error{NPME_ERR_ARG, "npme_dm_coefficients: NULL argument"};
+    const std::string ft(funcType);
+    int kind;
+    if (ft == "Laplace") kind = KIND_LAPLACE;
+    else if (ft == "Ralpha") kind = KIND_RALPHA;
+    else if (ft == "Helmholtz") kind = KIND_HELMHOLTZ;
+    else throw Error{NPME_ERR_KEYWORD, "funcType = " + ft + " is undefined"};
+    DMKernel k;
+    std::string err;
+    if (!dm_kernel_init(k, kind, alpha, k0_r, k0_i, Nder, Rdir, err)) throw Error{NPME_ERR_KEYWORD, err};
+    for (int i = 0; i < 2 * (Nder + 1); i++) {
+      a[i] = k.a[i];
+      b[i] = k.b[i];
+    }
+  });
+}
+
+int npme_ewald_rdir2beta(double Rdir, double tol, double *beta)
+{
+  return guarded([&] {
+    if (!beta) throw Error{NPME_ERR_ARG, "npme_ewald_rdir2beta: NULL argument"};
+    if (!(Rdir > 0) || !(tol > 0) || !(tol < 1)) throw Error{NPME_ERR_ARG, "needs Rdir > 0, 0 < tol < 1"};
+    *beta = rdir2beta(Rdir, tol);
+  });
+}
+
+int npme_get_table_error(const npme_handle *h, double err[2])
+{
+  if (!h || !err) {
+    set_error("npme_get_table_error: NULL argument");
+    return NPME_ERR_ARG;
+  }
+  err[0] = h->h.utErr[0];
+  err[1] = h->h.utErr[1];
+  return NPME_OK;
+}
+
 int npme_reset_coord(npme_handle *h, size_t nCharge, const double *coord)
 {
   return guarded([&] {
@@ -302,6 +403,10 @@ static void exact_host(npme_handle *h, double *V, size_t n, const double *charge
   if (!V || !charge) throw Error{NPME_ERR_ARG, "NULL V or charge"};
   std::lock_guard<std::mutex> lk(h->h.mtx);
   Handle &H = h->h;
+  if (H.userK) {  // the user's full kernel, through its callback on the host
+    user_exact_host(H.user, n, H.coordHost.data(), charge, n, nullptr, V);
+    return;
+  }
   const size_t qn = (cx ? 2 : 1) * n, vn = (cx ? 8 : 4) * n;
   NPME_HIP_CHECK(hipMemcpyAsync(H.d_qin, charge, qn * sizeof(double), hipMemcpyHostToDevice, H.stream));
   handle_exact(H, H.d_qin, n, nullptr, H.d_V);
@@ -333,6 +438,10 @@ int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget, const lon
     if (!H.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
     for (size_t t = 0; t < nTarget; t++)
       if (idx[t] < 0 || (size_t)idx[t] >= H.n) throw Error{NPME_ERR_ARG, "target index out of range"};
+    if (H.userK) {
+      user_exact_host(H.user, H.n, H.coordHost.data(), charge, nTarget, idx, V);
+      return;
+    }
     const bool cx = H.complexK;
     const size_t qn = (cx ? 2 : 1) * H.n, vw = cx ? 8 : 4;
     DevBuf d_idx(std::max<size_t>(nTarget, 1) * sizeof(long));
@@ -368,10 +477,12 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   if (!h || !buf) return NPME_ERR_ARG;
   const Handle &H = h->h;
   std::ostringstream os;
-  static const char *kn[4] = {"Laplace", "Ralpha", "Helmholtz", "Laplace"};
+  static const char *kn[6] = {"Laplace", "Ralpha", "Helmholtz", "Laplace", "user-defined real",
+                               "user-defined complex"};
   os << "NPME (MI355X) kernel " << kn[H.ker.kind]
-     << (H.ker.kind == KIND_LAPLACE_ERF ? " LaplaceOrig" : " DerivMatch") << ", precision "
-     << (H.fp64 ? "fp64" : "mixed") << "\n";
+     << (H.ker.kind == KIND_LAPLACE_ERF ? " LaplaceOrig"
+                                        : kind_is_user(H.ker.kind) ? " GenFunc" : " DerivMatch")
+     << ", precision " << (H.fp64 ? "fp64" : "mixed") << "\n";
   os << "  nCharge " << H.n << "  Rdir " << H.ker.Rdir << "  Nder " << H.ker.Nder << "  BnOrder "
      << H.Bn << "\n";
   os << "  N1 N2 N3 " << H.N[0] << " " << H.N[1] << " " << H.N[2] << "  n1 n2 n3 " << H.nblk[0]
@@ -380,6 +491,9 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
      << H.L[1] << " " << H.L[2] << "\n";
   os << "  fself " << H.fself[0] << " " << H.fself[1] << "\n";
   if (H.ker.kind == KIND_LAPLACE_ERF) os << "  beta " << H.ker.beta << "\n";
+  if (kind_is_user(H.ker.kind))
+    os << "  short-range table: " << kUserOct << " octaves x " << H.utS << " pieces of degree "
+       << kUserDeg << ", error " << H.utErr[0] << " (f) " << H.utErr[1] << " (f'/r)\n";
   os << "  cells " << H.nb[0] << "x" << H.nb[1] << "x" << H.nb[2] << " of " << H.csB
      << "  device GB " << H.deviceBytes / 1e9 << "\n";
   os << "  per-charge set-up on the " << (H.deviceSetup ? "device" : "host") << "\n";

@@ -64,7 +64,37 @@ template <typename F> inline int guarded(F &&f)
 // ------------------------------------------------------------- kernels
 // KIND_LAPLACE_ERF: 1/r with the Ewald split erf(beta r)/r + erfc(beta r)/r
 // (EwaldSplit = LaplaceOrig) instead of the derivative match
-enum KernelKind { KIND_LAPLACE = 0, KIND_RALPHA = 1, KIND_HELMHOLTZ = 2, KIND_LAPLACE_ERF = 3 };
+// KIND_USER / KIND_USER_CX: user-defined real / complex kernels
+// (NPME_InterfaceReal_GenFunc / NPME_InterfaceComplex_GenFunc with
+// NPME_KfuncReal / NPME_KfuncComplex plugins): the short-range part is
+// evaluated on the device from a host-built table
+enum KernelKind {
+  KIND_LAPLACE = 0,
+  KIND_RALPHA = 1,
+  KIND_HELMHOLTZ = 2,
+  KIND_LAPLACE_ERF = 3,
+  KIND_USER = 4,
+  KIND_USER_CX = 5
+};
+__host__ __device__ constexpr bool kind_is_complex(int kind)
+{
+  return kind == KIND_HELMHOLTZ || kind == KIND_USER_CX;
+}
+__host__ __device__ constexpr bool kind_is_user(int kind)
+{
+  return kind == KIND_USER || kind == KIND_USER_CX;
+}
+
+// short-range table of a user kernel: f(r) and g(r) = f'(r)/r (complex:
+// real and imaginary parts) on r = Rdir 2^-(k+1) (1 + x), x in [0, 1), per
+// octave k < kUserOct in S equal pieces, each a polynomial of degree
+// kUserDeg in t in [-1, 1]; coefficients per piece in the order
+// f, g (complex: Re f, Im f, Re g, Im g), lowest degree first
+constexpr int kUserOct = 32;
+constexpr int kUserDeg = 7;
+
+// highest B-spline order of the mixed-precision form
+constexpr long kMixedMaxBn = 12;
 
 // Derivative-match split of a radial kernel (host, FP64).
 struct DMKernel {
@@ -205,6 +235,13 @@ struct GridKey {
   }
 };
 
+// host-side state of a user-defined kernel set (borrowed callbacks)
+struct UserKernels {
+  bool complex = false;
+  npme_kfunc_real r[3] = {};      // func, funcLR, funcSR
+  npme_kfunc_complex c[3] = {};
+};
+
 struct Handle {
   std::mutex mtx;
   bool ready = false;  // the last set-up completed (device state valid)
@@ -222,6 +259,15 @@ struct Handle {
   // configuration
   Keywords kw;
   DMKernel ker;
+  // user-defined kernels: the callbacks, the short-range table (device, in
+  // the working precision) and its error, and the positions on the host for
+  // the exact sum, which evaluates the user's full kernel on the host
+  bool userK = false;
+  UserKernels user;
+  void *d_utab = nullptr;
+  int utS = 0;
+  double utErr[2] = {0, 0};
+  std::vector<double> coordHost;
   bool complexK = false;
   bool fp64 = false;
   long Bn = 6;
@@ -414,8 +460,19 @@ struct DirectParams {
   double cellTol; // bound on a charge's distance outside its cell's box
   int fxShift;    // mixed form: one cluster edge = 2^fxShift fixed-point units
   double fxInv;   // cluster edge / 2^fxShift
+  const void *utab = nullptr;  // user kernels: short-range table, S pieces per octave
+  int utS = 0;
 };
 double handle_count_pairs(Handle &h);
+// user-defined kernels, host side (genfunc.cpp)
+void user_fself(const UserKernels &u, double fself[2]);
+void user_check_radial(const UserKernels &u, double Rdir);
+void user_build_table(const UserKernels &u, double Rdir, std::vector<double> &coef, int &S,
+                      double err[2]);
+void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<double> &tab,
+                     std::vector<double> &F);
+void user_exact_host(const UserKernels &u, size_t n, const double *coord, const double *q,
+                     size_t nt, const long *idx, double *V);
 DirectParams direct_params(const Handle &h);
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s);

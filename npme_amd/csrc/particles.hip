@@ -31,6 +31,9 @@ struct DirectDev {
   int fxUnit;      // 2^S: one cluster edge in fixed-point units (mixed form)
   float fxInv;     // cluster edge / 2^S: length of one fixed-point unit
   float fxClose2;  // below this r^2 a pair adds the sub-unit residuals
+  const void *utab;  // user kernels: short-range table in the working precision
+  int utS;           // pieces per octave of the table
+  double invRdir;
 };
 
 // coefficient array of the working precision
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(256) void k_direct(DirectDev p, size_t n,
                                                 int nP, double csA, double *__restrict__ Vdir)
 {
   using V4 = typename V4t<Real>::T;
-  constexpr bool CX = (KIND == KIND_HELMHOLTZ);
+  constexpr bool CX = kind_is_complex(KIND);
   const int lane = threadIdx.x & 63;
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const int blk = wg * 4 + (threadIdx.x >> 6);
@@ -338,6 +341,9 @@ static DirectDev direct_dev(const DirectParams &p)
   // pairs closer than 2^24 units: one unit exceeds FP32 relative accuracy
   const double rc = std::ldexp(p.fxInv, 24);
   d.fxClose2 = (float)(rc * rc);
+  d.utab = p.utab;
+  d.utS = p.utS;
+  d.invRdir = 1.0 / p.Rdir;
   return d;
 }
 
@@ -364,11 +370,15 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
     if (p.kind == KIND_LAPLACE) NPME_DIRECT(double, KIND_LAPLACE, false);
     else if (p.kind == KIND_LAPLACE_ERF) NPME_DIRECT_D(double, KIND_LAPLACE_ERF, false, 8);
     else if (p.kind == KIND_RALPHA) NPME_DIRECT(double, KIND_RALPHA, false);
+    else if (p.kind == KIND_USER) NPME_DIRECT_D(double, KIND_USER, false, 4);
+    else if (p.kind == KIND_USER_CX) NPME_DIRECT_D(double, KIND_USER_CX, false, 4);
     else NPME_DIRECT(double, KIND_HELMHOLTZ, false);
   } else {
     if (p.kind == KIND_LAPLACE) NPME_DIRECT(float, KIND_LAPLACE, true);
     else if (p.kind == KIND_LAPLACE_ERF) NPME_DIRECT_D(float, KIND_LAPLACE_ERF, true, 8);
     else if (p.kind == KIND_RALPHA) NPME_DIRECT(float, KIND_RALPHA, true);
+    else if (p.kind == KIND_USER) NPME_DIRECT_D(float, KIND_USER, true, 4);
+    else if (p.kind == KIND_USER_CX) NPME_DIRECT_D(float, KIND_USER_CX, true, 4);
     else NPME_DIRECT(float, KIND_HELMHOLTZ, true);
   }
 #undef NPME_DIRECT_D
@@ -902,6 +912,48 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   NPME_HIP_CHECK(hipGetLastError());
 }
 
+// short-range part of a user kernel from its table: f[0], f[1] = Re, Im of
+// f(r) and f[2], f[3] = Re, Im of f'(r)/r (real kernels: f[1] = f[3] = 0).
+// r = Rdir 2^e m with m in [0.5, 1) gives octave k = -e and x = 2m - 1 in
+// [0, 1); r = Rdir takes the table's end, r > Rdir (pairs listed within a
+// relative 1e-9 of the cutoff) adds zero, as the reference's short-range
+// kernels do outside Rdir, and r below the last octave takes its start.
+// The piece and octave indices are clamped, so every table read stays
+// inside it for any r2.
+template <typename Real, bool CXU>
+__device__ __forceinline__ void user_sr(const DirectDev &p, Real r2, Real *f)
+{
+  constexpr int P = kUserDeg + 1, NQ = CXU ? 4 : 2;
+  const Real u = sqrt(r2) * (Real)p.invRdir;
+  int e;
+  const Real m = frexp(u, &e);
+  int k = -e;
+  Real x = (Real)2 * m - (Real)1;
+  const Real keep = (e > 1 || (e == 1 && m > (Real)0.5)) ? (Real)0 : (Real)1;
+  if (k < 0) {
+    k = 0;
+    x = (Real)1;
+  }
+  if (k >= kUserOct) {
+    k = kUserOct - 1;
+    x = (Real)0;
+  }
+  const int S = p.utS;
+  const Real xs = x * (Real)S;
+  int j = (int)xs;
+  j = min(max(j, 0), S - 1);
+  const Real t = (Real)2 * (xs - (Real)j) - (Real)1;
+  const Real *c = static_cast<const Real *>(p.utab) + ((size_t)k * S + j) * (NQ * P);
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    Real s = c[q * P + P - 1];
+#pragma unroll
+    for (int d = P - 2; d >= 0; d--) s = s * t + c[q * P + d];
+    f[CXU ? q : 2 * q] = s * keep;
+  }
+  if (!CXU) f[1] = f[3] = (Real)0;
+}
+
 // short-range DM pair term of one source (charge qr + i qim) at offset
 // (rx, ry, rz), r2 = |r|^2 < Rdir^2, added to part: V and grad (real), or
 // (re, im) of V, d/dx, d/dy, d/dz (complex)
@@ -934,6 +986,30 @@ __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Re
     part[1] += rx * fq;
     part[2] += ry * fq;
     part[3] += rz * fq;
+  } else if constexpr (KIND == KIND_USER) {
+    (void)topA;
+    (void)topB;
+    Real f[4];
+    user_sr<Real, false>(p, r2, f);
+    const Real fq = f[2] * qr;
+    part[0] += f[0] * qr;
+    part[1] += rx * fq;
+    part[2] += ry * fq;
+    part[3] += rz * fq;
+  } else if constexpr (KIND == KIND_USER_CX) {
+    (void)topA;
+    (void)topB;
+    Real f[4];
+    user_sr<Real, true>(p, r2, f);
+    part[0] += f[0] * qr - f[1] * qim;
+    part[1] += f[0] * qim + f[1] * qr;
+    const Real gr = f[2] * qr - f[3] * qim, gi = f[2] * qim + f[3] * qr;
+    part[2] += rx * gr;
+    part[3] += rx * gi;
+    part[4] += ry * gr;
+    part[5] += ry * gi;
+    part[6] += rz * gr;
+    part[7] += rz * gi;
   } else if constexpr (KIND == KIND_RALPHA) {
     const Real fe = pow(r2, (Real)(0.5 * p.alpha));
     const Real f0 = fe - horner_top<Real, MAXD>(topA, coef_a<Real>(p), r2);
@@ -989,7 +1065,7 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
                                              Real topB)
 {
   using V4 = typename V4t<Real>::T;
-  constexpr bool CX = (KIND == KIND_HELMHOLTZ);
+  constexpr bool CX = kind_is_complex(KIND);
   const size_t s = slice * 64 + lane;
   const bool tvalid = s < n;
   const size_t tt = tvalid ? s : slice * 64;
@@ -1175,11 +1251,15 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
     if (p.kind == KIND_LAPLACE) NPME_DNL(double, KIND_LAPLACE, false);
     else if (p.kind == KIND_LAPLACE_ERF) NPME_DNL_D(double, KIND_LAPLACE_ERF, false, 4);
     else if (p.kind == KIND_RALPHA) NPME_DNL(double, KIND_RALPHA, false);
+    else if (p.kind == KIND_USER) NPME_DNL_D(double, KIND_USER, false, 4);
+    else if (p.kind == KIND_USER_CX) NPME_DNL_D(double, KIND_USER_CX, false, 4);
     else NPME_DNL(double, KIND_HELMHOLTZ, false);
   } else {
     if (p.kind == KIND_LAPLACE) NPME_DNL(float, KIND_LAPLACE, true);
     else if (p.kind == KIND_LAPLACE_ERF) NPME_DNL_D(float, KIND_LAPLACE_ERF, true, 4);
     else if (p.kind == KIND_RALPHA) NPME_DNL(float, KIND_RALPHA, true);
+    else if (p.kind == KIND_USER) NPME_DNL_D(float, KIND_USER, true, 4);
+    else if (p.kind == KIND_USER_CX) NPME_DNL_D(float, KIND_USER_CX, true, 4);
     else NPME_DNL(float, KIND_HELMHOLTZ, true);
   }
 #undef NPME_DNL
@@ -1951,7 +2031,7 @@ __global__ __launch_bounds__(256) void k_exact(double alpha, double k0r, double 
                                                const long *__restrict__ idx,
                                                double *__restrict__ Vout)
 {
-  constexpr bool CX = (KIND == KIND_HELMHOLTZ);
+  constexpr bool CX = kind_is_complex(KIND);
   __shared__ double sx[256], sy[256], sz[256], sq[256], sqi[256];
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = t < nt;

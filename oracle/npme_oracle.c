@@ -167,6 +167,21 @@ int npo_kernel_init_erf(npo_kernel *k, double Rdir, double beta)
   return 0;
 }
 
+int npo_kernel_init_rfunc(npo_kernel *k, int isComplex, double cr, double ci, double Rdir,
+                          double beta)
+{
+  memset(k, 0, sizeof(*k));
+  if (!(Rdir > 0) || !(beta > 0)) return -1;
+  k->kind = isComplex ? NPO_RFUNC_CX : NPO_RFUNC;
+  k->alpha = 1.0;
+  k->k0r = cr;
+  k->k0i = isComplex ? ci : 0.0;
+  k->Nder = 2;
+  k->Rdir = Rdir;
+  k->beta = beta;
+  return 0;
+}
+
 /* erf(beta r)/r and (1/r) d/dr of it; the small-r limits by series */
 static void erf_over_r(double beta, double r, double *f0, double *f1)
 {
@@ -223,6 +238,32 @@ void npo_kernel_eval(const npo_kernel *k, int part, double r2, double *f0o,
   cplx f0 = 0, f1 = 0;
   const double R = k->Rdir;
   const double r = sqrt(fabs(r2));
+  if (k->kind == NPO_RFUNC || k->kind == NPO_RFUNC_CX) {
+    /* app/userDefineFunc.cpp:180-215 (f = r, df/dx = x/r) scaled by c;
+     * LR = c r erf(beta r), SR = c r erfc(beta r) (its header comment,
+     * :17-20); f1 = (1/r) df/dr, with its limit 4 beta/sqrt(pi) of the LR
+     * part at r = 0 */
+    const cplx c = k->k0r + I * k->k0i;
+    const double x = k->beta * r, g = 2.0 / sqrt(PI) * k->beta * exp(-x * x);
+    if (part == NPO_FULL) {
+      f0 = c * r;
+      f1 = c / r;
+    } else if (part == NPO_LR) {
+      f0 = c * r * erf(x);
+      f1 = (x < 1e-8) ? c * 2.0 * g : c * (erf(x) / r + g);
+    } else if (!(r > R)) {
+      f0 = c * r * erfc(x);
+      f1 = c * (erfc(x) / r - g);
+    }
+    if (k->kind == NPO_RFUNC_CX) {
+      f0o[0] = creal(f0); f0o[1] = cimag(f0);
+      f1o[0] = creal(f1); f1o[1] = cimag(f1);
+    } else {
+      f0o[0] = creal(f0);
+      f1o[0] = creal(f1);
+    }
+    return;
+  }
   if (k->kind == NPO_LAPLACE_ERF) {
     double l0, l1;
     erf_over_r(k->beta, r, &l0, &l1);
@@ -265,7 +306,7 @@ void npo_kernel_eval(const npo_kernel *k, int part, double r2, double *f0o,
       f1 = e1 - horner(k->b, k->Nder - 1, r2);
     }
   }
-  if (k->kind == NPO_HELMHOLTZ) {
+  if (NPO_IS_COMPLEX(k->kind)) {
     f0o[0] = creal(f0); f0o[1] = cimag(f0);
     f1o[0] = creal(f1); f1o[1] = cimag(f1);
   } else {
@@ -783,7 +824,7 @@ int npo_fgrid_compact(const npo_kernel *k, const long N[3], const double a[3],
               double z = az ? x2[2][i3] : x1[2][i3];
               double f0[2], f1[2];
               npo_kernel_eval(k, NPO_LR, x * x + y * y + z * z, f0, f1);
-              cplx fv = (k->kind == NPO_HELMHOLTZ) ? (f0[0] + I * f0[1]) : f0[0];
+              cplx fv = NPO_IS_COMPLEX(k->kind) ? (f0[0] + I * f0[1]) : f0[0];
               s += tx * ty * tz * fv;
             }
         F[(i1 * M[1] + i2) * M[2] + i3] = s;
@@ -1052,7 +1093,7 @@ static void ugrid_free(ugrid *g) { free(g->start); free(g->list); }
 void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
                    const double *q, double *V)
 {
-  const int cx = (k->kind == NPO_HELMHOLTZ);
+  const int cx = NPO_IS_COMPLEX(k->kind);
   const double R = k->Rdir;
   ugrid g;
   ugrid_build(&g, n, coord, R * 1.0000001);
@@ -1100,7 +1141,7 @@ void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
 void npo_exact(const npo_kernel *k, size_t n, const double *coord,
                const double *q, size_t nt, const long *idx, double *V)
 {
-  const int cx = (k->kind == NPO_HELMHOLTZ);
+  const int cx = NPO_IS_COMPLEX(k->kind);
   long t;
 #pragma omp parallel for schedule(dynamic, 16)
   for (t = 0; t < (long) nt; t++) {
@@ -1176,7 +1217,7 @@ int npo_calcv(const npo_kernel *k, const npo_recsum_parm *p, size_t n,
               const double *q, const double *coord, double *V)
 {
   double t0 = omp_get_wtime();
-  const int cx = (k->kind == NPO_HELMHOLTZ);
+  const int cx = NPO_IS_COMPLEX(k->kind);
   const int w = cx ? 8 : 4;
   npo_direct_sr(k, n, coord, q, V);
   double fs0[2], fs1[2];

@@ -27,6 +27,12 @@ extern "C" {
 #define NPO_RALPHA    1   /* f = r^alpha                     */
 #define NPO_HELMHOLTZ 2   /* f = exp(i k0 r)/r (complex)     */
 #define NPO_LAPLACE_ERF 3 /* f = 1/r with the Ewald split erf(beta r)/r + erfc(beta r)/r */
+/* f = c r with the split c r erf(beta r) + c r erfc(beta r): the user-defined
+ * kernel of app/userDefineFunc.cpp:17-20,180-215 (rFunc), c = k0r + i k0i;
+ * NPO_RFUNC is real (c = k0r), NPO_RFUNC_CX complex */
+#define NPO_RFUNC    4
+#define NPO_RFUNC_CX 5
+#define NPO_IS_COMPLEX(kind) ((kind) == NPO_HELMHOLTZ || (kind) == NPO_RFUNC_CX)
 
 /* kernel parts */
 #define NPO_FULL 0
@@ -56,6 +62,10 @@ int  npo_kernel_init(npo_kernel *k, int kind, double alpha, double k0r,
  * LR = erf(beta r)/r, SR = erfc(beta r)/r, beta from Rdir and tol. */
 double npo_rdir2beta(double Rdir, double tol);
 int  npo_kernel_init_erf(npo_kernel *k, double Rdir, double beta);
+/* the rFunc user kernel (NPO_RFUNC / NPO_RFUNC_CX) with coefficient c and
+ * split parameter beta; the short-range part is cut at r > Rdir */
+int  npo_kernel_init_rfunc(npo_kernel *k, int isComplex, double cr, double ci,
+                           double Rdir, double beta);
 /* evaluate one part of the kernel at r2: f0 and f1 = (1/r) df/dr.
  * Real kernels write f0[0], f1[0]; Helmholtz writes (re, im) pairs. */
 void npo_kernel_eval(const npo_kernel *k, int part, double r2,

@@ -13,7 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnpme_oracle.so")
 
-LAPLACE, RALPHA, HELMHOLTZ, LAPLACE_ERF = 0, 1, 2, 3
+LAPLACE, RALPHA, HELMHOLTZ, LAPLACE_ERF, RFUNC, RFUNC_CX = 0, 1, 2, 3, 4, 5
+_COMPLEX_KINDS = (HELMHOLTZ, RFUNC_CX)
 FULL, LR, SR = 0, 1, 2
 
 
@@ -26,7 +27,7 @@ class Kernel(C.Structure):
         n = self.Nder + 1
         a = np.array(self.a[:2 * n]).reshape(n, 2)
         b = np.array(self.b[:2 * n]).reshape(n, 2)
-        if self.kind == HELMHOLTZ:
+        if self.kind in _COMPLEX_KINDS:
             return a[:, 0] + 1j * a[:, 1], b[:, 0] + 1j * b[:, 1]
         return a[:, 0].copy(), b[:, 0].copy()
 
@@ -59,6 +60,7 @@ def lib():
         L.npo_rdir2beta.argtypes = [d, d]
         L.npo_rdir2beta.restype = d
         L.npo_kernel_init_erf.argtypes = [C.POINTER(Kernel), d, d]
+        L.npo_kernel_init_rfunc.argtypes = [C.POINTER(Kernel), C.c_int, d, d, d, d]
         L.npo_kernel_eval.argtypes = [C.POINTER(Kernel), i, d, p, p]
         L.npo_optimal_grid_size.restype = l
         L.npo_optimal_grid_size.argtypes = [l]
@@ -135,11 +137,22 @@ def kernel_erf(Rdir, beta):
     return k
 
 
+def kernel_rfunc(Rdir, beta, c=1.0, complex_kernel=False):
+    """The rFunc user kernel of app/userDefineFunc.cpp: f = c r, split as
+    c r erf(beta r) + c r erfc(beta r)"""
+    k = Kernel()
+    c = complex(c)
+    if lib().npo_kernel_init_rfunc(C.byref(k), int(bool(complex_kernel)), c.real, c.imag,
+                                   float(Rdir), float(beta)) != 0:
+        raise ValueError("bad kernel parameters")
+    return k
+
+
 def kernel_eval(k, part, r2):
     f0 = np.zeros(2)
     f1 = np.zeros(2)
     lib().npo_kernel_eval(C.byref(k), int(part), float(r2), _ptr(f0), _ptr(f1))
-    if k.kind == HELMHOLTZ:
+    if k.kind in _COMPLEX_KINDS:
         return complex(f0[0], f0[1]), complex(f1[0], f1[1])
     return f0[0], f1[0]
 
@@ -300,7 +313,7 @@ def _vshape(k, n):
 
 def direct_sr(k, coord, q):
     coord = _f64(coord)
-    cx = k.kind == HELMHOLTZ
+    cx = k.kind in _COMPLEX_KINDS
     qq = _c128_as_f64(q) if cx else _f64(q)
     V = np.zeros((len(coord), 4), np.complex128 if cx else np.float64)
     lib().npo_direct_sr(C.byref(k), len(coord), _ptr(coord), _ptr(qq), _ptr(V))
@@ -309,7 +322,7 @@ def direct_sr(k, coord, q):
 
 def exact(k, coord, q, idx=None):
     coord = _f64(coord)
-    cx = k.kind == HELMHOLTZ
+    cx = k.kind in _COMPLEX_KINDS
     qq = _c128_as_f64(q) if cx else _f64(q)
     if idx is None:
         nt, ip = len(coord), None
@@ -335,7 +348,7 @@ def calcv(k, N, Bn, coord, q, nblk=(20, 20, 20), dl=None, a=(4.0, 4.0, 4.0)):
         p.a[d] = float(a[d])
     p.Bn = int(Bn)
     coord = _f64(coord)
-    cx = k.kind == HELMHOLTZ
+    cx = k.kind in _COMPLEX_KINDS
     qq = _c128_as_f64(q) if cx else _f64(q)
     V = np.zeros((len(coord), 4), np.complex128 if cx else np.float64)
     rc = lib().npo_calcv(C.byref(k), C.byref(p), len(coord), _ptr(qq), _ptr(coord), _ptr(V))

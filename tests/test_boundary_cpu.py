@@ -82,6 +82,19 @@ def test_setup_rejects_odd_spline_order(Bn):
     assert "even" in N.npme_last_error()
 
 
+@pytest.mark.parametrize("Bn", [14, 16])
+def test_setup_rejects_mixed_above_order_12(Bn):
+    """The mixed form is provided up to spline order 12; higher orders are
+    refused with NPME_ERR_UNSUPPORTED before any device work unless
+    precision fp64 is asked for."""
+    import npme_amd as N
+    coord, q = uniform_box(500, 10.0)
+    k = _kw(BnOrder=Bn, N1=80, N2=80, N3=80, n1=8, n2=8, n3=8, nDeriv=6, precision="mixed")
+    assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q)
+    assert N.NPME_Interface_SetUpRealKernel(k, len(q), coord) is None
+    assert "fp64" in N.npme_last_error()
+
+
 @pytest.mark.parametrize("bad", [dict(calcType="fast"), dict(funcType="Yukawa"),
                                  dict(EwaldSplit="Other"), dict(funcType="Ralpha"),
                                  dict(funcType="Helmholtz", k0_r=1.0, k0_i=0.0)])
@@ -143,8 +156,9 @@ def test_cpp_dropin_header_host_calls(oracle, tmp_path):
 
 
 def test_cpp_header_files_and_descriptors(tmp_path):
-    """The src/ReadPrint.h / SupportFunctions.h helpers and the
-    parameterised kernel descriptors of the C++ header, compiled with
+    """The src/ReadPrint.h / SupportFunctions.h helpers and the built-in
+    kernel classes of the C++ header (host Calc: LR + SR = the kernel,
+    NPME_KernelFuncCheck, NPME_Interface_SelectKernelPtr), compiled with
     -Wall -Wextra -Werror and run on the host (tests/cpp/readprint_host.cpp)."""
     import shutil
     import subprocess
@@ -177,3 +191,23 @@ def test_server_request_board_threads(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_user_define_func_tool_host():
+    """test/06_userDefineFunc: the userDefineFunc tool (app/userDefineFunc.cpp
+    against the C++ header) evaluates its plugin rFunc on 8 random points in
+    both Calc forms and passes NPME_KernelFuncCheck (analytic against
+    numerical derivatives) on the host."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "npme_amd", "bin", "userDefineFunc")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split("\n")
+    xyz = [list(map(float, l.split("=")[1].split())) for l in lines if l.startswith("xyz =")]
+    rs = [float(l.split("=")[1]) for l in lines if l.startswith("r =") and "drdx" not in l]
+    grads = [l for l in lines if "drdx" in l]
+    assert len(xyz) == 8 and len(rs) == 8 and len(grads) == 8
+    for p, rv in zip(xyz, rs):
+        assert abs(rv - np.sqrt(sum(c * c for c in p))) < 1e-4
+    assert "NPME_KernelFuncCheck passed" in r.stdout

@@ -97,3 +97,19 @@ def test_calc_tool_pme_exact(tmp_path, tool, kwfile, kind, bound):
     V = h.calcv(q)
     Vf = N.NPME_ReadSingleBox_V(os.path.join(tmp, stem + "_V_pme.output"))
     assert np.all(np.abs(Vf - V) <= 1e-10 * np.abs(V) + 1e-300)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_user_define_func_calcv(tmp_path, prec):
+    """userDefineFunc (app/userDefineFunc.cpp, test/06): the C++ plugin
+    rFunc (f = r) and its erf/erfc parts through NPME_InterfaceReal_GenFunc
+    on the GPU, 2,000 charges on a 96^3 grid, against CalcV_exact (the
+    plugin's Calc on the host).  The oracle reaches 4.6e-8 on the same
+    configuration."""
+    out = _run([os.path.join(BIN, "userDefineFunc"), "calcv", "2000", prec], str(tmp_path))
+    assert "NPME_KernelFuncCheck passed" in out
+    m = re.search(r"relV (\S+) reldV (\S+)", out)
+    assert m, out
+    relV, reldV = float(m.group(1)), float(m.group(2))
+    print(prec, "relV", relV, "reldV", reldV)
+    assert relV < 1e-6 and reldV < 1e-5

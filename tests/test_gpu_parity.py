@@ -6,7 +6,8 @@ Tolerances (relative RMS, compareV metric):
   GPU CalcV_exact vs oracle exact ........... 1e-12
   mixed GPU path vs oracle CalcV ............ 5e-7 (V), 5e-6 (grad)
   mixed GPU path vs exact sum ............... 1e-6 (V) for 1/r at tol 1e-6
-  mixed GPU path vs oracle, every spline order 4..16 ... 1e-6 (V), 1e-5 (grad)
+  mixed GPU path vs oracle, every spline order 4..12 ... 1e-6 (V), 1e-5 (grad)
+  (the mixed form refuses orders above 12)
 The integer work (cell and cluster indices, working order, spline windows)
 is compared bit for bit with the oracle in test_gpu_indices.py; here the
 results must also be bit-identical run to run."""
@@ -68,8 +69,9 @@ def test_spline_orders_match_oracle(oracle, prec, Bn):
     compiled for (4, 6, 8, 10) and the generic order (12, 14, 16), on a
     given 64^3 grid (80^3 for order 16, which needs N/4 > 16), against the
     oracle's reciprocal + direct sum.  The B-spline weights are computed in
-    FP64 and rounded once, so the mixed form's rounding does not grow with
-    the order; it is held to the requested 1e-6 at every order."""
+    FP64 and rounded once.  The mixed form is held to the requested 1e-6 at
+    every order it accepts (up to 12: measured 2.5e-7 at orders 4-12); at
+    order 14 it measured 1.06e-6, so set-up refuses orders above 12."""
     import npme_amd as N
     coord, q = uniform_box(2500, 25.0)
     Ng = 80 if Bn == 16 else 64
@@ -78,6 +80,9 @@ def test_spline_orders_match_oracle(oracle, prec, Bn):
             N1=Ng, N2=Ng, N3=Ng, n1=8, n2=8, n3=8, nDeriv=6)
     assert N.NPME_AddMissingKeywords(k, len(q), coord, chargeReal=q), N.npme_last_error()
     h = N.NPME_Interface_SetUpRealKernel(k, len(q), coord)
+    if prec == "mixed" and Bn > 12:
+        assert h is None and "fp64" in N.npme_last_error()
+        return
     assert h is not None, N.npme_last_error()
     V = h.calcv(q)
     ko = oracle.kernel(oracle.LAPLACE, h.Get_Nder(), 5.0)
@@ -103,10 +108,12 @@ def test_exact_matches_oracle(oracle):
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_ralpha_anisotropic(oracle, prec):
-    """r^0.5, DM, anisotropic box, explicit grid (no RSEM row for 0.5)."""
+    """r^0.5, DM, anisotropic box, explicit grid (no RSEM row for 0.5).  The
+    grid 72x72x168 has a method error of 2.5e-7 against the exact sum (the
+    oracle on the same grid); a coarser 60x60x140 grid gives 1.09e-6."""
     coord, q = uniform_box(3000, (20.0, 20.0, 60.0))
     h = _real_setup(coord, q, prec, funcType="Ralpha", alpha=0.5, Rdir=6.0, nDeriv=6,
-                    N1=60, N2=60, N3=140, n1=15, n2=15, n3=14)
+                    N1=72, N2=72, N3=168, n1=12, n2=12, n3=12)
     V = h.calcv(q)
     k = oracle.kernel(oracle.RALPHA, 6, 6.0, alpha=0.5)
     Vo = oracle.calcv(k, _grid(h), 6, coord, q)
@@ -168,7 +175,8 @@ def test_helmholtz(oracle, prec):
 def test_helmholtz_high_orders(oracle, prec, Bn):
     """Complex charges at spline orders 14 and 16: the complex FP64 theta
     block of an interpolation tile needs tile edges 2 and 1 to stay inside
-    64 KB of LDS (an 80^3 grid, N/4 > 16)."""
+    64 KB of LDS (an 80^3 grid, N/4 > 16).  The mixed form is refused above
+    order 12."""
     import npme_amd as N
     coord, _ = uniform_box(2000, 22.0)
     rng = np.random.default_rng(12)
@@ -178,6 +186,9 @@ def test_helmholtz_high_orders(oracle, prec, Bn):
             tol=1e-6, precision=prec, N1=80, N2=80, N3=80, n1=8, n2=8, n3=8, nDeriv=8)
     assert N.NPME_AddMissingKeywords(k, 2000, coord, chargeComplex=qc), N.npme_last_error()
     h = N.NPME_Interface_SetUpComplexKernel(k, 2000, coord)
+    if prec == "mixed":
+        assert h is None and "fp64" in N.npme_last_error()
+        return
     assert h is not None, N.npme_last_error()
     V = h.calcv(qc)
     ko = oracle.kernel(oracle.HELMHOLTZ, 8, 5.0, k0=0.8 + 0.1j)

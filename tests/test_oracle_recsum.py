@@ -204,3 +204,66 @@ def test_whole_path_helmholtz(oracle):
     V = oracle.calcv(k, N, 8, coord, qc, nb)
     m = oracle.compare_v(oracle.exact(k, coord, qc), V)
     assert m["relV"] <= 1e-5, m
+
+
+@pytest.mark.parametrize("cx", [False, True])
+def test_rfunc_kernel_parts_and_bruteforce(oracle, cx):
+    """The rFunc user kernel of app/userDefineFunc.cpp (f = c r with the
+    split c r erf(beta r) + c r erfc(beta r)): the two parts add up to the
+    kernel, (1/r) df/dr matches central differences of each part, the
+    short-range part is zero beyond Rdir, and the direct and exact sums equal
+    numpy brute force."""
+    from scipy.special import erf, erfc
+    c = (1.0 + 0.5j) if cx else 1.0
+    k = oracle.kernel_rfunc(6.0, 0.5, c=c, complex_kernel=cx)
+    for r in (0.0, 1e-9, 0.3, 2.0, 5.9, 6.5):
+        r2 = r * r
+        f, f1 = oracle.kernel_eval(k, oracle.FULL, r2)
+        lr, lr1 = oracle.kernel_eval(k, oracle.LR, r2)
+        sr, sr1 = oracle.kernel_eval(k, oracle.SR, r2)
+        assert abs(lr - c * r * erf(0.5 * r)) < 1e-15 * (1 + abs(c * r))
+        if r <= 6.0:
+            assert abs(lr + sr - f) < 1e-14 * (1 + abs(f))
+            assert abs(sr - c * r * erfc(0.5 * r)) < 1e-15 * (1 + abs(c * r))
+        else:
+            assert sr == 0 and sr1 == 0
+        if r > 0.1:
+            h = 1e-5
+            for part, d in ((1, lr1), (2, sr1)):
+                if part == 2 and r > 6.0:
+                    continue
+                fp = oracle.kernel_eval(k, part, (r + h) ** 2)[0]
+                fm = oracle.kernel_eval(k, part, (r - h) ** 2)[0]
+                assert abs((fp - fm) / (2 * h) / r - d) < 1e-7 * (1 + abs(d))
+    # (1/r) d/dr of c r erf(beta r) tends to 4 beta c / sqrt(pi) at r = 0
+    assert abs(oracle.kernel_eval(k, 1, 0.0)[1] - c * 4 * 0.5 / np.sqrt(np.pi)) < 1e-15
+    coord, q = uniform_box(300, 12.0)
+    if cx:
+        q = q + 1j * np.roll(q, 5)
+    d = coord[:, None, :] - coord[None, :, :]
+    r = np.sqrt((d ** 2).sum(-1))
+    np.fill_diagonal(r, 1.0)
+    for fa, fa1, got in ((np.where(r <= 6.0, c * r * erfc(0.5 * r), 0),
+                          np.where(r <= 6.0, c * (erfc(0.5 * r) / r - 2 / np.sqrt(np.pi) * 0.5 *
+                                                  np.exp(-0.25 * r * r)), 0),
+                          oracle.direct_sr(k, coord, q)),
+                         (c * r, c / r, oracle.exact(k, coord, q))):
+        fa, fa1 = np.array(fa, dtype=complex), np.array(fa1, dtype=complex)
+        np.fill_diagonal(fa, 0)
+        np.fill_diagonal(fa1, 0)
+        exp = np.stack([fa @ q] + [(d[:, :, cc] * fa1) @ q for cc in range(3)], 1)
+        if not cx:
+            exp = exp.real
+        np.testing.assert_allclose(got, exp, atol=1e-10, rtol=1e-10)
+
+
+def test_rfunc_whole_path_converges_to_exact(oracle):
+    """The GenFunc path with the rFunc kernel converges to the exact sum as
+    the grid is refined (relative errors fall by more than 10x per step from
+    64^3 to 96^3) and reaches 1e-6 on the 96^3 grid the GPU tests use."""
+    coord, q = uniform_box(2000, 20.0)
+    k = oracle.kernel_rfunc(8.0, 0.5)
+    Ve = oracle.exact(k, coord, q)
+    errs = [oracle.compare_v(Ve, oracle.calcv(k, (g, g, g), 8, coord, q, nblk=(8, 8, 8)))["relV"]
+            for g in (64, 80, 96)]
+    assert errs[1] < errs[0] / 10 and errs[2] < errs[1] / 10 and errs[2] < 1e-6, errs
