@@ -173,6 +173,14 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen);
  * direct sum (the reference logs it as nInteract,
  * src/Interface.cpp:1329-1334). ---------------------------------------- */
 int npme_count_pairs(npme_handle *h, double *pairs);
+/* the same count without any device allocation, for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  No counterpart in the reference.  Its partial counts
+ * live in a static device array of 1 MiB per GPU (131,072 workgroup
+ * partials), so it accepts sets of at most 33,554,432 charges and returns
+ * NPME_ERR_ARG above that.  All calls on one GPU share that array: calls
+ * that overlap on one GPU race. */
+int npme_count_pairs_noalloc(npme_handle *h, double *pairs);
 
 /* ---- integer indexing of the last set-up, per charge in input order:
  * clusterKey[i] and cellKey[i] are the embedded cluster and cell indices of

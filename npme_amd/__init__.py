@@ -54,6 +54,7 @@ def lib():
         L.npme_print.argtypes = [p, C.c_char_p, s]
         L.npme_set_profiling.argtypes = [p, i]
         L.npme_count_pairs.argtypes = [p, p]
+        L.npme_count_pairs_noalloc.argtypes = [p, p]
         L.npme_create_genfunc_real.argtypes = [C.POINTER(p), p, s, p, p, p, p]
         L.npme_create_genfunc_complex.argtypes = [C.POINTER(p), p, s, p, p, p, p]
         L.npme_get_table_error.argtypes = [p, p]
@@ -291,6 +292,13 @@ class _InterfaceBase:
         """Ordered pairs i != j with r < Rdir (the direct-sum work)."""
         out = C.c_double()
         _check(lib().npme_count_pairs(self._h, C.byref(out)), "count_pairs")
+        return int(out.value)
+
+    def count_pairs_noalloc(self):
+        """The same count through npme_count_pairs_noalloc (no device
+        allocation; partials in the library's static device array)."""
+        out = C.c_double()
+        _check(lib().npme_count_pairs_noalloc(self._h, C.byref(out)), "count_pairs_noalloc")
         return int(out.value)
 
     def get_indices(self):

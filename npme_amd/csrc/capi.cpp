@@ -518,6 +518,15 @@ int npme_count_pairs(npme_handle *h, double *pairs)
   });
 }
 
+int npme_count_pairs_noalloc(npme_handle *h, double *pairs)
+{
+  return guarded([&] {
+    if (!h || !pairs) throw Error{NPME_ERR_ARG, "npme_count_pairs_noalloc: NULL argument"};
+    std::lock_guard<std::mutex> lk(h->h.mtx);
+    *pairs = handle_count_pairs_noalloc(h->h);
+  });
+}
+
 int npme_get_indices(npme_handle *h, size_t nCharge, unsigned long long *clusterKey,
                      unsigned long long *cellKey, long *mStart, long *order)
 {

@@ -464,6 +464,8 @@ struct DirectParams {
   int utS = 0;
 };
 double handle_count_pairs(Handle &h);
+// the same count with the partials in a static device array (noalloc.hip)
+double handle_count_pairs_noalloc(Handle &h);
 // user-defined kernels, host side (genfunc.cpp)
 void user_fself(const UserKernels &u, double fself[2]);
 void user_check_radial(const UserKernels &u, double Rdir);
