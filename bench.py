@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--slots", type=int, default=4, help="concurrent request slots of the server")
     p.add_argument("--pageable-requests", action="store_true",
                    help="serving client builds requests in pageable instead of page-locked memory")
+    p.add_argument("--stages-only", action="store_true",
+                   help="set up the N=1M workload and run --steps CalcV calls with the stages "
+                        "serialised on one stream and timed by HIP events (the timing the "
+                        "roofline uses); prints the stage times, for a profiler's kernel trace")
     p.add_argument("--calib-copy", action="store_true",
                    help="after the steps, copy a 256 MiB device buffer 3 times (a plain copy "
                         "kernel of known size, to calibrate profiler byte counters)")
@@ -281,7 +285,18 @@ def main():
     Ngrid = (h.Get_N1(), h.Get_N2(), h.Get_N3())
     Nder = h.Get_Nder()
     log("grid", Ngrid, "Nder", Nder, "setup %.3f s" % t_setup)
+    if a.stages_only:
+        stage_profile(h, dV, dq, max(1, a.warmup))
+        stages = stage_profile(h, dV, dq, a.steps)
+        print(json.dumps({"stage_ms_serialised": dict(zip(STAGES, np.round(stages, 4).tolist())),
+                          "ordered_pairs_in_cutoff": h.count_pairs(), "steps": a.steps}))
+        return 0
     ms = time_steps(h, dV, dq, a.steps, a.warmup)
+    # the same calls launching every kernel each time (no graph replay)
+    h.set_calcv_graph(False)
+    ms_nograph = time_steps(h, dV, dq, a.steps, a.warmup)
+    h.set_calcv_graph(True)
+    log("CalcV %.3f ms/step with graph replay, %.3f ms/step launching each call" % (ms, ms_nograph))
     if a.calib_copy:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -358,6 +373,9 @@ def main():
         roof = {"kernel": dom, "bound": "valu-fp32", "achieved": ach, "peak": FP32_PEAK,
                 "unit": "TFLOP/s", "frac": ach / FP32_PEAK, "traffic": None,
                 "note": "direct sum has no MFMA form; FP32 VALU peak"}
+    roof["timing"] = ("average launch time from HIP events on the library's stream with the "
+                      "stages serialised (npme_set_profiling); the timed CalcV runs the direct "
+                      "sum concurrently with the reciprocal sum")
     tf = os.path.join(ROOT, "profiles", "traffic_%s_1M.txt" % dom)
     if os.path.exists(tf) and n == 1000000:
         for line in open(tf):
@@ -409,6 +427,9 @@ def main():
         "host_buffers": {"ms_per_step": host_ms, "charges_per_s": n / (host_ms / 1e3),
                          "note": "npme_calcv_real from pageable host arrays: 8 MB charges in, "
                                  "32 MB V out over PCIe, median of 5"},
+        "launch": {"graph_replay_ms_per_step": ms, "kernel_launch_ms_per_step": ms_nograph,
+                   "note": "value uses the default mode: the CalcV launches captured once as a "
+                           "HIP graph and replayed"},
         "ordered_pairs_in_cutoff": P,
         "stage_ms": dict(zip(STAGES, np.round(stages, 4).tolist())),
         "stages": stage_info,

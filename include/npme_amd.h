@@ -196,6 +196,85 @@ int npme_count_pairs_noalloc(npme_handle *h, double *pairs);
 int npme_get_indices(npme_handle *h, size_t nCharge, unsigned long long *clusterKey,
                      unsigned long long *cellKey, long *mStart, long *order);
 
+/* ---- CalcV launch mode: on (default), the second and later CalcV calls of
+ * one set-up with the same charge and V pointers replay a HIP graph captured
+ * from the call's launches (the direct-sum and reciprocal branches joined on
+ * the device); off, every call launches the kernels.  Results are the same.
+ * No reference counterpart. ------------------------------------------------ */
+int npme_set_calcv_graph(npme_handle *h, int on);
+
+/* ---- two-box (translated-kernel) grids ------------------------------------
+ * The reference declares NPME_RecSumGrid_FullTrans_Func (src/RecSumGrid.h:
+ * 99-112, body src/RecSumGrid.cpp:663-794) and
+ * NPME_RSTP_CalcTheta_FullF_ReverseSign (src/RecSumV.h:102-113, body
+ * src/RecSumV.cpp:943-1009) for box-box interactions and calls neither from
+ * CalcV.  The two functions below are their device forms on host arrays in
+ * the reference's layouts; complex arrays are interleaved (re, im).
+ *
+ * npme_recsumgrid_fulltrans_func: f(r) = func(|r + R0|) on the full
+ * N1 x N2 x N3 grid of the box [-X - del1, X + del1) x ... (sample n at
+ * s L/N, s = n or n - N for n >= N/2; f[(n1 N2 + n2) N3 + n3]), summed over
+ * the 8 overlapped images with the top-hat weights of parameter a1..a3.  The
+ * kernel is the full kernel named by the keywords funcType (Laplace, Ralpha
+ * with alpha, Helmholtz with k0_r, k0_i), evaluated on the GPU in FP64;
+ * _user takes exactly one of func / cfunc (NPME_KfuncReal /
+ * NPME_KfuncComplex callbacks) and samples it on the host.  Where the
+ * reference prints rMin and exits (separation check, src/RecSumGrid.cpp:
+ * 694-705), these return NPME_ERR_ARG.
+ *
+ * npme_rstp_calctheta_fullF_reversesign: T = compact theta [N1/2][N2/2][N3/2]
+ * (DFT order, m < 0 at m + N/2) from the compact charge grid Qr (same
+ * layout) and F, the full forward DFT [N1][N2][N3] of the kernel grid, with
+ * the |lambda|^2 arrays of the axes (N_d values, DFT order); reverseSignX
+ * etc. use F as the grid of R0 with that component negated.  The reference's
+ * twiddle arrays C1..C3 (fixed by N) and its scratch X are not arguments; the
+ * convolution runs in FP64 on the GPU. */
+int npme_recsumgrid_fulltrans_func(double *f, long N1, double a1, double del1, long N2, double a2,
+                                   double del2, long N3, double a3, double del3, double X, double Y,
+                                   double Z, const npme_keywords *kernel, const double R0[3]);
+int npme_recsumgrid_fulltrans_func_user(double *f, long N1, double a1, double del1, long N2,
+                                        double a2, double del2, long N3, double a3, double del3,
+                                        double X, double Y, double Z, const npme_kfunc_real *func,
+                                        const npme_kfunc_complex *cfunc, const double R0[3]);
+int npme_rstp_calctheta_fullF_reversesign(double *T, const double *F, const double *Qr, long N1,
+                                          long N2, long N3, const double *lamda1_2,
+                                          const double *lamda2_2, const double *lamda3_2,
+                                          int reverseSignX, int reverseSignY, int reverseSignZ);
+
+/* ---- box pairs: V_t = sum_s q_s f(|r_t - r_s|) and its gradient at the
+ * targets t of one box from the sources s of another, entirely on the kernel
+ * grid (the reference has the two functions above and no driver: this
+ * pairing is the library's).  Each set is taken relative to the centre of
+ * its own bounding box, so the grid spans the larger box, not the pair; the
+ * kernel grid is the full kernel translated by the centre separation.  The
+ * keywords are those of npme_create (funcType, ..., BnOrder, precision);
+ * N1, N2, N3 are required (no error model), n1..n3, del1..3 (default
+ * Rdir, 8), a1..3 default as NPME_AddMissingKeywords.  Accuracy needs the
+ * grid box of edges 2 (X + del) to stay clear of the separation; set-up
+ * returns NPME_ERR_ARG when its distance rMin is below 1e-6.
+ * npme_boxpair_calcv_*: V [nTarget][4] (complex: [nTarget][4][2]);
+ * reverseSign (NULL: none) evaluates the pair with the separation negated
+ * along the flagged axes (the source box mirrored through the target box
+ * centre) with the same kernel grid.  npme_boxpair_separation gives R0 =
+ * target centre - source centre (f(r) = func(|r + R0|), r = the local
+ * target-minus-source separation) and rMin.  Box-pair handles reject
+ * npme_calcv_* and npme_reset_coord; npme_destroy releases them. */
+int npme_create_boxpair(npme_handle **out, const npme_keywords *kw, size_t nTarget,
+                        const double *coordTarget, size_t nSource, const double *coordSource);
+int npme_create_boxpair_genfunc_real(npme_handle **out, const npme_keywords *kw, size_t nTarget,
+                                     const double *coordTarget, size_t nSource,
+                                     const double *coordSource, const npme_kfunc_real *func);
+int npme_create_boxpair_genfunc_complex(npme_handle **out, const npme_keywords *kw, size_t nTarget,
+                                        const double *coordTarget, size_t nSource,
+                                        const double *coordSource, const npme_kfunc_complex *func);
+int npme_boxpair_reset_coord(npme_handle *h, size_t nTarget, const double *coordTarget,
+                             size_t nSource, const double *coordSource);
+int npme_boxpair_calcv_real(npme_handle *h, double *V, size_t nTarget, size_t nSource,
+                            const double *chargeSource, const int reverseSign[3]);
+int npme_boxpair_calcv_complex(npme_handle *h, double *V, size_t nTarget, size_t nSource,
+                               const double *chargeSource, const int reverseSign[3]);
+int npme_boxpair_separation(const npme_handle *h, double R0[3], double *rMin);
+
 /* ---- per-stage device times of the last CalcV when profiling is on:
  * ms[0] gather, [1] direct sum, [2] spread, [3..7] FFT passes A..E,
  * [8] interpolate+combine; returns the number of stages written. ------- */
@@ -205,9 +284,12 @@ int npme_get_stage_times(const npme_handle *h, double *ms, int n);
 /* ---- serving: a persistent loop of nSlots worker slots on the current
  * GPU that evaluates independent CalcV requests (any size and kernel)
  * concurrently.  A request is admitted first come first served when a slot
- * is free and its device-memory estimate fits the budget (memGB; <= 0
- * means 80% of the free device memory), runs NPME_AddMissingKeywords (RSEM
- * grid for its tol), the device set-up and CalcV, and writes V.  coord,
+ * is free and a provisional memory estimate fits the budget (memGB; <= 0
+ * means 80% of the free device memory); once NPME_AddMissingKeywords has
+ * chosen its grid (RSEM for its tol) its reservation becomes its own
+ * footprint (grids, per-charge arrays, neighbour list at its density),
+ * waiting for memory if that is larger, and after the device set-up the
+ * memory its set-up holds; then CalcV runs and V is written.  coord,
  * charge and V must stay valid until npme_server_wait returns for its
  * ticket.  A result does not depend on the other requests in flight.  No
  * reference counterpart: the reference evaluates one charge set per call
@@ -237,6 +319,12 @@ typedef struct {
    * choice (RSEM keywords), set-up, charges to the device, CalcV, V to the
    * host */
   double msKeywords, msSetup, msCopyIn, msCalc, msCopyOut;
+  /* device memory reserved for the request once its grid was chosen (its
+   * estimated footprint), GB; deviceGB is what its set-up holds */
+  double reservedGB;
+  /* number of requests whose CalcVs ran in the same graph launch as this
+   * one (1: it ran alone) */
+  int batch;
 } npme_request_info;
 int npme_server_create(npme_server **s, int nSlots, double memGB);
 /* queues a request; returns at once with its ticket */
@@ -246,6 +334,13 @@ int npme_server_submit(npme_server *s, const npme_request *r, long *ticket);
 int npme_server_wait(npme_server *s, long ticket, npme_request_info *info);
 /* seconds since the server was created */
 double npme_server_clock(const npme_server *s);
+/* small requests queued together are batched on one slot: up to maxBatch
+ * requests of at most maxCharges charges each run their set-ups in turn on
+ * handles of the slot, then their CalcVs as one captured HIP graph (one
+ * launch; the CalcVs run concurrently on the device), then their copies
+ * out.  Each result is the same as alone.  Defaults 8 and 32768; maxBatch 1
+ * turns batching off. */
+int npme_server_set_batching(npme_server *s, int maxBatch, size_t maxCharges);
 /* finishes the queued requests, then stops the slots */
 void npme_server_destroy(npme_server *s);
 /* page-locked host memory for request buffers (coord, charge, V): the

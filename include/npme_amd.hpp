@@ -1515,6 +1515,216 @@ inline void NPME_CompareV1Arrays(double &errorV, double &errordVdr, const size_t
   }
 }
 
+// ------------------------------------------------------------ two-box grids
+// NPME_RecSumGrid_FullTrans_Func (src/RecSumGrid.h:99-112): f(r) =
+// func(|r + R0|) on the full N1 x N2 x N3 grid.  A built-in full kernel
+// object (NPME_Kfunc_Laplace / _Ralpha / _Helmholtz) is sampled on the GPU,
+// any other object through its Calc on the host.  nProc is not used.  Where
+// the reference exits on a too small separation this returns false.
+namespace npme_detail {
+inline bool fulltrans_builtin(const NPME_BuiltinKernel &d, _Complex double *f, const long N[3],
+                              const double a[3], const double del[3], const double X[3],
+                              const double R0[3], bool PRINT, std::ostream &os)
+{
+  NPME_KeywordInput k;
+  if (d.family == NPME_BuiltinKernel::LAPLACE) {
+    k.AddKeyword("funcType", "Laplace");
+  } else if (d.family == NPME_BuiltinKernel::RALPHA) {
+    k.AddKeyword("funcType", "Ralpha");
+    k.AddKeyword("alpha", d.alpha);
+  } else {
+    k.AddKeyword("funcType", "Helmholtz");
+    k.AddKeyword("k0_r", d.k0_r);
+    k.AddKeyword("k0_i", d.k0_i);
+  }
+  const int rc = npme_recsumgrid_fulltrans_func(reinterpret_cast<double *>(f), N[0], a[0], del[0], N[1],
+                                                a[1], del[1], N[2], a[2], del[2], X[0], X[1], X[2],
+                                                k.c_handle(), R0);
+  if (rc != NPME_OK) os << "Error in NPME_RecSumGrid_FullTrans_Func.\n" << npme_last_error() << "\n";
+  else if (PRINT) os << "\nNPME_RecSumGrid_FullTrans_Func\n";
+  return rc == NPME_OK;
+}
+}  // namespace npme_detail
+
+inline bool NPME_RecSumGrid_FullTrans_Func(_Complex double *f, const long int N1, const double a1,
+                                           const double del1, const long int N2, const double a2,
+                                           const double del2, const long int N3, const double a3,
+                                           const double del3, const double X, const double Y,
+                                           const double Z, const NPME_KfuncReal &func,
+                                           const double R0[3], int nProc, bool PRINT,
+                                           std::ostream &os)
+{
+  (void)nProc;
+  const long N[3] = {N1, N2, N3};
+  const double a[3] = {a1, a2, a3}, del[3] = {del1, del2, del3}, Xb[3] = {X, Y, Z};
+  const NPME_BuiltinKernel *d = func.npme_builtin();
+  if (d && d->part == NPME_BuiltinKernel::FULL)
+    return npme_detail::fulltrans_builtin(*d, f, N, a, del, Xb, R0, PRINT, os);
+  const npme_kfunc_real c = {const_cast<NPME_KfuncReal *>(&func), npme_detail::tramp_value_r,
+                             npme_detail::tramp_grad_r};
+  const int rc = npme_recsumgrid_fulltrans_func_user(reinterpret_cast<double *>(f), N1, a1, del1, N2, a2,
+                                                     del2, N3, a3, del3, X, Y, Z, &c, nullptr, R0);
+  if (rc != NPME_OK) os << "Error in NPME_RecSumGrid_FullTrans_Func.\n" << npme_last_error() << "\n";
+  else if (PRINT) os << "\nNPME_RecSumGrid_FullTrans_Func\n";
+  return rc == NPME_OK;
+}
+
+inline bool NPME_RecSumGrid_FullTrans_Func(_Complex double *f, const long int N1, const double a1,
+                                           const double del1, const long int N2, const double a2,
+                                           const double del2, const long int N3, const double a3,
+                                           const double del3, const double X, const double Y,
+                                           const double Z, const NPME_KfuncComplex &func,
+                                           const double R0[3], int nProc, bool PRINT,
+                                           std::ostream &os)
+{
+  (void)nProc;
+  const long N[3] = {N1, N2, N3};
+  const double a[3] = {a1, a2, a3}, del[3] = {del1, del2, del3}, Xb[3] = {X, Y, Z};
+  const NPME_BuiltinKernel *d = func.npme_builtin();
+  if (d && d->part == NPME_BuiltinKernel::FULL)
+    return npme_detail::fulltrans_builtin(*d, f, N, a, del, Xb, R0, PRINT, os);
+  const npme_kfunc_complex c = {const_cast<NPME_KfuncComplex *>(&func), npme_detail::tramp_value_c,
+                                npme_detail::tramp_grad_c};
+  const int rc = npme_recsumgrid_fulltrans_func_user(reinterpret_cast<double *>(f), N1, a1, del1, N2, a2,
+                                                     del2, N3, a3, del3, X, Y, Z, nullptr, &c, R0);
+  if (rc != NPME_OK) os << "Error in NPME_RecSumGrid_FullTrans_Func.\n" << npme_last_error() << "\n";
+  else if (PRINT) os << "\nNPME_RecSumGrid_FullTrans_Func\n";
+  return rc == NPME_OK;
+}
+
+// NPME_RSTP_CalcTheta_FullF_ReverseSign (src/RecSumV.h:102-113): the compact
+// theta T from the compact charge grid Qr and the full DFT F of the kernel
+// grid, on the GPU.  The scratch X and the twiddles C1..C3 (fixed by N) are
+// not used, nor is nProc.  The reference returns nothing and cannot fail;
+// here a failure (no device, bad sizes) prints the error and leaves T as it
+// was.
+inline void NPME_RSTP_CalcTheta_FullF_ReverseSign(
+    _Complex double *T, _Complex double *X, const _Complex double *F, const _Complex double *Qr,
+    const long int N1, const long int N2, const long int N3, const _Complex double *C1,
+    const _Complex double *C2, const _Complex double *C3, const double *lamda1_2,
+    const double *lamda2_2, const double *lamda3_2, bool reverseSignX, bool reverseSignY,
+    bool reverseSignZ, int nProc, bool PRINT, std::ostream &os)
+{
+  (void)X;
+  (void)C1;
+  (void)C2;
+  (void)C3;
+  (void)nProc;
+  const int rc = npme_rstp_calctheta_fullF_reversesign(
+      reinterpret_cast<double *>(T), reinterpret_cast<const double *>(F),
+      reinterpret_cast<const double *>(Qr), N1, N2, N3, lamda1_2, lamda2_2, lamda3_2,
+      reverseSignX ? 1 : 0, reverseSignY ? 1 : 0, reverseSignZ ? 1 : 0);
+  if (rc != NPME_OK) os << "Error in NPME_RSTP_CalcTheta_FullF_ReverseSign.\n" << npme_last_error() << "\n";
+  else if (PRINT) os << "\n    NPME_RSTP_CalcT_FullF\n";
+}
+
+// A box pair (no reference counterpart; the reference has the two functions
+// above and no driver): V at the targets of one box from the sources of
+// another, entirely on the translated kernel grid.  The kernel is the
+// keywords' funcType (func == nullptr), a built-in full kernel object, or a
+// user kernel; a user kernel object is borrowed and must outlive the pair.
+class NPME_BoxPair {
+ public:
+  NPME_BoxPair() {}
+  NPME_BoxPair(const NPME_BoxPair &) = delete;
+  NPME_BoxPair &operator=(const NPME_BoxPair &) = delete;
+  ~NPME_BoxPair() { npme_destroy(_h); }
+
+  bool SetUp(const NPME_KeywordInput &keyword, size_t nTarget, const double *coordTarget,
+             size_t nSource, const double *coordSource, const NPME_KfuncReal *func, bool printLog,
+             std::ostream &os)
+  {
+    reset();
+    const NPME_BuiltinKernel *d = func ? func->npme_builtin() : nullptr;
+    int rc;
+    if (!func || (d && d->part == NPME_BuiltinKernel::FULL)) {
+      NPME_KeywordInput k = keyword;
+      if (d) {
+        k.AddKeyword("funcType", d->family == NPME_BuiltinKernel::RALPHA ? "Ralpha" : "Laplace");
+        if (d->family == NPME_BuiltinKernel::RALPHA) k.AddKeyword("alpha", d->alpha);
+      }
+      rc = npme_create_boxpair(&_h, k.c_handle(), nTarget, coordTarget, nSource, coordSource);
+    } else {
+      _cr = {const_cast<NPME_KfuncReal *>(func), npme_detail::tramp_value_r, npme_detail::tramp_grad_r};
+      rc = npme_create_boxpair_genfunc_real(&_h, keyword.c_handle(), nTarget, coordTarget, nSource,
+                                            coordSource, &_cr);
+    }
+    return done(rc, nTarget, nSource, false, printLog, os);
+  }
+  bool SetUp(const NPME_KeywordInput &keyword, size_t nTarget, const double *coordTarget,
+             size_t nSource, const double *coordSource, const NPME_KfuncComplex *func,
+             bool printLog, std::ostream &os)
+  {
+    reset();
+    const NPME_BuiltinKernel *d = func ? func->npme_builtin() : nullptr;
+    int rc;
+    if (d && d->part == NPME_BuiltinKernel::FULL) {
+      NPME_KeywordInput k = keyword;
+      k.AddKeyword("funcType", "Helmholtz");
+      k.AddKeyword("k0_r", d->k0_r);
+      k.AddKeyword("k0_i", d->k0_i);
+      rc = npme_create_boxpair(&_h, k.c_handle(), nTarget, coordTarget, nSource, coordSource);
+    } else if (!func) {
+      rc = npme_create_boxpair(&_h, keyword.c_handle(), nTarget, coordTarget, nSource, coordSource);
+    } else {
+      _cc = {const_cast<NPME_KfuncComplex *>(func), npme_detail::tramp_value_c, npme_detail::tramp_grad_c};
+      rc = npme_create_boxpair_genfunc_complex(&_h, keyword.c_handle(), nTarget, coordTarget, nSource,
+                                               coordSource, &_cc);
+    }
+    return done(rc, nTarget, nSource, true, printLog, os);
+  }
+  // V[nTarget][4]; reverseSign (nullptr: none) mirrors the source box through
+  // the target box centre along the flagged axes, with the same grid
+  bool CalcV(double *V, const double *chargeSource, const bool *reverseSign, bool printLog,
+             std::ostream &os)
+  {
+    int rv[3];
+    return npme_report(npme_boxpair_calcv_real(_h, V, _nT, _nS, chargeSource, flags(reverseSign, rv)),
+                       printLog, os);
+  }
+  bool CalcV(_Complex double *V, const _Complex double *chargeSource, const bool *reverseSign,
+             bool printLog, std::ostream &os)
+  {
+    int rv[3];
+    return npme_report(npme_boxpair_calcv_complex(_h, reinterpret_cast<double *>(V), _nT, _nS,
+                                                  reinterpret_cast<const double *>(chargeSource),
+                                                  flags(reverseSign, rv)),
+                       printLog, os);
+  }
+  // R0 = target centre - source centre (f(r) = func(|r + R0|)) and the
+  // distance of the grid box from the singularity
+  bool Separation(double R0[3], double &rMin) const { return npme_boxpair_separation(_h, R0, &rMin) == NPME_OK; }
+
+ private:
+  void reset()
+  {
+    npme_destroy(_h);
+    _h = nullptr;
+    _nT = _nS = 0;
+  }
+  bool done(int rc, size_t nT, size_t nS, bool cx, bool printLog, std::ostream &os)
+  {
+    (void)cx;
+    if (rc != NPME_OK) {
+      _h = nullptr;
+      return npme_report(rc, printLog, os);
+    }
+    _nT = nT;
+    _nS = nS;
+    return true;
+  }
+  static const int *flags(const bool *r, int *o)
+  {
+    if (!r) return nullptr;
+    for (int d = 0; d < 3; d++) o[d] = r[d] ? 1 : 0;
+    return o;
+  }
+  npme_handle *_h = nullptr;
+  size_t _nT = 0, _nS = 0;
+  npme_kfunc_real _cr = {};
+  npme_kfunc_complex _cc = {};
+};
+
 }  // namespace NPME_Library
 
 

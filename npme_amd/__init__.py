@@ -53,11 +53,25 @@ def lib():
         L.npme_get_params.argtypes = [p, p, p, p, p, p, p]
         L.npme_print.argtypes = [p, C.c_char_p, s]
         L.npme_set_profiling.argtypes = [p, i]
+        L.npme_set_calcv_graph.argtypes = [p, i]
         L.npme_count_pairs.argtypes = [p, p]
         L.npme_count_pairs_noalloc.argtypes = [p, p]
         L.npme_create_genfunc_real.argtypes = [C.POINTER(p), p, s, p, p, p, p]
         L.npme_create_genfunc_complex.argtypes = [C.POINTER(p), p, s, p, p, p, p]
         L.npme_get_table_error.argtypes = [p, p]
+        L.npme_create_boxpair.argtypes = [C.POINTER(p), p, s, p, s, p]
+        L.npme_create_boxpair_genfunc_real.argtypes = [C.POINTER(p), p, s, p, s, p, p]
+        L.npme_create_boxpair_genfunc_complex.argtypes = [C.POINTER(p), p, s, p, s, p, p]
+        L.npme_boxpair_reset_coord.argtypes = [p, s, p, s, p]
+        L.npme_boxpair_calcv_real.argtypes = [p, p, s, s, p, p]
+        L.npme_boxpair_calcv_complex.argtypes = [p, p, s, s, p, p]
+        L.npme_boxpair_separation.argtypes = [p, p, p]
+        L.npme_recsumgrid_fulltrans_func.argtypes = [p, C.c_long, d, d, C.c_long, d, d, C.c_long, d,
+                                                     d, d, d, d, p, p]
+        L.npme_recsumgrid_fulltrans_func_user.argtypes = [p, C.c_long, d, d, C.c_long, d, d,
+                                                          C.c_long, d, d, d, d, d, p, p, p]
+        L.npme_rstp_calctheta_fullF_reversesign.argtypes = [p, p, p, C.c_long, C.c_long, C.c_long,
+                                                            p, p, p, i, i, i]
         L.npme_get_stage_times.argtypes = [p, p, i]
         L.npme_get_indices.argtypes = [p, s, p, p, p, p]
         L.npme_debug_fail_alloc.argtypes = [C.c_long]
@@ -68,6 +82,7 @@ def lib():
         L.npme_server_clock.argtypes = [p]
         L.npme_server_clock.restype = d
         L.npme_server_destroy.argtypes = [p]
+        L.npme_server_set_batching.argtypes = [p, i, s]
         L.npme_host_alloc.argtypes = [s, C.POINTER(p)]
         L.npme_host_free.argtypes = [p]
         pp = C.POINTER(p)
@@ -93,7 +108,8 @@ class _RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int), ("tArrival", C.c_double), ("tSubmit", C.c_double),
                 ("tStart", C.c_double), ("tEnd", C.c_double), ("N", C.c_long * 3),
                 ("deviceGB", C.c_double), ("msKeywords", C.c_double), ("msSetup", C.c_double),
-                ("msCopyIn", C.c_double), ("msCalc", C.c_double), ("msCopyOut", C.c_double)]
+                ("msCopyIn", C.c_double), ("msCalc", C.c_double), ("msCopyOut", C.c_double),
+                ("reservedGB", C.c_double), ("batch", C.c_int)]
 
 
 def npme_last_error():
@@ -318,6 +334,10 @@ class _InterfaceBase:
     def set_profiling(self, on=True):
         lib().npme_set_profiling(self._h, int(on))
 
+    def set_calcv_graph(self, on):
+        """Replay a captured graph for repeated CalcV calls (default on)."""
+        _check(lib().npme_set_calcv_graph(self._h, 1 if on else 0), "set_calcv_graph")
+
     def stage_times(self):
         ms = np.zeros(16)
         k = lib().npme_get_stage_times(self._h, _ptr(ms), 16)
@@ -454,6 +474,131 @@ class NPME_InterfaceComplex_GenFunc(_GenFuncBase):
     _funcType = ("Helmholtz",)
 
 
+# ------------------------------------------------------------ two-box grids
+def _user_kernel_ptrs(func):
+    """(real, complex) C callback pointers of an NPME_KfuncReal /
+    NPME_KfuncComplex"""
+    c = C.cast(C.pointer(func._c()), C.c_void_p)
+    return (None, c) if isinstance(func, NPME_KfuncComplex) else (c, None)
+
+
+def _raise_calc_error(func, rc, destroy=None):
+    if func is not None and getattr(func, "_calc_error", None) is not None:
+        err, func._calc_error = func._calc_error, None
+        if rc == NPME_OK and destroy is not None:
+            destroy()
+        raise err
+
+
+def NPME_RecSumGrid_FullTrans_Func(N, a, dl, X, func, R0):
+    """f[N1][N2][N3] (complex): func(|r + R0|) on the full grid of the box
+    [-X - del, X + del) per axis with the 8-image top-hat extension
+    (src/RecSumGrid.cpp:663-794).  func is an NPME_KeywordInput naming a
+    built-in kernel (funcType ..., sampled on the GPU in FP64) or an
+    NPME_KfuncReal / NPME_KfuncComplex (sampled on the host)."""
+    N = [int(x) for x in N]
+    f = np.zeros(N, np.complex128)
+    args = []
+    for d in range(3):
+        args += [N[d], float(a[d]), float(dl[d])]
+    args += [float(X[0]), float(X[1]), float(X[2])]
+    R = np.ascontiguousarray(R0, dtype=np.float64)
+    if isinstance(func, NPME_KeywordInput):
+        rc = lib().npme_recsumgrid_fulltrans_func(_ptr(f), *args, func._h, _ptr(R))
+    else:
+        func._calc_error = None
+        rp, cp = _user_kernel_ptrs(func)
+        rc = lib().npme_recsumgrid_fulltrans_func_user(_ptr(f), *args, rp, cp, _ptr(R))
+        _raise_calc_error(func, rc)
+    _check(rc, "NPME_RecSumGrid_FullTrans_Func")
+    return f
+
+
+def NPME_RSTP_CalcTheta_FullF_ReverseSign(F, Qr, lamda, reverseSign=(0, 0, 0)):
+    """Compact theta [N1/2][N2/2][N3/2] (src/RecSumV.cpp:943-1009) on the
+    GPU: F the full forward DFT [N1][N2][N3] of the kernel grid, Qr the
+    compact charge grid (DFT order), lamda = (|lambda_1|^2, |lambda_2|^2,
+    |lambda_3|^2); reverseSign uses F as the grid of R0 with those
+    components negated."""
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    Q = np.ascontiguousarray(Qr, dtype=np.complex128)
+    N1, N2, N3 = F.shape
+    if Q.shape != (N1 // 2, N2 // 2, N3 // 2):
+        raise ValueError("Qr must be [N1/2][N2/2][N3/2]")
+    lam = [np.ascontiguousarray(l, dtype=np.float64) for l in lamda]
+    for l, n in zip(lam, F.shape):
+        if l.shape != (n,):
+            raise ValueError("lamda arrays must have N1, N2, N3 values")
+    T = np.zeros_like(Q)
+    _check(lib().npme_rstp_calctheta_fullF_reversesign(
+        _ptr(T), _ptr(F), _ptr(Q), N1, N2, N3, _ptr(lam[0]), _ptr(lam[1]), _ptr(lam[2]),
+        int(bool(reverseSign[0])), int(bool(reverseSign[1])), int(bool(reverseSign[2]))),
+        "NPME_RSTP_CalcTheta_FullF_ReverseSign")
+    return T
+
+
+class NPME_BoxPair:
+    """V at the targets of one box from the sources of another, entirely on
+    the translated kernel grid (no reference driver; the reference has the
+    two functions above).  keyword: funcType etc. as for SetUp plus N1-N3;
+    func: None (the keywords' built-in kernel) or a user NPME_KfuncReal /
+    NPME_KfuncComplex, borrowed for the object's life."""
+
+    def __init__(self, keyword, coordTarget, coordSource, func=None):
+        self._L = lib()
+        self._h = None
+        cT, cS = _coord(coordTarget), _coord(coordSource)
+        self.nTarget, self.nSource = len(cT), len(cS)
+        h = C.c_void_p()
+        if func is None:
+            self._complex = keyword.GetValue("funcType") == "Helmholtz"
+            rc = lib().npme_create_boxpair(C.byref(h), keyword._h, self.nTarget, _ptr(cT),
+                                           self.nSource, _ptr(cS))
+        else:
+            self._complex = isinstance(func, NPME_KfuncComplex)
+            func._calc_error = None
+            create = (lib().npme_create_boxpair_genfunc_complex if self._complex
+                      else lib().npme_create_boxpair_genfunc_real)
+            rc = create(C.byref(h), keyword._h, self.nTarget, _ptr(cT), self.nSource, _ptr(cS),
+                        C.cast(C.pointer(func._c()), C.c_void_p))
+            _raise_calc_error(func, rc, lambda: lib().npme_destroy(h))
+            self._kfunc = func
+        _check(rc, "npme_create_boxpair")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.npme_destroy(self._h)
+            self._h = None
+
+    def calcv(self, chargeSource, reverseSign=(0, 0, 0)):
+        dt = np.complex128 if self._complex else np.float64
+        q = np.ascontiguousarray(chargeSource, dtype=dt)
+        if q.shape != (self.nSource,):
+            raise ValueError("chargeSource must have nSource values")
+        V = np.zeros((self.nTarget, 4), dt)
+        rv = (C.c_int * 3)(*[int(bool(r)) for r in reverseSign])
+        f = lib().npme_boxpair_calcv_complex if self._complex else lib().npme_boxpair_calcv_real
+        _check(f(self._h, _ptr(V), self.nTarget, self.nSource, _ptr(q), rv), "boxpair_calcv")
+        return V
+
+    def reset_coord(self, coordTarget, coordSource):
+        cT, cS = _coord(coordTarget), _coord(coordSource)
+        _check(lib().npme_boxpair_reset_coord(self._h, len(cT), _ptr(cT), len(cS), _ptr(cS)),
+               "boxpair_reset_coord")
+        self.nTarget, self.nSource = len(cT), len(cS)
+
+    def separation(self):
+        """(R0, rMin): R0 = target centre - source centre, f(r) = func(|r + R0|)"""
+        R0 = np.zeros(3)
+        r = C.c_double()
+        _check(lib().npme_boxpair_separation(self._h, _ptr(R0), C.byref(r)), "boxpair_separation")
+        return R0, r.value
+
+    def set_calcv_graph(self, on):
+        _check(lib().npme_set_calcv_graph(self._h, int(bool(on))), "set_calcv_graph")
+
+
 class NPME_InterfaceComplex_Helmholtz_DM(_InterfaceBase):
     _complex = True
     _funcType = ("Helmholtz",)
@@ -483,11 +628,20 @@ class NPME_Server:
     when a slot and its device-memory estimate are free.  No reference
     counterpart."""
 
-    def __init__(self, nSlots=4, memGB=0.0):
+    def __init__(self, nSlots=4, memGB=0.0, maxBatch=None, batchMaxN=None):
         self._L = lib()  # kept for __del__, which may run during interpreter shutdown
         self._s = C.c_void_p()
         _check(lib().npme_server_create(C.byref(self._s), int(nSlots), float(memGB)), "server_create")
         self._live = {}
+        if maxBatch is not None or batchMaxN is not None:
+            self.set_batching(8 if maxBatch is None else maxBatch,
+                              32768 if batchMaxN is None else batchMaxN)
+
+    def set_batching(self, maxBatch, batchMaxN=32768):
+        """Small queued requests (at most batchMaxN charges) run in batches of
+        up to maxBatch with one graph launch for their CalcVs; 1 turns it off."""
+        _check(lib().npme_server_set_batching(self._s, int(maxBatch), int(batchMaxN)),
+               "server_set_batching")
 
     def clock(self):
         """Seconds since the server was created (the clock of tArrival)."""
@@ -526,6 +680,7 @@ class NPME_Server:
             raise NPMEError(rc, "request %d" % ticket)
         return V, {"tArrival": info.tArrival, "tSubmit": info.tSubmit, "tStart": info.tStart,
                    "tEnd": info.tEnd, "grid": tuple(info.N), "deviceGB": info.deviceGB,
+                   "reservedGB": info.reservedGB, "batch": info.batch,
                    "ms": {"keywords": info.msKeywords, "setup": info.msSetup,
                           "copy_in": info.msCopyIn, "calcv": info.msCalc,
                           "copy_out": info.msCopyOut}}

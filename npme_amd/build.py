@@ -15,7 +15,7 @@ LIB = os.path.join(HERE, "libnpme_amd.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 SOURCES = ["geometry.hip", "fft.hip", "particles.hip", "setup.cpp", "host_math.cpp", "capi.cpp",
-           "devmem.cpp", "server.cpp", "fileio.cpp", "noalloc.hip", "genfunc.cpp"]
+           "devmem.cpp", "server.cpp", "fileio.cpp", "noalloc.hip", "genfunc.cpp", "twobox.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("NPME_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,

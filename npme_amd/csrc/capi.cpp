@@ -9,13 +9,6 @@
 
 #include "npme_internal.h"
 
-struct npme_keywords {
-  npme::Keywords kw;
-};
-struct npme_handle {
-  npme::Handle h;
-};
-
 namespace npme {
 static thread_local std::string g_err;
 void set_error(const std::string &m) { g_err = m; }
@@ -332,6 +325,7 @@ int npme_reset_coord(npme_handle *h, size_t nCharge, const double *coord)
 {
   return guarded([&] {
     if (!h || !coord) throw Error{NPME_ERR_ARG, "npme_reset_coord: NULL argument"};
+    if (h->h.trans) throw Error{NPME_ERR_ARG, "a box-pair handle is reset with npme_boxpair_reset_coord"};
     std::lock_guard<std::mutex> lk(h->h.mtx);
     Keywords kw = h->h.kw;
     handle_setup(h->h, kw, nCharge, coord);
@@ -349,6 +343,7 @@ void npme_destroy(npme_handle *h)
 static void check_call(npme_handle *h, size_t nCharge, bool wantComplex)
 {
   if (!h) throw Error{NPME_ERR_ARG, "NULL handle"};
+  if (h->h.trans) throw Error{NPME_ERR_ARG, "a box-pair handle is evaluated with npme_boxpair_calcv_*"};
   if (!h->h.ready) throw Error{NPME_ERR_ARG, "no valid set-up: the last SetUp or ResetCoord failed"};
   if (nCharge != h->h.n) {
     std::ostringstream os;
@@ -392,6 +387,7 @@ int npme_calcv_device(npme_handle *h, double *dV, const double *dCharge)
 {
   return guarded([&] {
     if (!h || !dV || !dCharge) throw Error{NPME_ERR_ARG, "npme_calcv_device: NULL argument"};
+    if (h->h.trans) throw Error{NPME_ERR_ARG, "a box-pair handle is evaluated with npme_boxpair_calcv_*"};
     std::lock_guard<std::mutex> lk(h->h.mtx);
     handle_calcv(h->h, dCharge, dV);
   });
@@ -449,7 +445,9 @@ int npme_calcv_exact_subset(npme_handle *h, double *V, size_t nTarget, const lon
     NPME_HIP_CHECK(hipMemcpyAsync(H.d_qin, charge, qn * sizeof(double), hipMemcpyHostToDevice, H.stream));
     NPME_HIP_CHECK(hipMemcpyAsync(d_idx.p, idx, nTarget * sizeof(long), hipMemcpyHostToDevice, H.stream));
     handle_exact(H, H.d_qin, nTarget, (const long *)d_idx.p, (double *)d_out.p);
-    NPME_HIP_CHECK(hipMemcpy(V, d_out.p, nTarget * vw * sizeof(double), hipMemcpyDeviceToHost));
+    NPME_HIP_CHECK(hipMemcpyAsync(V, d_out.p, nTarget * vw * sizeof(double), hipMemcpyDeviceToHost,
+                                  H.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(H.stream));
   });
 }
 
@@ -567,6 +565,15 @@ int npme_get_indices(npme_handle *h, size_t nCharge, unsigned long long *cluster
 }
 
 void npme_debug_fail_alloc(long k) { dev_fail_after(k); }
+
+int npme_set_calcv_graph(npme_handle *h, int on)
+{
+  return guarded([&] {
+    if (!h) throw Error{NPME_ERR_ARG, "NULL handle"};
+    std::lock_guard<std::mutex> lk(h->h.mtx);
+    h->h.useGraph = on != 0;
+  });
+}
 
 int npme_set_profiling(npme_handle *h, int on)
 {

@@ -455,14 +455,53 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
   }
 }
 
+// G of pass C.  GM 0: real G at (|k1|, |k2|, k3); GM 1: complex G at
+// (|k1|, |k2|, k3 or |k3|) -- the radial kernels, whose grid is even; GM 2:
+// complex G at (k1, k2, k3) over the full N1 x N2 x G3 grid (translated
+// kernels), where gfl selects k -> -k per axis (bits 1, 2, 4) and the
+// conjugate (bit 8).  Layout G[a2][a3 / 8][a1][a3 % 8], a1 over A1 values.
+struct GIndex {
+  long base;
+  int A1;
+};
+__device__ __forceinline__ int g_neg(int k, int N) { return k ? N - k : 0; }
+template <int GM>
+__device__ __forceinline__ GIndex g_index(int N, int N2, int Kv, int N3fold, int gfl, bool valid,
+                                          int colg, int C3)
+{
+  const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
+  int a2, a3, G3, A1;
+  if constexpr (GM == 2) {
+    a2 = (gfl & 2) ? g_neg(k2, N2) : k2;
+    a3 = (gfl & 4) ? g_neg(k3, N3fold) : k3;
+    G3 = Kv;
+    A1 = N;
+  } else {
+    a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
+    a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
+    G3 = N3fold > 0 ? (N3fold / 2 + 1) : Kv;
+    A1 = N / 2 + 1;
+  }
+  GIndex r;
+  r.base = (((long)a2 * ((G3 + 7) >> 3) + (a3 >> 3)) * A1) * 8 + (a3 & 7);
+  r.A1 = A1;
+  return r;
+}
+template <int GM> __device__ __forceinline__ int g_a1(int n, int N, int gfl)
+{
+  if constexpr (GM == 2) return (gfl & 1) ? g_neg(n, N) : n;
+  else return (n <= N / 2) ? n : N - n;
+}
+
 // pass C: forward along axis 1, X <- X G, backward, keep the first Lin
-// outputs.  Columns c = k2 * C3 + k3; G index (|k1|, |k2|, k3 or |k3|).
-template <typename Real, bool CG, int RC>
-__global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int pos, int N2, int G2,
+// outputs.  Columns c = k2 * C3 + k3.
+template <typename Real, int GM, int RC>
+__global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int pos, int N2, int gfl,
                                                  int C3, int Kv, int N3fold, PlanDev p,
                                                  const typename Cx<Real>::T *tw, const void *Gv,
                                                  typename Cx<Real>::T *__restrict__ data, int lgB)
 {
+  constexpr bool CG = GM > 0;
   using T = typename Cx<Real>::T;
   extern __shared__ char smem_raw[];
   T *lds = reinterpret_cast<T *>(smem_raw);
@@ -478,14 +517,7 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   // multiply by G and conjugate (for the backward transform)
-  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : Kv;
-  const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
-  const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
-  const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
-  // G[a2][a3 / 8][a1][a3 % 8] with a1 in [0, N/2] (k_make_G)
-  const long gbase = (((long)a2 * ((G3 + 7) >> 3) + (a3 >> 3)) * (N / 2 + 1)) * 8 + (a3 & 7);
-  const int gstep = 8;
-  (void)G2;
+  const GIndex gi = g_index<GM>(N, N2, Kv, N3fold, gfl, valid, colg, C3);
   constexpr int E = emax<Real>();
   using GT = typename std::conditional<CG, T, Real>::type;
   GT gv[E];
@@ -493,8 +525,9 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   for (int u = 0; u < E; u++) {  // the G loads of the thread, in flight together
     const int n = L.tj + u * L.tpc;
     if (n < N) {
-      const int a1 = (n <= N / 2) ? n : N - n;
-      gv[u] = reinterpret_cast<const GT *>(Gv)[(long)a1 * gstep + gbase];
+      gv[u] = reinterpret_cast<const GT *>(Gv)[(long)g_a1<GM>(n, N, gfl) * 8 + gi.base];
+      if constexpr (GM == 2)
+        if (gfl & 8) gv[u] = cconj(gv[u]);
     }
   }
 #pragma unroll
@@ -681,12 +714,13 @@ __global__ __launch_bounds__(NT) void k_fft8_cols(CGeom g, const typename Cx<Rea
 
 // pass C: forward along axis 1 (the first N/2 rows non-zero), times G,
 // backward, the first N/2 rows kept; columns c = k2 * C3 + k3, Kv valid k3
-template <typename Real, int L, bool CG>
-__global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, int G2, int C3, int Kv,
+template <typename Real, int L, int GM>
+__global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, int gfl, int C3, int Kv,
                                                   int N3fold, const typename Cx<Real>::T *tw,
                                                   const void *Gv,
                                                   typename Cx<Real>::T *__restrict__ data)
 {
+  constexpr bool CG = GM > 0;
   using T = typename Cx<Real>::T;
   using P = P8<L>;
   constexpr int N = P::N;
@@ -698,14 +732,7 @@ __global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, in
   T *col = data + (valid ? colg : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, P::B * N, N, tw);
   // the G values of the thread's rows, fetched up front
-  const int G3 = N3fold > 0 ? (N3fold / 2 + 1) : Kv;
-  const int k2 = valid ? colg / C3 : 0, k3 = valid ? colg - k2 * C3 : 0;
-  const int a2 = (k2 <= N2 / 2) ? k2 : N2 - k2;
-  const int a3 = (N3fold > 0 && k3 > N3fold / 2) ? N3fold - k3 : k3;
-  // G[a2][a3 / 8][a1][a3 % 8] with a1 in [0, N/2] (k_make_G)
-  const long gbase = (((long)a2 * ((G3 + 7) >> 3) + (a3 >> 3)) * (N / 2 + 1)) * 8 + (a3 & 7),
-             gstep = 8;
-  (void)G2;
+  const GIndex gi = g_index<GM>(N, N2, Kv, N3fold, gfl, valid, colg, C3);
   using GT = typename std::conditional<CG, T, Real>::type;
   GT gv[P::BPT][8];
   T v[P::BPT][8];
@@ -715,8 +742,9 @@ __global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, in
 #pragma unroll
     for (int r = 0; r < 8; r++) {
       const int n = j + r * P::NB;
-      const int a1 = (n <= N / 2) ? n : N - n;
-      gv[u][r] = reinterpret_cast<const GT *>(Gv)[a1 * gstep + gbase];
+      gv[u][r] = reinterpret_cast<const GT *>(Gv)[(long)g_a1<GM>(n, N, gfl) * 8 + gi.base];
+      if constexpr (GM == 2)
+        if (gfl & 8) gv[u][r] = cconj(gv[u][r]);
       v[u][r] = (r < 4 && valid) ? col[(long)n * pos] : mk<T>(0, 0);
     }
   }
@@ -995,56 +1023,69 @@ void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
 
 void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
                        int N2, int G2, long C3, long Kv, const FFTPlan1D &plan, const void *tw,
-                       const void *G, void *data, hipStream_t s, bool pow8)
+                       const void *G, void *data, hipStream_t s, bool pow8, bool fullG, int gflags)
 {
-  // C3: entries per axis-2 row in the data (padded), Kv of them valid; for
-  // complex kernels the axis-3 index is folded (|k3|) against N3 = Kv.
+  // C3: entries per axis-2 row in the data (padded), Kv of them valid.
+  // complexG: complex charges and kernel, the axis-3 index runs over
+  // N3 = Kv (folded to |k3| for a radial kernel's G); else the R2C data of
+  // real charges, k3 in [0, Kv).  fullG: G is complex over the full grid
+  // (GM 2), for either data; the k3 reversal (gflags bit 4) needs complex
+  // data.
+  (void)G2;
   check_int(nCols * (long)Lin, "FFT pass");
+  const int gm = fullG ? 2 : complexG ? 1 : 0;
+  if (fullG && !complexG && (gflags & 4))
+    throw Error{NPME_ERR_UNSUPPORTED, "axis-3 reversal of G needs complex data"};
   const int lgB = lg_batch(N, fp64, false);
   const dim3 grid((unsigned)((nCols + (1 << lgB) - 1) >> lgB));
   const int rc = radix_class(plan);
   const size_t sh = pass_lds(N, lgB, fp64, false);
+  // the axis-3 length of complex data (0 for the R2C data of real kernels)
   const int N3fold = complexG ? (int)Kv : 0;
   const int L8 = pow8 ? pow8_level(plan) : 0;
   if (L8 > 0 && 2 * Lin == N) {
-#define NPME_C8(R, LV, CGV)                                                                    \
-  hipLaunchKernelGGL((k_fft8_conv<R, LV, CGV>), grid, dim3(NT), sh, s, (int)nCols,             \
-                     (int)pos_stride, N2, G2, (int)C3, (int)Kv, N3fold,                         \
+#define NPME_C8(R, LV, GMV)                                                                    \
+  hipLaunchKernelGGL((k_fft8_conv<R, LV, GMV>), grid, dim3(NT), sh, s, (int)nCols,             \
+                     (int)pos_stride, N2, gflags, (int)C3, (int)Kv, N3fold,                     \
                      (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data)
-#define NPME_C8L(R, CGV)                                                                       \
+#define NPME_C8L(R, GMV)                                                                       \
   do {                                                                                         \
-    if (L8 == 2) NPME_C8(R, 2, CGV);                                                           \
-    else NPME_C8(R, 3, CGV);                                                                   \
+    if (L8 == 2) NPME_C8(R, 2, GMV);                                                           \
+    else NPME_C8(R, 3, GMV);                                                                   \
   } while (0)
-    if (fp64) {
-      if (complexG) NPME_C8L(double, true);
-      else NPME_C8L(double, false);
-    } else {
-      if (complexG) NPME_C8L(float, true);
-      else NPME_C8L(float, false);
-    }
+#define NPME_C8G(R)                                                                            \
+  do {                                                                                         \
+    if (gm == 2) NPME_C8L(R, 2);                                                               \
+    else if (gm == 1) NPME_C8L(R, 1);                                                          \
+    else NPME_C8L(R, 0);                                                                       \
+  } while (0)
+    if (fp64) NPME_C8G(double);
+    else NPME_C8G(float);
 #undef NPME_C8
 #undef NPME_C8L
+#undef NPME_C8G
     NPME_HIP_CHECK(hipGetLastError());
     return;
   }
-#define NPME_CONV(R, CGV, RCV)                                                                 \
-  hipLaunchKernelGGL((k_fft_conv<R, CGV, RCV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,     \
-                     (int)pos_stride, N2, G2, (int)C3, (int)Kv, N3fold, to_dev(plan),           \
+#define NPME_CONV(R, GMV, RCV)                                                                 \
+  hipLaunchKernelGGL((k_fft_conv<R, GMV, RCV>), grid, dim3(NT), sh, s, N, Lin, (int)nCols,     \
+                     (int)pos_stride, N2, gflags, (int)C3, (int)Kv, N3fold, to_dev(plan),       \
                      (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data, lgB)
-#define NPME_CONV2(R, CGV)                                                                     \
+#define NPME_CONV2(R, GMV)                                                                     \
   do {                                                                                         \
-    if (rc == RC_R8) NPME_CONV(R, CGV, RC_R8);                                                 \
-    else if (rc == RC_POW2) NPME_CONV(R, CGV, RC_POW2);                                        \
-    else NPME_CONV(R, CGV, RC_ALL);                                                            \
+    if (rc == RC_R8) NPME_CONV(R, GMV, RC_R8);                                                 \
+    else if (rc == RC_POW2) NPME_CONV(R, GMV, RC_POW2);                                        \
+    else NPME_CONV(R, GMV, RC_ALL);                                                            \
   } while (0)
-  if (fp64) {
-    if (complexG) NPME_CONV2(double, true);
-    else NPME_CONV2(double, false);
-  } else {
-    if (complexG) NPME_CONV2(float, true);
-    else NPME_CONV2(float, false);
-  }
+#define NPME_CONVG(R)                                                                          \
+  do {                                                                                         \
+    if (gm == 2) NPME_CONV2(R, 2);                                                             \
+    else if (gm == 1) NPME_CONV2(R, 1);                                                        \
+    else NPME_CONV2(R, 0);                                                                     \
+  } while (0)
+  if (fp64) NPME_CONVG(double);
+  else NPME_CONVG(float);
+#undef NPME_CONVG
 #undef NPME_CONV
 #undef NPME_CONV2
   NPME_HIP_CHECK(hipGetLastError());

@@ -272,18 +272,18 @@ void user_build_table(const UserKernels &u, double Rdir, std::vector<double> &co
   }
 }
 
-// f_LR on the compact real-space kernel grid: the same images and weights as
-// the built-in kernels' device sampling (per axis d the table holds x1, x2,
-// T1, T2 of Mf[d] points each); F is [Mf0][Mf1][Mf2] doubles (real) or
-// (re, im) pairs (complex)
-void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<double> &tab,
-                     std::vector<double> &F)
+// f_0 of the user's func (which = 0) or funcLR (which = 1) on a grid given by
+// per-axis tables [x1 | x2 | T1 | T2] of n[d] points: the same images and
+// weights as the built-in kernels' device sampling; F is [n0][n1][n2]
+// doubles (real) or (re, im) pairs (a complex kernel, or cxOut)
+void user_fgrid_host_n(const UserKernels &u, int which, const long n[3], const std::vector<double> &tab,
+                       bool cxOut, std::vector<double> &F)
 {
-  const double *t0 = tab.data(), *t1 = t0 + 4 * Mf[0], *t2 = t1 + 4 * Mf[1];
-  const int cw = u.complex ? 2 : 1;
-  F.assign((size_t)Mf[0] * Mf[1] * Mf[2] * cw, 0.0);
-  const long rows = Mf[0] * Mf[1];
-  const size_t maxPts = 8 * (size_t)Mf[2];
+  const double *t0 = tab.data(), *t1 = t0 + 4 * n[0], *t2 = t1 + 4 * n[1];
+  const int cw = (u.complex || cxOut) ? 2 : 1;
+  F.assign((size_t)n[0] * n[1] * n[2] * cw, 0.0);
+  const long rows = n[0] * n[1];
+  const size_t maxPts = 8 * (size_t)n[2];
   struct Buf {
     std::vector<double> x, y, z, fr, fi, w;
     std::vector<long> at;
@@ -301,12 +301,12 @@ void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<d
   }
   parallel_for(rows, [&](long row, int th) {
     Buf &b = bufs[th];
-    const long i1 = row / Mf[1], i2 = row % Mf[1];
-    const double X[2] = {t0[i1], t0[Mf[0] + i1]}, TX[2] = {t0[2 * Mf[0] + i1], t0[3 * Mf[0] + i1]};
-    const double Y[2] = {t1[i2], t1[Mf[1] + i2]}, TY[2] = {t1[2 * Mf[1] + i2], t1[3 * Mf[1] + i2]};
+    const long i1 = row / n[1], i2 = row % n[1];
+    const double X[2] = {t0[i1], t0[n[0] + i1]}, TX[2] = {t0[2 * n[0] + i1], t0[3 * n[0] + i1]};
+    const double Y[2] = {t1[i2], t1[n[1] + i2]}, TY[2] = {t1[2 * n[1] + i2], t1[3 * n[1] + i2]};
     size_t np = 0;
-    for (long i3 = 0; i3 < Mf[2]; i3++) {
-      const double Z[2] = {t2[i3], t2[Mf[2] + i3]}, TZ[2] = {t2[2 * Mf[2] + i3], t2[3 * Mf[2] + i3]};
+    for (long i3 = 0; i3 < n[2]; i3++) {
+      const double Z[2] = {t2[i3], t2[n[2] + i3]}, TZ[2] = {t2[2 * n[2] + i3], t2[3 * n[2] + i3]};
       for (int ax = 0; ax < 2; ax++)
         for (int ay = 0; ay < 2; ay++)
           for (int az = 0; az < 2; az++) {
@@ -324,15 +324,25 @@ void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<d
     }
     if (np == 0) return;
     const std::vector<double> x(b.x.begin(), b.x.begin() + np);
-    eval_value(u, 1, np, x.data(), b.y.data(), b.z.data(), b.fr.data(), b.fi.data());
-    double *Frow = F.data() + (size_t)row * Mf[2] * cw;
+    eval_value(u, which, np, x.data(), b.y.data(), b.z.data(), b.fr.data(), b.fi.data());
+    double *Frow = F.data() + (size_t)row * n[2] * cw;
     for (size_t p = 0; p < np; p++) {
       Frow[b.at[p] * cw] += b.w[p] * b.fr[p];
       if (cw == 2) Frow[b.at[p] * 2 + 1] += b.w[p] * b.fi[p];
     }
   });
   for (double v : F)
-    if (!std::isfinite(v)) throw Error{NPME_ERR_UNSUPPORTED, "funcLR is not finite on the kernel grid"};
+    if (!std::isfinite(v))
+      throw Error{NPME_ERR_UNSUPPORTED,
+                  std::string(which == 0 ? "func" : "funcLR") + " is not finite on the kernel grid"};
+}
+
+// f_LR on the compact real-space kernel grid (per axis d the table holds x1,
+// x2, T1, T2 of Mf[d] points each)
+void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<double> &tab,
+                     std::vector<double> &F)
+{
+  user_fgrid_host_n(u, 1, Mf, tab, false, F);
 }
 
 // V[t] = sum over j != i of f(r_i - r_j) q_j and its gradient for the

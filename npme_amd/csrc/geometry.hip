@@ -130,7 +130,9 @@ __global__ __launch_bounds__(256) void k_charge_setup(size_t n, const int *__res
   int cnt = 1;
   for (int d = 0; d < 3; d++) {
     const double x = coord[3 * (size_t)i + d];
-    loc[d] = (x + (-g.mn[d])) - (double)(gcv[d] / (int)g.nP) * g.csA;
+    // the list walk reads positions in one frame (the cells' low corner),
+    // the cell-row sum relative to the charge's cluster corner
+    loc[d] = g.globalFrame ? x + (-g.mn[d]) : (x + (-g.mn[d])) - (double)(gcv[d] / (int)g.nP) * g.csA;
     const double w = g.NoverL[d] * (x - g.R[d]);
     const long m0 = (w < 0) ? (long)(-w) + 1 : (long)(-w);
     if (m0 < -g.N[d] / 4 || m0 + g.Bn - 1 > g.N[d] / 4 - 1) {
@@ -153,12 +155,13 @@ __global__ __launch_bounds__(256) void k_charge_setup(size_t n, const int *__res
     float res[3];
     for (int d = 0; d < 3; d++) {
       const double u = loc[d] * g.fxScale;
-      const long r = lround(u);
-      if (r < -(1L << 20) || r > (1L << g.fxShift) + (1L << 20)) {
+      const long r = g.globalFrame ? (fabs(u) < 4.0e18 ? llround(u) : 0) : lround(u);
+      if (g.globalFrame ? !(fabs(u) < 4.0e18) : (r < -(1L << 20) || r > (1L << g.fxShift) + (1L << 20))) {
         err[0] = GEO_ERR_FIXED;
         v[d] = 0;
       } else {
-        v[d] = (int)r;
+        // the list walk's frame wraps modulo 2^32
+        v[d] = (int)(unsigned)(unsigned long)r;
       }
       res[d] = (float)(u - (double)r);
     }
@@ -676,4 +679,30 @@ double stencil_candidates_device(const int nb[3], int nNeigh, const int *d_cellS
   return v;
 }
 
+}  // namespace npme
+
+namespace npme {
+__global__ __launch_bounds__(256) void k_sentinel_near(size_t n, const int4 *__restrict__ src, int4 v,
+                                                      int thr, int *__restrict__ hit)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int4 p = src[s];
+  const int dx = (int)((unsigned)p.x - (unsigned)v.x), dy = (int)((unsigned)p.y - (unsigned)v.y),
+            dz = (int)((unsigned)p.z - (unsigned)v.z);
+  if (abs(dx) < thr && abs(dy) < thr && abs(dz) < thr) hit[0] = 1;
+}
+
+bool sentinel_near_any(size_t n, const void *d_src, int4 v, int thr, hipStream_t s)
+{
+  DevBuf hit(sizeof(int));
+  int h = 0;
+  NPME_HIP_CHECK(hipMemsetAsync(hit.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_sentinel_near, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     (const int4 *)d_src, v, thr, (int *)hit.p);
+  NPME_HIP_CHECK(hipGetLastError());
+  NPME_HIP_CHECK(hipMemcpyAsync(&h, hit.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  NPME_HIP_CHECK(hipStreamSynchronize(s));
+  return h != 0;
+}
 }  // namespace npme

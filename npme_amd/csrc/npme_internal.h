@@ -225,13 +225,17 @@ struct GridKey {
   bool fp64 = false, complexK = false;
   int kind = -1, Nder = 0;
   double alpha = 0, k0r = 0, k0i = 0, Rdir = 0, beta = 0;
+  bool trans = false;           // translated full kernel (box pair)
+  double D[3] = {0, 0, 0};      // its translation
   bool operator==(const GridKey &o) const
   {
     for (int d = 0; d < 3; d++)
-      if (N[d] != o.N[d] || X[d] != o.X[d] || del[d] != o.del[d] || ta[d] != o.ta[d]) return false;
+      if (N[d] != o.N[d] || X[d] != o.X[d] || del[d] != o.del[d] || ta[d] != o.ta[d] ||
+          D[d] != o.D[d])
+        return false;
     return Bn == o.Bn && fp64 == o.fp64 && complexK == o.complexK && kind == o.kind &&
            Nder == o.Nder && alpha == o.alpha && k0r == o.k0r && k0i == o.k0i && Rdir == o.Rdir &&
-           beta == o.beta;
+           beta == o.beta && trans == o.trans;
   }
 };
 
@@ -337,6 +341,25 @@ struct Handle {
   // cells and sort, neighbour list, per-charge arrays and tile lists, FFT
   // buffers and kernel grid
   double setupMs[6] = {0, 0, 0, 0, 0, 0};
+  // CalcV as a graph: the second CalcV call with the same charge and V
+  // pointers on one set-up captures the launches (the direct-sum branch and
+  // the reciprocal branch joined by an event) and later calls replay it.
+  // Any set-up or release of the device state drops the graph.
+  bool useGraph = true;
+  hipEvent_t evFork = nullptr, evJoin = nullptr;  // fork/join of the two streams
+  hipGraphExec_t calcvExec = nullptr;
+  const void *graphQ = nullptr, *graphV = nullptr;  // pointers of the last CalcV
+  int graphCalls = 0;  // CalcV calls in a row with those pointers
+  // box pair (two point sets, translated kernel): the charges are the
+  // targets followed by the sources, each relative to the centre of its own
+  // bounding box; the kernel grid holds the full kernel at r + transD with
+  // transD = source centre - target centre, G over the full grid; no direct
+  // sum and no self term.  transFlags: the k-reversal / conjugation bits of
+  // the convolution pass for the reversal transRev (bits x, y, z) of CalcV.
+  bool trans = false;
+  double transD[3] = {0, 0, 0};
+  size_t nTarget = 0;
+  int transRev = 0, transFlags = 0;
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> ev;
@@ -345,6 +368,10 @@ struct Handle {
 };
 
 void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord);
+// box pairs: the reversal (bits x, y, z) used by the next CalcV
+void handle_set_trans_rev(Handle &h, int rev);
+// exp(-2 pi i t / N), t = 0 .. N-1, as (re, im) pairs in FP64
+std::vector<double> twiddles(int N);
 // NPME_AddMissingKeywords: defaults, validation, RSEM grid (throws Error)
 void add_missing_keywords(Keywords &kw, size_t nCharge, bool isChargeReal, const double *coord,
                           const double *chargeReal, const double *chargeComplex,
@@ -366,7 +393,11 @@ struct GeoParams {
   int ts[3];               // spreading tile edge per axis (16 x 8 x 8 or 8^3)
   int nt[3];               // spreading tiles per axis
   int TI, nti[3];          // interpolation tiles
+  int globalFrame;         // positions relative to mn (neighbour-list walk), not to the cluster
 };
+// whether any of the n int4 fixed-point positions lies within thr units of
+// v on all three axes (wrapped int32 differences)
+bool sentinel_near_any(size_t n, const void *d_src, int4 v, int thr, hipStream_t s);
 void geometry_cells_device(const GeoParams &g, size_t n, const double *d_coord, int *d_perm,
                            int4 *d_cellOf, int *d_cellSE, hipStream_t s);
 void geometry_charges_device(const GeoParams &g, bool fp64, size_t n, const double *d_coord,
@@ -409,6 +440,11 @@ void handle_staged_bounds(Handle &h, double mn[3], double mx[3]);
 // destroys the handle's streams and timing events (after handle_free_device)
 void handle_destroy_streams(Handle &h);
 void handle_calcv(Handle &h, const double *d_q, double *d_Vout);
+// CalcV of m handles (charges in d_qin, V to d_V of each) as one captured
+// graph launched on sb (events: one start, one per handle)
+void handle_calcv_batch(Handle *const *hs, int m, hipStream_t sb, hipEvent_t evStart,
+                        const hipEvent_t *evDone);
+
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout);
 
 
@@ -432,10 +468,13 @@ void fft_pass_r2c_rows(bool fp64, int N, int Lin, long nRows, long inRowStride,
 void fft_pass_c2r_rows(bool fp64, int N, int Lout, long nRows, long inRowStride,
                        long outRowStride, const FFTPlan1D &plan, const void *tw, const void *in,
                        void *out, hipStream_t s);
-// middle pass: forward FFT along the axis, multiply by G, backward FFT
+// middle pass: forward FFT along the axis, multiply by G, backward FFT.
+// fullG: G is complex over the full N1 x N2 x Kv grid of a translated
+// kernel, gflags: k -> -k on axis 1, 2, 3 (bits 1, 2, 4), conjugate (bit 8)
 void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, long pos_stride,
                        int N2, int G2, long K3, long Kv, const FFTPlan1D &plan, const void *tw,
-                       const void *G, void *data, hipStream_t s, bool pow8 = true);
+                       const void *G, void *data, hipStream_t s, bool pow8 = true,
+                       bool fullG = false, int gflags = 0);
 // even-extension DFT of the compact kernel grid along one axis (set-up,
 // FP64): complex lines, and real lines ([.] doubles) two per transform
 void fft_pass_even(int N, long nCols, long nInner, long outer, long inner, long pos,
@@ -462,10 +501,11 @@ struct DirectParams {
   double fxInv;   // cluster edge / 2^fxShift
   const void *utab = nullptr;  // user kernels: short-range table, S pieces per octave
   int utS = 0;
+  // neighbour-list entries are source byte offsets: log2 of the source
+  // element size (5 for FP64 double4, 4 for the mixed form's 16 bytes)
+  int nlEshift = 4;
 };
 double handle_count_pairs(Handle &h);
-// the same count with the partials in a static device array (noalloc.hip)
-double handle_count_pairs_noalloc(Handle &h);
 // user-defined kernels, host side (genfunc.cpp)
 void user_fself(const UserKernels &u, double fself[2]);
 void user_check_radial(const UserKernels &u, double Rdir);
@@ -475,6 +515,8 @@ void user_fgrid_host(const UserKernels &u, const long Mf[3], const std::vector<d
                      std::vector<double> &F);
 void user_exact_host(const UserKernels &u, size_t n, const double *coord, const double *q,
                      size_t nt, const long *idx, double *V);
+// the same count with the partials in a static device array (noalloc.hip)
+double handle_count_pairs_noalloc(Handle &h);
 DirectParams direct_params(const Handle &h);
 void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                    void *src, void *qi, hipStream_t s);
@@ -483,7 +525,8 @@ void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const 
 void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);
-unsigned nlist_pad(size_t sentinel);  // padding entry of the neighbour list (sentinel source = n)
+// padding entry of the neighbour list: the sentinel source n, as a byte offset
+unsigned nlist_pad(size_t sentinel, int eshift);
 // scratch of one list build, from its count pass to its fill pass: the
 // hit masks of the wavefront-per-cell walk and their per-cell offsets
 struct NlistWork {
@@ -514,7 +557,38 @@ void launch_exact(int kind, double alpha, double k0r, double k0i, size_t n,
 // double2 for Helmholtz
 void launch_fgrid_real_space(const DMKernel &k, const long Mf[3], const double *d_tab,
                              void *F, hipStream_t s);
+// translated full kernel grid (box pairs): f(r) = func(|r + D|) sampled on
+// the full grid from per-axis tables [x1 + D | x2 + D | T1 | T2], (re, im)
+// per point; G = F lam^2 / (N1 N2 N3) over N1 x N2 x G3 (complex)
+void launch_fgrid_trans(const DMKernel &km, const long N[3], const double *d_tab, double2 *F,
+                        hipStream_t s);
+void launch_make_G_full(bool fp64, const long N[3], long G3, const double2 *F, const double *lam,
+                        void *G, hipStream_t s);
+// host tables of the translated grid (src/RecSumGrid.cpp:663-794,
+// :1024-1042, :1193-1231) and the separation check
+// (src/SupportFunctions.cpp:687-718)
+std::vector<double> fulltrans_tables(const long N[3], const double ta[3], const double del[3],
+                                     const double X[3], const double D[3]);
+double min_distance_r0(const double R0[3], const double l[3]);
+// f_0 of a user kernel (which: 0 func, 1 funcLR) on a grid given by
+// per-axis tables [x1 | x2 | T1 | T2] of n[d] values: F[n0][n1][n2] doubles
+// (real) or (re, im) pairs (complex, or cxOut)
+void user_fgrid_host_n(const UserKernels &u, int which, const long n[3], const std::vector<double> &tab,
+                       bool cxOut, std::vector<double> &F);
+// forward 3-D FFT in FP64 of a complex [N0][N1][N2] grid (twiddles tw64 of
+// each axis); a holds the input, b is scratch; returns the buffer holding
+// the result
+double2 *fft3d_forward_fp64(const long N[3], const FFTPlan1D plan[3], const void *const tw64[3],
+                            double2 *a, double2 *b, hipStream_t s);
 void launch_make_G(bool fp64, bool complexK, const long Mf[3], long K3, const double *F,
                    const double *lam, void *G, hipStream_t s);
 
 }  // namespace npme
+
+// the opaque types of the C ABI
+struct npme_keywords {
+  npme::Keywords kw;
+};
+struct npme_handle {
+  npme::Handle h;
+};

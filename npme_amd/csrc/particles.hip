@@ -31,6 +31,10 @@ struct DirectDev {
   int fxUnit;      // 2^S: one cluster edge in fixed-point units (mixed form)
   float fxInv;     // cluster edge / 2^S: length of one fixed-point unit
   float fxClose2;  // below this r^2 a pair adds the sub-unit residuals
+  int eshift;        // neighbour-list entries: source byte offsets, log2 of the source size
+  double fxInvD;     // fxInv in FP64 (scale of the list walk's gradient sums)
+  float fxClose2u;   // list walk: squared distance in fixed-point units below which
+                     // a pair adds the residuals (2^48: pairs closer than 2^24 units)
   const void *utab;  // user kernels: short-range table in the working precision
   int utS;           // pieces per octave of the table
   double invRdir;
@@ -344,6 +348,9 @@ static DirectDev direct_dev(const DirectParams &p)
   d.utab = p.utab;
   d.utS = p.utS;
   d.invRdir = 1.0 / p.Rdir;
+  d.eshift = p.nlEshift;
+  d.fxInvD = p.fxInv;
+  d.fxClose2u = 281474976710656.0f;  // 2^48
   return d;
 }
 
@@ -403,9 +410,7 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
 // loop needs no test for it.  The pairs are found in FP64 with a cutoff
 // inflated by 1e-9 relative and every entry is evaluated (the short-range
 // function vanishes at Rdir with its first Nder derivatives).
-constexpr unsigned NL_PAD = 0xfc000000u;
-unsigned nlist_pad(size_t sentinel) { return NL_PAD | (unsigned)sentinel; }
-constexpr int NL_SHIFT = 26;
+unsigned nlist_pad(size_t sentinel, int eshift) { return (unsigned)sentinel << eshift; }
 
 // walk the cells of the stencil of sorted target s in FP64 input
 // coordinates; COUNT mode counts, FILL mode writes entries
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
           }
           if (in) {
             if (FILL) {
-              const unsigned ent = (code << NL_SHIFT) | (unsigned)j;
+              const unsigned ent = (unsigned)j << p.eshift;
               const int e = k & 3;
               grp.x = e == 0 ? ent : grp.x;
               grp.y = e == 1 ? ent : grp.y;
@@ -492,7 +497,7 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
         }
       }
   if (FILL && (k & 3) != 0) {  // the last, partial group, padded
-    const unsigned pad = NL_PAD | (unsigned)n;
+    const unsigned pad = (unsigned)n << p.eshift;
     const int e = k & 3;
     if (e <= 1) grp.y = pad;
     if (e <= 2) grp.z = pad;
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
           const double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
           hit = rx * rx + ry * ry + rz * rz < R2;
         }
-        ent = ((code & ~NLW_ALLIN) << NL_SHIFT) | (unsigned)j;
+        ent = (unsigned)j << p.eshift;
       }
     }
     const unsigned long long m = __ballot(hit);
@@ -668,7 +673,7 @@ __device__ __forceinline__ int nl_stencil_scan(const DirectDev &p, const int2 *_
 
 // candidate t of the scanned stencil: source index and list entry
 __device__ __forceinline__ int nl_candidate(int t, int M, const int *shStart, const int *shPre,
-                                            const unsigned char *shCode, unsigned &ent)
+                                            const unsigned char *shCode, int eshift, unsigned &ent)
 {
   // last cell whose first candidate is at or before t (empty cells share
   // their start with the next cell, so it is the cell holding t)
@@ -679,7 +684,8 @@ __device__ __forceinline__ int nl_candidate(int t, int M, const int *shStart, co
     else hi = mid - 1;
   }
   const int j = shStart[lo] + (t - shPre[lo]);
-  ent = ((unsigned)shCode[lo] << NL_SHIFT) | (unsigned)j;
+  (void)shCode;
+  ent = (unsigned)j << eshift;
   return j;
 }
 
@@ -730,7 +736,7 @@ __global__ __launch_bounds__(256) void k_nlist_cell_count(DirectDev p, long ncel
       double ox = 1.0e150, oy = 1.0e150, oz = 1.0e150;
       if (t < T) {
         unsigned ent;
-        j = nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
+        j = nl_candidate(t, M, shStart[w], shPre[w], shCode[w], p.eshift, ent);
         const double4 o = xs[j];
         ox = o.x;
         oy = o.y;
@@ -799,7 +805,7 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
       unsigned ent = 0;
-      if (t < T) (void)nl_candidate(t, M, shStart[w], shPre[w], shCode[w], ent);
+      if (t < T) (void)nl_candidate(t, M, shStart[w], shPre[w], shCode[w], p.eshift, ent);
       const unsigned long long *mrow = masks + mbase + (long)(t0 >> 6) * home.y + tb;
       const unsigned long long mym = lane < mt ? mrow[lane] : 0ull;
       for (int i = 0; i < mt; i++) {
@@ -862,6 +868,7 @@ void launch_nlist(bool fill, const DirectParams &p, size_t n, const int4 *cell, 
   d.mn1 = p.mn[1];
   d.mn2 = p.mn[2];
   d.cellTol = p.cellTol;
+  d.eshift = p.nlEshift;
   // 1: a thread per target (stencils wider than NLW_NN), 2: a wavefront per
   // target (small sets), 3: a wavefront per cell
   const int pick = p.nNeigh > NLW_NN ? 1 : n <= NLIST_WAVE_MAX ? 2 : 3;
@@ -1070,17 +1077,19 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   const bool tvalid = s < n;
   const size_t tt = tvalid ? s : slice * 64;
   const V4 me = src[tt];
-  // fixed-point target position shifted by one cluster edge, so that the
-  // offset code c in {0, 1, 2} enters as (me + unit) - c * unit - o
+  // positions are in one frame for all charges (FP64: relative to the low
+  // corner of the cells; mixed: int32 fixed point of that, wrapped modulo
+  // 2^32), so a pair's difference needs no cluster offset.  Listed pairs lie
+  // within Rdir < 2^31 units, where the wrapped int32 difference is exact.
   int mx = 0, my = 0, mz = 0;
   float4 mres = make_float4(0, 0, 0, 0);
   if constexpr (FIXP) {
-    mx = __float_as_int(me.x) + p.fxUnit;
-    my = __float_as_int(me.y) + p.fxUnit;
-    mz = __float_as_int(me.z) + p.fxUnit;
+    mx = __float_as_int(me.x);
+    my = __float_as_int(me.y);
+    mz = __float_as_int(me.z);
     mres = res[tt];
   }
-  const Real cA = (Real)csA;
+  (void)csA;
   // the slice is the wavefront's: its length is uniform, so the loop and
   // the bounds of the entry loads branch on scalars
   const int len = __builtin_amdgcn_readfirstlane(sliceLen[slice]);
@@ -1091,9 +1100,13 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   // step unrolled twice), so while step i is evaluated the source gathers
   // of step i+1 and the entries of step i+2 are in flight without register
   // copies of loads in flight.  Lanes past n hold padding entries only.
+  // An entry is the byte offset of its source in src (and, for the mixed
+  // form, in res, whose elements have the same 16-byte size).
   constexpr int U = 4;
-  const unsigned ushift = (unsigned)p.fxShift;
   static_assert(U == 4, "one group of 4 list entries per step");
+  const char *srcb = reinterpret_cast<const char *>(src);
+  const char *resb = reinterpret_cast<const char *>(res);
+  constexpr int ESH = sizeof(V4) == 32 ? 5 : 4;
   auto load_ent = [&](int k0, unsigned *ent) {  // k0 a multiple of 4, len too
     uint4 g = make_uint4(pad, pad, pad, pad);
     if (k0 < len) g = e4[(long)(k0 >> 2) * 64];
@@ -1105,42 +1118,57 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   auto gather = [&](const unsigned *ent, V4 *o, Real *oqi) {
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int j = (int)(ent[u] & ((1u << NL_SHIFT) - 1));
-      o[u] = src[j];
-      if constexpr (CX) oqi[u] = qi[j];
+      o[u] = *reinterpret_cast<const V4 *>(srcb + ent[u]);
+      if constexpr (CX) oqi[u] = qi[ent[u] >> ESH];
     }
   };
+  // the mixed form keeps the pair offsets in fixed-point units: the squared
+  // distance is scaled once (with a floor of 1e-24 that keeps the padding
+  // sentinel's terms finite and changes no real pair), and the gradient
+  // sums are scaled by the unit length once per target
+  const Real u2 = (Real)(p.fxInvD * p.fxInvD);
   auto eval = [&](const unsigned *ent, const V4 *o, const Real *oqi) {
+    Real rx[U], ry[U], rz[U], r2[U];
+    if constexpr (FIXP) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const unsigned en = ent[u];
-      Real rx, ry, rz, r2;
-      if constexpr (FIXP) {
-        // (me + unit) - o - code * unit, exact in int32
-        const int dx = (mx - __float_as_int(o[u].x)) - (int)(((en >> NL_SHIFT) & 3u) << ushift);
-        const int dy = (my - __float_as_int(o[u].y)) - (int)(((en >> (NL_SHIFT + 2)) & 3u) << ushift);
-        const int dz = (mz - __float_as_int(o[u].z)) - (int)((en >> (NL_SHIFT + 4)) << ushift);
-        rx = (Real)dx * (Real)p.fxInv;
-        ry = (Real)dy * (Real)p.fxInv;
-        rz = (Real)dz * (Real)p.fxInv;
-        r2 = rx * rx + ry * ry + rz * rz;
-        if (r2 < (Real)p.fxClose2) {  // rare: add the sub-unit residuals
-          const float4 ro = res[en & ((1u << NL_SHIFT) - 1)];
-          rx = ((Real)dx + (mres.x - ro.x)) * (Real)p.fxInv;
-          ry = ((Real)dy + (mres.y - ro.y)) * (Real)p.fxInv;
-          rz = ((Real)dz + (mres.z - ro.z)) * (Real)p.fxInv;
-          r2 = rx * rx + ry * ry + rz * rz;
-        }
-      } else {
-        const int cx = (int)((en >> NL_SHIFT) & 3), cy = (int)((en >> (NL_SHIFT + 2)) & 3),
-                  cz = (int)(en >> (NL_SHIFT + 4));
-        rx = me.x - (o[u].x + (Real)(cx - 1) * cA);
-        ry = me.y - (o[u].y + (Real)(cy - 1) * cA);
-        rz = me.z - (o[u].z + (Real)(cz - 1) * cA);
-        r2 = rx * rx + ry * ry + rz * rz;
+      for (int u = 0; u < U; u++) {
+        rx[u] = (Real)(mx - __float_as_int(o[u].x));
+        ry[u] = (Real)(my - __float_as_int(o[u].y));
+        rz[u] = (Real)(mz - __float_as_int(o[u].z));
+        r2[u] = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
       }
-      pair_sr<Real, KIND, MAXD>(p, rx, ry, rz, r2, o[u].w, CX ? oqi[u] : (Real)0, part, topA, topB);
+      // rare (pairs closer than 2^24 units): add the sub-unit residuals of
+      // both positions; one wave-uniform test per step on the closest pair
+      const Real rmin = fmin(fmin(r2[0], r2[1]), fmin(r2[2], r2[3]));
+      if (__builtin_amdgcn_ballot_w64(rmin < (Real)p.fxClose2u)) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (r2[u] < (Real)p.fxClose2u) {
+            const float4 ro = *reinterpret_cast<const float4 *>(resb + ent[u]);
+            const int dx = mx - __float_as_int(o[u].x), dy = my - __float_as_int(o[u].y),
+                      dz = mz - __float_as_int(o[u].z);
+            rx[u] = (Real)dx + (mres.x - ro.x);
+            ry[u] = (Real)dy + (mres.y - ro.y);
+            rz[u] = (Real)dz + (mres.z - ro.z);
+            r2[u] = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) r2[u] = fma(r2[u], u2, (Real)1.0e-24);
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        rx[u] = me.x - o[u].x;
+        ry[u] = me.y - o[u].y;
+        rz[u] = me.z - o[u].z;
+        r2[u] = rx[u] * rx[u] + ry[u] * ry[u] + rz[u] * rz[u];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      pair_sr<Real, KIND, MAXD>(p, rx[u], ry[u], rz[u], r2[u], o[u].w, CX ? oqi[u] : (Real)0, part,
+                                topA, topB);
   };
   auto carry = [&](int k0) {
     if (((k0 + U) & 15) == 0) {  // FP64 carry every 16 entries
@@ -1184,6 +1212,10 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   if (!tvalid) return;
 #pragma unroll
   for (int c = 0; c < (CX ? 8 : 4); c++) acc[c] += (double)part[c];
+  if constexpr (FIXP) {  // gradient sums from fixed-point units to lengths
+#pragma unroll
+    for (int c = (CX ? 2 : 1); c < (CX ? 8 : 4); c++) acc[c] *= p.fxInvD;
+  }
   if (CX) {
 #pragma unroll
     for (int c = 0; c < 8; c++) Vdir[8 * s + c] = acc[c];
@@ -1226,7 +1258,7 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
                       const void *qi, const long *sliceStart, const int *sliceLen, const unsigned *list,
                       double csA, double *Vdir, hipStream_t s)
 {
-  const unsigned pad = nlist_pad(n);
+  const unsigned pad = nlist_pad(n, p.nlEshift);
   const DirectDev d = direct_dev(p);
   // a workgroup (4 wavefronts) per 4 slices
   const long nwg = (long)((n + 255) / 256);
@@ -2276,6 +2308,112 @@ void launch_make_G(bool fp64, bool complexK, const long Mf[3], long K3, const do
     else NPME_G(float, false);
   }
 #undef NPME_G
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+// ------------------------------------------------ translated kernel grid
+// The full kernel (not its long-range part) at r2 = |r|^2
+template <int KIND>
+__device__ __forceinline__ void full_eval(const LRDev &k, double r2, double &fr, double &fi)
+{
+  const double r = sqrt(r2);
+  fi = 0;
+  if (KIND == KIND_RALPHA) {
+    fr = pow(r2, 0.5 * k.alpha);
+  } else if (KIND == KIND_HELMHOLTZ) {
+    double sn, cn;
+    sincos(k.k0r * r, &sn, &cn);
+    const double d = exp(-k.k0i * r) / r;
+    fr = d * cn;
+    fi = d * sn;
+  } else {  // KIND_LAPLACE, KIND_LAPLACE_ERF
+    fr = 1.0 / r;
+  }
+}
+
+// f(r) = func(|r + D|) on the full N0 x N1 x N2 grid as the sum of the 8
+// overlapped images with their top-hat weights; tab per axis d holds
+// [x1 + D | x2 + D | T1 | T2] of N_d values (axes consecutive).  Images of
+// zero weight are skipped.  Output (re, im) per point.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_fgrid_trans(LRDev k, long N0, long N1, long N2,
+                                                     const double *__restrict__ tab,
+                                                     double2 *__restrict__ Fout)
+{
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= N0 * N1 * N2) return;
+  const long i3 = id % N2, i2 = (id / N2) % N1, i1 = id / (N2 * N1);
+  const double *t0 = tab, *t1 = tab + 4 * N0, *t2 = t1 + 4 * N1;
+  const double X[2] = {t0[i1], t0[N0 + i1]}, TX[2] = {t0[2 * N0 + i1], t0[3 * N0 + i1]};
+  const double Y[2] = {t1[i2], t1[N1 + i2]}, TY[2] = {t1[2 * N1 + i2], t1[3 * N1 + i2]};
+  const double Z[2] = {t2[i3], t2[N2 + i3]}, TZ[2] = {t2[2 * N2 + i3], t2[3 * N2 + i3]};
+  double sr = 0, si = 0;
+#pragma unroll
+  for (int ax = 0; ax < 2; ax++)
+#pragma unroll
+    for (int ay = 0; ay < 2; ay++)
+#pragma unroll
+      for (int az = 0; az < 2; az++) {
+        const double w = TX[ax] * TY[ay] * TZ[az];
+        if (w == 0.0) continue;
+        double fr, fi;
+        full_eval<KIND>(k, X[ax] * X[ax] + Y[ay] * Y[ay] + Z[az] * Z[az], fr, fi);
+        sr += w * fr;
+        si += w * fi;
+      }
+  Fout[id] = make_double2(sr, si);
+}
+
+void launch_fgrid_trans(const DMKernel &km, const long N[3], const double *d_tab, double2 *F,
+                        hipStream_t s)
+{
+  LRDev k{};
+  k.kind = km.kind;
+  k.alpha = km.alpha;
+  k.k0r = km.k0r;
+  k.k0i = km.k0i;
+  const long tot = N[0] * N[1] * N[2];
+  const dim3 g((unsigned)((tot + 255) / 256));
+  if (k.kind == KIND_RALPHA)
+    hipLaunchKernelGGL(k_fgrid_trans<KIND_RALPHA>, g, dim3(256), 0, s, k, N[0], N[1], N[2], d_tab, F);
+  else if (k.kind == KIND_HELMHOLTZ)
+    hipLaunchKernelGGL(k_fgrid_trans<KIND_HELMHOLTZ>, g, dim3(256), 0, s, k, N[0], N[1], N[2], d_tab, F);
+  else if (k.kind == KIND_LAPLACE || k.kind == KIND_LAPLACE_ERF)
+    hipLaunchKernelGGL(k_fgrid_trans<KIND_LAPLACE>, g, dim3(256), 0, s, k, N[0], N[1], N[2], d_tab, F);
+  else
+    throw Error{NPME_ERR_UNSUPPORTED, "no device form of this kernel's translated grid"};
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+// G = F lam1 lam2 lam3 scale over N0 x N1 x G3 (a3 < G3 <= N2 of F), F
+// complex [N0][N1][N2]; lam holds the full |lambda|^2 arrays of the three
+// axes (N0, N1, N2 values); layout G[a2][a3 / 8][a1][a3 % 8]
+template <typename Real>
+__global__ void k_make_G_full(long N0, long N1, long N2, long G3, const double2 *__restrict__ F,
+                              const double *__restrict__ lam, double scale, Real *__restrict__ G)
+{
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= N0 * N1 * G3) return;
+  const long a3 = id % G3, a2 = (id / G3) % N1, a1 = id / (G3 * N1);
+  const double l = lam[a1] * lam[N0 + a2] * lam[N0 + N1 + a3] * scale;
+  const double2 f = F[(a1 * N1 + a2) * N2 + a3];
+  const long o = ((a2 * ((G3 + 7) / 8) + (a3 >> 3)) * N0 + a1) * 8 + (a3 & 7);
+  G[2 * o] = (Real)(f.x * l);
+  G[2 * o + 1] = (Real)(f.y * l);
+}
+
+void launch_make_G_full(bool fp64, const long N[3], long G3, const double2 *F, const double *lam,
+                        void *G, hipStream_t s)
+{
+  const long tot = N[0] * N[1] * G3;
+  const double scale = 1.0 / ((double)N[0] * (double)N[1] * (double)N[2]);
+  const dim3 g((unsigned)((tot + 255) / 256));
+  if (fp64)
+    hipLaunchKernelGGL(k_make_G_full<double>, g, dim3(256), 0, s, N[0], N[1], N[2], G3, F, lam, scale,
+                       (double *)G);
+  else
+    hipLaunchKernelGGL(k_make_G_full<float>, g, dim3(256), 0, s, N[0], N[1], N[2], G3, F, lam, scale,
+                       (float *)G);
   NPME_HIP_CHECK(hipGetLastError());
 }
 

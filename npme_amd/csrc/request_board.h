@@ -69,6 +69,18 @@ template <typename Job> class RequestBoard {
     }
   }
 
+  // runs change under the lock once ok holds (both run under the lock;
+  // finish and adjust wake the waiters), then wakes the workers
+  void adjust(const std::function<bool()> &ok, const std::function<void()> &change)
+  {
+    {
+      std::unique_lock<std::mutex> lk(mtx);
+      cvWork.wait(lk, ok);
+      change();
+    }
+    cvWork.notify_all();
+  }
+
   // marks a job done (fin runs under the lock) and wakes its waiter and the
   // workers (admission may have changed)
   void finish(Job *j, const std::function<void(Job &)> &fin)

@@ -49,8 +49,17 @@ template <typename T> static T *dalloc(Handle &h, size_t count)
   return (T *)p;
 }
 
+static void drop_graph(Handle &h)
+{
+  if (h.calcvExec) (void)hipGraphExecDestroy(h.calcvExec);
+  h.calcvExec = nullptr;
+  h.graphQ = h.graphV = nullptr;
+  h.graphCalls = 0;
+}
+
 void handle_free_device(Handle &h, bool keepG)
 {
+  drop_graph(h);
   void *ptrs[] = {h.d_itileStart, h.d_itileList, h.d_perm, h.d_src, h.d_srcRes, h.d_blockBox, h.d_segStart, h.d_nlist,
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2,
@@ -96,6 +105,8 @@ static void ensure_streams(Handle &h)
   NPME_HIP_CHECK(hipGetDevice(&h.device));
   if (!h.stream) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking));
   if (!h.stream2) NPME_HIP_CHECK(hipStreamCreateWithFlags(&h.stream2, hipStreamNonBlocking));
+  if (!h.evFork) NPME_HIP_CHECK(hipEventCreateWithFlags(&h.evFork, hipEventDisableTiming));
+  if (!h.evJoin) NPME_HIP_CHECK(hipEventCreateWithFlags(&h.evJoin, hipEventDisableTiming));
 }
 
 void handle_stage_coords(Handle &h, size_t n, const double *coord)
@@ -151,7 +162,7 @@ static unsigned long coord2cell(double x, double cs, unsigned long ncell)
   return v;
 }
 
-static std::vector<double> twiddles(int N)
+std::vector<double> twiddles(int N)
 {
   const double PI = 3.14159265358979323846;
   std::vector<double> t(2 * (size_t)N);
@@ -185,6 +196,73 @@ static double tsmooth(double x, double X, double Xmax, double a)
   if (x < -X) return hramp((x + X + del) / del, a);
   if (x <= X) return 1.0;
   return hramp((-x + X + del) / del, a);
+}
+
+// per-axis tables of the full (translated) kernel grid
+// (NPME_RecSumGrid_FullTrans_Func, src/RecSumGrid.cpp:663-794): sample n at
+// s L/N with s = n, or n - N for n >= N/2 (src/RecSumGrid.cpp:1024-1042);
+// its second image and the top-hat weights as NPME_CalcOverlapSmoothFunc
+// (src/RecSumGrid.cpp:1193-1231); both image coordinates shifted by D after
+// the weights are taken (src/RecSumGrid.cpp:77-100).  Per axis
+// [x1 + D | x2 + D | T1 | T2].
+std::vector<double> fulltrans_tables(const long N[3], const double ta[3], const double del[3],
+                                     const double X[3], const double D[3])
+{
+  std::vector<double> tab;
+  for (int d = 0; d < 3; d++) {
+    const long n = N[d];
+    const double Xmax = X[d] + del[d], Xt = 2 * Xmax - X[d], dx = 2 * Xmax / n;
+    std::vector<double> x1(n), x2(n), T1(n), T2(n);
+    for (long i = 0; i < n; i++) {
+      const double x = (double)(i >= n / 2 ? i - n : i) * dx;
+      if (x > X[d]) {
+        T1[i] = tsmooth(x, X[d], Xt, ta[d]);
+        x2[i] = x - 2 * Xmax;
+        T2[i] = tsmooth(x2[i], X[d], Xt, ta[d]);
+      } else if (x > -X[d]) {
+        T1[i] = 1.0;
+        x2[i] = 1.0;  // weight 0: never evaluated
+        T2[i] = 0.0;
+      } else {
+        T1[i] = tsmooth(x, X[d], Xt, ta[d]);
+        x2[i] = x + 2 * Xmax;
+        T2[i] = tsmooth(x2[i], X[d], Xt, ta[d]);
+      }
+      x1[i] = x + D[d];
+      x2[i] += D[d];
+    }
+    tab.insert(tab.end(), x1.begin(), x1.end());
+    tab.insert(tab.end(), x2.begin(), x2.end());
+    tab.insert(tab.end(), T1.begin(), T1.end());
+    tab.insert(tab.end(), T2.begin(), T2.end());
+  }
+  return tab;
+}
+
+// min |r + R0| over the box of edges l centred at the origin
+// (NPME_CalcMinDistance_R0_r, src/SupportFunctions.cpp:687-718)
+double min_distance_r0(const double R0[3], const double l[3])
+{
+  double s = 0;
+  for (int d = 0; d < 3; d++) {
+    const double m = std::fabs(R0[d]);
+    if (!(m < 0.5 * l[d])) s += (0.5 * l[d] - m) * (0.5 * l[d] - m);
+  }
+  return std::sqrt(s);
+}
+
+double2 *fft3d_forward_fp64(const long N[3], const FFTPlan1D plan[3], const void *const tw64[3],
+                            double2 *a, double2 *b, hipStream_t s)
+{
+  const long N0 = N[0], N1 = N[1], N2 = N[2];
+  // rows along axis 3, columns along axis 2 (stride N2), columns along axis 1
+  const PassGeom g3{(int)N2, (int)N2, (int)N2, N0 * N1, N0 * N1, 0, N2, 1, 0, N2, 1};
+  const PassGeom g2{(int)N1, (int)N1, (int)N1, N0 * N2, N2, N1 * N2, 1, N2, N1 * N2, 1, N2};
+  const PassGeom g1{(int)N0, (int)N0, (int)N0, N1 * N2, N1 * N2, 0, 1, N1 * N2, 0, 1, N1 * N2};
+  fft_pass_complex(true, g3, plan[2], tw64[2], a, b, false, s);
+  fft_pass_complex(true, g2, plan[1], tw64[1], b, a, false, s);
+  fft_pass_complex(true, g1, plan[0], tw64[0], a, b, false, s);
+  return b;
 }
 
 // ---------------------------------------------------------------- neighbour list
@@ -244,9 +322,13 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
   h.d_nlist = dalloc<unsigned>(h, entries);
   h.d_sliceStart = dalloc<long>(h, nSlices + 1);
   h.d_sliceLen = dalloc<int>(h, nSlices);
-  NPME_HIP_CHECK(hipMemcpy(h.d_sliceStart, sliceStart.data(), (nSlices + 1) * sizeof(long), hipMemcpyHostToDevice));
-  NPME_HIP_CHECK(hipMemcpy(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int), hipMemcpyHostToDevice));
-  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(h.n), entries, h.stream));
+  // copies on the handle's stream, never the null stream: a null-stream
+  // operation can invalidate a graph capture running on another thread
+  NPME_HIP_CHECK(hipMemcpyAsync(h.d_sliceStart, sliceStart.data(), (nSlices + 1) * sizeof(long),
+                                hipMemcpyHostToDevice, h.stream));
+  NPME_HIP_CHECK(hipMemcpyAsync(h.d_sliceLen, sliceLen.data(), nSlices * sizeof(int),
+                                hipMemcpyHostToDevice, h.stream));
+  NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(h.n, h.fp64 ? 5 : 4), entries, h.stream));
   launch_nlist(true, dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, nP, nullptr, h.d_sliceStart,
                h.d_nlist, work, h.stream);
   NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
@@ -259,6 +341,53 @@ static void *adopt(Handle &h, DevBuf &b)
 {
   h.deviceBytes += dev_block_bytes(b.bytes);
   return b.release();
+}
+
+// the list walk's fixed point: 2^30 units per cluster edge.  Positions are
+// wrapped modulo 2^32, and a listed pair is closer than Rdir <= one cluster
+// edge (nNeigh <= nCellClust1D) < 2^31 units, so its int32 difference is exact
+constexpr int kListShift = 30;
+
+// the zero-charge source n that pads the neighbour list.  FP64: outside the
+// cells' box by two cluster edges on every axis, so it is at least that far
+// from every charge.  Mixed (wrapped frame, where no point is far from all
+// charges): charge 0's position moved by large odd constants, moved again
+// while any charge sits within 2^20 units (1e-3 cluster edges) of it on all
+// three axes, so every term it adds is finite (and zero).
+static void place_sentinel(Handle &h)
+{
+  const size_t n = h.n;
+  if (h.fp64) {
+    const double4 v = make_double4(-2.0 * h.csA, -2.0 * h.csA, -2.0 * h.csA, 0.0);
+    NPME_HIP_CHECK(hipMemcpyAsync(static_cast<double4 *>(h.d_src) + n, &v, sizeof(v),
+                                  hipMemcpyHostToDevice, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+    return;
+  }
+  int4 c0;
+  NPME_HIP_CHECK(hipMemcpyAsync(&c0, h.d_src, sizeof(c0), hipMemcpyDeviceToHost, h.stream));
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  if (!h.useNlist) {
+    const int4 z = make_int4(0, 0, 0, 0);
+    NPME_HIP_CHECK(hipMemcpyAsync(static_cast<int4 *>(h.d_src) + n, &z, sizeof(z), hipMemcpyHostToDevice,
+                                  h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before z goes out of scope
+    return;
+  }
+  unsigned x = (unsigned)c0.x, y = (unsigned)c0.y, z = (unsigned)c0.z;
+  for (int attempt = 0;; attempt++) {
+    x += 0x9E3779B9u;
+    y += 0x7F4A7C15u;
+    z += 0x85EBCA6Bu;
+    const int4 v = make_int4((int)x, (int)y, (int)z, 0);  // w: FP32 charge 0
+    if (!sentinel_near_any(n, h.d_src, v, 1 << 20, h.stream)) {
+      NPME_HIP_CHECK(hipMemcpyAsync(static_cast<int4 *>(h.d_src) + n, &v, sizeof(v),
+                                    hipMemcpyHostToDevice, h.stream));
+      NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before v goes out of scope
+      return;
+    }
+    if (attempt == 63) throw Error{NPME_ERR_UNSUPPORTED, "no place for the list's padding source"};
+  }
 }
 
 static int fixed_shift(int maxOff)
@@ -315,7 +444,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
       list = false;
     } else {
       h.d_nlist = dalloc<unsigned>(h, entries);
-      NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n), std::max<size_t>(entries, 1), s));
+      NPME_HIP_CHECK(hipMemsetD32Async((hipDeviceptr_t)h.d_nlist, nlist_pad(n, f64 ? 5 : 4), std::max<size_t>(entries, 1), s));
       launch_nlist(true, dp, n, h.d_cell, h.d_cellStart, h.d_xs, (int)h.nCellClust1D, nullptr,
                    h.d_sliceStart, h.d_nlist, work, s);
       h.nlistEntries = entries;
@@ -338,12 +467,12 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   // positions, spline windows and tile lists; the fixed-point scale
   // depends on how far the direct sum's sources reach (list: the adjacent
   // clusters)
-  h.fxShift = list ? fixed_shift(1) : fixed_shift(maxOff);
+  h.fxShift = list ? kListShift : fixed_shift(maxOff);
   g.fxShift = h.fxShift;
   g.fxScale = std::ldexp(1.0, h.fxShift) / h.csA;
+  g.globalFrame = list ? 1 : 0;
   // [n + 1]: entry n is the neighbour list's zero-charge sentinel
   h.d_src = dalloc<char>(h, 4 * rs * (n + 1));
-  NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_src + 4 * rs * n, 0, 4 * rs, s));
   if (!f64) {
     h.d_srcRes = dalloc<float4>(h, n + 1);
     NPME_HIP_CHECK(hipMemsetAsync(static_cast<float4 *>(h.d_srcRes) + n, 0, sizeof(float4), s));
@@ -369,6 +498,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   h.d_V = dalloc<double>(h, vw * n);
   h.d_qin = dalloc<double>(h, (h.complexK ? 2 : 1) * n);
   NPME_HIP_CHECK(hipStreamSynchronize(s));
+  place_sentinel(h);
   h.setupMs[4] = ms_since(t0);
   return true;
 }
@@ -558,7 +688,8 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   const bool f64 = h.fp64;
   const size_t rs = f64 ? 8 : 4;
   auto up = [&](void *dst, const void *src, size_t bytes) {
-    NPME_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    NPME_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // the host arrays are temporaries
   };
   h.d_perm = dalloc<int>(h, n);
   up(h.d_perm, perm.data(), n * sizeof(int));
@@ -606,26 +737,33 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
   // ---- direct sum: the neighbour list when it fits, else the cell rows;
   // the fixed-point scale depends on which of the two reads the positions
   if (nlPossible) build_neighbor_list(h, nlmode);
-  h.fxShift = h.useNlist ? fixed_shift(1) : fixed_shift(maxOff);
+  h.fxShift = h.useNlist ? kListShift : fixed_shift(maxOff);
+  if (h.useNlist) {
+    // the list walk reads positions in one frame: relative to the low
+    // corner of the cells (the cell-row sum keeps them per cluster)
+    for (size_t s = 0; s < n; s++)
+      for (int d = 0; d < 3; d++) loc[4 * s + d] = coord[3 * (size_t)perm[s] + d] + (-mn[d]);
+  }
   // [n + 1]: entry n is the neighbour list's zero-charge sentinel
   h.d_src = dalloc<char>(h, 4 * rs * (n + 1));
-  NPME_HIP_CHECK(hipMemsetAsync((char *)h.d_src + 4 * rs * n, 0, 4 * rs, h.stream));
   if (f64) {
     up(h.d_src, loc.data(), 4 * n * sizeof(double));
   } else {
-    // int32 fixed point relative to the cluster corner (2^S units per
-    // cluster edge) in the x, y, z words; the w word receives the charge
-    // and the sub-unit residual of each coordinate
+    // int32 fixed point (2^S units per cluster edge) in the x, y, z words,
+    // relative to the cluster corner for the cell rows and to the cells'
+    // low corner, wrapped modulo 2^32, for the list; the w word receives
+    // the charge, and the sub-unit residual of each coordinate is kept
     std::vector<int32_t> fx(4 * n, 0);
     std::vector<float> fr(4 * n, 0.0f);
     const double scale = std::ldexp(1.0, h.fxShift) / csA;
     for (size_t s = 0; s < n; s++)
       for (int d = 0; d < 3; d++) {
         const double u = loc[4 * s + d] * scale;
-        const long v = std::lround(u);
-        if (v < -(1L << 20) || v > (1L << h.fxShift) + (1L << 20))
+        const long long v = std::llround(u);
+        if (h.useNlist ? !(std::fabs(u) < 4.0e18)
+                       : (v < -(1L << 20) || v > (1L << h.fxShift) + (1L << 20)))
           throw Error{NPME_ERR_ARG, "position outside its cluster"};
-        fx[4 * s + d] = (int32_t)v;
+        fx[4 * s + d] = (int32_t)(uint32_t)(unsigned long long)v;
         fr[4 * s + d] = (float)(u - (double)v);
       }
     up(h.d_src, fx.data(), 4 * n * sizeof(int32_t));
@@ -633,6 +771,7 @@ static void particles_host(Handle &h, GeoParams &g, const std::string &nlmode, b
     NPME_HIP_CHECK(hipMemsetAsync(static_cast<float4 *>(h.d_srcRes) + n, 0, sizeof(float4), h.stream));
     up(h.d_srcRes, fr.data(), 4 * n * sizeof(float));
   }
+  place_sentinel(h);
 }
 
 // FFT buffers, twiddles and the kernel multiplier G = F lam^2 / (N1 N2 N3)
@@ -688,6 +827,8 @@ static void grid_setup(Handle &h)
   key.k0i = h.ker.k0i;
   key.Rdir = h.ker.Rdir;
   key.beta = h.ker.beta;
+  key.trans = h.trans;
+  for (int d = 0; d < 3; d++) key.D[d] = h.trans ? h.transD[d] : 0.0;
   // a user kernel's callbacks may change between set-ups: its G is never reused
   h.gReused = h.gValid && h.d_G && key == h.gKey && !kind_is_user(h.ker.kind);
   if (h.gReused) {
@@ -702,6 +843,56 @@ static void grid_setup(Handle &h)
   h.deviceBytes -= h.gBytes;
   h.gValid = false;
   h.gBytes = 0;
+
+  if (h.trans) {
+    // box pair: the full kernel at r + D on the full grid, its 3-D DFT, and
+    // G over the full N1 x N2 x G3 grid (complex)
+    if (min_distance_r0(h.transD, h.L) < 1.0e-6) {
+      std::ostringstream os;
+      os << "box-box separation (" << h.transD[0] << ", " << h.transD[1] << ", " << h.transD[2]
+         << ") should be increased: the kernel grid box of edges (" << h.L[0] << ", " << h.L[1] << ", "
+         << h.L[2] << ") reaches the singularity";
+      throw Error{NPME_ERR_ARG, os.str()};
+    }
+    const std::vector<double> tab = fulltrans_tables(h.N, h.ta, h.del, h.X, h.transD);
+    const size_t offTab = put(tab.data(), tab.size() * sizeof(double));
+    std::vector<double> lam;
+    for (int d = 0; d < 3; d++) {
+      std::vector<double> l(h.N[d]);
+      lambda2(h.N[d], h.Bn, l.data());
+      lam.insert(lam.end(), l.begin(), l.end());
+    }
+    const size_t offLam = put(lam.data(), lam.size() * sizeof(double));
+    h.d_tables = dalloc<char>(h, blob.size());
+    NPME_HIP_CHECK(hipMemcpyAsync(h.d_tables, blob.data(), blob.size(), hipMemcpyHostToDevice, h.stream));
+    char *tb = static_cast<char *>(h.d_tables);
+    const void *tw64[3];
+    for (int d = 0; d < 3; d++) {
+      h.d_tw[d] = tb + offTw[d];
+      tw64[d] = tb + offTw64[d];
+    }
+    const size_t tot = (size_t)h.N[0] * h.N[1] * h.N[2];
+    DevBuf dA(tot * sizeof(double2)), dB(tot * sizeof(double2));
+    std::vector<double> Fhost;
+    if (kind_is_user(h.ker.kind)) {
+      user_fgrid_host_n(h.user, 0, h.N, tab, true, Fhost);
+      NPME_HIP_CHECK(hipMemcpyAsync(dA.p, Fhost.data(), Fhost.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, h.stream));
+    } else {
+      launch_fgrid_trans(h.ker, h.N, (const double *)(tb + offTab), (double2 *)dA.p, h.stream);
+    }
+    double2 *F = fft3d_forward_fp64(h.N, h.plan, tw64, (double2 *)dA.p, (double2 *)dB.p, h.stream);
+    const long G3 = h.complexK ? h.N[2] : h.N[2] / 2 + 1;
+    const size_t totG = (size_t)h.N[0] * h.N[1] * ((G3 + 7) / 8) * 8;
+    const size_t before = h.deviceBytes;
+    h.d_G = dalloc<char>(h, cs * totG);
+    h.gBytes = h.deviceBytes - before;
+    launch_make_G_full(f64, h.N, G3, F, (const double *)(tb + offLam), h.d_G, h.stream);
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before the temporaries return to the pool
+    h.gKey = key;
+    h.gValid = true;
+    return;
+  }
 
   // kernel grid: real-space compact grid of f_LR with the smooth Fourier
   // extension, then an even-extension DFT along each axis, all FP64
@@ -862,9 +1053,11 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.ker = DMKernel{};
     h.ker.kind = kind;
     h.ker.Rdir = Rdir;
-    user_check_radial(h.user, Rdir);
-    user_build_table(h.user, Rdir, utab, h.utS, h.utErr);
-    user_fself(h.user, h.fself);
+    if (!h.trans) {  // a box pair uses only func, on the kernel grid
+      user_check_radial(h.user, Rdir);
+      user_build_table(h.user, Rdir, utab, h.utS, h.utErr);
+      user_fself(h.user, h.fself);
+    }
     h.coordHost.assign(coord, coord + 3 * n);
   } else if (kind == KIND_LAPLACE_ERF) {
     // NPME_InterfaceReal_Laplace_Original::SetUp (src/Interface.cpp:2152-2230)
@@ -880,6 +1073,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     dm_eval_lr(h.ker, 0.0, h.fself, f1);
     h.coordHost.clear();
   }
+  if (h.trans) h.fself[0] = h.fself[1] = 0.0;  // a box pair has no self term
 
   // NPME_CheckFFTParm (src/RecSumSupportFunctions.cpp:37-135)
   for (int d = 0; d < 3; d++) {
@@ -922,6 +1116,7 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (smode != "device" && smode != "host") kw_error("setup must be 'device' or 'host'");
   if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
     kw_error("neighborList must be 'auto', 'on' or 'off'");
+  if (h.trans) nlmode = "off";  // no direct sum between the boxes
   const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
   const bool devSetup = smode == "device";
   const bool staged = devSetup && h.coordStaged && h.stagedN == n;
@@ -1041,6 +1236,12 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     }
   }
   grid_setup(h);
+  if (h.trans) {
+    // the interpolation adds the (absent) direct sum: zero, once
+    const size_t vb = h.n * 4 * (h.complexK ? 2 : 1) * sizeof(double);
+    NPME_HIP_CHECK(hipMemsetAsync(h.d_Vdir, 0, vb, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  }
   h.setupMs[5] = ms_since(t0);
   h.ready = true;
 }
@@ -1053,8 +1254,12 @@ static void require_ready(const Handle &h)
 
 void handle_destroy_streams(Handle &h)
 {
+  drop_graph(h);
   for (hipEvent_t e : h.ev) (void)hipEventDestroy(e);
   h.ev.clear();
+  if (h.evFork) (void)hipEventDestroy(h.evFork);
+  if (h.evJoin) (void)hipEventDestroy(h.evJoin);
+  h.evFork = h.evJoin = nullptr;
   if (h.stream2) (void)hipStreamDestroy(h.stream2);
   if (h.stream) (void)hipStreamDestroy(h.stream);
   h.stream = h.stream2 = nullptr;
@@ -1100,6 +1305,7 @@ DirectParams direct_params(const Handle &h)
   dp.fxInv = std::ldexp(h.csA, -h.fxShift);
   dp.utab = h.d_utab;
   dp.utS = h.utS;
+  dp.nlEshift = h.fp64 ? 5 : 4;
   return dp;
 }
 
@@ -1116,23 +1322,23 @@ double handle_count_pairs(Handle &h)
   return (double)tot;
 }
 
-void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
+// The launches of one CalcV.  The direct sum (gather of the packed sources,
+// pair sum into Vdir) runs on stream2 while the reciprocal sum runs on
+// stream: stream2 waits for a fork event recorded on stream, and stream
+// waits for a join event recorded on stream2 before the interpolation,
+// which reads Vdir.  Both events are recorded before their waits are
+// queued, so a wait never depends on work queued after it.  Without split
+// (per-stage timing) everything runs on stream in order, with the stage
+// events.
+static void calcv_launches(Handle &h, const double *d_q, double *d_Vout, bool split)
 {
-  require_ready(h);
   const bool f64 = h.fp64, cx = h.complexK;
   hipStream_t s = h.stream;
   const long M1 = h.M[0], M2 = h.M[1], M3 = h.M[2], N1 = h.N[0], N2 = h.N[1], N3 = h.N[2];
   const long K3 = h.K3, Kp = h.Kp;
-  // The direct sum (gather of the packed sources, pair sum into Vdir) runs
-  // on stream2 while the reciprocal sum runs on stream: both are queued at
-  // once and the GPU shares its CUs between them.  The two paths read
-  // disjoint inputs (each gathers the charges itself); the host waits for
-  // stream2 before queuing the interpolation, which reads Vdir, so no
-  // stream waits on another on the device.  With per-stage timing on,
-  // everything runs on stream in order.
-  const bool split = !h.profiling;
   hipStream_t s2 = split ? h.stream2 : s;
   const DirectParams dp = direct_params(h);
+  const bool doDirect = !h.trans;  // a box pair has no direct sum
   auto direct = [&] {
     launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s2);
     if (h.useNlist)
@@ -1142,10 +1348,15 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
       launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
                     (int)h.nCellClust1D, h.csA, h.d_Vdir, s2);
   };
-  if (split) direct();
+  if (split && doDirect) {
+    NPME_HIP_CHECK(hipEventRecord(h.evFork, s));
+    NPME_HIP_CHECK(hipStreamWaitEvent(s2, h.evFork, 0));
+    direct();
+    NPME_HIP_CHECK(hipEventRecord(h.evJoin, s2));
+  }
   int st = 0;
   mark(h, st++);
-  if (!split) {
+  if (!split && doDirect) {
     // the same kernels in order, timed one by one: gather, direct
     launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
     mark(h, st++);
@@ -1170,7 +1381,8 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s, h.fftPow8);
     mark(h, st++);
     fft_pass_convolve(f64, false, (int)N1, (int)M1, N2 * Kp, N2 * Kp, (int)N2, (int)(N2 / 2 + 1),
-                      Kp, K3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8);
+                      Kp, K3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8, h.trans,
+                      h.transFlags);
     mark(h, st++);
     PassGeom d{(int)N2, (int)N2, (int)M2, M1 * Kp, Kp, N2 * Kp, 1, Kp, M2 * Kp, 1, Kp};
     d.validInner = K3;
@@ -1188,7 +1400,8 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     fft_pass_complex(f64, b, h.plan[1], h.d_tw[1], h.d_buf1, h.d_buf2, false, s, h.fftPow8);
     mark(h, st++);
     fft_pass_convolve(f64, true, (int)N1, (int)M1, N2 * Kp, N2 * Kp, (int)N2, (int)(N2 / 2 + 1),
-                      Kp, N3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8);
+                      Kp, N3, h.plan[0], h.d_tw[0], h.d_G, h.d_buf2, s, h.fftPow8, h.trans,
+                      h.transFlags);
     mark(h, st++);
     PassGeom d{(int)N2, (int)N2, (int)M2, M1 * Kp, Kp, N2 * Kp, 1, Kp, M2 * Kp, 1, Kp};
     d.validInner = N3;
@@ -1199,12 +1412,12 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     mark(h, st++);
   }
   const double C[3] = {N1 / h.L[0], N2 / h.L[1], N3 / h.L[2]};
-  if (split) NPME_HIP_CHECK(hipStreamSynchronize(s2));  // Vdir is complete
+  if (split && doDirect) NPME_HIP_CHECK(hipStreamWaitEvent(s, h.evJoin, 0));  // Vdir is complete
   launch_interp(f64, cx, h.Bn, h.M, C, h.TI, h.nti, h.d_itileStart, h.d_itileList, h.d_ms,
                 h.d_frac, h.d_grid, h.d_Vdir, h.d_perm, d_q, h.fself, d_Vout, s);
   mark(h, st++);
-  NPME_HIP_CHECK(hipStreamSynchronize(s));
-  if (h.profiling) {
+  if (!split) {
+    NPME_HIP_CHECK(hipStreamSynchronize(s));
     h.nstage = st - 1;
     for (int i = 0; i + 1 < st; i++) {
       float ms = 0;
@@ -1213,6 +1426,110 @@ void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
     }
   }
 }
+
+// the CalcV launches captured from stream as one graph
+static hipGraphExec_t capture_calcv(Handle &h, const double *d_q, double *d_Vout)
+{
+  NPME_HIP_CHECK(hipStreamBeginCapture(h.stream, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    calcv_launches(h, d_q, d_Vout, true);
+  } catch (...) {
+    if (hipStreamEndCapture(h.stream, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  NPME_HIP_CHECK(hipStreamEndCapture(h.stream, &g));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  NPME_HIP_CHECK(e);
+  return ex;
+}
+
+// the CalcV of m set-up handles (charges in each handle's d_qin, V to its
+// d_V) captured as one graph and launched once on stream sb: each handle's
+// stream waits for an event recorded on sb, the handles' launches are
+// recorded, and sb waits for each handle's last launch; the handles' CalcVs
+// then run concurrently on the device
+void handle_calcv_batch(Handle *const *hs, int m, hipStream_t sb, hipEvent_t evStart,
+                        const hipEvent_t *evDone)
+{
+  for (int k = 0; k < m; k++) require_ready(*hs[k]);
+  NPME_HIP_CHECK(hipStreamBeginCapture(sb, hipStreamCaptureModeThreadLocal));
+  hipGraph_t g = nullptr;
+  try {
+    NPME_HIP_CHECK(hipEventRecord(evStart, sb));
+    for (int k = 0; k < m; k++) {
+      Handle &h = *hs[k];
+      NPME_HIP_CHECK(hipStreamWaitEvent(h.stream, evStart, 0));
+      calcv_launches(h, h.d_qin, h.d_V, true);
+      NPME_HIP_CHECK(hipEventRecord(evDone[k], h.stream));
+      NPME_HIP_CHECK(hipStreamWaitEvent(sb, evDone[k], 0));
+    }
+  } catch (...) {
+    if (hipStreamEndCapture(sb, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  const hipError_t ec = hipStreamEndCapture(sb, &g);
+  if (ec != hipSuccess || !g) {
+    // an invalidated capture (HIP invalidates a capture when a
+    // synchronising operation touches its streams, possibly from another
+    // thread): the same launches, issued directly
+    (void)hipGetLastError();
+    for (int k = 0; k < m; k++) {
+      Handle &h = *hs[k];
+      calcv_launches(h, h.d_qin, h.d_V, true);
+    }
+    for (int k = 0; k < m; k++) NPME_HIP_CHECK(hipStreamSynchronize(hs[k]->stream));
+    return;
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  NPME_HIP_CHECK(e);
+  const hipError_t el = hipGraphLaunch(ex, sb);
+  const hipError_t es = el == hipSuccess ? hipStreamSynchronize(sb) : el;
+  (void)hipGraphExecDestroy(ex);
+  NPME_HIP_CHECK(es);
+}
+
+void handle_set_trans_rev(Handle &h, int rev)
+{
+  // G holds DFT(g) of the translated kernel; reversing axis d uses G at
+  // -k_d.  The R2C data of real charges stores k3 >= 0 only: G(k1, k2, -k3)
+  // = conj(G(-k1, -k2, k3)) for the real g of a real kernel.
+  int f = rev & 7;
+  if (!h.complexK && (rev & 4)) f = ((~rev) & 3) | 8;
+  if (f != h.transFlags) drop_graph(h);  // the flags are arguments of a captured launch
+  h.transRev = rev & 7;
+  h.transFlags = f;
+}
+
+void handle_calcv(Handle &h, const double *d_q, double *d_Vout)
+{
+  require_ready(h);
+  if (h.profiling) {
+    calcv_launches(h, d_q, d_Vout, false);
+    return;
+  }
+  // repeated calls with the same pointers on one set-up replay a graph
+  if (d_q == h.graphQ && d_Vout == h.graphV) {
+    h.graphCalls++;
+  } else {
+    drop_graph(h);
+    h.graphQ = d_q;
+    h.graphV = d_Vout;
+    h.graphCalls = 1;
+  }
+  if (h.useGraph && h.graphCalls >= 2) {
+    if (!h.calcvExec) h.calcvExec = capture_calcv(h, d_q, d_Vout);
+    NPME_HIP_CHECK(hipGraphLaunch(h.calcvExec, h.stream));
+  } else {
+    calcv_launches(h, d_q, d_Vout, true);
+  }
+  NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+}
+
 
 void handle_exact(Handle &h, const double *d_q, size_t nt, const long *d_idx, double *d_Vout)
 {

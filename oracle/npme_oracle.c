@@ -916,11 +916,13 @@ int npo_spread_basic(size_t n, const double *q, int isComplex,
   return 0;
 }
 
-void npo_theta_8parity(const long N[3], long Bn, const double *Fd,
-                       const double *Qd, double *Td)
+static void theta_parity(const long N[3], long Bn, const double *Fd, int fullF,
+                         const int rev[3], const double *Qd, double *Td)
 /* src/RecSumV.cpp:766-840 (CalcTheta), src/RecSumQ.cpp:456-522 (C twiddle,
  * CalcQfft), src/RecSumV.cpp:639-707 (compact-F k-space multiply),
- * :709-761 (twiddle accumulate) */
+ * :709-761 (twiddle accumulate); with fullF the k-space multiply of
+ * src/RecSumV.cpp:851-941 (full F[N1][N2][N3] indexed at the signed
+ * frequency 2m+a, negated on the axes with rev set) */
 {
   const cplx *F = (const cplx *) Fd;
   const cplx *Q = (const cplx *) Qd;
@@ -964,12 +966,22 @@ void npo_theta_8parity(const long N[3], long Bn, const double *Fd,
           long m1 = (n1 >= M1 / 2) ? n1 - M1 : n1;
           long m2 = (n2 >= M2 / 2) ? n2 - M2 : n2;
           long t1 = 2 * m1 + a1, t2 = 2 * m2 + a2;
-          long k1 = dft2arr(t1, N[0]), k2 = dft2arr(t2, N[1]);
+          long k1 = dft2arr(fullF && rev[0] ? -t1 : t1, N[0]);
+          long k2 = dft2arr(fullF && rev[1] ? -t2 : t2, N[1]);
           double cl = lam[0][k1] * lam[1][k2] * norm;
           for (long n3 = 0; n3 < M3; n3++) {
             long m3 = (n3 >= M3 / 2) ? n3 - M3 : n3;
-            long t3 = 2 * m3 + a3, k3 = dft2arr(t3, N[2]);
+            long t3 = 2 * m3 + a3;
+            long k3 = dft2arr(t3, N[2]);
             long fi = (labs(t1) * F2 + labs(t2)) * F3 + labs(t3);
+            if (fullF) {
+              /* reversal: nTot = DFT2Array(-mTot) on that axis; the
+               * |lambda|^2 factors follow the same index */
+              const long r1 = dft2arr(rev[0] ? -t1 : t1, N[0]);
+              const long r2 = dft2arr(rev[1] ? -t2 : t2, N[1]);
+              k3 = dft2arr(rev[2] ? -t3 : t3, N[2]);
+              fi = (r1 * N[1] + r2) * N[2] + k3;
+            }
             long idx = c * M3 + n3;
             X[idx] = cl * conj(X[idx]) * F[fi] * lam[2][k3];
           }
@@ -989,6 +1001,102 @@ void npo_theta_8parity(const long N[3], long Bn, const double *Fd,
       }
   free(X);
   for (int d = 0; d < 3; d++) { free(lam[d]); free(C[d]); }
+}
+
+void npo_theta_8parity(const long N[3], long Bn, const double *Fd,
+                       const double *Qd, double *Td)
+{
+  const int none[3] = {0, 0, 0};
+  theta_parity(N, Bn, Fd, 0, none, Qd, Td);
+}
+
+void npo_theta_fullF_reverse(const long N[3], long Bn, const double *Fd,
+                             const double *Qd, const int rev[3], double *Td)
+/* NPME_RSTP_CalcTheta_FullF_ReverseSign (src/RecSumV.cpp:943-1009) with
+ * NPME_RSTP_CalcT_FullF_Contribution_ReverseSign (:851-941) */
+{
+  theta_parity(N, Bn, Fd, 1, rev, Qd, Td);
+}
+
+/* ======================================================================
+ * Two-box (translated) kernel grid: f(r) = func(|r + R0|) on the full
+ * N1 x N2 x N3 grid of the box [-X - del, X + del) per axis, sample n at
+ * s L/N with s = n, or n - N for n >= N/2 (src/RecSumGrid.cpp:1024-1042),
+ * as the sum over the 8 overlapped images with the smooth top-hat weights
+ * (src/RecSumGrid.cpp:45-162, :1193-1231), the image coordinates shifted by
+ * R0 after the weights are taken (:77-100); NPME_RecSumGrid_FullTrans_Func
+ * (src/RecSumGrid.cpp:663-794).  Images of zero weight are skipped (they
+ * add nothing; the reference evaluates them at a dummy coordinate).
+ * Returns -2 when, for the full kernel, min |r + R0| over the grid box,
+ * computed as NPME_CalcMinDistance_R0_r (src/SupportFunctions.cpp:687-718),
+ * is below 1e-6 (the reference exits; it checks whatever func it is given,
+ * this restatement only the singular full kernel), -1 for a sample outside
+ * the extension.
+ * Pinned by: at R0 = 0 its DFT equals the even extension of the compact
+ * grid npo_fgrid_compact; the reversal identity of npo_theta_fullF_reverse;
+ * the box-pair potential against the exact sum.
+ * ==================================================================== */
+double npo_min_distance_r0(const double R0[3], const double l[3])
+{
+  double s = 0;
+  for (int d = 0; d < 3; d++) {
+    const double m = fabs(R0[d]);
+    if (!(m < 0.5 * l[d])) s += (0.5 * l[d] - m) * (0.5 * l[d] - m);
+  }
+  return sqrt(s);
+}
+
+int npo_fgrid_fulltrans(const npo_kernel *k, int part, const long N[3],
+                        const double a[3], const double del[3], const double X[3],
+                        const double R0[3], double *fd)
+{
+  cplx *f = (cplx *) fd;
+  double l[3];
+  for (int d = 0; d < 3; d++) l[d] = 2 * (X[d] + del[d]);
+  /* the check guards the singular full kernel; the long-range part is
+   * smooth everywhere, so it is sampled at any R0 (R0 = 0 included) */
+  if (part == NPO_FULL && npo_min_distance_r0(R0, l) < 1.0e-6) return -2;
+  double *x1[3], *x2[3], *T1[3], *T2[3];
+  int err = 0;
+  for (int d = 0; d < 3; d++) {
+    x1[d] = (double *) malloc(sizeof(double) * N[d]);
+    x2[d] = (double *) malloc(sizeof(double) * N[d]);
+    T1[d] = (double *) malloc(sizeof(double) * N[d]);
+    T2[d] = (double *) malloc(sizeof(double) * N[d]);
+    const double Xmax = X[d] + del[d], dx = l[d] / N[d];
+    for (long i = 0; i < N[d]; i++) x1[d][i] = (double) (i >= N[d] / 2 ? i - N[d] : i) * dx;
+    if (overlap_smooth(N[d], x1[d], X[d], Xmax, a[d], T1[d], T2[d], x2[d])) err = -1;
+    for (long i = 0; i < N[d]; i++) {
+      x1[d][i] += R0[d];
+      x2[d][i] += R0[d];
+    }
+  }
+  if (!err) {
+    long c;
+#pragma omp parallel for schedule(static)
+    for (c = 0; c < N[0] * N[1]; c++) {
+      const long i1 = c / N[1], i2 = c % N[1];
+      for (long i3 = 0; i3 < N[2]; i3++) {
+        cplx s = 0;
+        for (int ax = 0; ax < 2; ax++)
+          for (int ay = 0; ay < 2; ay++)
+            for (int az = 0; az < 2; az++) {
+              const double w = (ax ? T2[0][i1] : T1[0][i1]) * (ay ? T2[1][i2] : T1[1][i2]) *
+                               (az ? T2[2][i3] : T1[2][i3]);
+              if (w == 0.0) continue;
+              const double x = ax ? x2[0][i1] : x1[0][i1];
+              const double y = ay ? x2[1][i2] : x1[1][i2];
+              const double z = az ? x2[2][i3] : x1[2][i3];
+              double f0[2], f1[2];
+              npo_kernel_eval(k, part, x * x + y * y + z * z, f0, f1);
+              s += w * (NPO_IS_COMPLEX(k->kind) ? (f0[0] + I * f0[1]) : f0[0]);
+            }
+        f[c * N[2] + i3] = s;
+      }
+    }
+  }
+  for (int d = 0; d < 3; d++) { free(x1[d]); free(x2[d]); free(T1[d]); free(T2[d]); }
+  return err;
 }
 
 int npo_interp_basic(size_t n, const double *coord, const double *thetad,

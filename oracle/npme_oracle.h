@@ -130,6 +130,20 @@ int npo_spread_basic(size_t n, const double *q, int isComplex,
  * src/RecSumQ.cpp:456-522) */
 void npo_theta_8parity(const long N[3], long Bn, const double *F,
                        const double *Q, double *T);
+/* 8-parity compact convolution with the full Fourier grid F[N1][N2][N3]
+ * and the optional sign reversal per axis (src/RecSumV.cpp:851-1009):
+ * rev[d] != 0 uses F as the grid of R0 with component d negated */
+void npo_theta_fullF_reverse(const long N[3], long Bn, const double *F,
+                             const double *Q, const int rev[3], double *T);
+/* translated kernel grid f(r) = part(|r + R0|) on the full grid
+ * (src/RecSumGrid.cpp:663-794), real space, f[N1][N2][N3] complex;
+ * 0 ok, -1 sample outside the extension, -2 R0 too close (rMin < 1e-6) */
+int npo_fgrid_fulltrans(const npo_kernel *k, int part, const long N[3],
+                        const double a[3], const double del[3], const double X[3],
+                        const double R0[3], double *f);
+/* min |r + R0| over the box of edges l centred at 0
+ * (src/SupportFunctions.cpp:687-718) */
+double npo_min_distance_r0(const double R0[3], const double l[3]);
 /* brute-force interpolation (src/RecSumV.cpp:44-149): V1[n][4] complex */
 int npo_interp_basic(size_t n, const double *coord, const double *theta,
                      const long N[3], const double L[3], long Bn,

@@ -78,6 +78,10 @@ def lib():
         L.npo_spread_basic.argtypes = [sz, p, i, p, p, p, l, p, p]
         L.npo_spline_start.argtypes = [sz, p, p, p, l, p, p]
         L.npo_theta_8parity.argtypes = [p, l, p, p, p]
+        L.npo_theta_fullF_reverse.argtypes = [p, l, p, p, p, p]
+        L.npo_fgrid_fulltrans.argtypes = [C.POINTER(Kernel), i, p, p, p, p, p, p]
+        L.npo_min_distance_r0.restype = d
+        L.npo_min_distance_r0.argtypes = [p, p]
         L.npo_interp_basic.argtypes = [sz, p, p, p, p, l, p, p]
         L.npo_direct_sr.argtypes = [C.POINTER(Kernel), sz, p, p, p]
         L.npo_exact.argtypes = [C.POINTER(Kernel), sz, p, p, sz, p, p]
@@ -293,6 +297,62 @@ def theta_8parity(N, Bn, F, Q):
     T = np.zeros_like(Q)
     lib().npo_theta_8parity(_ptr(N), int(Bn), _ptr(F), _ptr(Q), _ptr(T))
     return T
+
+
+def theta_fullF_reverse(N, Bn, F, Q, rev=(0, 0, 0)):
+    """8-parity theta with the full Fourier grid F[N1][N2][N3] and per-axis
+    sign reversal (src/RecSumV.cpp:851-1009)"""
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    F = np.ascontiguousarray(F, dtype=np.complex128)
+    Q = np.ascontiguousarray(Q, dtype=np.complex128)
+    rv = np.ascontiguousarray([int(bool(r)) for r in rev], dtype=np.int32)
+    T = np.zeros_like(Q)
+    lib().npo_theta_fullF_reverse(_ptr(N), int(Bn), _ptr(F), _ptr(Q), _ptr(rv), _ptr(T))
+    return T
+
+
+def fgrid_fulltrans(k, N, a, dl, X, R0, part=FULL):
+    """real-space translated kernel grid f(r) = part(|r + R0|), [N1][N2][N3]
+    complex (src/RecSumGrid.cpp:663-794)"""
+    N = np.ascontiguousarray(N, dtype=np.int64)
+    f = np.zeros(tuple(int(x) for x in N), np.complex128)
+    rc = lib().npo_fgrid_fulltrans(C.byref(k), int(part), _ptr(N), _ptr(_f64(a)), _ptr(_f64(dl)),
+                                   _ptr(_f64(X)), _ptr(_f64(R0)), _ptr(f))
+    if rc == -2:
+        raise ValueError("box-box separation R0 is too small")
+    if rc != 0:
+        raise ValueError("translated kernel grid failed")
+    return f
+
+
+def min_distance_r0(R0, l):
+    return lib().npo_min_distance_r0(_ptr(_f64(R0)), _ptr(_f64(l)))
+
+
+def boxpair(k, N, Bn, coordT, coordS, qS, dl, a=(4.0, 4.0, 4.0), rev=(0, 0, 0), part=FULL):
+    """V at the targets from the sources through the translated full grid:
+    both sets in one local frame (each set relative to the centre of its own
+    bounding box), the kernel grid of R0 = cT - cS (reversed per rev),
+    spread -> theta_fullF_reverse -> interpolate.  V[nT][4] complex.
+
+    The grid index of a charge runs against its position (the window start
+    is -floor(w), src/RecSumSupportFunctions.h:123-127), so the correlation
+    over grid indices that theta computes, sum_n Q(n) f(n - p), evaluates f
+    at the target-minus-source separation: V_t = sum_s q_s func(|t - s +
+    R0|) in local coordinates, which is the global separation for
+    R0 = cT - cS."""
+    coordT, coordS = _f64(coordT), _f64(coordS)
+    _, cT = box_dim_center(coordT)
+    _, cS = box_dim_center(coordS)
+    lt, ls = coordT - cT, coordS - cS
+    X0, R0c = box_dim_center(np.vstack([lt, ls]))
+    X, R = pme_corr_box(X0, R0c, N, dl, Bn)
+    L = 2 * (X + np.asarray(dl, float))
+    f = fgrid_fulltrans(k, N, a, dl, X, cT - cS, part)
+    F = fft3d(f, -1)
+    Q = spread_basic(qS, ls, N, L, Bn, R)
+    T = theta_fullF_reverse(N, Bn, F, Q, rev)
+    return interp_basic(lt, T, N, L, Bn, R)
 
 
 def interp_basic(coord, theta, N, L, Bn, R):

@@ -280,7 +280,12 @@ def test_direct_sum_paths_agree(oracle, prec, n, box, nNeigh, nClust):
     """The neighbour-list and the cell-row direct sums give the same V.  The
     cases build the list with each walk: a wavefront per target (up to
     32,768 charges), a wavefront per cell (above), both also with the
-    widest stencil they take (nNeigh 3), and a thread per target (nNeigh 4)."""
+    widest stencil they take (nNeigh 3), and a thread per target (nNeigh 4).
+    In FP64 the two sums agree to rounding.  In the mixed form each is
+    checked against the FP64 path: the two FP32 sums round differently, so
+    their mutual difference is up to the sum of their errors (measured at
+    60000 charges in 84^3: list 1.8e-7 / 1.22e-6, cell 1.8e-7 / 1.02e-6 in
+    V / grad V, difference 7e-8 / 1.25e-6)."""
     print("case direct paths", n, prec, nNeigh, flush=True)
     coord, q = uniform_box(n, box)
     kw = dict(nNeigh=nNeigh, nCellClust1D=nClust)
@@ -288,8 +293,14 @@ def test_direct_sum_paths_agree(oracle, prec, n, box, nNeigh, nClust):
     hc = _real_setup(coord, q, prec, Rdir=7.0, neighborList="off", **kw)
     assert "neighbour list" in hl.Print() and "cell rows" in hc.Print()
     Vl, Vc = hl.calcv(q), hc.calcv(q)
-    m = oracle.compare_v(Vl, Vc)
-    assert m["relV"] < (1e-13 if prec == "fp64" else 2e-7) and m["reldV"] < (1e-13 if prec == "fp64" else 1e-6), m
+    if prec == "fp64":
+        m = oracle.compare_v(Vl, Vc)
+        assert m["relV"] < 1e-13 and m["reldV"] < 1e-13, m
+        return
+    ref = _real_setup(coord, q, "fp64", Rdir=7.0, neighborList="on", **kw).calcv(q)
+    for V in (Vl, Vc):
+        m = oracle.compare_v(ref, V)
+        assert m["relV"] < 5e-7 and m["reldV"] < 1.5e-6, m
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])

@@ -166,3 +166,62 @@ def test_server_request_equals_library_call():
             assert np.array_equal(V, h.calcv(q))
     finally:
         S.close()
+
+
+def test_server_reservation_follows_footprint():
+    """Admission by each request's own footprint: once its grid is chosen a
+    request reserves the grids, per-charge arrays and neighbour list at its
+    density, and after set-up what the set-up holds.  The reservation is an
+    upper estimate of the resident bytes (within a factor 4 for uniform
+    sets); a dense clustered request reserves more per charge than a
+    uniform one of the same size; under a budget smaller than two requests
+    every request still completes with the result it has alone."""
+    import npme_amd as N
+    rng = np.random.default_rng(41)
+    reqs = _requests([4000, 20000, 60000, 9000])
+    # the same charge count packed 8x denser (a bigger neighbour list)
+    n = 20000
+    L = (n / 0.8) ** (1.0 / 3.0)
+    dense = ("Laplace", rng.uniform(0.0, L, (n, 3)), rng.uniform(-1, 1, n), 1e-6, {})
+    print("case server footprint", flush=True)
+    S = N.NPME_Server(nSlots=2)
+    out = _run(S, reqs + [dense])
+    S.close()
+    for (ft, c, q, tol, kw), (V, inf) in zip(reqs + [dense], out):
+        print(ft, len(q), "reserved %.4f GB resident %.4f GB" % (inf["reservedGB"], inf["deviceGB"]))
+        assert inf["deviceGB"] <= inf["reservedGB"] * 1.25 + 0.01
+        assert inf["reservedGB"] <= 4 * inf["deviceGB"] + 0.02
+    per = [inf["reservedGB"] / len(q) for (_, _, q, _, _), (_, inf) in zip(reqs + [dense], out)]
+    assert per[-1] > 2 * per[1]  # dense 20k against uniform 20k (same kernel and tol)
+    # a budget below two requests' footprints: requests wait for memory,
+    # none deadlocks, and every result equals the unconstrained one
+    tight = max(inf["reservedGB"] for _, inf in out) * 1.5
+    S2 = N.NPME_Server(nSlots=3, memGB=tight)
+    out2 = _run(S2, reqs + [dense])
+    S2.close()
+    for (V, _), (V2, _) in zip(out, out2):
+        assert np.array_equal(V, V2)
+
+
+def test_server_batches_equal_alone():
+    """Small requests queued together run as batches (their set-ups in turn,
+    their CalcVs as one graph launch): with one slot and a burst of 20 small
+    requests of every kernel, some run in batches of several, and every
+    result is bit-identical to the same request run with batching off."""
+    import npme_amd as N
+    sizes = [600, 900, 1500, 2500, 4000, 700, 1200, 3000, 800, 2000,
+             1000, 1800, 650, 2200, 3500, 900, 1100, 2800, 750, 1600]
+    reqs = _requests(sizes, seed=53)
+    print("case server batches: %d requests on 1 slot" % len(reqs), flush=True)
+    S = N.NPME_Server(nSlots=1, maxBatch=8, batchMaxN=32768)
+    out = _run(S, reqs)
+    S.close()
+    sizes_seen = [inf["batch"] for _, inf in out]
+    print("batch sizes", sizes_seen)
+    assert max(sizes_seen) > 1
+    S1 = N.NPME_Server(nSlots=1, maxBatch=1)
+    alone = _run(S1, reqs)
+    S1.close()
+    assert all(inf["batch"] == 1 for _, inf in alone)
+    for (V, _), (Va, _), (ft, _, _, _, _) in zip(out, alone, reqs):
+        assert np.array_equal(V, Va), ft
