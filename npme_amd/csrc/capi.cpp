@@ -499,7 +499,11 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
      << H.setupMs[2] << ", list " << H.setupMs[3] << ", charges " << H.setupMs[4] << ", grid "
      << H.setupMs[5] << "\n";
   os << "  kernel grid G: " << (H.gReused ? "reused from the previous set-up" : "computed") << "\n";
-  if (H.useNlist)
+  if (H.trans)
+    os << "  direct sum: none (box pair on the translated kernel grid)\n";
+  else if (H.ker.kind == KIND_LAPLACE_ERF)
+    os << "  direct sum: the reference's interacting cell pairs, every point pair (FP64)\n";
+  else if (H.useNlist)
     os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";
   else
     os << "  direct sum: cell rows, " << H.nSegments << " segments\n";

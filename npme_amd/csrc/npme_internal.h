@@ -333,6 +333,7 @@ struct Handle {
   double *d_qin = nullptr;       // [n] (x2) input charges staging
   double *d_coordS = nullptr;    // [n][3] FP64 input coordinates, input order (exact sums)
   double4 *d_xs = nullptr;       // [n] FP64 coordinates in sorted order (pair search)
+  unsigned *d_cellFlags = nullptr;  // LaplaceOrig: interacting (cell, stencil offset) bits
   FFTPlan1D plan[3];
   size_t deviceBytes = 0;
   bool deviceSetup = false;      // per-charge set-up ran on the device
@@ -506,6 +507,14 @@ struct DirectParams {
   int nlEshift = 4;
 };
 double handle_count_pairs(Handle &h);
+// LaplaceOrig's direct sum over the reference's interacting cell pairs (no
+// cutoff): bit words per cell, the flags, and the FP64 sum into Vdir
+int refpair_words(int nNeigh);
+void launch_cellpair_contact(const DirectParams &pp, const int *cellStart, const double4 *xs,
+                             unsigned *flags, hipStream_t s);
+void launch_direct_refpairs(const DirectParams &pp, size_t n, const int4 *cell, const int *cellStart,
+                            const double4 *xs, const int *perm, const double *q,
+                            const unsigned *flags, double *Vdir, hipStream_t s);
 // user-defined kernels, host side (genfunc.cpp)
 void user_fself(const UserKernels &u, double fself[2]);
 void user_check_radial(const UserKernels &u, double Rdir);

@@ -1347,6 +1347,131 @@ void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const
   NPME_HIP_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------- LaplaceOrig: the reference's pairs
+// The reference's erfc direct sum has no cutoff of its own: it adds every
+// point pair of the cell pairs of its interaction list
+// (src/PotentialLaplace.cpp:267-318, :613-668 with the list of
+// NPME_FindInteractionList_C, src/PartitionBox.cpp:789-922).  A cell pair
+// (c, c') is listed when c' = c, or when c' is in the +-nNeigh stencil of c,
+// the two cell boxes are closer than 1.0001 Rdir (from the cells' integer
+// offsets: sum_d max(0, |dn_d| - 1)^2 <= nNeigh^2) and some point pair of
+// the two cells is closer than Rdir (squared distance below 1.0001 Rdir^2,
+// Rdir = nNeigh * cell edge).  k_cellpair_contact sets one bit per (cell,
+// stencil offset) for the listed pairs; k_direct_refpairs sums, per target,
+// every source of the flagged cells in FP64.
+__global__ __launch_bounds__(256) void k_cellpair_contact(DirectDev p, long ncell, int W, double R2,
+                                                          const int2 *__restrict__ cellSE,
+                                                          const double4 *__restrict__ xs,
+                                                          unsigned *__restrict__ flags)
+{
+  const int nN = p.nNeigh, S = 2 * nN + 1, nOff = S * S * S;
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= ncell * nOff) return;
+  const long c = id / nOff;
+  const int o = (int)(id - c * nOff);
+  const int2 se = cellSE[c];
+  if (se.y == 0) return;
+  const int cz = (int)(c % p.nb2), cy = (int)((c / p.nb2) % p.nb1), cx = (int)(c / ((long)p.nb1 * p.nb2));
+  const int dx = o / (S * S) - nN, dy = (o / S) % S - nN, dz = o % S - nN;
+  const int gx = cx + dx, gy = cy + dy, gz = cz + dz;
+  if (gx < 0 || gy < 0 || gz < 0 || gx >= p.nb0 || gy >= p.nb1 || gz >= p.nb2) return;
+  const int ex = max(abs(dx) - 1, 0), ey = max(abs(dy) - 1, 0), ez = max(abs(dz) - 1, 0);
+  if (ex * ex + ey * ey + ez * ez > nN * nN) return;
+  const long c2 = ((long)gx * p.nb1 + gy) * p.nb2 + gz;
+  const int2 se2 = cellSE[c2];
+  if (se2.y == 0) return;
+  bool on = c2 == c;
+  for (int i = se.x; i < se.x + se.y && !on; i++) {
+    const double4 a = xs[i];
+    for (int j = se2.x; j < se2.x + se2.y; j++) {
+      const double4 b = xs[j];
+      const double rx = a.x - b.x, ry = a.y - b.y, rz = a.z - b.z;
+      if (rx * rx + ry * ry + rz * rz < R2) {
+        on = true;
+        break;
+      }
+    }
+  }
+  if (on) atomicOr(&flags[c * W + (o >> 5)], 1u << (o & 31));
+}
+
+__global__ __launch_bounds__(256) void k_direct_refpairs(DirectDev p, size_t n, int W,
+                                                         const int4 *__restrict__ cell,
+                                                         const int2 *__restrict__ cellSE,
+                                                         const double4 *__restrict__ xs,
+                                                         const int *__restrict__ perm,
+                                                         const double *__restrict__ q,
+                                                         const unsigned *__restrict__ flags,
+                                                         double *__restrict__ Vdir)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int nN = p.nNeigh, S = 2 * nN + 1, nOff = S * S * S;
+  const int4 c = cell[s];
+  const long cl = ((long)c.x * p.nb1 + c.y) * p.nb2 + c.z;
+  const double4 me = xs[s];
+  double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int o = 0; o < nOff; o++) {
+    if (!((flags[cl * W + (o >> 5)] >> (o & 31)) & 1u)) continue;
+    const int gx = c.x + o / (S * S) - nN, gy = c.y + (o / S) % S - nN, gz = c.z + o % S - nN;
+    const int2 se = cellSE[((long)gx * p.nb1 + gy) * p.nb2 + gz];
+    for (int j = se.x; j < se.x + se.y; j++) {
+      if ((size_t)j == s) continue;
+      const double4 b = xs[j];
+      const double rx = me.x - b.x, ry = me.y - b.y, rz = me.z - b.z;
+      pair_sr<double, KIND_LAPLACE_ERF, 8>(p, rx, ry, rz, rx * rx + ry * ry + rz * rz, q[perm[j]],
+                                           0.0, part, 0.0, 0.0);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) Vdir[4 * s + k] = part[k];
+}
+
+static DirectDev refpair_dev(const DirectParams &pp)
+{
+  DirectDev d{};
+  d.kind = KIND_LAPLACE_ERF;
+  d.nNeigh = pp.nNeigh;
+  d.Rdir = pp.Rdir;
+  d.beta = pp.beta;
+  d.nb0 = pp.nb[0];
+  d.nb1 = pp.nb[1];
+  d.nb2 = pp.nb[2];
+  return d;
+}
+
+int refpair_words(int nNeigh)
+{
+  const int S = 2 * nNeigh + 1;
+  return (S * S * S + 31) / 32;
+}
+
+void launch_cellpair_contact(const DirectParams &pp, const int *cellStart, const double4 *xs,
+                             unsigned *flags, hipStream_t s)
+{
+  const DirectDev d = refpair_dev(pp);
+  const long ncell = (long)pp.nb[0] * pp.nb[1] * pp.nb[2];
+  const int S = 2 * pp.nNeigh + 1, W = refpair_words(pp.nNeigh);
+  // the reference's test: Rdir recomputed as nNeigh cell edges, squared
+  // distance below 1.0001 Rdir^2 (src/PartitionBox.cpp:767-787, :841-842)
+  const double cs = pp.Rdir / pp.nNeigh, Rr = pp.nNeigh * cs, R2 = Rr * Rr * 1.0001;
+  NPME_HIP_CHECK(hipMemsetAsync(flags, 0, (size_t)ncell * W * sizeof(unsigned), s));
+  const long tot = ncell * S * S * S;
+  hipLaunchKernelGGL(k_cellpair_contact, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, d, ncell, W,
+                     R2, (const int2 *)cellStart, xs, flags);
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
+void launch_direct_refpairs(const DirectParams &pp, size_t n, const int4 *cell, const int *cellStart,
+                            const double4 *xs, const int *perm, const double *q,
+                            const unsigned *flags, double *Vdir, hipStream_t s)
+{
+  const DirectDev d = refpair_dev(pp);
+  hipLaunchKernelGGL(k_direct_refpairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, n,
+                     refpair_words(pp.nNeigh), cell, (const int2 *)cellStart, xs, perm, q, flags, Vdir);
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ B-splines
 // M_n(t + j), j = 0..n-1 (Cox-de Boor over all shifts), n <= 16.
 // The recursion always runs in FP64 and the weights are rounded once to the

@@ -64,7 +64,7 @@ void handle_free_device(Handle &h, bool keepG)
                   h.d_sliceStart, h.d_sliceLen, h.d_qi, h.d_cell, h.d_cellStart, h.d_ms, h.d_frac,
                   h.d_tileStart, h.d_tileList, h.d_grid, h.d_buf1, h.d_buf2,
                   h.d_tables, h.d_Vdir, h.d_V, h.d_qin, h.d_coordS, h.d_xs,
-                  h.d_qs, h.d_qsi, h.d_utab};
+                  h.d_qs, h.d_qsi, h.d_utab, h.d_cellFlags};
   // queued work of this handle may still read its buffers
   if (h.stream) (void)hipStreamSynchronize(h.stream);
   if (h.stream2) (void)hipStreamSynchronize(h.stream2);
@@ -96,6 +96,7 @@ void handle_free_device(Handle &h, bool keepG)
   h.d_Vdir = h.d_V = h.d_qin = h.d_coordS = nullptr;
   h.d_xs = nullptr;
   h.d_utab = nullptr;
+  h.d_cellFlags = nullptr;
   h.coordStaged = h.stagedBoundsReady = false;
   h.deviceBytes = h.d_G ? h.gBytes : 0;
 }
@@ -1117,6 +1118,8 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
     kw_error("neighborList must be 'auto', 'on' or 'off'");
   if (h.trans) nlmode = "off";  // no direct sum between the boxes
+  // LaplaceOrig sums the reference's cell pairs, not a cutoff list
+  if (kind == KIND_LAPLACE_ERF) nlmode = "off";
   const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
   const bool devSetup = smode == "device";
   const bool staged = devSetup && h.coordStaged && h.stagedN == n;
@@ -1235,6 +1238,12 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
       NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before f is released
     }
   }
+  if (h.ker.kind == KIND_LAPLACE_ERF) {
+    const size_t ncell = (size_t)h.nb[0] * h.nb[1] * h.nb[2];
+    h.d_cellFlags = dalloc<unsigned>(h, ncell * refpair_words((int)h.nNeigh));
+    launch_cellpair_contact(direct_params(h), h.d_cellStart, h.d_xs, h.d_cellFlags, h.stream);
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
+  }
   grid_setup(h);
   if (h.trans) {
     // the interpolation adds the (absent) direct sum: zero, once
@@ -1339,9 +1348,13 @@ static void calcv_launches(Handle &h, const double *d_q, double *d_Vout, bool sp
   hipStream_t s2 = split ? h.stream2 : s;
   const DirectParams dp = direct_params(h);
   const bool doDirect = !h.trans;  // a box pair has no direct sum
+  const bool refPairs = h.ker.kind == KIND_LAPLACE_ERF;
   auto direct = [&] {
     launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s2);
-    if (h.useNlist)
+    if (refPairs)
+      launch_direct_refpairs(dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, h.d_perm, d_q, h.d_cellFlags,
+                             h.d_Vdir, s2);
+    else if (h.useNlist)
       launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
                        h.d_nlist, h.csA, h.d_Vdir, s2);
     else
@@ -1360,7 +1373,10 @@ static void calcv_launches(Handle &h, const double *d_q, double *d_Vout, bool sp
     // the same kernels in order, timed one by one: gather, direct
     launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
     mark(h, st++);
-    if (h.useNlist)
+    if (refPairs)
+      launch_direct_refpairs(dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, h.d_perm, d_q, h.d_cellFlags,
+                             h.d_Vdir, s);
+    else if (h.useNlist)
       launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
                        h.d_nlist, h.csA, h.d_Vdir, s);
     else

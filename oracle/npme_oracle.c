@@ -1246,6 +1246,130 @@ void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
   ugrid_free(&g);
 }
 
+/* ======================================================================
+ * LaplaceOrig's direct sum over the reference's pair set.
+ * NPME_PotLaplace_SR_Original_DirectSum_V1 (src/PotentialLaplace.cpp:
+ * 320-380, pair/self :613-700) adds erfc(beta r)/r for EVERY point pair of
+ * the cell pairs its interaction list keeps, with no cutoff of its own.
+ * The list (NPME_FindInteractionList_C, src/PartitionBox.cpp:789-922, type
+ * 'C' of src/Constant.h:61) over the B cells of the embedded partition
+ * (edge Rdir/nNeigh, src/PartitionEmbeddedBox.cpp:47-203) keeps each
+ * occupied cell with itself, and a pair of occupied cells in each other's
+ * +-nNeigh stencil when the cell boxes are closer than 1.0001 Rdir
+ * (NPME_CalcMinDistanceRectVolumes of the cell centres) and some point pair
+ * of the two cells has a squared distance below 1.0001 Rdir^2, with Rdir
+ * recomputed as nNeigh cell edges (:767-787, :841-842).  Cells are indexed
+ * as npo_cell_index does.
+ * Pinned by: a brute-force restatement of the same pair rule in numpy on
+ * small inputs (tests), and: the pair set contains every pair closer than
+ * Rdir, so the sum minus the cutoff sum is the tail beyond Rdir.
+ * ==================================================================== */
+typedef struct { long cell; size_t i; } npo_cellpt;
+
+static int cellpt_cmp(const void *a, const void *b)
+{
+  const npo_cellpt *x = (const npo_cellpt *) a, *y = (const npo_cellpt *) b;
+  if (x->cell != y->cell) return x->cell < y->cell ? -1 : 1;
+  return x->i < y->i ? -1 : (x->i > y->i);
+}
+
+static size_t cell_lower(const npo_cellpt *v, size_t n, long c)
+{
+  size_t lo = 0, hi = n;
+  while (lo < hi) {
+    size_t m = (lo + hi) / 2;
+    if (v[m].cell < c) lo = m + 1; else hi = m;
+  }
+  return lo;
+}
+
+int npo_direct_erfc_refpairs(const npo_kernel *k, size_t n, const double *coord,
+                             const double *q, long nNeigh, long nCellClust1D,
+                             double *V)
+{
+  const double csB = k->Rdir / nNeigh;
+  const unsigned long nP = (unsigned long) nCellClust1D;
+  const double csA = nP * csB;
+  double mn[3], mx[3];
+  for (int d = 0; d < 3; d++) mn[d] = mx[d] = coord[d];
+  for (size_t i = 1; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      const double x = coord[3 * i + d];
+      if (mn[d] > x) mn[d] = x;
+      if (mx[d] < x) mx[d] = x;
+    }
+  long nb[3];
+  unsigned long nA[3];
+  for (int d = 0; d < 3; d++) {
+    nA[d] = (unsigned long) ((mx[d] - mn[d]) / csA) + 1;
+    nb[d] = (long) (nA[d] * nP);
+  }
+  long *g = (long *) malloc(sizeof(long) * 3 * n);
+  npo_cellpt *v = (npo_cellpt *) malloc(sizeof(npo_cellpt) * n);
+  for (size_t i = 0; i < n; i++) {
+    for (int d = 0; d < 3; d++) {
+      unsigned long a, b;
+      const double x1 = coord[3 * i + d] + (-mn[d]);
+      if (coord2cell(x1, csA, nA[d], &a) || coord2cell(x1 - a * csA, csB, nP, &b)) {
+        free(g); free(v);
+        return -1;
+      }
+      g[3 * i + d] = (long) (a * nP + b);
+    }
+    v[i].cell = (g[3 * i] * nb[1] + g[3 * i + 1]) * nb[2] + g[3 * i + 2];
+    v[i].i = i;
+  }
+  qsort(v, n, sizeof(npo_cellpt), cellpt_cmp);
+  const double Rr = nNeigh * csB, R2 = Rr * Rr * 1.0001;
+  const long nN = nNeigh;
+  long ii;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (ii = 0; ii < (long) n; ii++) {
+    const size_t i = (size_t) ii;
+    const long *gi = &g[3 * i];
+    const long ci = (gi[0] * nb[1] + gi[1]) * nb[2] + gi[2];
+    const size_t a0 = cell_lower(v, n, ci), a1 = cell_lower(v, n, ci + 1);
+    double acc[4] = {0, 0, 0, 0};
+    for (long dx = -nN; dx <= nN; dx++)
+      for (long dy = -nN; dy <= nN; dy++)
+        for (long dz = -nN; dz <= nN; dz++) {
+          const long gx = gi[0] + dx, gy = gi[1] + dy, gz = gi[2] + dz;
+          if (gx < 0 || gy < 0 || gz < 0 || gx >= nb[0] || gy >= nb[1] || gz >= nb[2]) continue;
+          const long ex = labs(dx) > 1 ? labs(dx) - 1 : 0, ey = labs(dy) > 1 ? labs(dy) - 1 : 0,
+                     ez = labs(dz) > 1 ? labs(dz) - 1 : 0;
+          if (sqrt((double) (ex * ex + ey * ey + ez * ez)) * csB >= Rr * 1.0001) continue;
+          const long c2 = (gx * nb[1] + gy) * nb[2] + gz;
+          const size_t b0 = cell_lower(v, n, c2), b1 = cell_lower(v, n, c2 + 1);
+          if (b0 == b1) continue;
+          int on = c2 == ci;
+          for (size_t s = a0; s < a1 && !on; s++)
+            for (size_t t = b0; t < b1; t++) {
+              const double *p1 = &coord[3 * v[s].i], *p2 = &coord[3 * v[t].i];
+              const double rx = p1[0] - p2[0], ry = p1[1] - p2[1], rz = p1[2] - p2[2];
+              if (rx * rx + ry * ry + rz * rz < R2) { on = 1; break; }
+            }
+          if (!on) continue;
+          for (size_t t = b0; t < b1; t++) {
+            const size_t j = v[t].i;
+            if (j == i) continue;
+            const double rx = coord[3 * i] - coord[3 * j], ry = coord[3 * i + 1] - coord[3 * j + 1],
+                         rz = coord[3 * i + 2] - coord[3 * j + 2];
+            const double r2 = rx * rx + ry * ry + rz * rz, r = sqrt(r2), x = k->beta * r;
+            const double e = erfc(x) / r;
+            const double f1 = -(k->beta * 2.0 / sqrt(PI) * exp(-x * x) + e) / r2;
+            acc[0] += e * q[j];
+            acc[1] += rx * f1 * q[j];
+            acc[2] += ry * f1 * q[j];
+            acc[3] += rz * f1 * q[j];
+          }
+        }
+    for (int c = 0; c < 4; c++) V[4 * i + c] = acc[c];
+  }
+  free(g);
+  free(v);
+  return 0;
+}
+
 void npo_exact(const npo_kernel *k, size_t n, const double *coord,
                const double *q, size_t nt, const long *idx, double *V)
 {

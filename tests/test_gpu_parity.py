@@ -127,16 +127,23 @@ def test_ralpha_anisotropic(oracle, prec):
 @pytest.mark.parametrize("compactF", [1, 0])
 def test_laplace_orig_split(oracle, prec, compactF):
     """EwaldSplit LaplaceOrig (erf/erfc split, beta from Rdir and tol) on the
-    RSEM grid: against the oracle's erfc-split CalcV (same beta, same grid)
-    and against the exact sum, which it meets less well than the derivative
-    match on that grid (as in the reference).  useCompactF 0 gives the same
-    result as 1 (the full F of a radial kernel is the compact F extended)."""
+    RSEM grid: against the oracle's NPME_InterfaceReal_Laplace_Original
+    CalcV (same beta, same grid, and the reference's direct-sum pair set:
+    every point pair of the interacting cell pairs, no cutoff) and against
+    the exact sum, which it meets less well than the derivative match on
+    that grid (as in the reference).  The direct sum runs in FP64 in both
+    forms.  useCompactF 0 gives the same result as 1 (the full F of a radial
+    kernel is the compact F extended)."""
     coord, q = uniform_box(3000, 30.0)
     h = _real_setup(coord, q, prec, Rdir=8.0, EwaldSplit="LaplaceOrig", useCompactF=compactF)
     V = h.calcv(q)
-    assert "LaplaceOrig" in h.Print()
+    assert "LaplaceOrig" in h.Print() and "interacting cell pairs" in h.Print()
     k = oracle.kernel_erf(8.0, oracle.rdir2beta(8.0, 1e-6))
-    Vo = oracle.calcv(k, _grid(h), 6, coord, q)
+    Vo = oracle.calcv_laplace_orig(k, _grid(h), 6, coord, q, nNeigh=2, nCellClust1D=3)
+    # the reference's pair set differs from a cutoff at Rdir by the tail of
+    # the kept pairs beyond Rdir
+    tail = oracle.compare_v(oracle.calcv(k, _grid(h), 6, coord, q), Vo)["relV"]
+    assert tail > 1e-9
     mo = oracle.compare_v(Vo, V)
     me = oracle.compare_v(oracle.exact(oracle.kernel(oracle.LAPLACE, 6, 8.0), coord, q), V)
     print(prec, compactF, "vs oracle", mo, "vs exact", me)
