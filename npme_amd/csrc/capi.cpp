@@ -501,7 +501,7 @@ int npme_print(const npme_handle *h, char *buf, size_t buflen)
   os << "  kernel grid G: " << (H.gReused ? "reused from the previous set-up" : "computed") << "\n";
   if (H.trans)
     os << "  direct sum: none (box pair on the translated kernel grid)\n";
-  else if (H.ker.kind == KIND_LAPLACE_ERF)
+  else if (kind_ref_pairs(H.ker.kind))
     os << "  direct sum: the reference's interacting cell pairs, every point pair (FP64)\n";
   else if (H.useNlist)
     os << "  direct sum: neighbour list, " << H.nlistEntries << " entries\n";

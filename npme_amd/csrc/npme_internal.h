@@ -271,6 +271,7 @@ struct Handle {
   void *d_utab = nullptr;
   int utS = 0;
   double utErr[2] = {0, 0};
+  double utR = 0;  // the table's radius: the longest pair of interacting cells
   std::vector<double> coordHost;
   bool complexK = false;
   bool fp64 = false;
@@ -502,6 +503,7 @@ struct DirectParams {
   double fxInv;   // cluster edge / 2^fxShift
   const void *utab = nullptr;  // user kernels: short-range table, S pieces per octave
   int utS = 0;
+  double utR = 0;              // its radius (r / utR in (0, 1])
   // neighbour-list entries are source byte offsets: log2 of the source
   // element size (5 for FP64 double4, 4 for the mixed form's 16 bytes)
   int nlEshift = 4;
@@ -515,6 +517,9 @@ void launch_cellpair_contact(const DirectParams &pp, const int *cellStart, const
 void launch_direct_refpairs(const DirectParams &pp, size_t n, const int4 *cell, const int *cellStart,
                             const double4 *xs, const int *perm, const double *q,
                             const unsigned *flags, double *Vdir, hipStream_t s);
+// kernels whose direct sum takes the reference's pair set (no cutoff):
+// LaplaceOrig's erfc and the user-defined short-range parts
+inline bool kind_ref_pairs(int kind) { return kind == KIND_LAPLACE_ERF || kind == KIND_USER || kind == KIND_USER_CX; }
 // user-defined kernels, host side (genfunc.cpp)
 void user_fself(const UserKernels &u, double fself[2]);
 void user_check_radial(const UserKernels &u, double Rdir);

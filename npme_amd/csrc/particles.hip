@@ -1347,10 +1347,11 @@ void launch_count_pairs(const DirectParams &p, size_t n, const int4 *cell, const
   NPME_HIP_CHECK(hipGetLastError());
 }
 
-// ------------------------------------------- LaplaceOrig: the reference's pairs
-// The reference's erfc direct sum has no cutoff of its own: it adds every
-// point pair of the cell pairs of its interaction list
-// (src/PotentialLaplace.cpp:267-318, :613-668 with the list of
+// ------------------------------- LaplaceOrig, user kernels: the reference's pairs
+// The reference's erfc direct sum and its GenFunc direct sum (funcSR.Calc)
+// have no cutoff of their own: they add every point pair of the cell pairs
+// of its interaction list (src/PotentialLaplace.cpp:267-318, :613-668,
+// src/PotentialGenFunc.cpp:140-320, :523-645, with the list of
 // NPME_FindInteractionList_C, src/PartitionBox.cpp:789-922).  A cell pair
 // (c, c') is listed when c' = c, or when c' is in the +-nNeigh stencil of c,
 // the two cell boxes are closer than 1.0001 Rdir (from the cells' integer
@@ -1395,6 +1396,7 @@ __global__ __launch_bounds__(256) void k_cellpair_contact(DirectDev p, long ncel
   if (on) atomicOr(&flags[c * W + (o >> 5)], 1u << (o & 31));
 }
 
+template <int KIND>
 __global__ __launch_bounds__(256) void k_direct_refpairs(DirectDev p, size_t n, int W,
                                                          const int4 *__restrict__ cell,
                                                          const int2 *__restrict__ cellSE,
@@ -1410,6 +1412,7 @@ __global__ __launch_bounds__(256) void k_direct_refpairs(DirectDev p, size_t n, 
   const int4 c = cell[s];
   const long cl = ((long)c.x * p.nb1 + c.y) * p.nb2 + c.z;
   const double4 me = xs[s];
+  constexpr bool CX = kind_is_complex(KIND);
   double part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int o = 0; o < nOff; o++) {
     if (!((flags[cl * W + (o >> 5)] >> (o & 31)) & 1u)) continue;
@@ -1419,24 +1422,29 @@ __global__ __launch_bounds__(256) void k_direct_refpairs(DirectDev p, size_t n, 
       if ((size_t)j == s) continue;
       const double4 b = xs[j];
       const double rx = me.x - b.x, ry = me.y - b.y, rz = me.z - b.z;
-      pair_sr<double, KIND_LAPLACE_ERF, 8>(p, rx, ry, rz, rx * rx + ry * ry + rz * rz, q[perm[j]],
-                                           0.0, part, 0.0, 0.0);
+      const size_t oj = (size_t)perm[j];
+      const double qr = CX ? q[2 * oj] : q[oj], qim = CX ? q[2 * oj + 1] : 0.0;
+      pair_sr<double, KIND, 8>(p, rx, ry, rz, rx * rx + ry * ry + rz * rz, qr, qim, part, 0.0, 0.0);
     }
   }
 #pragma unroll
-  for (int k = 0; k < 4; k++) Vdir[4 * s + k] = part[k];
+  for (int k = 0; k < (CX ? 8 : 4); k++) Vdir[(CX ? 8 : 4) * s + k] = part[k];
 }
 
 static DirectDev refpair_dev(const DirectParams &pp)
 {
   DirectDev d{};
-  d.kind = KIND_LAPLACE_ERF;
+  d.kind = pp.kind;
   d.nNeigh = pp.nNeigh;
   d.Rdir = pp.Rdir;
   d.beta = pp.beta;
   d.nb0 = pp.nb[0];
   d.nb1 = pp.nb[1];
   d.nb2 = pp.nb[2];
+  // user kernels: the FP64 short-range table over (0, utR]
+  d.utab = pp.utab;
+  d.utS = pp.utS;
+  d.invRdir = pp.utR > 0 ? 1.0 / pp.utR : 0.0;
   return d;
 }
 
@@ -1467,8 +1475,19 @@ void launch_direct_refpairs(const DirectParams &pp, size_t n, const int4 *cell, 
                             const unsigned *flags, double *Vdir, hipStream_t s)
 {
   const DirectDev d = refpair_dev(pp);
-  hipLaunchKernelGGL(k_direct_refpairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d, n,
-                     refpair_words(pp.nNeigh), cell, (const int2 *)cellStart, xs, perm, q, flags, Vdir);
+  const dim3 g((unsigned)((n + 255) / 256));
+  const int W = refpair_words(pp.nNeigh);
+  if (pp.kind == KIND_USER)
+    hipLaunchKernelGGL(k_direct_refpairs<KIND_USER>, g, dim3(256), 0, s, d, n, W, cell,
+                       (const int2 *)cellStart, xs, perm, q, flags, Vdir);
+  else if (pp.kind == KIND_USER_CX)
+    hipLaunchKernelGGL(k_direct_refpairs<KIND_USER_CX>, g, dim3(256), 0, s, d, n, W, cell,
+                       (const int2 *)cellStart, xs, perm, q, flags, Vdir);
+  else if (pp.kind == KIND_LAPLACE_ERF)
+    hipLaunchKernelGGL(k_direct_refpairs<KIND_LAPLACE_ERF>, g, dim3(256), 0, s, d, n, W, cell,
+                       (const int2 *)cellStart, xs, perm, q, flags, Vdir);
+  else
+    throw Error{NPME_ERR_UNSUPPORTED, "no reference-pair direct sum for this kernel"};
   NPME_HIP_CHECK(hipGetLastError());
 }
 

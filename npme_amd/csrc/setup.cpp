@@ -1056,7 +1056,12 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.ker.Rdir = Rdir;
     if (!h.trans) {  // a box pair uses only func, on the kernel grid
       user_check_radial(h.user, Rdir);
-      user_build_table(h.user, Rdir, utab, h.utS, h.utErr);
+      // the reference calls funcSR on every point pair of the interacting
+      // cell pairs (src/PotentialGenFunc.cpp:523-645), up to the farthest
+      // points of cells nNeigh apart on each axis: sqrt(3) (nNeigh + 1)
+      // cell edges
+      h.utR = std::sqrt(3.0) * (double)(h.nNeigh + 1) * (Rdir / (double)h.nNeigh) * (1.0 + 1.0e-6);
+      user_build_table(h.user, h.utR, utab, h.utS, h.utErr);
       user_fself(h.user, h.fself);
     }
     h.coordHost.assign(coord, coord + 3 * n);
@@ -1118,8 +1123,9 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   if (nlmode != "auto" && nlmode != "on" && nlmode != "off")
     kw_error("neighborList must be 'auto', 'on' or 'off'");
   if (h.trans) nlmode = "off";  // no direct sum between the boxes
-  // LaplaceOrig sums the reference's cell pairs, not a cutoff list
-  if (kind == KIND_LAPLACE_ERF) nlmode = "off";
+  // LaplaceOrig and user kernels sum the reference's cell pairs, not a
+  // cutoff list
+  if (kind_ref_pairs(kind)) nlmode = "off";
   const bool nlPossible = nlmode != "off" && h.nNeigh <= h.nCellClust1D && n < (1u << 26);
   const bool devSetup = smode == "device";
   const bool staged = devSetup && h.coordStaged && h.stagedN == n;
@@ -1225,20 +1231,14 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
   h.deviceSetup = done;
   ms_since(t0);
   if (!utab.empty()) {
-    // the short-range table in the working precision
-    if (h.fp64) {
-      h.d_utab = dalloc<double>(h, utab.size());
-      NPME_HIP_CHECK(hipMemcpyAsync(h.d_utab, utab.data(), utab.size() * sizeof(double),
-                                    hipMemcpyHostToDevice, h.stream));
-    } else {
-      const std::vector<float> f(utab.begin(), utab.end());
-      h.d_utab = dalloc<float>(h, f.size());
-      NPME_HIP_CHECK(hipMemcpyAsync(h.d_utab, f.data(), f.size() * sizeof(float),
-                                    hipMemcpyHostToDevice, h.stream));
-      NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before f is released
-    }
+    // the short-range table in FP64: the reference-pair direct sum
+    // evaluates it in FP64 in both forms
+    h.d_utab = dalloc<double>(h, utab.size());
+    NPME_HIP_CHECK(hipMemcpyAsync(h.d_utab, utab.data(), utab.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, h.stream));
+    NPME_HIP_CHECK(hipStreamSynchronize(h.stream));  // before utab is released
   }
-  if (h.ker.kind == KIND_LAPLACE_ERF) {
+  if (kind_ref_pairs(h.ker.kind)) {
     const size_t ncell = (size_t)h.nb[0] * h.nb[1] * h.nb[2];
     h.d_cellFlags = dalloc<unsigned>(h, ncell * refpair_words((int)h.nNeigh));
     launch_cellpair_contact(direct_params(h), h.d_cellStart, h.d_xs, h.d_cellFlags, h.stream);
@@ -1314,6 +1314,7 @@ DirectParams direct_params(const Handle &h)
   dp.fxInv = std::ldexp(h.csA, -h.fxShift);
   dp.utab = h.d_utab;
   dp.utS = h.utS;
+  dp.utR = h.utR;
   dp.nlEshift = h.fp64 ? 5 : 4;
   return dp;
 }
@@ -1348,7 +1349,7 @@ static void calcv_launches(Handle &h, const double *d_q, double *d_Vout, bool sp
   hipStream_t s2 = split ? h.stream2 : s;
   const DirectParams dp = direct_params(h);
   const bool doDirect = !h.trans;  // a box pair has no direct sum
-  const bool refPairs = h.ker.kind == KIND_LAPLACE_ERF;
+  const bool refPairs = kind_ref_pairs(h.ker.kind);
   auto direct = [&] {
     launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s2);
     if (refPairs)

@@ -1247,15 +1247,19 @@ void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
 }
 
 /* ======================================================================
- * LaplaceOrig's direct sum over the reference's pair set.
+ * Direct sums over the reference's pair set, for the short-range kernels
+ * that carry no cutoff of their own: LaplaceOrig's erfc(beta r)/r and the
+ * GenFunc user kernels (here rFunc's c r erfc(beta r)).
  * NPME_PotLaplace_SR_Original_DirectSum_V1 (src/PotentialLaplace.cpp:
- * 320-380, pair/self :613-700) adds erfc(beta r)/r for EVERY point pair of
- * the cell pairs its interaction list keeps, with no cutoff of its own.
- * The list (NPME_FindInteractionList_C, src/PartitionBox.cpp:789-922, type
- * 'C' of src/Constant.h:61) over the B cells of the embedded partition
- * (edge Rdir/nNeigh, src/PartitionEmbeddedBox.cpp:47-203) keeps each
- * occupied cell with itself, and a pair of occupied cells in each other's
- * +-nNeigh stencil when the cell boxes are closer than 1.0001 Rdir
+ * 320-380, pair/self :613-700) and NPME_PotGenFunc_DirectSum_V1 /
+ * _Pair_V1 (src/PotentialGenFunc.cpp:140-320, :523-645, funcSR.Calc on
+ * every pair) add the kernel for EVERY point pair of the cell pairs the
+ * interaction list keeps.  The list (NPME_FindInteractionList_C,
+ * src/PartitionBox.cpp:789-922, type 'C' of src/Constant.h:61) over the B
+ * cells of the embedded partition (edge Rdir/nNeigh,
+ * src/PartitionEmbeddedBox.cpp:47-203) keeps each occupied cell with
+ * itself, and a pair of occupied cells in each other's +-nNeigh stencil
+ * when the cell boxes are closer than 1.0001 Rdir
  * (NPME_CalcMinDistanceRectVolumes of the cell centres) and some point pair
  * of the two cells has a squared distance below 1.0001 Rdir^2, with Rdir
  * recomputed as nNeigh cell edges (:767-787, :841-842).  Cells are indexed
@@ -1283,10 +1287,28 @@ static size_t cell_lower(const npo_cellpt *v, size_t n, long c)
   return lo;
 }
 
-int npo_direct_erfc_refpairs(const npo_kernel *k, size_t n, const double *coord,
-                             const double *q, long nNeigh, long nCellClust1D,
-                             double *V)
+/* the short-range part without a cutoff: f0 and f1 = (1/r) df/dr */
+static void sr_nocut(const npo_kernel *k, double r2, cplx *f0, cplx *f1)
 {
+  const double r = sqrt(r2), x = k->beta * r;
+  const double g = 2.0 / sqrt(PI) * k->beta * exp(-x * x);
+  if (k->kind == NPO_LAPLACE_ERF) {
+    const double e = erfc(x) / r;
+    *f0 = e;
+    *f1 = -(g + e) / r2;
+  } else { /* NPO_RFUNC, NPO_RFUNC_CX: c r erfc(beta r) */
+    const cplx c = k->k0r + I * k->k0i;
+    *f0 = c * r * erfc(x);
+    *f1 = c * (erfc(x) / r - g);
+  }
+}
+
+int npo_direct_refpairs(const npo_kernel *k, size_t n, const double *coord,
+                        const double *qd, long nNeigh, long nCellClust1D,
+                        double *Vd)
+{
+  if (k->kind != NPO_LAPLACE_ERF && k->kind != NPO_RFUNC && k->kind != NPO_RFUNC_CX) return -2;
+  const int cx = NPO_IS_COMPLEX(k->kind);
   const double csB = k->Rdir / nNeigh;
   const unsigned long nP = (unsigned long) nCellClust1D;
   const double csA = nP * csB;
@@ -1329,7 +1351,7 @@ int npo_direct_erfc_refpairs(const npo_kernel *k, size_t n, const double *coord,
     const long *gi = &g[3 * i];
     const long ci = (gi[0] * nb[1] + gi[1]) * nb[2] + gi[2];
     const size_t a0 = cell_lower(v, n, ci), a1 = cell_lower(v, n, ci + 1);
-    double acc[4] = {0, 0, 0, 0};
+    cplx acc[4] = {0, 0, 0, 0};
     for (long dx = -nN; dx <= nN; dx++)
       for (long dy = -nN; dy <= nN; dy++)
         for (long dz = -nN; dz <= nN; dz++) {
@@ -1354,16 +1376,23 @@ int npo_direct_erfc_refpairs(const npo_kernel *k, size_t n, const double *coord,
             if (j == i) continue;
             const double rx = coord[3 * i] - coord[3 * j], ry = coord[3 * i + 1] - coord[3 * j + 1],
                          rz = coord[3 * i + 2] - coord[3 * j + 2];
-            const double r2 = rx * rx + ry * ry + rz * rz, r = sqrt(r2), x = k->beta * r;
-            const double e = erfc(x) / r;
-            const double f1 = -(k->beta * 2.0 / sqrt(PI) * exp(-x * x) + e) / r2;
-            acc[0] += e * q[j];
-            acc[1] += rx * f1 * q[j];
-            acc[2] += ry * f1 * q[j];
-            acc[3] += rz * f1 * q[j];
+            cplx f0, f1;
+            sr_nocut(k, rx * rx + ry * ry + rz * rz, &f0, &f1);
+            const cplx qj = cx ? qd[2 * j] + I * qd[2 * j + 1] : qd[j];
+            acc[0] += f0 * qj;
+            acc[1] += rx * f1 * qj;
+            acc[2] += ry * f1 * qj;
+            acc[3] += rz * f1 * qj;
           }
         }
-    for (int c = 0; c < 4; c++) V[4 * i + c] = acc[c];
+    for (int c = 0; c < 4; c++) {
+      if (cx) {
+        Vd[8 * i + 2 * c] = creal(acc[c]);
+        Vd[8 * i + 2 * c + 1] = cimag(acc[c]);
+      } else {
+        Vd[4 * i + c] = creal(acc[c]);
+      }
+    }
   }
   free(g);
   free(v);

@@ -155,13 +155,15 @@ int npo_interp_basic(size_t n, const double *coord, const double *theta,
  * PotentialHelmholtz.cpp:194-483).  V is [n][4] real or complex. */
 void npo_direct_sr(const npo_kernel *k, size_t n, const double *coord,
                    const double *q, double *V);
-/* LaplaceOrig's direct sum over the reference's interacting cell pairs,
- * every point pair of them, no cutoff (src/PotentialLaplace.cpp:320-380,
- * src/PartitionBox.cpp:789-922); V[n][4] real; -1 if a point leaves the
- * cell grid */
-int npo_direct_erfc_refpairs(const npo_kernel *k, size_t n, const double *coord,
-                             const double *q, long nNeigh, long nCellClust1D,
-                             double *V);
+/* direct sum over the reference's interacting cell pairs, every point pair
+ * of them, no cutoff, for the short-range kernels without a cutoff of their
+ * own: NPO_LAPLACE_ERF (src/PotentialLaplace.cpp:320-380) and the rFunc
+ * user kernels (src/PotentialGenFunc.cpp:140-320), with the list of
+ * src/PartitionBox.cpp:789-922; V[n][4] real or complex; -1 if a point
+ * leaves the cell grid, -2 for another kernel kind */
+int npo_direct_refpairs(const npo_kernel *k, size_t n, const double *coord,
+                        const double *q, long nNeigh, long nCellClust1D,
+                        double *V);
 /* exact O(N^2) sum of the full kernel for the targets idx[0..nt)
  * (src/PotentialLaplace.cpp:40-132, PotentialHelmholtz.cpp:41-135,
  * PotentialGenFunc.cpp:321-522).  idx NULL means all targets. */

@@ -85,7 +85,7 @@ def lib():
         L.npo_interp_basic.argtypes = [sz, p, p, p, p, l, p, p]
         L.npo_direct_sr.argtypes = [C.POINTER(Kernel), sz, p, p, p]
         L.npo_exact.argtypes = [C.POINTER(Kernel), sz, p, p, sz, p, p]
-        L.npo_direct_erfc_refpairs.argtypes = [C.POINTER(Kernel), sz, p, p, l, l, p]
+        L.npo_direct_refpairs.argtypes = [C.POINTER(Kernel), sz, p, p, l, l, p]
         L.npo_count_pairs.restype = d
         L.npo_count_pairs.argtypes = [sz, p, d]
         L.npo_calcv.argtypes = [C.POINTER(Kernel), C.POINTER(RecSumParm), sz, p, p, p]
@@ -381,23 +381,30 @@ def direct_sr(k, coord, q):
     return V
 
 
-def direct_erfc_refpairs(k, coord, q, nNeigh=2, nCellClust1D=3):
-    """LaplaceOrig's direct sum over the reference's interacting cell pairs
-    (no cutoff)"""
-    coord, q = _f64(coord), _f64(q)
-    V = np.zeros((len(coord), 4))
-    if lib().npo_direct_erfc_refpairs(C.byref(k), len(coord), _ptr(coord), _ptr(q), int(nNeigh),
-                                      int(nCellClust1D), _ptr(V)) != 0:
-        raise ValueError("point outside the cell grid")
+def direct_refpairs(k, coord, q, nNeigh=2, nCellClust1D=3):
+    """direct sum over the reference's interacting cell pairs, no cutoff
+    (LaplaceOrig's erfc kernel and the rFunc user kernels)"""
+    coord = _f64(coord)
+    cx = k.kind in _COMPLEX_KINDS
+    qq = _c128_as_f64(q) if cx else _f64(q)
+    V = np.zeros((len(coord), 4), np.complex128 if cx else np.float64)
+    rc = lib().npo_direct_refpairs(C.byref(k), len(coord), _ptr(coord), _ptr(qq), int(nNeigh),
+                                   int(nCellClust1D), _ptr(V))
+    if rc != 0:
+        raise ValueError("direct_refpairs failed (%d)" % rc)
     return V
 
 
-def calcv_laplace_orig(k, N, Bn, coord, q, nblk=(20, 20, 20), dl=None, a=(4.0, 4.0, 4.0),
-                       nNeigh=2, nCellClust1D=3):
-    """CalcV of NPME_InterfaceReal_Laplace_Original: the reciprocal sum and
-    self term of calcv with the reference's direct sum"""
+def calcv_ref(k, N, Bn, coord, q, nblk=(20, 20, 20), dl=None, a=(4.0, 4.0, 4.0),
+              nNeigh=2, nCellClust1D=3):
+    """CalcV with the reference's direct-sum pair set: the reciprocal sum and
+    self term of calcv, and the direct sum over the interacting cell pairs
+    (NPME_InterfaceReal_Laplace_Original; GenFunc with the rFunc kernels)"""
     V = calcv(k, N, Bn, coord, q, nblk=nblk, dl=dl, a=a)
-    return V - direct_sr(k, coord, q) + direct_erfc_refpairs(k, coord, q, nNeigh, nCellClust1D)
+    return V - direct_sr(k, coord, q) + direct_refpairs(k, coord, q, nNeigh, nCellClust1D)
+
+
+calcv_laplace_orig = calcv_ref
 
 
 def exact(k, coord, q, idx=None):

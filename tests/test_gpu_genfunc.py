@@ -6,8 +6,10 @@ version f = c r, as Python plugins (NPME_KfuncReal / NPME_KfuncComplex
 subclasses whose Calc works in place on the library's arrays).
 
 Checked through the C ABI against the oracle's restatement of the same
-GenFunc path (the rFunc kernels evaluated directly, no table) on the same
-96^3 grid, and against the exact sum:
+GenFunc path (the rFunc kernels evaluated directly, no table, and the
+reference's direct-sum pair set: funcSR on every point pair of the
+interacting cell pairs, src/PotentialGenFunc.cpp:523-645) on the same 96^3
+grid, and against the exact sum:
   fp64 GPU path vs oracle CalcV ............. 1e-10 (V and grad): the device
                                                short-range table's error is
                                                below 1e-12
@@ -123,7 +125,7 @@ def test_genfunc_real_rfunc(oracle, prec):
     assert ef < 1e-12 and eg < 1e-11
     V = h.calcv(q)
     ko = oracle.kernel_rfunc(RDIR, BETA)
-    mo = oracle.compare_v(oracle.calcv(ko, (NG, NG, NG), 8, coord, q, nblk=(8, 8, 8)), V)
+    mo = oracle.compare_v(oracle.calcv_ref(ko, (NG, NG, NG), 8, coord, q, nblk=(8, 8, 8)), V)
     Ve = oracle.exact(ko, coord, q)
     me = oracle.compare_v(Ve, V)
     print(prec, "vs oracle", mo, "vs exact", me)
@@ -155,7 +157,7 @@ def test_genfunc_complex_rfunc(oracle, prec):
                    _complex_kernel("sr", c)), N.npme_last_error()
     V = h.calcv(qc)
     ko = oracle.kernel_rfunc(RDIR, BETA, c=c, complex_kernel=True)
-    mo = oracle.compare_v(oracle.calcv(ko, (NG, NG, NG), 8, coord, qc, nblk=(8, 8, 8)), V)
+    mo = oracle.compare_v(oracle.calcv_ref(ko, (NG, NG, NG), 8, coord, qc, nblk=(8, 8, 8)), V)
     Ve = oracle.exact(ko, coord, qc)
     me = oracle.compare_v(Ve, V)
     print(prec, "vs oracle", mo, "vs exact", me)

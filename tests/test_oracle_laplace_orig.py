@@ -1,5 +1,7 @@
-"""Pins the oracle's LaplaceOrig direct sum over the reference's pair set
-(npo_direct_erfc_refpairs): a brute-force numpy restatement of the
+"""Pins the oracle's direct sum over the reference's pair set
+(npo_direct_refpairs, LaplaceOrig's erfc kernel and the GenFunc rFunc user
+kernel, whose short-range parts have no cutoff): a brute-force numpy
+restatement of the
 interaction-list rule of NPME_FindInteractionList_C (src/PartitionBox.cpp:
 789-922) over the embedded B cells, summing erfc(beta r)/r over every point
 pair of the kept cell pairs (src/PotentialLaplace.cpp:613-700); and the
@@ -22,7 +24,12 @@ def _cells(coord, Rdir, nNeigh, nClust):
     return a * nClust + b
 
 
-def _brute(coord, q, Rdir, beta, nNeigh, nClust):
+def _erfc_over_r(beta, r):
+    e = erfc(beta * r) / r
+    return e, -(beta * 2 / np.sqrt(np.pi) * np.exp(-(beta * r) ** 2) + e) / r ** 2
+
+
+def _brute(coord, q, Rdir, beta, nNeigh, nClust, fg=None):
     g = _cells(coord, Rdir, nNeigh, nClust)
     n = len(q)
     d = coord[:, None, :] - coord[None, :, :]
@@ -44,9 +51,9 @@ def _brute(coord, q, Rdir, beta, nNeigh, nClust):
     keep = np.array([[i != j and contact.get((keys[i], keys[j]), False) for j in range(n)]
                      for i in range(n)])
     r = np.sqrt(np.where(keep, r2, 1.0))
-    e = np.where(keep, erfc(beta * r) / r, 0.0)
-    f1 = np.where(keep, -(beta * 2 / np.sqrt(np.pi) * np.exp(-(beta * r) ** 2) + e) / r ** 2, 0.0)
-    V = np.zeros((n, 4))
+    e, f1 = (fg or _erfc_over_r)(beta, r)
+    e, f1 = np.where(keep, e, 0.0), np.where(keep, f1, 0.0)
+    V = np.zeros((n, 4), np.result_type(e, q))
     V[:, 0] = e @ q
     for c in range(3):
         V[:, 1 + c] = (d[:, :, c] * f1) @ q
@@ -59,7 +66,7 @@ def test_refpairs_match_brute_force_rule(oracle, nNeigh, nClust):
     Rdir = 4.0
     beta = oracle.rdir2beta(Rdir, 1e-6)
     k = oracle.kernel_erf(Rdir, beta)
-    V = oracle.direct_erfc_refpairs(k, coord, q, nNeigh, nClust)
+    V = oracle.direct_refpairs(k, coord, q, nNeigh, nClust)
     Vb, keep, r2 = _brute(coord, q, Rdir, beta, nNeigh, nClust)
     np.testing.assert_allclose(V, Vb, rtol=1e-12, atol=1e-12 * np.abs(Vb).max())
     # complete: every pair inside the cutoff is kept; some beyond it are too
@@ -70,3 +77,24 @@ def test_refpairs_match_brute_force_rule(oracle, nNeigh, nClust):
     Vc = oracle.direct_sr(k, coord, q)
     assert np.abs(V - Vc).max() > 0
     assert np.abs(V - Vc).max() < 1e-6 * np.abs(V).max()
+
+
+@pytest.mark.parametrize("c", [1.0, 0.6 + 0.8j])
+def test_refpairs_rfunc_match_brute_force_rule(oracle, c):
+    """the user kernel c r with the split c r erf + c r erfc (rFunc of
+    app/userDefineFunc.cpp): GenFunc calls funcSR on every kept pair"""
+    coord, q = uniform_box(500, (12.0, 13.0, 10.0))
+    cx = isinstance(c, complex)
+    if cx:
+        q = q + 1j * np.random.default_rng(3).standard_normal(len(q))
+    Rdir, beta = 4.0, 0.5
+    k = oracle.kernel_rfunc(Rdir, beta, c=c, complex_kernel=cx)
+
+    def fg(b, r):
+        return (c * r * erfc(b * r),
+                c * (erfc(b * r) / r - 2 / np.sqrt(np.pi) * b * np.exp(-(b * r) ** 2)))
+
+    V = oracle.direct_refpairs(k, coord, q, 2, 3)
+    Vb, keep, r2 = _brute(coord, q, Rdir, beta, 2, 3, fg)
+    np.testing.assert_allclose(V, Vb, rtol=1e-12, atol=1e-12 * np.abs(Vb).max())
+    assert (keep & (r2 > Rdir * Rdir)).any()
