@@ -66,8 +66,8 @@ int npme_add_missing_keywords(npme_keywords *kw, size_t nCharge,
  *      EwaldSplit), then ::SetUp (src/Interface.cpp:1606-1671, :1847-1956,
  *      :2661-2791).  The keyword set must already hold every value
  *      (npme_add_missing_keywords).  Extension keyword: precision = mixed
- *      (default; FP32 spreading/FFT/direct sum with compensated
- *      accumulation) or fp64. ------------------------------------------ */
+ *      (default; FP32 spreading and FFT, FP32 direct-sum pair terms summed
+ *      in FP32 partials of 16 that are carried into FP64) or fp64. ------- */
 int npme_create(npme_handle **h, const npme_keywords *kw, size_t nCharge,
                 const double *coord);
 
@@ -102,8 +102,11 @@ typedef struct {
  * smooth long-range part, sampled on the kernel grid on the host at every
  * set-up (and fself = funcLR(0, 0, 0)), funcSR the short-range part, which
  * must be radial: it is tabulated on the host as piecewise polynomials in r
- * over [0, Rdir] and evaluated from that table by the device direct sum
- * (npme_get_table_error reports the table's error).  The keywords must hold
+ * over [0, R_t], R_t = sqrt(3) (nNeigh + 1) Rdir / nNeigh, the longest pair
+ * of the interacting cell pairs on which the reference calls it
+ * (src/PotentialGenFunc.cpp:523-645), and evaluated from that table by the
+ * device direct sum over those pairs in FP64 (npme_get_table_error reports
+ * the table's error).  The keywords must hold
  * the grid and split (Rdir, BnOrder, N1-3, n1-3, del1-3, a1-3, nNeigh,
  * nCellClust1D); funcType, EwaldSplit and nDeriv are not used. */
 int npme_create_genfunc_real(npme_handle **h, const npme_keywords *kw, size_t nCharge,
