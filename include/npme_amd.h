@@ -67,7 +67,7 @@ int npme_add_missing_keywords(npme_keywords *kw, size_t nCharge,
  *      :2661-2791).  The keyword set must already hold every value
  *      (npme_add_missing_keywords).  Extension keyword: precision = mixed
  *      (default; FP32 spreading and FFT, FP32 direct-sum pair terms summed
- *      in FP32 partials of 16 that are carried into FP64) or fp64. ------- */
+ *      in FP32 partials of 8 that are carried into FP64) or fp64. ------- */
 int npme_create(npme_handle **h, const npme_keywords *kw, size_t nCharge,
                 const double *coord);
 

@@ -214,25 +214,26 @@ __global__ __launch_bounds__(256) void k_starts(long nt, const unsigned *__restr
   start[t] = (int)lower_bound_d(skey, m, (unsigned)t);
 }
 
-// neighbour-list slices: the length of a slice is its longest list; the
-// scan input is 64 * length (the last slot stays 0 for the total)
+// neighbour-list slices of kNlSlice targets: a slice's per-lane length
+// comes from its longest list; the scan input is 64 * that length (the last
+// slot stays 0 for the total)
 __global__ __launch_bounds__(256) void k_slice_len(size_t n, const int *__restrict__ count,
                                                    int *__restrict__ sliceLen,
                                                    long *__restrict__ sliceWords)
 {
   const size_t sl = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t nSlices = (n + 63) / 64;
+  const size_t nSlices = (n + kNlSlice - 1) / kNlSlice;
   if (sl > nSlices) return;
   if (sl == nSlices) {
     sliceWords[sl] = 0;
     return;
   }
   int m = 0;
-  const size_t e = (64 * sl + 64 < n) ? 64 * sl + 64 : n;
-  for (size_t s = 64 * sl; s < e; s++) m = max(m, count[s]);
-  m = (m + 3) & ~3;  // the list stores groups of 4 entries per target
-  sliceLen[sl] = m;
-  sliceWords[sl] = 64L * m;
+  const size_t b = kNlSlice * sl, e = (b + kNlSlice < n) ? b + kNlSlice : n;
+  for (size_t s = b; s < e; s++) m = max(m, count[s]);
+  const int ml = nlist_lane_len(m);
+  sliceLen[sl] = ml;
+  sliceWords[sl] = 64L * ml;
 }
 
 __global__ __launch_bounds__(256) void k_sort_coords(size_t n, const int *__restrict__ perm,
@@ -558,7 +559,7 @@ long scan_words_device(const long *d_in, long *d_out, size_t m, hipStream_t s)
 size_t nlist_slices_device(size_t n, const int *d_count, int *d_sliceLen, long *d_sliceStart,
                            hipStream_t s)
 {
-  const size_t nSlices = (n + 63) / 64;
+  const size_t nSlices = (n + kNlSlice - 1) / kNlSlice;
   DevBuf words((nSlices + 1) * sizeof(long));
   hipLaunchKernelGGL(k_slice_len, grid_for(nSlices + 1), dim3(256), 0, s, n, d_count, d_sliceLen,
                      (long *)words.p);

@@ -541,6 +541,20 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
                    double *Vdir, hipStream_t s);
 // padding entry of the neighbour list: the sentinel source n, as a byte offset
 unsigned nlist_pad(size_t sentinel, int eshift);
+// neighbour-list slices: a team of kNlTeam lanes walks one target's list
+// (lane j of the team takes entries j, j + kNlTeam, ...), so a slice (one
+// wavefront) holds kNlSlice targets.  A slice's length is its longest list
+// rounded up to 4 kNlTeam; sliceLen holds that length divided by kNlTeam,
+// the entries each lane walks (a multiple of 4), and a slice takes
+// 64 sliceLen words
+constexpr int kNlTeamLog = 2;
+constexpr int kNlTeam = 1 << kNlTeamLog;
+constexpr int kNlSlice = 64 / kNlTeam;
+// per-lane length of a slice whose longest list has m entries
+__host__ __device__ constexpr int nlist_lane_len(int m)
+{
+  return ((m + 4 * kNlTeam - 1) / (4 * kNlTeam)) * 4;
+}
 // scratch of one list build, from its count pass to its fill pass: the
 // hit masks of the wavefront-per-cell walk and their per-cell offsets
 struct NlistWork {

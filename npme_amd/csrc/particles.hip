@@ -395,22 +395,37 @@ void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, 
 
 // ------------------------------------------------------------------ neighbour list
 // The geometry is fixed between CalcV calls, so the pairs inside the cutoff
-// are found once at set-up and stored as a sliced-ELLPACK list: the targets
-// are cut into slices of 64 consecutive sorted charges (one wavefront), each
-// slice is padded to its longest list rounded up to 4, and the entries go in
-// groups of 4 per lane: entry k of lane l sits at
-// sliceStart + (k / 4) * 256 + 4 l + k % 4, so group g of a wavefront is
-// 1 KB contiguous, read by one 16-byte load per lane (and written so by
-// the list build).  An entry packs the source index (low 26 bits) with the offset of
-// the source's cluster from the target's cluster, dx+1, dy+1 and dz+1 in
-// two bits each (bits 26, 28, 30).  A padding entry is the sentinel source
-// n (position 0 in its frame, charge 0, stored after the n charges) with
-// the code 3 on every axis, an offset of two clusters: its distance is
-// finite and non-zero and its charge zero, so it adds exactly zero and the
-// loop needs no test for it.  The pairs are found in FP64 with a cutoff
-// inflated by 1e-9 relative and every entry is evaluated (the short-range
-// function vanishes at Rdir with its first Nder derivatives).
+// are found once at set-up and stored as a sliced-ELLPACK list.  The sorted
+// targets are cut into slices of kNlSlice consecutive charges (one
+// wavefront), and a team of kNlTeam lanes walks each target's list: lane
+// t kNlTeam + j of a slice takes entries j, j + kNlTeam, j + 2 kNlTeam, ...
+// of the slice's target t.  A slice is padded to its longest list rounded
+// up to 4 kNlTeam, and each lane's entries go in groups of 4: entry k of
+// target t sits at
+//   sliceStart + 256 (k / (4 kNlTeam)) + 4 (t kNlTeam + k % kNlTeam)
+//              + (k / kNlTeam) % 4,
+// so group g of a wavefront is 1 KB contiguous, read by one 16-byte load
+// per lane.  The lanes of a team read consecutive entries of one list at
+// each step, which the build keeps in stencil-cell order and ascending
+// source order within a cell, so one gather of a wavefront touches a few
+// cache lines per team rather than one per lane.  An entry is the byte
+// offset of its source in the sorted source array.  A padding entry is the
+// sentinel source n (stored after the n charges, charge 0, placed so that
+// its distance to every target is finite and non-zero): it adds exactly
+// zero and the loop needs no test for it.  The pairs are found in FP64 with
+// a cutoff inflated by 1e-9 relative and every entry is evaluated (the
+// short-range function vanishes at Rdir with its first Nder derivatives).
 unsigned nlist_pad(size_t sentinel, int eshift) { return (unsigned)sentinel << eshift; }
+
+// word of entry 0 of sorted target s, and of its entry k
+__device__ __forceinline__ long nl_target_base(const long *__restrict__ sliceStart, size_t s)
+{
+  return sliceStart[s / kNlSlice] + 4L * kNlTeam * (long)(s % kNlSlice);
+}
+__device__ __forceinline__ long nl_slot(long base, int k)
+{
+  return base + 256L * (k >> (2 + kNlTeamLog)) + 4 * (k & (kNlTeam - 1)) + ((k >> kNlTeamLog) & 3);
+}
 
 // walk the cells of the stencil of sorted target s in FP64 input
 // coordinates; COUNT mode counts, FILL mode writes entries
@@ -456,9 +471,8 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
   const double R2 = R * R;
   const int nN = p.nNeigh;
   const int ax = c.x / nP, ay = c.y / nP, az = c.z / nP;
-  uint4 *out = nullptr;  // group g of this target at out[g * 64]
-  if (FILL) out = reinterpret_cast<uint4 *>(list + sliceStart[s >> 6]) + (s & 63);
-  uint4 grp = make_uint4(0, 0, 0, 0);
+  long base = 0;  // the list is filled with padding before this pass
+  if (FILL) base = nl_target_base(sliceStart, s);
   int k = 0;
   for (int dx = -nN; dx <= nN; dx++)
     for (int dy = -nN; dy <= nN; dy++)
@@ -483,27 +497,11 @@ __global__ __launch_bounds__(256) void k_nlist(DirectDev p, size_t n, const int4
             in = rx * rx + ry * ry + rz * rz < R2;
           }
           if (in) {
-            if (FILL) {
-              const unsigned ent = (unsigned)j << p.eshift;
-              const int e = k & 3;
-              grp.x = e == 0 ? ent : grp.x;
-              grp.y = e == 1 ? ent : grp.y;
-              grp.z = e == 2 ? ent : grp.z;
-              grp.w = ent;
-              if (e == 3) out[(long)(k >> 2) * 64] = grp;
-            }
+            if (FILL) list[nl_slot(base, k)] = (unsigned)j << p.eshift;
             k++;
           }
         }
       }
-  if (FILL && (k & 3) != 0) {  // the last, partial group, padded
-    const unsigned pad = (unsigned)n << p.eshift;
-    const int e = k & 3;
-    if (e <= 1) grp.y = pad;
-    if (e <= 2) grp.z = pad;
-    grp.w = pad;
-    out[(long)(k >> 2) * 64] = grp;
-  }
   if (!FILL) count[s] = k;
 }
 
@@ -578,8 +576,8 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
   }
   __syncthreads();
   if (!live) return;  // after the only barrier
-  unsigned *out = nullptr;  // entry k of target s: out[256 (k / 4) + k % 4]
-  if (FILL) out = list + sliceStart[s >> 6] + 4 * (s & 63);
+  long base = 0;
+  if (FILL) base = nl_target_base(sliceStart, s);
   const unsigned long long below = (1ull << lane) - 1;
   int k = 0;
   for (int t0 = 0; t0 < T; t0 += 64) {
@@ -611,7 +609,7 @@ __global__ __launch_bounds__(256) void k_nlist_wave(DirectDev p, size_t n,
     const unsigned long long m = __ballot(hit);
     if (FILL && hit) {
       const int pos = k + __popcll(m & below);
-      out[256L * (pos >> 2) + (pos & 3)] = ent;
+      list[nl_slot(base, pos)] = ent;
     }
     k += __popcll(m);
   }
@@ -797,10 +795,10 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
   for (int tb = 0; tb < home.y; tb += 64) {
     const int mt = min(64, home.y - tb);
     const int s0 = home.x + tb;
-    // lane i: list offset of target s0 + i (entries of the target's group
-    // column start there), taken per target with readlane
+    // lane i: list offset of target s0 + i (its entry 0), taken per
+    // target with readlane
     const int si = s0 + min(lane, mt - 1);
-    const long obase = sliceStart[si >> 6] + 4 * (long)(si & 63);
+    const long obase = nl_target_base(sliceStart, (size_t)si);
     int kreg = 0;
     for (int t0 = 0; t0 < T; t0 += 64) {
       const int t = t0 + lane;
@@ -816,7 +814,7 @@ __global__ __launch_bounds__(256) void k_nlist_cell_expand(DirectDev p, long nce
         if ((m >> lane) & 1ull) {
           const int pos = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, (unsigned)ki));
-          list[ob + 256L * (pos >> 2) + (pos & 3)] = ent;
+          list[nl_slot(ob, pos)] = ent;
         }
       }
       kreg += __popcll(mym);
@@ -1057,9 +1055,10 @@ __device__ __forceinline__ void pair_sr(const DirectDev &p, Real rx, Real ry, Re
   }
 }
 
-// direct sum over the neighbour list: one target per lane, the 64 targets of
-// a slice walk their lists in lockstep (lists of one slice have similar
-// lengths, so little of the wavefront idles)
+// direct sum over the neighbour list: a team of kNlTeam lanes per target,
+// the kNlSlice targets of a slice walk their lists in lockstep (lists of one
+// slice have similar lengths, so little of the wavefront idles); the team's
+// sums are added across its lanes at the end
 template <typename Real, int KIND, bool FIXP, int MAXD>
 __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_t slice, int lane,
                                              const typename V4t<Real>::T *__restrict__ src,
@@ -1073,9 +1072,9 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
 {
   using V4 = typename V4t<Real>::T;
   constexpr bool CX = kind_is_complex(KIND);
-  const size_t s = slice * 64 + lane;
+  const size_t s = slice * kNlSlice + (lane >> kNlTeamLog);
   const bool tvalid = s < n;
-  const size_t tt = tvalid ? s : slice * 64;
+  const size_t tt = tvalid ? s : slice * kNlSlice;
   const V4 me = src[tt];
   // positions are in one frame for all charges (FP64: relative to the low
   // corner of the cells; mixed: int32 fixed point of that, wrapped modulo
@@ -1093,7 +1092,7 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
   // the slice is the wavefront's: its length is uniform, so the loop and
   // the bounds of the entry loads branch on scalars
   const int len = __builtin_amdgcn_readfirstlane(sliceLen[slice]);
-  const uint4 *e4 = reinterpret_cast<const uint4 *>(list + sliceStart[slice]) + (s & 63);
+  const uint4 *e4 = reinterpret_cast<const uint4 *>(list + sliceStart[slice]) + lane;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   Real part[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   // U entries per step.  Two buffer sets alternate (the loop body is the
@@ -1171,7 +1170,7 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
                                 topA, topB);
   };
   auto carry = [&](int k0) {
-    if (((k0 + U) & 15) == 0) {  // FP64 carry every 16 entries
+    if (((k0 + U) & 7) == 0) {  // FP64 carry every 8 entries
 #pragma unroll
       for (int c = 0; c < (CX ? 8 : 4); c++) {
         acc[c] += (double)part[c];
@@ -1209,9 +1208,13 @@ __device__ __forceinline__ void direct_slice(const DirectDev &p, size_t n, size_
     eval(DB, OB, QB);
     carry(k0 + U);
   }
-  if (!tvalid) return;
 #pragma unroll
-  for (int c = 0; c < (CX ? 8 : 4); c++) acc[c] += (double)part[c];
+  for (int c = 0; c < (CX ? 8 : 4); c++) {
+    acc[c] += (double)part[c];
+#pragma unroll
+    for (int o = 1; o < kNlTeam; o <<= 1) acc[c] += __shfl_xor(acc[c], o);
+  }
+  if (!tvalid || (lane & (kNlTeam - 1)) != 0) return;
   if constexpr (FIXP) {  // gradient sums from fixed-point units to lengths
 #pragma unroll
     for (int c = (CX ? 2 : 1); c < (CX ? 8 : 4); c++) acc[c] *= p.fxInvD;
@@ -1241,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_direct_nl(DirectDev p, size_t n,
                                                    double csA, double *__restrict__ Vdir)
 {
   const int lane = threadIdx.x & 63;
-  const size_t nSlices = (n + 63) / 64;
+  const size_t nSlices = (n + kNlSlice - 1) / kNlSlice;
   const size_t w = (size_t)xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
   const size_t nw = (size_t)gridDim.x * 4;
   const size_t per = nSlices / nw, extra = nSlices - per * nw;
@@ -1261,7 +1264,7 @@ void launch_direct_nl(bool fp64, const DirectParams &p, size_t n, const void *sr
   const unsigned pad = nlist_pad(n, p.nlEshift);
   const DirectDev d = direct_dev(p);
   // a workgroup (4 wavefronts) per 4 slices
-  const long nwg = (long)((n + 255) / 256);
+  const long nwg = (long)((n + 4 * kNlSlice - 1) / (4 * kNlSlice));
   const dim3 grid((unsigned)std::max<long>(nwg, 1));
 #define NPME_DNL_D(R, K, HL, D)                                                                 \
   hipLaunchKernelGGL((k_direct_nl<R, K, HL, D>), grid, dim3(256), 0, s, d, n,                   \

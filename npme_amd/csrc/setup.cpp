@@ -281,11 +281,11 @@ static bool nlist_clearly_too_big(Handle &h, const std::string &mode)
   if (mode == "on") return false;
   size_t freeB = 0, totalB = 0;
   NPME_HIP_CHECK(hipMemGetInfo(&freeB, &totalB));
-  // no list exceeds 64 ceil(n / 64) slots per slice entry, n - 1 + 3
-  // entries per slot (padding to groups of 4): when even that fits, the
+  // no list exceeds kNlSlice ceil(n / kNlSlice) targets of n - 1 entries
+  // each, plus fewer than 4 kNlTeam of padding: when even that fits, the
   // estimate is not needed
   const double n = (double)h.n;
-  if ((n + 64.0) * (n + 2.0) * sizeof(unsigned) <= (double)(freeB / 4)) return false;
+  if ((n + kNlSlice) * (n + 4.0 * kNlTeam) * sizeof(unsigned) <= (double)(freeB / 4)) return false;
   const double cand = stencil_candidates_device(h.nb, (int)h.nNeigh, h.d_cellStart, h.stream);
   const double side = (2.0 * h.nNeigh + 1.0) * h.csB, R = h.ker.Rdir;
   const double est = cand * (4.18879020478639098 * R * R * R) / (side * side * side);
@@ -306,15 +306,14 @@ static void build_neighbor_list(Handle &h, const std::string &mode)
     NPME_HIP_CHECK(hipMemcpyAsync(cnt.data(), d_count.p, h.n * sizeof(int), hipMemcpyDeviceToHost, h.stream));
     NPME_HIP_CHECK(hipStreamSynchronize(h.stream));
   }
-  const size_t nSlices = (h.n + 63) / 64;
+  const size_t nSlices = (h.n + kNlSlice - 1) / kNlSlice;
   std::vector<long> sliceStart(nSlices + 1, 0);
   std::vector<int> sliceLen(nSlices, 0);
   for (size_t sl = 0; sl < nSlices; sl++) {
     int m = 0;
-    for (size_t s = 64 * sl; s < std::min(h.n, 64 * sl + 64); s++) m = std::max(m, cnt[s]);
-    m = (m + 3) & ~3;  // the list stores groups of 4 entries per target
-    sliceLen[sl] = m;
-    sliceStart[sl + 1] = sliceStart[sl] + 64L * m;
+    for (size_t s = kNlSlice * sl; s < std::min(h.n, kNlSlice * sl + kNlSlice); s++) m = std::max(m, cnt[s]);
+    sliceLen[sl] = nlist_lane_len(m);
+    sliceStart[sl + 1] = sliceStart[sl] + 64L * sliceLen[sl];
   }
   const size_t entries = (size_t)sliceStart[nSlices];
   size_t freeB = 0, totalB = 0;
@@ -428,7 +427,7 @@ static bool particles_device(Handle &h, GeoParams &g, const std::string &nlmode,
   bool list = nlPossible && !nlist_clearly_too_big(h, nlmode);
   if (list) {
     const DirectParams dp = direct_params(h);
-    const size_t nSlices = (n + 63) / 64;
+    const size_t nSlices = (n + kNlSlice - 1) / kNlSlice;
     size_t entries = 0;
     h.d_sliceStart = dalloc<long>(h, nSlices + 1);
     h.d_sliceLen = dalloc<int>(h, nSlices);

@@ -291,8 +291,8 @@ def test_direct_sum_paths_agree(oracle, prec, n, box, nNeigh, nClust):
     In FP64 the two sums agree to rounding.  In the mixed form each is
     checked against the FP64 path: the two FP32 sums round differently, so
     their mutual difference is up to the sum of their errors (measured at
-    60000 charges in 84^3: list 1.8e-7 / 1.22e-6, cell 1.8e-7 / 1.02e-6 in
-    V / grad V, difference 7e-8 / 1.25e-6)."""
+    60000 charges in 84^3: list 1.8e-7 / 1.04e-6, cell 1.8e-7 / 1.02e-6 in
+    V / grad V)."""
     print("case direct paths", n, prec, nNeigh, flush=True)
     coord, q = uniform_box(n, box)
     kw = dict(nNeigh=nNeigh, nCellClust1D=nClust)
