@@ -47,6 +47,27 @@ template <typename T> __device__ __forceinline__ T cmul(T a, T b)
 template <typename T> __device__ __forceinline__ T cconj(T a) { return mk<T>(a.x, -a.y); }
 template <typename T> __device__ __forceinline__ T mul_mi(T a) { return mk<T>(a.y, -a.x); }  // * -i
 
+// p[idx] when ok, else zero, with no branch around the load: the index is
+// clamped to 0 (p[0] must be readable) and the value selected afterwards.
+// A load under a per-lane condition becomes an exec-masked block whose
+// result is copied out at once, which waits for that load before the next
+// one is issued; with the select, a thread's loads are all in flight together
+template <typename T> __device__ __forceinline__ T load_or_zero(const T *p, long idx, bool ok)
+{
+  const T x = p[ok ? idx : 0];
+  return ok ? x : mk<T>(0, 0);
+}
+__device__ __forceinline__ float load_or_zero(const float *p, long idx, bool ok)
+{
+  const float x = p[ok ? idx : 0];
+  return ok ? x : 0.f;
+}
+__device__ __forceinline__ double load_or_zero(const double *p, long idx, bool ok)
+{
+  const double x = p[ok ? idx : 0];
+  return ok ? x : 0.0;
+}
+
 constexpr int NT = 256;  // threads per workgroup
 
 struct PlanDev {
@@ -303,15 +324,18 @@ __device__ __forceinline__ Lane lane_rows(int lgB, int N)
 
 // loads of one column into LDS: all of a thread's (at most EMAX) elements
 // are fetched into registers first, so its global loads are in flight
-// together, then stored; get(n) returns element n (n < N)
+// together, then stored; get(n) returns element n (n < N) with no branch
+// around its loads
 template <typename T, int EMAX, typename F>
 __device__ __forceinline__ void stage_in(T *lds, const Lane &L, int N, F &&get)
 {
   T v[EMAX];
+  // get runs for every slot (element 0 in place of those past N), so no
+  // load sits behind a per-lane branch
 #pragma unroll
   for (int u = 0; u < EMAX; u++) {
     const int n = L.tj + u * L.tpc;
-    if (n < N) v[u] = get(n);
+    v[u] = get(n < N ? n : 0);
   }
 #pragma unroll
   for (int u = 0; u < EMAX; u++) {
@@ -366,12 +390,8 @@ __global__ __launch_bounds__(NT) void k_fft_complex(CGeom g, PlanDev p, const ty
   const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (ROWS ? N + 1 : N), N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
-    T v = mk<T>(0, 0);
-    if (valid && n < g.Lin) {
-      v = src[(long)n * g.inPos];
-      if (backward) v = cconj(v);
-    }
-    return v;
+    const T v = load_or_zero(src, (long)n * g.inPos, valid && n < g.Lin);
+    return backward ? cconj(v) : v;
   });
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
@@ -397,10 +417,11 @@ __global__ __launch_bounds__(NT) void k_fft_r2c(int N, int Lin, int nPairs, int 
   const Lane L = lane_rows(lgB, N);
   const int pr = (xcd_wg() << lgB) + L.c;
   const bool valid = pr < nPairs;
-  const Real *ra = in + (long)(2 * pr) * inRow, *rb = ra + inRow;
+  const Real *ra = in + (valid ? (long)(2 * pr) * inRow : 0), *rb = ra + (valid ? inRow : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (N + 1), N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
-    return (valid && n < Lin) ? mk<T>(ra[n], rb[n]) : mk<T>(0, 0);
+    const bool ok = valid && n < Lin;
+    return mk<T>(load_or_zero(ra, n, ok), load_or_zero(rb, n, ok));
   });
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
@@ -431,18 +452,13 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
   const int pr = (xcd_wg() << lgB) + L.c;
   const bool valid = pr < nPairs;
   const int h = N / 2;
-  const T *ia = in + (long)(2 * pr) * inRow, *ib = ia + inRow;
+  const T *ia = in + (valid ? (long)(2 * pr) * inRow : 0), *ib = ia + (valid ? inRow : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (N + 1), N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
-    T v = mk<T>(0, 0);
-    if (valid) {
-      const int m = (n <= h) ? n : N - n;
-      const T a = ia[m], b = ib[m];
-      // Z = Xa + i Xb (n <= N/2), conj(Xa) + i conj(Xb) above; store conj(Z)
-      if (n <= h) v = mk<T>(a.x - b.y, -(a.y + b.x));
-      else v = mk<T>(a.x + b.y, a.y - b.x);
-    }
-    return v;
+    const int m = (n <= h) ? n : N - n;
+    const T a = load_or_zero(ia, m, valid), b = load_or_zero(ib, m, valid);
+    // Z = Xa + i Xb (n <= N/2), conj(Xa) + i conj(Xb) above; store conj(Z)
+    return (n <= h) ? mk<T>(a.x - b.y, -(a.y + b.x)) : mk<T>(a.x + b.y, a.y - b.x);
   });
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
@@ -512,7 +528,7 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
   T *col = data + (valid ? colg : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * N, N, tw);
   stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
-    return (valid && n < Lin) ? col[(long)n * pos] : mk<T>(0, 0);
+    return load_or_zero(col, (long)n * pos, valid && n < Lin);
   });
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
@@ -524,11 +540,10 @@ __global__ __launch_bounds__(NT) void k_fft_conv(int N, int Lin, int nCols, int 
 #pragma unroll
   for (int u = 0; u < E; u++) {  // the G loads of the thread, in flight together
     const int n = L.tj + u * L.tpc;
-    if (n < N) {
-      gv[u] = reinterpret_cast<const GT *>(Gv)[(long)g_a1<GM>(n, N, gfl) * 8 + gi.base];
-      if constexpr (GM == 2)
-        if (gfl & 8) gv[u] = cconj(gv[u]);
-    }
+    const int nn = n < N ? n : 0;  // unconditional load (the value past N is unused)
+    gv[u] = reinterpret_cast<const GT *>(Gv)[(long)g_a1<GM>(nn, N, gfl) * 8 + gi.base];
+    if constexpr (GM == 2)
+      if (gfl & 8) gv[u] = cconj(gv[u]);
   }
 #pragma unroll
   for (int u = 0; u < E; u++) {
@@ -687,13 +702,16 @@ __global__ __launch_bounds__(NT) void k_fft8_cols(CGeom g, const typename Cx<Rea
     const int j = tj + u * P::TPC;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-      T x = mk<T>(0, 0);
+      // the rows loaded are all below Lin (Lin = N, or N / 2 with HIN), and
+      // a column past the last reads column 0 and is not stored, so the
+      // loads need no condition and are all in flight together
       const int n = j + r * P::NB;
-      if ((!HIN || r < 4) && valid && n < g.Lin) {
-        x = src[(long)n * g.inPos];
-        if (backward) x = cconj(x);
+      if (HIN && r >= 4) {
+        v[u][r] = mk<T>(0, 0);
+      } else {
+        const T x = src[(long)n * g.inPos];
+        v[u][r] = backward ? cconj(x) : x;
       }
-      v[u][r] = x;
     }
   }
   // the barrier of first8 also orders the twiddle copy before its use
@@ -745,7 +763,9 @@ __global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, in
       gv[u][r] = reinterpret_cast<const GT *>(Gv)[(long)g_a1<GM>(n, N, gfl) * 8 + gi.base];
       if constexpr (GM == 2)
         if (gfl & 8) gv[u][r] = cconj(gv[u][r]);
-      v[u][r] = (r < 4 && valid) ? col[(long)n * pos] : mk<T>(0, 0);
+      // rows below N / 2 hold data; an invalid column reads column 0 and is
+      // not stored, so the loads need no condition
+      v[u][r] = r < 4 ? col[(long)n * pos] : mk<T>(0, 0);
     }
   }
   first8<T, L, true>(lds, c, tj, v);

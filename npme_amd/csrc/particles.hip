@@ -2036,46 +2036,41 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
             e3 = (int)min((long)E, M3 - u3);
   constexpr int W = CX ? 2 : 1;
   {
-    // (i, j, k) of element idx of the e1 x e2 x e3 block, advanced by the
-    // block size with carries instead of a division per element; 8
-    // elements per thread are loaded into registers before they are stored
-    // to LDS, so their global loads are in flight together
-    const int nb = blockDim.x, tot = e1 * e2 * e3;
-    const int dk = nb % e3, dj = (nb / e3) % e2, di = nb / (e3 * e2);
-    int idx = threadIdx.x, k = idx % e3, j = (idx / e3) % e2, i = idx / (e3 * e2);
-    const Real *g0 = grid + W * (((long)u1 * M2 + u2) * M3 + u3);
-    while (idx < tot) {
-      Real v[8][W];
-      int l[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++) {
-        l[u] = -1;
-        if (idx < tot) {
-          const long g = ((long)i * M2 + j) * M3 + k;
-          l[u] = (i * E + j) * E + k;
-#pragma unroll
-          for (int w = 0; w < W; w++) v[u][w] = g0[W * g + w];
-        }
-        idx += nb;
-        k += dk;
-        j += dj;
-        if (k >= e3) {
-          k -= e3;
-          j++;
-        }
-        i += di;
-        if (j >= e2) {
-          j -= e2;
-          i++;
-        }
+    // the e1 x e2 x e3 block of theta, copied from global memory straight
+    // into LDS (global_load_lds: lane l of a wavefront writes dword l of the
+    // 64-dword run the wavefront names, no registers in between), so every
+    // load of the block is in flight at once.  Element (i, j, k) sits at
+    // element ((i e2 + j) e3 + k) of the block, D dwords each; dword
+    // d = b + threadIdx.x of the copy is part d % D of element d / D, and as
+    // d advances by the block size each thread's part stays the same while
+    // its element advances by nb / D, with carries instead of divisions
+    constexpr int D = (int)sizeof(Real) * W / 4;
+    const int nb = blockDim.x, tot = e1 * e2 * e3, totw = tot * D, step = nb / D;
+    const int dk = step % e3, dj = (step / e3) % e2, di = step / (e3 * e2);
+    const int part = threadIdx.x % D;
+    int e = threadIdx.x / D, k = e % e3, j = (e / e3) % e2, i = e / (e3 * e2);
+    const unsigned *g0 = reinterpret_cast<const unsigned *>(grid + W * (((long)u1 * M2 + u2) * M3 + u3));
+    unsigned *thw = reinterpret_cast<unsigned *>(th) + (threadIdx.x & ~63);
+    for (int b = 0; b < totw; b += nb) {  // uniform per workgroup
+      if (b + (int)threadIdx.x < totw) {
+        const unsigned *src = g0 + (long)D * (((long)i * M2 + j) * M3 + k) + part;
+        __builtin_amdgcn_global_load_lds(
+            (const void __attribute__((address_space(1))) *)src,
+            (void __attribute__((address_space(3))) *)(thw + b), 4, 0, 0);
       }
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (l[u] >= 0) {
-#pragma unroll
-          for (int w = 0; w < W; w++) th[W * l[u] + w] = v[u][w];
-        }
+      k += dk;
+      j += dj;
+      if (k >= e3) {
+        k -= e3;
+        j++;
+      }
+      i += di;
+      if (j >= e2) {
+        j -= e2;
+        i++;
+      }
     }
+    __builtin_amdgcn_s_waitcnt(0);  // the copies have landed in LDS
   }
   __syncthreads();
   for (int c = beg + threadIdx.x; c < end; c += blockDim.x) {
@@ -2108,7 +2103,7 @@ __global__ __launch_bounds__(256) void k_interp_tile(int BnRun, int T, long M1, 
 #pragma unroll
       for (int j2 = 0; j2 < BA; j2++) {
         if (j2 >= Bn) break;
-        const Real *row = th + W * (((l1 + j1) * E + (l2 + j2)) * E + l3);
+        const Real *row = th + W * (((l1 + j1) * e2 + (l2 + j2)) * e3 + l3);
         Real tb = 0, tdb = 0, tbi = 0, tdbi = 0;
 #pragma unroll
         for (int j3 = 0; j3 < BA; j3++) {
