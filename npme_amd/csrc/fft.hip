@@ -604,6 +604,23 @@ template <int L> struct P8 {
   static constexpr int BPT = NB / TPC;      // butterflies per thread
 };
 
+// LDS slot of (row, column c) of a radix-8 workgroup: rows of B columns.
+// For the 512-point FP32 kernels a row (8 columns of 8 bytes) covers half of
+// the LDS banks, and which half follows the row's parity; the first stage's
+// writes of one wavefront go to rows 8 apart, all of one parity, so the rows
+// of every odd group of 8 are stored pairwise swapped (row ^ 1): the
+// wavefront's writes then alternate halves.  The other stages access whole
+// groups of 8 consecutive rows, which the swap only permutes.
+template <typename T, int L> constexpr bool lds_swap8() { return L == 3 && sizeof(T) == 8; }
+template <typename T, int L> __device__ __forceinline__ int slot8(int row, int c)
+{
+  using P = P8<L>;
+  if constexpr (lds_swap8<T, L>()) return ((row ^ ((row >> 3) & 1)) << P::LGB) + c;
+  else return (row << P::LGB) + c;
+}
+// LDS elements of the columns (the twiddle table follows them)
+template <typename T, int L> constexpr int lds_elems8() { return P8<L>::N << P8<L>::LGB; }
+
 // middle stage NS of a column: read x[j + r NB], twiddle, DFT8, write
 // y[(j - k) 8 + k + r NS] (LDS index * B + c), with a barrier between the
 // reads and the writes
@@ -616,7 +633,7 @@ __device__ __forceinline__ void mid8(T *lds, int c, int tj, const T *tw)
   for (int u = 0; u < P::BPT; u++) {
     const int j = tj + u * P::TPC;
 #pragma unroll
-    for (int r = 0; r < 8; r++) v[u][r] = lds[((j + r * P::NB) << P::LGB) + c];
+    for (int r = 0; r < 8; r++) v[u][r] = lds[slot8<T, L>(j + r * P::NB, c)];
   }
   __syncthreads();
 #pragma unroll
@@ -628,7 +645,7 @@ __device__ __forceinline__ void mid8(T *lds, int c, int tj, const T *tw)
     DFT<T, 8>::run(v[u]);
     const int o = (j - k) * 8 + k;
 #pragma unroll
-    for (int r = 0; r < 8; r++) lds[((o + r * NS) << P::LGB) + c] = v[u][r];
+    for (int r = 0; r < 8; r++) lds[slot8<T, L>(o + r * NS, c)] = v[u][r];
   }
   __syncthreads();
 }
@@ -642,7 +659,7 @@ __device__ __forceinline__ void last8(T *lds, int c, int tj, const T *tw, T (*v)
   for (int u = 0; u < P::BPT; u++) {
     const int j = tj + u * P::TPC;
 #pragma unroll
-    for (int r = 0; r < 8; r++) v[u][r] = lds[((j + r * P::NB) << P::LGB) + c];
+    for (int r = 0; r < 8; r++) v[u][r] = lds[slot8<T, L>(j + r * P::NB, c)];
   }
 #pragma unroll
   for (int u = 0; u < P::BPT; u++) {
@@ -665,7 +682,7 @@ __device__ __forceinline__ void first8(T *lds, int c, int tj, T (*v)[8])
     if constexpr (HALF) dft8_half(v[u]);
     else DFT<T, 8>::run(v[u]);
 #pragma unroll
-    for (int r = 0; r < 8; r++) lds[((8 * j + r) << P::LGB) + c] = v[u][r];
+    for (int r = 0; r < 8; r++) lds[slot8<T, L>(8 * j + r, c)] = v[u][r];
   }
   __syncthreads();
 }
@@ -694,7 +711,7 @@ __global__ __launch_bounds__(NT) void k_fft8_cols(CGeom g, const typename Cx<Rea
   const int o0 = colg / g.nInner;
   const bool valid = colg < g.nCols && colg - o0 * g.nInner < g.validInner;
   const int o = valid ? o0 : 0, i = valid ? colg - o0 * g.nInner : 0;
-  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, P::B * P::N, P::N, tw);
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, lds_elems8<T, L>(), P::N, tw);
   const T *src = in + ((long)o * g.inOuter + (long)i * g.inInner);
   T v[P::BPT][8];
 #pragma unroll
@@ -748,7 +765,7 @@ __global__ __launch_bounds__(NT) void k_fft8_conv(int nCols, int pos, int N2, in
   const int colg = (xcd_wg() << P::LGB) + c;
   const bool valid = colg < nCols && colg % C3 < Kv;
   T *col = data + (valid ? colg : 0);
-  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, P::B * N, N, tw);
+  const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, lds_elems8<T, L>(), N, tw);
   // the G values of the thread's rows, fetched up front
   const GIndex gi = g_index<GM>(N, N2, Kv, N3fold, gfl, valid, colg, C3);
   using GT = typename std::conditional<CG, T, Real>::type;
@@ -892,6 +909,10 @@ static size_t pass_lds(int N, int lgB, bool fp64, bool rows)
   return lds_bytes(N, lgB, fp64, rows) + (fp64 ? 0 : (size_t)N * 8);
 }
 
+// LDS of a radix-8 pass (N = 64 or 512): the columns, plus the twiddle
+// table in FP32
+static size_t pass_lds8(int N, bool fp64) { return pass_lds(N, N == 64 ? 6 : 3, fp64, false); }
+
 // L when the plan is L radix-8 stages of N = 8^L with L in {2, 3}, else 0
 static int pow8_level(const FFTPlan1D &p)
 {
@@ -940,7 +961,7 @@ void fft_pass_complex(bool fp64, const PassGeom &g, const FFTPlan1D &plan, const
   const int L8 = pow8 ? pow8_level(plan) : 0;
   if (!rows && L8 > 0 && (g.Lin == g.N || 2 * g.Lin == g.N) && g.Lout <= g.N) {
     // N = 8^L: the compile-time kernel (4096 / N columns per workgroup)
-    const size_t sh8 = pass_lds(g.N, L8 == 2 ? 6 : 3, fp64, false);
+    const size_t sh8 = pass_lds8(g.N, fp64);
     const bool half = 2 * g.Lin == g.N;
 #define NPME_F8(R, LV, H)                                                                      \
   hipLaunchKernelGGL((k_fft8_cols<R, LV, H>), grid, dim3(NT), sh8, s, c,                       \
@@ -1064,8 +1085,9 @@ void fft_pass_convolve(bool fp64, bool complexG, int N, int Lin, long nCols, lon
   const int N3fold = complexG ? (int)Kv : 0;
   const int L8 = pow8 ? pow8_level(plan) : 0;
   if (L8 > 0 && 2 * Lin == N) {
+    const size_t sh8 = pass_lds8(N, fp64);
 #define NPME_C8(R, LV, GMV)                                                                    \
-  hipLaunchKernelGGL((k_fft8_conv<R, LV, GMV>), grid, dim3(NT), sh, s, (int)nCols,             \
+  hipLaunchKernelGGL((k_fft8_conv<R, LV, GMV>), grid, dim3(NT), sh8, s, (int)nCols,           \
                      (int)pos_stride, N2, gflags, (int)C3, (int)Kv, N3fold,                     \
                      (const typename Cx<R>::T *)tw, G, (typename Cx<R>::T *)data)
 #define NPME_C8L(R, GMV)                                                                       \

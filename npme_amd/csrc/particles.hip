@@ -1530,8 +1530,26 @@ template <typename Real> __device__ __forceinline__ void bsp_shifts(int n, doubl
   for (int j = 0; j < n; j++) m[j] = (Real)w[j];
 }
 
+// one level of the unnormalised recursion, order k - 1 -> k, in place:
+// U_k(t + j) = (t + j) U_{k-1}(t + j) + (k - t - j) U_{k-1}(t + j - 1), with
+// U_k = (k - 1)! M_k; the end terms j = 0 and j = k - 1 have one factor
+template <int K> __device__ __forceinline__ void bsp_level(double t, double *w)
+{
+  const double top = (1.0 - t) * w[K - 2];
+#pragma unroll
+  for (int j = K - 2; j >= 1; j--) w[j] = (t + j) * w[j] + ((double)(K - j) - t) * w[j - 1];
+  w[0] *= t;
+  w[K - 1] = top;
+}
+template <int N> struct Fact {
+  static constexpr double v = N * Fact<N - 1>::v;
+};
+template <> struct Fact<0> {
+  static constexpr double v = 1.0;
+};
+
 // the recursion with the order known at compile time (fully unrolled,
-// registers); w[0..N-1] = M_N(t + j)
+// registers); w[0..N-1] = M_N(t + j), scaled by 1 / (N - 1)! once
 template <int N> __device__ __forceinline__ void bsp_rec_c(double t, double *w)
 {
   if constexpr (N == 1) {
@@ -1539,17 +1557,25 @@ template <int N> __device__ __forceinline__ void bsp_rec_c(double t, double *w)
   } else {
     w[0] = t;
     w[1] = 1.0 - t;
+    if constexpr (N >= 3) bsp_level<3>(t, w);
+    if constexpr (N >= 4) bsp_level<4>(t, w);
+    if constexpr (N >= 5) bsp_level<5>(t, w);
+    if constexpr (N >= 6) bsp_level<6>(t, w);
+    if constexpr (N >= 7) bsp_level<7>(t, w);
+    if constexpr (N >= 8) bsp_level<8>(t, w);
+    if constexpr (N >= 9) bsp_level<9>(t, w);
+    if constexpr (N >= 10) bsp_level<10>(t, w);
+    if constexpr (N >= 11) bsp_level<11>(t, w);
+    if constexpr (N >= 12) bsp_level<12>(t, w);
+    if constexpr (N >= 13) bsp_level<13>(t, w);
+    if constexpr (N >= 14) bsp_level<14>(t, w);
+    if constexpr (N >= 15) bsp_level<15>(t, w);
+    if constexpr (N >= 16) bsp_level<16>(t, w);
+    static_assert(N <= 16, "spline orders up to 16");
+    if constexpr (N >= 3) {
+      constexpr double sc = 1.0 / Fact<N - 1>::v;
 #pragma unroll
-    for (int k = 3; k <= N; k++) {
-      const double inv = 1.0 / (double)(k - 1);
-      double prev = 0.0;
-#pragma unroll
-      for (int j = 0; j < k; j++) {
-        const double left = (j < k - 1) ? w[j] : 0.0;
-        const double val = ((t + j) * left + ((double)k - t - j) * prev) * inv;
-        prev = left;
-        w[j] = val;
-      }
+      for (int j = 0; j < N; j++) w[j] *= sc;
     }
   }
 }
@@ -1575,15 +1601,14 @@ __device__ __forceinline__ void bsp_weights_grad_c(double t, Real *B, Real *dB)
   for (int j = 0; j < N; j++)
     dB[j] = (Real)(((j < N - 1) ? w[j] : 0.0) - ((j >= 1) ? w[j - 1] : 0.0));
   {
-    // one more level of the recursion: order N - 1 -> N
-    const double inv = 1.0 / (double)(N - 1);
-    double prev = 0.0;
+    // one more level of the recursion: order N - 1 -> N (the end terms
+    // have one factor)
+    constexpr double inv = 1.0 / (double)(N - 1);
+    const double top = (1.0 - t) * w[N - 2];
 #pragma unroll
-    for (int j = 0; j < N; j++) {
-      const double left = (j < N - 1) ? w[j] : 0.0;
-      w[j] = ((t + j) * left + ((double)N - t - j) * prev) * inv;
-      prev = left;
-    }
+    for (int j = N - 2; j >= 1; j--) w[j] = ((t + j) * w[j] + ((double)(N - j) - t) * w[j - 1]) * inv;
+    w[0] *= t * inv;
+    w[N - 1] = top * inv;
   }
 #pragma unroll
   for (int j = 0; j < N; j++) B[j] = (Real)w[j];
@@ -1734,10 +1759,13 @@ __global__ __launch_bounds__(64) void k_spread_mfma(int BnRun, long M1, long M2,
 {
   constexpr int TX = 16, TY = 8, TZ = 8;
   // rows of 16 floats: the 16 lanes of an operand group read one row, the
-  // four groups of a step read four consecutive rows (no bank conflicts)
-  __shared__ __align__(16) float sA[64][TX];
-  __shared__ __align__(16) float sW2[64][TY];
-  __shared__ __align__(16) float sW3[64][TZ];
+  // four groups of a step read four consecutive rows (no bank conflicts).
+  // The three row blocks are one 8 KB block, cleared per chunk by the
+  // wavefront in contiguous 1 KB runs
+  __shared__ __align__(16) float stage[64 * (TX + TY + TZ)];
+  float(*sA)[TX] = reinterpret_cast<float(*)[TX]>(stage);
+  float(*sW2)[TY] = reinterpret_cast<float(*)[TY]>(stage + 64 * TX);
+  float(*sW3)[TZ] = reinterpret_cast<float(*)[TZ]>(stage + 64 * (TX + TY));
   const int Bn = BN > 0 ? BN : BnRun;
   constexpr int BA = BN > 0 ? BN : 16;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -1771,13 +1799,11 @@ __global__ __launch_bounds__(64) void k_spread_mfma(int BnRun, long M1, long M2,
     // w2[5], w2[7]): a lane's four B columns j = 2b + h (h = n16 / 8) are
     // then one 16-byte read
     {
+      // the workgroup is this one wavefront, whose LDS accesses execute in
+      // program order, so the window writes below follow the clearing
       const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-      for (int k = 0; k < TX; k += 4) *reinterpret_cast<float4 *>(&sA[l][k]) = z4;
-#pragma unroll
-      for (int k = 0; k < TY; k += 4) *reinterpret_cast<float4 *>(&sW2[l][k]) = z4;
-#pragma unroll
-      for (int k = 0; k < TZ; k += 4) *reinterpret_cast<float4 *>(&sW3[l][k]) = z4;
+      for (int k = 0; k < (TX + TY + TZ) / 4; k++) reinterpret_cast<float4 *>(stage)[64 * k + l] = z4;
     }
     if (l < cnt) {
       float sh[BA];
@@ -1852,9 +1878,12 @@ __global__ __launch_bounds__(64) void k_spread_mfma64(int BnRun, long M1, long M
                                                       double *__restrict__ grid)
 {
   constexpr int TX = 16, TY = 8, TZ = 8;
-  __shared__ __align__(16) double sA[64][TX];
-  __shared__ __align__(16) double sW2[64][TY];
-  __shared__ __align__(16) double sW3[64][TZ];
+  // one 16 KB block of rows, cleared per chunk by the wavefront in
+  // contiguous runs (as in k_spread_mfma)
+  __shared__ __align__(16) double stage[64 * (TX + TY + TZ)];
+  double(*sA)[TX] = reinterpret_cast<double(*)[TX]>(stage);
+  double(*sW2)[TY] = reinterpret_cast<double(*)[TY]>(stage + 64 * TX);
+  double(*sW3)[TZ] = reinterpret_cast<double(*)[TZ]>(stage + 64 * (TX + TY));
   const int Bn = BN > 0 ? BN : BnRun;
   constexpr int BA = BN > 0 ? BN : 16;
   const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -1879,12 +1908,11 @@ __global__ __launch_bounds__(64) void k_spread_mfma64(int BnRun, long M1, long M
     const int cnt = min(64, end - base);
     // row l as in k_spread_mfma: w2 stored (w2[0], w2[2], w2[4], w2[6],
     // w2[1], w2[3], w2[5], w2[7])
+    {
+      const double2 z2 = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int k = 0; k < TX; k++) sA[l][k] = 0;
-#pragma unroll
-    for (int k = 0; k < TY; k++) sW2[l][k] = 0;
-#pragma unroll
-    for (int k = 0; k < TZ; k++) sW3[l][k] = 0;
+      for (int k = 0; k < (TX + TY + TZ) / 2; k++) reinterpret_cast<double2 *>(stage)[64 * k + l] = z2;
+    }
     if (l < cnt) {
       double sh[BA];
       if constexpr (BN > 0) bsp_shifts_c<double, BN>(cf.x, sh);
