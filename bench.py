@@ -48,6 +48,10 @@ def parse():
     p.add_argument("--slots", type=int, default=4, help="concurrent request slots of the server")
     p.add_argument("--pageable-requests", action="store_true",
                    help="serving client builds requests in pageable instead of page-locked memory")
+    p.add_argument("--serving-only", action="store_true",
+                   help="run only the serving stream and print its latency JSON (the child "
+                        "process of a full run)")
+    p.add_argument("--crash-log", default="", help=argparse.SUPPRESS)
     p.add_argument("--stages-only", action="store_true",
                    help="set up the N=1M workload and run --steps CalcV calls with the stages "
                         "serialised on one stream and timed by HIP events (the timing the "
@@ -278,8 +282,45 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
                         % (rate, seed, "page-locked (pool of 24)" if pinned else "pageable")}
 
 
+def serving_stream_child(a):
+    """The serving stream in a child process: a crash there (the process
+    ends on a signal) leaves the rest of the bench line intact and is
+    reported in it, with the child's fault-handler traceback."""
+    import subprocess
+    import tempfile
+    fd, crash = tempfile.mkstemp(prefix="npme_stream_", suffix=".log")
+    os.close(fd)
+    cmd = [sys.executable, os.path.abspath(__file__), "--serving-only", "--crash-log", crash,
+           "--stream-requests", str(a.stream_requests), "--stream-rate", str(a.stream_rate),
+           "--slots", str(a.slots)]
+    if a.pageable_requests:
+        cmd.append("--pageable-requests")
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=900, text=True)
+        lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+        if r.returncode == 0 and lines:
+            return json.loads(lines[-1])["latency"]
+        tail = open(crash).read()[-4000:]
+        log("serving stream process failed with status %d" % r.returncode)
+        sys.stderr.write(tail)
+        return {"error": "serving stream process exited with status %d" % r.returncode,
+                "traceback": tail}
+    except subprocess.TimeoutExpired:
+        return {"error": "serving stream process exceeded 900 s"}
+    finally:
+        os.unlink(crash)
+
+
 def main():
+    import faulthandler
     a = parse()
+    if a.serving_only:
+        faulthandler.enable(file=open(a.crash_log, "w") if a.crash_log else sys.stderr,
+                            all_threads=True)
+        lat = serving_stream(a.stream_requests, a.stream_rate, a.slots, pinned=not a.pageable_requests)
+        print(json.dumps({"latency": lat}))
+        return 0
+    faulthandler.enable(all_threads=True)
     if a.gpus != 1:
         print("bench.py measures one MI355X: --gpus %d is not supported (only --gpus 1)" % a.gpus,
               file=sys.stderr)
@@ -399,9 +440,9 @@ def main():
     latency = None
     if a.stream_requests > 0:
         log("serving stream: %d requests at %g/s on %d slots" % (a.stream_requests, a.stream_rate, a.slots))
-        latency = serving_stream(a.stream_requests, a.stream_rate, a.slots,
-                                 pinned=not a.pageable_requests)
-        log("stream: p50 %.2f ms p99 %.2f ms" % (latency["p50_ms"], latency["p99_ms"]))
+        latency = serving_stream_child(a)
+        if "p50_ms" in latency:
+            log("stream: p50 %.2f ms p99 %.2f ms" % (latency["p50_ms"], latency["p99_ms"]))
 
     cpu = None
     if not a.no_cpu_baseline:
