@@ -395,7 +395,7 @@ def main():
     Mt = M1 * M2 * M3
     Fc = (Ngrid[0] // 2 + 1) * (Ngrid[1] // 2 + 1) * K3
     algo = {
-        "gather": ("hbm", n * (4 + 8 + 4)),
+        "gather": ("hbm", n * (4 + 8 + 4 + 4)),
         "direct": ("fp32", P * (20 + 4 * Nder)),
         "spread": ("hbm", n * 16 + 4 * Mt),
         "fftA_r2c_z": ("hbm", 4 * Mt + 8 * M1 * M2 * K3),

@@ -532,10 +532,12 @@ void user_exact_host(const UserKernels &u, size_t n, const double *coord, const 
 // the same count with the partials in a static device array (noalloc.hip)
 double handle_count_pairs_noalloc(Handle &h);
 DirectParams direct_params(const Handle &h);
-void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
-                   void *src, void *qi, hipStream_t s);
 void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                      void *qre, void *qim, hipStream_t s);
+// the charges in sorted order for both sums in one pass: the direct sum's
+// packed source charges (src[s].w, qi) and the spreading copy (qre, qim)
+void launch_gather_both(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
+                        void *src, void *qi, void *qre, void *qim, hipStream_t s);
 void launch_direct(bool fp64, const DirectParams &p, size_t n, const void *src, const void *res,
                    const void *qi, const int4 *cell, const int *segStart, const void *segs, int nP, double csA,
                    double *Vdir, hipStream_t s);

@@ -83,21 +83,6 @@ __device__ __forceinline__ void horner_fix_cx(const Real *c, Real r2, Real &re, 
 }
 
 // ------------------------------------------------------------------ gather
-template <typename Real, bool CX>
-__global__ void k_gather(size_t n, const int *__restrict__ perm, const double *__restrict__ q,
-                         typename V4t<Real>::T *__restrict__ src, Real *__restrict__ qi)
-{
-  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int o = perm[s];
-  if (CX) {
-    src[s].w = (Real)q[2 * (size_t)o];
-    qi[s] = (Real)q[2 * (size_t)o + 1];
-  } else {
-    src[s].w = (Real)q[o];
-  }
-}
-
 // charges in sorted order for the reciprocal sum (re and im in separate
 // arrays), so spreading does not read the direct sum's packed sources
 template <typename Real, bool CX>
@@ -115,6 +100,47 @@ __global__ void k_gather_q(size_t n, const int *__restrict__ perm, const double 
   }
 }
 
+// both at once: one read of each charge (the random gather through perm)
+// writes the direct sum's packed source charge and the spreading copy
+template <typename Real, bool CX>
+__global__ void k_gather_both(size_t n, const int *__restrict__ perm, const double *__restrict__ q,
+                              typename V4t<Real>::T *__restrict__ src, Real *__restrict__ qi,
+                              Real *__restrict__ qre, Real *__restrict__ qim)
+{
+  const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int o = perm[s];
+  if (CX) {
+    const Real re = (Real)q[2 * (size_t)o], im = (Real)q[2 * (size_t)o + 1];
+    src[s].w = re;
+    qi[s] = im;
+    qre[s] = re;
+    qim[s] = im;
+  } else {
+    const Real re = (Real)q[o];
+    src[s].w = re;
+    qre[s] = re;
+  }
+}
+
+void launch_gather_both(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
+                        void *src, void *qi, void *qre, void *qim, hipStream_t s)
+{
+  const dim3 grid((unsigned)((n + 255) / 256));
+#define NPME_GB(R, CXV)                                                                        \
+  hipLaunchKernelGGL((k_gather_both<R, CXV>), grid, dim3(256), 0, s, n, perm, q_in,             \
+                     (typename V4t<R>::T *)src, (R *)qi, (R *)qre, (R *)qim)
+  if (fp64) {
+    if (complexK) NPME_GB(double, true);
+    else NPME_GB(double, false);
+  } else {
+    if (complexK) NPME_GB(float, true);
+    else NPME_GB(float, false);
+  }
+#undef NPME_GB
+  NPME_HIP_CHECK(hipGetLastError());
+}
+
 void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
                      void *qre, void *qim, hipStream_t s)
 {
@@ -129,28 +155,6 @@ void launch_gather_q(bool fp64, bool complexK, size_t n, const int *perm, const 
     else NPME_GQ(float, false);
   }
 #undef NPME_GQ
-  NPME_HIP_CHECK(hipGetLastError());
-}
-
-void launch_gather(bool fp64, bool complexK, size_t n, const int *perm, const double *q_in,
-                   void *src, void *qi, hipStream_t s)
-{
-  const dim3 grid((unsigned)((n + 255) / 256));
-  if (fp64) {
-    if (complexK)
-      hipLaunchKernelGGL((k_gather<double, true>), grid, dim3(256), 0, s, n, perm, q_in,
-                         (double4 *)src, (double *)qi);
-    else
-      hipLaunchKernelGGL((k_gather<double, false>), grid, dim3(256), 0, s, n, perm, q_in,
-                         (double4 *)src, (double *)qi);
-  } else {
-    if (complexK)
-      hipLaunchKernelGGL((k_gather<float, true>), grid, dim3(256), 0, s, n, perm, q_in,
-                         (float4 *)src, (float *)qi);
-    else
-      hipLaunchKernelGGL((k_gather<float, false>), grid, dim3(256), 0, s, n, perm, q_in,
-                         (float4 *)src, (float *)qi);
-  }
   NPME_HIP_CHECK(hipGetLastError());
 }
 

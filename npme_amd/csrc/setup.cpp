@@ -1349,45 +1349,44 @@ static void calcv_launches(Handle &h, const double *d_q, double *d_Vout, bool sp
   const DirectParams dp = direct_params(h);
   const bool doDirect = !h.trans;  // a box pair has no direct sum
   const bool refPairs = kind_ref_pairs(h.ker.kind);
-  auto direct = [&] {
-    launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s2);
+  // the charges in sorted order: one gather on stream writes the direct
+  // sum's packed sources and the spreading copy (a box pair has no direct
+  // sum, so only the spreading copy)
+  auto gather = [&] {
+    if (doDirect)
+      launch_gather_both(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, h.d_qs, h.d_qsi, s);
+    else
+      launch_gather_q(f64, cx, h.n, h.d_perm, d_q, h.d_qs, h.d_qsi, s);
+  };
+  auto direct = [&](hipStream_t ds) {
     if (refPairs)
       launch_direct_refpairs(dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, h.d_perm, d_q, h.d_cellFlags,
-                             h.d_Vdir, s2);
+                             h.d_Vdir, ds);
     else if (h.useNlist)
       launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
-                       h.d_nlist, h.csA, h.d_Vdir, s2);
+                       h.d_nlist, h.csA, h.d_Vdir, ds);
     else
       launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
-                    (int)h.nCellClust1D, h.csA, h.d_Vdir, s2);
+                    (int)h.nCellClust1D, h.csA, h.d_Vdir, ds);
   };
-  if (split && doDirect) {
-    NPME_HIP_CHECK(hipEventRecord(h.evFork, s));
-    NPME_HIP_CHECK(hipStreamWaitEvent(s2, h.evFork, 0));
-    direct();
-    NPME_HIP_CHECK(hipEventRecord(h.evJoin, s2));
-  }
   int st = 0;
   mark(h, st++);
+  gather();
+  if (split && doDirect) {
+    NPME_HIP_CHECK(hipEventRecord(h.evFork, s));  // after the gather
+    NPME_HIP_CHECK(hipStreamWaitEvent(s2, h.evFork, 0));
+    direct(s2);
+    NPME_HIP_CHECK(hipEventRecord(h.evJoin, s2));
+  }
+  mark(h, st++);  // stage "gather"
   if (!split && doDirect) {
-    // the same kernels in order, timed one by one: gather, direct
-    launch_gather(f64, cx, h.n, h.d_perm, d_q, h.d_src, h.d_qi, s);
-    mark(h, st++);
-    if (refPairs)
-      launch_direct_refpairs(dp, h.n, h.d_cell, h.d_cellStart, h.d_xs, h.d_perm, d_q, h.d_cellFlags,
-                             h.d_Vdir, s);
-    else if (h.useNlist)
-      launch_direct_nl(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_sliceStart, h.d_sliceLen,
-                       h.d_nlist, h.csA, h.d_Vdir, s);
-    else
-      launch_direct(f64, dp, h.n, h.d_src, h.d_srcRes, h.d_qi, h.d_cell, h.d_segStart, h.d_blockBox,
-                    (int)h.nCellClust1D, h.csA, h.d_Vdir, s);
+    // the direct sum in order on stream, timed as its own stage
+    direct(s);
     mark(h, st++);
   }
-  launch_gather_q(f64, cx, h.n, h.d_perm, d_q, h.d_qs, h.d_qsi, s);
   launch_spread(f64, cx, h.Bn, h.M, h.T, h.nt, h.d_tileStart, h.d_tileList, h.d_qs, h.d_qsi,
                 h.d_ms, h.d_frac, h.d_grid, s);
-  mark(h, st++);  // stage "spread" includes the charge gather of the reciprocal sum
+  mark(h, st++);  // stage "spread"
   if (!cx) {
     fft_pass_r2c_rows(f64, (int)N3, (int)M3, M1 * M2, M3, Kp, h.plan[2], h.d_tw[2], h.d_grid,
                       h.d_buf1, s);
