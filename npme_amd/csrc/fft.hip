@@ -454,12 +454,33 @@ __global__ __launch_bounds__(NT) void k_fft_c2r(int N, int Lout, int nPairs, int
   const int h = N / 2;
   const T *ia = in + (valid ? (long)(2 * pr) * inRow : 0), *ib = ia + (valid ? inRow : 0);
   const T *twl = twiddles_in<T, twl_lds<Real>()>(lds, (1 << lgB) * (N + 1), N, tw);
-  stage_in<T, emax<Real>()>(lds, L, N, [&](int n) {
-    const int m = (n <= h) ? n : N - n;
-    const T a = load_or_zero(ia, m, valid), b = load_or_zero(ib, m, valid);
-    // Z = Xa + i Xb (n <= N/2), conj(Xa) + i conj(Xb) above; store conj(Z)
-    return (n <= h) ? mk<T>(a.x - b.y, -(a.y + b.x)) : mk<T>(a.x + b.y, a.y - b.x);
-  });
+  {
+    // each stored element m = 0 .. N/2 of the two Hermitian rows is loaded
+    // once and gives both rows n = m and n = N - m of the column:
+    // Z = Xa + i Xb (n <= N/2), conj(Xa) + i conj(Xb) above; stored conj(Z).
+    // The thread's loads are all issued (index clamped, no branch around
+    // them) before the LDS stores; nu, the slots a thread needs, is uniform
+    constexpr int EH = emax<Real>() / 2 + 1;
+    const int nu = (h + L.tpc) / L.tpc;
+    T a[EH], b[EH];
+#pragma unroll
+    for (int u = 0; u < EH; u++) {
+      if (u < nu) {
+        const int m = L.tj + u * L.tpc;
+        const bool ok = valid && m <= h;
+        a[u] = load_or_zero(ia, m <= h ? m : 0, ok);
+        b[u] = load_or_zero(ib, m <= h ? m : 0, ok);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < EH; u++) {
+      const int m = L.tj + u * L.tpc;
+      if (u < nu && m <= h) {
+        lds[L.col.at(m)] = mk<T>(a[u].x - b[u].y, -(a[u].y + b[u].x));
+        if (m > 0 && m < h) lds[L.col.at(N - m)] = mk<T>(a[u].x + b[u].y, a[u].y - b[u].x);
+      }
+    }
+  }
   __syncthreads();
   fft_columns<T, emax<Real>(), RC>(lds, L.col, L.tj, L.tpc, p, twl);
   if (!valid) return;
