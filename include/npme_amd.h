@@ -279,8 +279,10 @@ int npme_boxpair_calcv_complex(npme_handle *h, double *V, size_t nTarget, size_t
 int npme_boxpair_separation(const npme_handle *h, double R0[3], double *rMin);
 
 /* ---- per-stage device times of the last CalcV when profiling is on:
- * ms[0] gather, [1] direct sum, [2] spread, [3..7] FFT passes A..E,
- * [8] interpolate+combine; returns the number of stages written. ------- */
+ * ms[0] gather (the charges in sorted order for both sums), [1] direct
+ * sum, [2] spread, [3..7] FFT passes A..E, [8] interpolate+combine (a box
+ * pair, which has no direct sum, omits [1]); returns the number of stages
+ * written. ------- */
 int npme_set_profiling(npme_handle *h, int on);
 int npme_get_stage_times(const npme_handle *h, double *ms, int n);
 

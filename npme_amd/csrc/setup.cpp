@@ -1197,16 +1197,21 @@ void handle_setup(Handle &h, const Keywords &kw, size_t n, const double *coord)
     h.nt[d] = g.nt[d] = (int)((h.M[d] + h.T[d] - 1) / h.T[d]);
   }
   {
-    // largest power-of-two tile edge (16 at most) whose theta block plus
-    // halo fits 64 KB; every accepted order fits at some edge >= 1
-    // (complex FP64 at order 16: edge 1, 16^3 points of 16 bytes)
+    // largest tile edge of 16, 12, 8, 6, 4, 2, 1 whose theta block plus halo
+    // fits 64 KB (FP64 real at order 6: 12, a 17^3 block of 39 KB, which
+    // reads the grid 2.8 times over against 4.3 times for edge 8); every
+    // accepted order fits at some edge >= 1 (complex FP64 at order 16:
+    // edge 1, 16^3 points of 16 bytes)
     const long es = (h.fp64 ? 8 : 4) * (h.complexK ? 2 : 1);
     auto bytes = [&](int TI) {
       const long E = TI + h.Bn - 1;
       return E * E * E * es;
     };
-    h.TI = 16;
-    while (h.TI > 1 && bytes(h.TI) > 65536) h.TI /= 2;
+    const int edges[] = {16, 12, 8, 6, 4, 2, 1};
+    for (int e : edges) {
+      h.TI = e;
+      if (bytes(e) <= 65536) break;
+    }
     int maxLds = 0;
     NPME_HIP_CHECK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, h.device));
     if (bytes(h.TI) > std::min<long>(65536, maxLds)) {
