@@ -165,7 +165,20 @@ __device__ __forceinline__ void sincos_r(double x, double *s, double *c) { sinco
 // v_rsq_f32 (1 ulp): r2 of a pair is never denormal, so the range fix-up
 // of rsqrtf is not needed
 template <> __device__ __forceinline__ float rsqrt_r<float>(float x) { return __builtin_amdgcn_rsqf(x); }
-template <> __device__ __forceinline__ double rsqrt_r<double>(double x) { return 1.0 / sqrt(x); }
+// FP64: the hardware estimate (v_rsq_f64, about 23 bits) refined by two
+// Newton steps y <- y + y (1 - x y^2) / 2, whose quadratic convergence
+// reaches full FP64 precision (about 1 ulp; the correctly rounded
+// 1 / sqrt(x) costs a square root and a division with their fix-ups)
+template <> __device__ __forceinline__ double rsqrt_r<double>(double x)
+{
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
 
 // Horner in r2 of a real coefficient set (stride-2 storage: re parts)
 template <typename Real>
