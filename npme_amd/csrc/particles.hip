@@ -1,11 +1,15 @@
 // Particle-side kernels of the NPME CalcV pipeline on gfx950:
-//   gather   charges into cell-sorted order
-//   direct   short-range DM sum over the cell list (one target per lane,
-//            per-target cell pruning, FP32 pair math with FP64 per-cell
-//            accumulation in the mixed form)
-//   spread   B-spline charge spreading, one 8x8x8 grid tile per wavefront,
-//            the tile accumulated in registers (no atomics; every grid point
-//            sums its charges in list order, so results are deterministic)
+//   gather   charges into cell-sorted order, for both sums in one pass
+//   direct   short-range DM sum: the neighbour-list walk (a team of 4 lanes
+//            per target, FP32 pair math carried into FP64 in the mixed
+//            form), the cell-row walk for sets whose list does not fit, and
+//            the reference's interacting cell pairs for the erfc and user
+//            kernels; the list build (count and fill passes)
+//   spread   B-spline charge spreading, one grid tile per wavefront (16 x 8
+//            x 8 on the matrix cores for real charges, 8^3 on the VALUs for
+//            complex ones), the tile accumulated in registers (no atomics;
+//            every grid point sums its charges in list order, so results
+//            are deterministic)
 //   interp   B-spline interpolation of theta + combine with the direct sum
 //            and the self term, scattered back to input order in FP64
 //   exact    O(N^2) FP64 sum of the full kernel (CalcV_exact)
