@@ -59,7 +59,8 @@ struct Server {
   RequestBoard<Job> board;
   std::vector<std::thread> workers;
   size_t budget = 0, reserved = 0;
-  int running = 0;
+  int running = 0;  // admitted requests (a batch counts each member)
+  int active = 0;   // slots holding admitted requests (a batch counts once)
   std::set<long> growing;  // running requests waiting to enlarge their reservation
   // small requests queued together run as one batch on a slot: their
   // set-ups in turn, then their CalcVs captured as one graph
@@ -111,8 +112,11 @@ static size_t footprint_bytes(size_t n, const long N[3], bool fp64, bool cx, dou
 }
 
 // step 2: the reservation of a running request becomes need; a raise that
-// does not fit waits until it fits, or the request runs alone, or every
-// other running request is waiting to raise too (then the oldest goes on)
+// does not fit waits until it fits, or its slot is the only one at work, or
+// every other working slot is waiting to raise too (then the oldest goes on).
+// Slots, not requests, are counted: a batch's members are set up one after
+// another on one slot, so at most one of them waits at a time, and counting
+// the others would leave every slot waiting for good
 static void set_reservation(Server *S, Job &j, size_t need)
 {
   if (need <= j.reserve) {
@@ -126,8 +130,8 @@ static void set_reservation(Server *S, Job &j, size_t need)
   S->board.adjust(
       [&] {
         const bool fits = S->reserved - j.reserve + need <= S->budget;
-        const bool stuck = (int)S->growing.size() >= S->running && *S->growing.begin() == j.ticket;
-        return fits || S->running == 1 || stuck;
+        const bool stuck = (int)S->growing.size() >= S->active && *S->growing.begin() == j.ticket;
+        return fits || S->active == 1 || stuck;
       },
       [&] {
         S->growing.erase(j.ticket);
@@ -322,7 +326,10 @@ static void worker_main(Server *S)
       // budget then runs alone)
       Job *j = S->board.take(
           [S](const Job &hd) { return S->running == 0 || S->reserved + hd.reserve <= S->budget; },
-          onTake);
+          [S, &onTake](Job &jb) {
+            onTake(jb);
+            S->active++;
+          });
       if (!j) break;
       std::vector<Job *> batch{j};
       // small requests queued behind it join it (while the budget allows)
@@ -352,6 +359,7 @@ static void worker_main(Server *S)
           jb.tEnd = S->now();
           S->reserved -= jb.reserve;
           S->running--;
+          if (k + 1 == batch.size()) S->active--;
         });
     }
   }

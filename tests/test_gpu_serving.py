@@ -225,3 +225,39 @@ def test_server_batches_equal_alone():
     assert all(inf["batch"] == 1 for _, inf in alone)
     for (V, _), (Va, _), (ft, _, _, _, _) in zip(out, alone, reqs):
         assert np.array_equal(V, Va), ft
+
+
+def test_server_batch_raise_over_budget():
+    """A batch whose members' footprints exceed the budget once their grids
+    are known runs to the end: the members are set up in turn on one slot,
+    so a member waiting to raise its reservation is the only work waiting
+    and goes on (as a request that runs alone does).  A large request holds
+    the one slot while two dense 20k requests (list-heavy footprints above
+    their provisional estimates) queue behind it and then run as a batch of
+    two under a 0.25 GB budget; their results equal the same requests run
+    alone."""
+    import npme_amd as N
+    rng = np.random.default_rng(71)
+    n0 = 200000
+    L0 = (n0 / 0.1) ** (1.0 / 3.0)
+    q0 = rng.uniform(-1.0, 1.0, n0)
+    blocker = ("Laplace", rng.uniform(0.0, L0, (n0, 3)), q0 - q0.mean(), 1e-4, {})
+    dense = []
+    for _ in range(2):
+        n = 20000
+        L = (n / 1.0) ** (1.0 / 3.0)
+        q = rng.uniform(-1.0, 1.0, n)
+        dense.append(("Laplace", rng.uniform(0.0, L, (n, 3)), q - q.mean(), 1e-6, {}))
+    print("case server batch raise over a 0.25 GB budget", flush=True)
+    S = N.NPME_Server(nSlots=1, memGB=0.25, maxBatch=8, batchMaxN=32768)
+    out = _run(S, [blocker] + dense)
+    S.close()
+    print("batch sizes", [inf["batch"] for _, inf in out],
+          "reserved GB", [round(inf["reservedGB"], 3) for _, inf in out])
+    assert [inf["batch"] for _, inf in out[1:]] == [2, 2]
+    assert all(inf["reservedGB"] > 0.125 for _, inf in out[1:])  # raised past half the budget
+    S1 = N.NPME_Server(nSlots=1, maxBatch=1)
+    alone = _run(S1, dense)
+    S1.close()
+    for (V, _), (Va, _) in zip(out[1:], alone):
+        assert np.array_equal(V, Va)
