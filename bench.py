@@ -127,7 +127,39 @@ def accuracy(h, q, V, n):
             "max_absV_over_meanV": float(np.max(np.abs(d[:, 0])) / magV)}
 
 
-def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
+class StallWatch:
+    """Calls on_stall once, from a thread of its own, when mark() has not
+    been called for stall_s seconds between start() and stop()."""
+
+    def __init__(self, stall_s, on_stall, poll_s=5.0):
+        import threading
+        self.stall_s, self.on_stall, self.poll_s = stall_s, on_stall, poll_s
+        self.last = time.monotonic()
+        self.count = 0
+        self._stop = threading.Event()
+        self._th = threading.Thread(target=self._run, daemon=True)
+
+    def mark(self):
+        self.last = time.monotonic()
+        self.count += 1
+
+    def start(self):
+        self.last = time.monotonic()
+        self._th.start()
+        return self
+
+    def stop(self):
+        self._stop.set()
+        self._th.join()
+
+    def _run(self):
+        while not self._stop.wait(self.poll_s):
+            if time.monotonic() - self.last > self.stall_s:
+                self.on_stall()
+                return
+
+
+def serving_stream(nreq, rate, slots, seed=2001, pinned=True, diag=None, stall_s=60.0):
     """Mixed-size request stream through the serving loop: Poisson arrivals
     at `rate`; N log-uniform in [500, 1e6]; positions uniform in a cube at
     density 0.1; charges U[-1,1] (zero net) or complex N(0,1); kernel 70%
@@ -138,7 +170,11 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
     The client builds each request in buffers from a pool of `npool`
     max-size buffer sets, reused as requests complete: page-locked host
     memory (N.pinned_empty) by default, as a serving front end registers
-    its request buffers, or ordinary pageable memory."""
+    its request buffers, or ordinary pageable memory.
+
+    A stream in which no request completes for stall_s seconds is ended:
+    the tracebacks of every thread go to diag (a file) and the process
+    exits with status 3, so a stalled stream is reported, not waited for."""
     import queue
     import threading
     import npme_amd as N
@@ -187,9 +223,22 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
             except Exception as e:  # recorded, reported in the JSON
                 errors.append(str(e))
             free.put(slot_of[i])
+            watch.mark()
             if (i + 1) % 1000 == 0:
                 log("stream: %d/%d requests done" % (i + 1, nreq))
 
+    def stalled():
+        import faulthandler
+        msg = "serving stream: no request completed for %.0f s (%d of %d done)" % (
+            stall_s, watch.count, nreq)
+        log(msg)
+        if diag is not None:
+            diag.write(msg + "\n")
+            diag.flush()
+            faulthandler.dump_traceback(file=diag, all_threads=True)
+        os._exit(3)
+
+    watch = StallWatch(stall_s, stalled).start()
     th = threading.Thread(target=consume)
     th.start()
 
@@ -235,6 +284,7 @@ def serving_stream(nreq, rate, slots, seed=2001, pinned=True):
         ready.release()
     pt.join()
     th.join()
+    watch.stop()
     S.close()
     gc.enable()
     ok = [i for i in range(nreq) if infos[i] is not None]
@@ -303,8 +353,9 @@ def serving_stream_child(a):
         tail = open(crash).read()[-4000:]
         log("serving stream process failed with status %d" % r.returncode)
         sys.stderr.write(tail)
-        return {"error": "serving stream process exited with status %d" % r.returncode,
-                "traceback": tail}
+        err = ("serving stream stalled: no request completed for 60 s" if r.returncode == 3 else
+               "serving stream process exited with status %d" % r.returncode)
+        return {"error": err, "traceback": tail}
     except subprocess.TimeoutExpired:
         return {"error": "serving stream process exceeded 900 s"}
     finally:
@@ -315,9 +366,10 @@ def main():
     import faulthandler
     a = parse()
     if a.serving_only:
-        faulthandler.enable(file=open(a.crash_log, "w") if a.crash_log else sys.stderr,
-                            all_threads=True)
-        lat = serving_stream(a.stream_requests, a.stream_rate, a.slots, pinned=not a.pageable_requests)
+        diag = open(a.crash_log, "w") if a.crash_log else sys.stderr
+        faulthandler.enable(file=diag, all_threads=True)
+        lat = serving_stream(a.stream_requests, a.stream_rate, a.slots, pinned=not a.pageable_requests,
+                             diag=diag)
         print(json.dumps({"latency": lat}))
         return 0
     faulthandler.enable(all_threads=True)

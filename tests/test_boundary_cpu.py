@@ -211,3 +211,19 @@ def test_user_define_func_tool_host():
     for p, rv in zip(xyz, rs):
         assert abs(rv - np.sqrt(sum(c * c for c in p))) < 1e-4
     assert "NPME_KernelFuncCheck passed" in r.stdout
+
+
+def test_bench_stall_watch():
+    """The serving stream's watchdog fires once when progress stops for
+    longer than its limit, and never while progress is marked."""
+    import time
+    import bench
+    fired = []
+    w = bench.StallWatch(0.3, lambda: fired.append(time.monotonic()), poll_s=0.02).start()
+    for _ in range(20):  # progress every 0.05 s for 1 s
+        time.sleep(0.05)
+        w.mark()
+    assert fired == [] and w.count == 20
+    time.sleep(0.6)
+    w.stop()
+    assert len(fired) == 1
